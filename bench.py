@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark: site-iterations/s of generate()'s loop on one MI355X.
+
+Workload (single terrain, FP64): 1,000,000 seeded uniform sites in
+[0,1000]^2, one Lloyd step, Delaunay graph; erodibility = clip(10^fBm, 0.1,
+10); outlets where a second fBm field < -0.2 (about 20% of sites); default
+uplift, m and max_iteration (run to convergence).
+
+A step is one full solve: elevations reset to the jittered initial state in
+HBM, then outer iterations until no elevation changes (src/lem/generator.rs:
+140-210), including the lake-traversal visit order that a fresh generate()
+call computes.  value = sites * iterations / seconds, over the timed steps.
+
+Printed: one JSON line with the metric, the roofline of the dominant kernel
+(algorithmic bytes per launch / HIP-event-timed launch duration) and the CPU
+baseline (the oracle, one core, on a capped number of iterations).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name: str, n: int, nnz: int) -> float:
+    """Algorithmic (compulsory) bytes of one launch: every array the kernel
+    must touch, read or written once; gathers and re-reads are not counted."""
+    table = {
+        # row_ptr 4 + h 8 + is_outlet 1, write next 4 + dnext 8; col 4 + len 8 per entry
+        "k_receivers": 25 * n + 12 * nnz,
+        # row_ptr 4 + prev area 8, write coff/ccnt/hc 12, clist 4; col 4 + next 4 per entry
+        "k_children": 28 * n + 8 * nnz,
+        # jc pc jn lw read + written
+        "k_chain_jump": 32 * n,
+        "k_chain_init": 28 * n,
+        # perm 4 + coff 4 + ccnt 4 + hc 4 + area 8 + A write 8 + clist 4 + children A 8 (per site)
+        "k_da_round": 44 * n,
+        # perm 4 + next 4 + jn 4 + x 8 + U 8 + tan 8 + dnext 8 + h 8 + writes tau 8 + h 8
+        "k_tau_round": 68 * n,
+        "k_celerity": 45 * n,
+    }
+    return float(table.get(name, 0))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sites", type=int, default=1_000_000)
+    ap.add_argument("--width", type=float, default=1000.0)
+    ap.add_argument("--cpu-iters", type=int, default=16, help="iterations of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this solve runs on one GPU; --gpus > 1 is not supported "
+                                   "(terrains shard across GPUs without communication, see DESIGN.md)"}))
+        return 2
+
+    import ctypes
+
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceTerrain
+    from fastlem_amd.generator import context, jitter_elevations
+    from fastlem_amd.workloads import config_fbm
+
+    t0 = time.time()
+    w = config_fbm(args.sites, args.width)
+    n = w.n
+    nnz = len(w.model.col)
+    log(f"workload built: n={n} nnz={nnz} outlets={len(w.outlets())} in {time.time() - t0:.1f}s")
+    init = jitter_elevations(w.base, 0)
+    t = DeviceTerrain.from_workload(w, init_elevation=init)
+    L = _native.lib()
+    ctx = context()
+
+    try:
+        import torch
+
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None
+
+    iters = None
+    for i in range(args.warmup):
+        L.fl_terrain_invalidate_cache(t._h)
+        ts = time.time()
+        iters = t.solve()
+        log(f"warmup {i}: {iters} iterations in {time.time() - ts:.3f}s")
+
+    sync()
+    times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        L.fl_terrain_invalidate_cache(t._h)
+        ts = time.perf_counter()
+        it = t.solve()
+        times.append(time.perf_counter() - ts)
+        if iters is not None and it != iters:
+            log(f"warning: iteration count changed {iters} -> {it}")
+        iters = it
+        log(f"step {i}: {it} iterations in {times[-1]:.4f}s")
+    sync()
+    elapsed = time.perf_counter() - t_start
+    site_iters = float(n) * iters * args.steps
+    value = site_iters / elapsed
+
+    # roofline of the dominant kernel: HIP events around every launch on the
+    # library's stream, one extra solve
+    L.fl_ctx_set_profiling(ctx, 1)
+    L.fl_ctx_reset_profile(ctx)
+    t.solve()
+    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_chain_len", "k_bucket_count",
+             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_round", "k_celerity", "k_tau_round",
+             "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
+    prof = {}
+    for nm in names:
+        cnt = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms))
+        if cnt.value:
+            prof[nm] = (cnt.value, ms.value)
+    L.fl_ctx_set_profiling(ctx, 0)
+    total_kernel_ms = sum(v[1] for v in prof.values())
+    for nm, (cnt, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us")
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    dcnt, dms = prof[dom]
+    avg_s = dms / dcnt / 1e3
+    b = kernel_bytes(dom, n, nnz)
+    achieved = b / avg_s / 1e9 if b else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import flo
+
+        rp, col, ln = w.model.graph()
+        g = flo.Graph(rp, col, ln)
+        ts = time.perf_counter()
+        _, cit, _ = flo.generate(g, w.model.areas(), w.erodibility, w.uplift, init, w.outlets(),
+                                 max_iteration=args.cpu_iters)
+        cs = time.perf_counter() - ts
+        cpu = {"value": float(n) * cit / cs, "unit": "site-iterations/s", "cores": 1, "kind": "port",
+               "sample": f"oracle generate() capped at max_iteration={cit} on the same {n}-site terrain "
+                         f"({cs:.1f} s, iteration 1 includes the lake heap)"}
+        log(f"cpu baseline: {cit} iterations in {cs:.2f}s")
+
+    line = {
+        "metric": "site-iterations/s (1M-site FP64 terrain, generate() loop to convergence)",
+        "value": value,
+        "unit": "site-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fbm-terrain sites={n} box=[0,{args.width:g}]^2 lloyd=1 "
+                               f"K=clip(10^fBm,0.1,10) ocean=fBm<-0.2 outlets={len(w.outlets())} nnz={nnz}",
+                   "iterations": iters},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

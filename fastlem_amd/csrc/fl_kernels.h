@@ -1,0 +1,89 @@
+// fl_kernels.h -- device buffers of one terrain and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+// Per-kernel timing with HIP events recorded on the launch stream.
+struct fl_profiler {
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<uint64_t, double>> totals;
+  hipEvent_t get();
+  void record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b);
+  void flush();
+  void reset();
+  ~fl_profiler();
+};
+
+// Set by the C ABI for the duration of a call on a profiled context.
+extern thread_local fl_profiler* fl_current_profiler;
+
+enum { FL_FLAG_LAKE = 0, FL_FLAG_JUMP = 1, FL_FLAG_CHANGED = 2, FL_NUM_FLAGS = 4 };
+enum { FL_CNT_CHILDREN = 0, FL_CNT_MAXLD = 1, FL_NUM_COUNTERS = 4 };
+
+// Read-only inputs of one terrain, resident in HBM.
+struct fl_dev_terrain {
+  uint32_t n = 0;
+  uint32_t nnz = 0;
+  uint32_t* row_ptr = nullptr;   // [n+1]
+  uint32_t* col = nullptr;       // [nnz]
+  double* len = nullptr;         // [nnz]
+  double* area = nullptr;        // [n]
+  double* erod = nullptr;        // [n]
+  double* uplift = nullptr;      // [n]
+  double* tan_ms = nullptr;      // [n] tan(max_slope), NaN when None
+  double* init_h = nullptr;      // [n]
+  uint8_t* is_outlet = nullptr;  // [n]
+  uint32_t* rank = nullptr;      // [n] lake-traversal visit rank (UINT32_MAX: never)
+  uint32_t* order = nullptr;     // [n] visited sites by rank
+};
+
+// Per-iteration state and scratch of one terrain.
+struct fl_dev_work {
+  double* h = nullptr;      // [n] elevations
+  double* A = nullptr;      // [n] drainage area (kept: heavy-child hint)
+  double* tau = nullptr;    // [n] response time
+  double* x = nullptr;      // [n] 1/celerity * distance
+  double* dnext = nullptr;  // [n] distance to the receiver
+  uint32_t* next = nullptr; // [n]
+  // lakes
+  uint32_t* sub = nullptr;
+  uint32_t* sub2 = nullptr;
+  uint64_t* lake_key = nullptr;
+  // decomposition
+  uint32_t* coff = nullptr;
+  uint32_t* ccnt = nullptr;
+  uint32_t* clist = nullptr;
+  uint32_t* hc = nullptr;
+  uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;
+  uint32_t *jc2 = nullptr, *pc2 = nullptr, *jn2 = nullptr, *lw2 = nullptr;
+  uint32_t* clen = nullptr;
+  uint32_t *bcnt = nullptr, *bnodes = nullptr, *boff = nullptr, *bnoff = nullptr, *bcur = nullptr,
+           *bncur = nullptr;                                  // [n+2] light-depth buckets
+  uint32_t *chain_base = nullptr, *chain_len = nullptr;      // [n] chains by bucket
+  uint32_t* head_base = nullptr;                             // [n]
+  uint32_t* perm = nullptr;                                  // [n] chain-major site order
+  int* flags = nullptr;                                      // [FL_NUM_FLAGS]
+  uint32_t* counters = nullptr;                              // [FL_NUM_COUNTERS]
+};
+
+void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
+void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
+void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t nb);
+void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1);
+void fl_launch_celerity(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1);
+void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);

@@ -1,0 +1,382 @@
+// fl_setup.cpp -- one-time host setup for the uplift-erosion solve.
+//
+// Everything here runs once per model (or once per generate() call for the
+// jitter) on the host; none of it is inside the iteration loop:
+//   * a restatement of rand 0.8.5's StdRng (ChaCha12, rand_chacha 0.3) with
+//     SeedableRng::seed_from_u64 / from_seed, Rng::gen::<f64>() and
+//     Rng::gen_range(f64..f64), used by
+//       src/models/surface/builder.rs:38-52 (from_random_sites) and
+//       src/lem/generator.rs:134-138 (elevation jitter);
+//   * clipped Voronoi cell areas and centroids (the cell areas of
+//     src/models/surface/builder.rs:236-247 and the Lloyd step of :155-200);
+//   * the adjacency build of src/models/surface/builder.rs:249-268 (edge
+//     (a,b) added only when a<b, insertion-ordered rows);
+//   * the static visit order of the lake-removal heap
+//     (src/lem/stream_tree.rs:184-240), which depends only on the graph and
+//     the outlet order.
+//
+// Built with -ffp-contract=off: every floating-point expression is evaluated
+// as written, as rustc does.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "fl_internal.h"
+
+// ------------------------------------------------------------ ChaCha12 ----
+
+namespace {
+
+inline uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+
+inline void quarter(uint32_t* x, int a, int b, int c, int d) {
+  x[a] += x[b]; x[d] ^= x[a]; x[d] = rotl32(x[d], 16);
+  x[c] += x[d]; x[b] ^= x[c]; x[b] = rotl32(x[b], 12);
+  x[a] += x[b]; x[d] ^= x[a]; x[d] = rotl32(x[d], 8);
+  x[c] += x[d]; x[b] ^= x[c]; x[b] = rotl32(x[b], 7);
+}
+
+}  // namespace
+
+// One ChaCha block: `rounds` rounds (12 for StdRng, 20 for RFC 8439) over the
+// 16-word input state; out = permuted state + input state.
+extern "C" void fl_chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]) {
+  uint32_t x[16];
+  std::memcpy(x, in, sizeof(x));
+  for (int r = 0; r < rounds; r += 2) {
+    quarter(x, 0, 4, 8, 12);
+    quarter(x, 1, 5, 9, 13);
+    quarter(x, 2, 6, 10, 14);
+    quarter(x, 3, 7, 11, 15);
+    quarter(x, 0, 5, 10, 15);
+    quarter(x, 1, 6, 11, 12);
+    quarter(x, 2, 7, 8, 13);
+    quarter(x, 3, 4, 9, 14);
+  }
+  for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
+}
+
+struct fl_rng {
+  uint32_t key[8];
+  uint64_t counter;   // next block counter
+  uint32_t buf[64];   // four blocks, as rand_chacha buffers them
+  uint32_t index;     // next word in buf (64 = empty)
+};
+
+static void rng_refill(fl_rng* r) {
+  for (int b = 0; b < 4; b++) {
+    uint32_t st[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+    for (int k = 0; k < 8; k++) st[4 + k] = r->key[k];
+    st[12] = (uint32_t)(r->counter & 0xffffffffu);
+    st[13] = (uint32_t)(r->counter >> 32);
+    st[14] = 0;  // stream id 0
+    st[15] = 0;
+    fl_chacha_block(12, st, r->buf + 16 * b);
+    r->counter++;
+  }
+  r->index = 0;
+}
+
+extern "C" void fl_rng_from_seed(fl_rng* r, const uint8_t seed[32]) {
+  for (int k = 0; k < 8; k++)
+    r->key[k] = (uint32_t)seed[4 * k] | ((uint32_t)seed[4 * k + 1] << 8) |
+                ((uint32_t)seed[4 * k + 2] << 16) | ((uint32_t)seed[4 * k + 3] << 24);
+  r->counter = 0;
+  r->index = 64;
+}
+
+// rand_core 0.6 SeedableRng::seed_from_u64: PCG32 output words fill the seed.
+extern "C" void fl_rng_seed_from_u64(fl_rng* r, uint64_t state) {
+  const uint64_t MUL = 6364136223846793005ull;
+  const uint64_t INC = 11634580027462260723ull;
+  uint8_t seed[32];
+  for (int c = 0; c < 8; c++) {
+    state = state * MUL + INC;
+    uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    uint32_t x = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+    seed[4 * c + 0] = (uint8_t)(x);
+    seed[4 * c + 1] = (uint8_t)(x >> 8);
+    seed[4 * c + 2] = (uint8_t)(x >> 16);
+    seed[4 * c + 3] = (uint8_t)(x >> 24);
+  }
+  fl_rng_from_seed(r, seed);
+}
+
+extern "C" uint64_t fl_rng_next_u64(fl_rng* r) {
+  // BlockRng::next_u64 with only 64-bit draws: the index stays even.
+  if (r->index >= 64) rng_refill(r);
+  uint64_t lo = r->buf[r->index], hi = r->buf[r->index + 1];
+  r->index += 2;
+  return (hi << 32) | lo;
+}
+
+// Standard distribution for f64: 53 high bits times 2^-53.
+extern "C" double fl_rng_gen_f64(fl_rng* r) {
+  uint64_t v = fl_rng_next_u64(r) >> 11;
+  return (double)v * (1.0 / 9007199254740992.0);
+}
+
+// UniformFloat::<f64>::sample_single(low, high): [1,2) mantissa trick,
+// value0_1 * scale + low, rejected while not < high.
+extern "C" double fl_rng_gen_range_f64(fl_rng* r, double low, double high) {
+  double scale = high - low;
+  for (;;) {
+    uint64_t bits = (fl_rng_next_u64(r) >> 12) | 0x3ff0000000000000ull;
+    double v12;
+    std::memcpy(&v12, &bits, sizeof(v12));
+    double v01 = v12 - 1.0;
+    double res = v01 * scale + low;
+    if (res < high) return res;
+  }
+}
+
+extern "C" size_t fl_rng_state_size(void) { return sizeof(fl_rng); }
+
+// Fill out[i] = base[i] + gen::<f64>() * f64::EPSILON with a StdRng seeded by
+// seed_from_u64(seed) (generator.rs:134-138 uses seed 0).
+extern "C" void fl_jitter_elevations(uint64_t seed, size_t n, const double* base, double* out) {
+  fl_rng r;
+  fl_rng_seed_from_u64(&r, seed);
+  const double eps = 2.220446049250313e-16;  // f64::EPSILON
+  for (size_t i = 0; i < n; i++) out[i] = base[i] + fl_rng_gen_f64(&r) * eps;
+}
+
+// n sites drawn as from_random_sites(n, min, max): StdRng::from_seed([0;32]),
+// x then y per site (builder.rs:38-52).
+extern "C" void fl_random_sites(size_t n, double xmin, double ymin, double xmax, double ymax, double* xs,
+                                double* ys) {
+  fl_rng r;
+  uint8_t seed[32] = {0};
+  fl_rng_from_seed(&r, seed);
+  for (size_t i = 0; i < n; i++) {
+    xs[i] = fl_rng_gen_range_f64(&r, xmin, xmax);
+    ys[i] = fl_rng_gen_range_f64(&r, ymin, ymax);
+  }
+}
+
+// ------------------------------------------------------- Voronoi cells ----
+
+namespace {
+
+struct P2 {
+  double x, y;
+};
+
+// Keep the part of polygon `in` with (p - m) . nrm <= 0.
+void clip_halfplane(const std::vector<P2>& in, std::vector<P2>& out, P2 m, P2 nrm) {
+  out.clear();
+  size_t k = in.size();
+  for (size_t a = 0; a < k; a++) {
+    const P2& p = in[a];
+    const P2& q = in[(a + 1) % k];
+    double fp = (p.x - m.x) * nrm.x + (p.y - m.y) * nrm.y;
+    double fq = (q.x - m.x) * nrm.x + (q.y - m.y) * nrm.y;
+    if (fp <= 0) out.push_back(p);
+    if ((fp < 0 && fq > 0) || (fp > 0 && fq < 0)) {
+      double t = fp / (fp - fq);
+      out.push_back(P2{p.x + (q.x - p.x) * t, p.y + (q.y - p.y) * t});
+    }
+  }
+}
+
+}  // namespace
+
+// Voronoi cell of every site clipped to the box [xmin,xmax]x[ymin,ymax]:
+// cell i = box intersected with the bisector half-planes of i's Delaunay
+// neighbours (tris: ntri triangles of site indices).  Writes the shoelace
+// area and the polygon centroid (the Lloyd target).
+extern "C" int fl_voronoi_cells(size_t n, const double* xs, const double* ys, size_t ntri, const int64_t* tris,
+                                double xmin, double ymin, double xmax, double ymax, double* area_out,
+                                double* cx_out, double* cy_out) {
+  std::vector<uint32_t> deg(n + 1, 0);
+  for (size_t t = 0; t < ntri; t++)
+    for (int e = 0; e < 3; e++) {
+      int64_t a = tris[3 * t + e];
+      if (a < 0 || (size_t)a >= n) return FL_ERR_INVALID;
+      deg[a] += 2;
+    }
+  std::vector<uint64_t> off(n + 1, 0);
+  for (size_t i = 0; i < n; i++) off[i + 1] = off[i] + deg[i];
+  std::vector<uint32_t> nb(off[n]);
+  std::vector<uint64_t> fill(off.begin(), off.end() - 1);
+  for (size_t t = 0; t < ntri; t++)
+    for (int e = 0; e < 3; e++) {
+      int64_t a = tris[3 * t + e], b = tris[3 * t + (e + 1) % 3], c = tris[3 * t + (e + 2) % 3];
+      nb[fill[a]++] = (uint32_t)b;
+      nb[fill[a]++] = (uint32_t)c;
+    }
+  std::vector<P2> poly, tmp;
+  for (size_t i = 0; i < n; i++) {
+    poly.assign({P2{xmin, ymin}, P2{xmax, ymin}, P2{xmax, ymax}, P2{xmin, ymax}});
+    P2 s{xs[i], ys[i]};
+    for (uint64_t k = off[i]; k < off[i + 1] && !poly.empty(); k++) {
+      uint32_t j = nb[k];
+      P2 o{xs[j], ys[j]};
+      P2 m{(s.x + o.x) * 0.5, (s.y + o.y) * 0.5};
+      P2 nrm{o.x - s.x, o.y - s.y};
+      clip_halfplane(poly, tmp, m, nrm);
+      poly.swap(tmp);
+    }
+    double a2 = 0, cx = 0, cy = 0;
+    size_t k = poly.size();
+    for (size_t a = 0; a < k; a++) {
+      const P2& p = poly[a];
+      const P2& q = poly[(a + 1) % k];
+      double cr = p.x * q.y - q.x * p.y;
+      a2 += cr;
+      cx += (p.x + q.x) * cr;
+      cy += (p.y + q.y) * cr;
+    }
+    area_out[i] = std::fabs(a2) / 2.0;
+    if (a2 != 0) {
+      cx_out[i] = cx / (3.0 * a2);
+      cy_out[i] = cy / (3.0 * a2);
+    } else {
+      cx_out[i] = s.x;
+      cy_out[i] = s.y;
+    }
+  }
+  return FL_OK;
+}
+
+// ------------------------------------------------------------ adjacency ----
+
+// builder.rs:249-268: for each triangle (a,b,c) in order add (a,b) if a<b,
+// (b,c) if b<c, (c,a) if c<a; add_edge appends to both rows, so rows keep
+// insertion order.  Edge attribute = Site2D::distance (sites.rs:27-29).
+// row_ptr must hold n+1 entries; col/len at least 6*ntri... call with
+// col == NULL first to get the entry count in row_ptr[n].
+extern "C" int fl_build_adjacency(size_t n, const double* xs, const double* ys, size_t ntri, const int64_t* tris,
+                                  uint32_t* row_ptr, uint32_t* col, double* len) {
+  std::vector<uint64_t> deg(n, 0);
+  auto each_edge = [&](auto&& fn) {
+    for (size_t t = 0; t < ntri; t++) {
+      int64_t a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+      if (a < b) fn(a, b);
+      if (b < c) fn(b, c);
+      if (c < a) fn(c, a);
+    }
+  };
+  for (size_t t = 0; t < 3 * ntri; t++)
+    if (tris[t] < 0 || (size_t)tris[t] >= n) return FL_ERR_INVALID;
+  each_edge([&](int64_t a, int64_t b) {
+    deg[a]++;
+    deg[b]++;
+  });
+  uint64_t total = 0;
+  row_ptr[0] = 0;
+  for (size_t i = 0; i < n; i++) {
+    total += deg[i];
+    if (total > 0xffffffffull) return FL_ERR_TOO_LARGE;
+    row_ptr[i + 1] = (uint32_t)total;
+  }
+  if (!col) return FL_OK;
+  std::vector<uint32_t> fill(row_ptr, row_ptr + n);
+  each_edge([&](int64_t a, int64_t b) {
+    double dx = xs[a] - xs[b], dy = ys[a] - ys[b];
+    double d = std::sqrt(dx * dx + dy * dy);
+    col[fill[a]] = (uint32_t)b;
+    len[fill[a]++] = d;
+    col[fill[b]] = (uint32_t)a;
+    len[fill[b]++] = d;
+  });
+  return FL_OK;
+}
+
+// ---------------------------------------------------- lake visit order ----
+
+namespace {
+
+struct Ridge {
+  uint32_t index;
+  double dist;
+};
+
+// RidgeElement ordering (stream_tree.rs:34-38): a <= b iff a.dist >= b.dist,
+// so the max-heap yields the shortest edge first.
+inline bool le(const Ridge& a, const Ridge& b) { return a.dist >= b.dist; }
+
+// std::collections::BinaryHeap mechanics: push = append + sift_up (stop when
+// elem <= parent); pop = take root, move last to root, sift_down_to_bottom
+// (descend to the larger child, right on ties) then sift_up.
+struct RustHeap {
+  std::vector<Ridge> d;
+  size_t sift_up(size_t start, size_t pos) {
+    Ridge e = d[pos];
+    while (pos > start) {
+      size_t parent = (pos - 1) / 2;
+      if (le(e, d[parent])) break;
+      d[pos] = d[parent];
+      pos = parent;
+    }
+    d[pos] = e;
+    return pos;
+  }
+  void push(Ridge x) {
+    d.push_back(x);
+    sift_up(0, d.size() - 1);
+  }
+  bool pop(Ridge* out) {
+    if (d.empty()) return false;
+    Ridge item = d.back();
+    d.pop_back();
+    if (!d.empty()) {
+      std::swap(item, d[0]);
+      size_t end = d.size(), pos = 0;
+      Ridge e = d[0];
+      size_t child = 1;
+      while (end >= 2 && child <= end - 2) {
+        child += le(d[child], d[child + 1]) ? 1 : 0;
+        d[pos] = d[child];
+        pos = child;
+        child = 2 * pos + 1;
+      }
+      if (child == end - 1) {
+        d[pos] = d[child];
+        pos = child;
+      }
+      d[pos] = e;
+      sift_up(0, pos);
+    }
+    *out = item;
+    return true;
+  }
+};
+
+}  // namespace
+
+// Visit order of the lake-removal traversal: sites in the order in which the
+// heap first pops them.  rank_out[i] = position of i in that order, or
+// UINT32_MAX when i is never reached (no path to an outlet).  The flips the
+// reference performs during the traversal never change what is pushed, so
+// this order is a function of (graph, outlet order) alone.
+extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const double* len,
+                                   const uint32_t* outlets, size_t n_outlets, uint32_t* rank_out,
+                                   uint64_t* visited_out) {
+  RustHeap h;
+  h.d.reserve(n);
+  std::vector<uint8_t> visited(n, 0);
+  for (size_t i = 0; i < n; i++) rank_out[i] = UINT32_MAX;
+  for (size_t k = 0; k < n_outlets; k++) {
+    if (outlets[k] >= n) return FL_ERR_INVALID;
+    h.push(Ridge{outlets[k], 0.0});
+  }
+  uint32_t cnt = 0;
+  Ridge el;
+  while (h.pop(&el)) {
+    uint32_t i = el.index;
+    if (visited[i]) continue;
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; e++) {
+      uint32_t j = col[e];
+      if (visited[j]) continue;
+      if (std::isnan(len[e])) return FL_ERR_NAN_LENGTH;
+      h.push(Ridge{j, len[e]});
+    }
+    visited[i] = 1;
+    rank_out[i] = cnt++;
+  }
+  if (visited_out) *visited_out = cnt;
+  return FL_OK;
+}

@@ -1,0 +1,149 @@
+/*
+ * fastlem_hip.h -- C ABI of the MI355X (gfx950) uplift-erosion solve.
+ *
+ * This library replaces the body of fastlem's TerrainGenerator::generate()
+ * (src/lem/generator.rs:118-210 of fastlem 0.1.4): receivers, lake labelling
+ * and joining, per-outlet drainage basins, drainage area, response time and
+ * elevation, iterated until no elevation changes or max_iteration.  The
+ * caller keeps the model build, parameter validation (generator.rs:91-116),
+ * outlet resolution (generator.rs:120-132) and the elevation jitter
+ * (generator.rs:134-138) and passes their results in.
+ *
+ * Conventions
+ *   - Every function returns int32_t: FL_OK (0) or a negative FL_ERR_* code;
+ *     fl_last_error() returns the calling thread's last message.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Indices are uint32_t; a terrain may hold up to 2^32-1 sites and 2^32-1
+ *     directed adjacency entries.
+ *   - Arithmetic is IEEE binary64 evaluated exactly as the reference writes
+ *     it (no contraction), so results are bit-identical to the reference
+ *     given the same inputs.
+ *   - A context is bound to one HIP device and one stream; calls on one
+ *     context must not overlap.  Different contexts are independent.
+ */
+#ifndef FASTLEM_HIP_H
+#define FASTLEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FL_OK = 0,
+  FL_ERR_INVALID = -1,     /* bad argument (null pointer, index out of range) */
+  FL_ERR_HIP = -2,         /* HIP runtime failure; message has the HIP error */
+  FL_ERR_OUT_OF_MEMORY = -3,
+  FL_ERR_TOO_LARGE = -4,   /* sizes beyond the uint32 index range */
+  FL_ERR_NAN_LENGTH = -5,  /* NaN edge length: the reference heap panics
+                              (stream_tree.rs:36 unwraps partial_cmp) */
+  FL_ERR_NO_DEVICE = -6,
+  FL_ERR_BUDGET = -7,      /* request exceeds the server's HBM budget */
+};
+
+#define FL_UNBOUNDED_ITERATIONS 0xffffffffu /* max_iteration = None */
+
+/* One terrain as generate() sees it through the Model trait
+ * (src/core/traits.rs:13-20) plus the resolved per-site parameters
+ * (src/core/parameters.rs:24-30). */
+typedef struct fl_terrain_desc {
+  uint64_t n;                    /* model.num() (model.rs:42-44) */
+  uint64_t nnz;                  /* total adjacency entries = row_ptr[n] */
+  const uint32_t *row_ptr;       /* [n+1] CSR row offsets */
+  const uint32_t *col;           /* [nnz] neighbors_of(i) site indices, in order */
+  const double *length;          /* [nnz] neighbors_of(i) edge attributes */
+  const double *area;            /* [n] model.areas() */
+  const double *erodibility;     /* [n] TopographicalParameters::erodibility */
+  const double *uplift_rate;     /* [n] TopographicalParameters::uplift_rate */
+  const uint8_t *has_max_slope;  /* [n] max_slope.is_some(); NULL = all None */
+  const double *max_slope;       /* [n] max_slope in radians (tan taken here) */
+  const double *init_elevation;  /* [n] base_elevation + jitter (generator.rs:134-138) */
+  const uint32_t *outlets;       /* [n_outlets] resolved outlets, generator.rs:120-132 order */
+  uint64_t n_outlets;
+  uint32_t max_iteration;        /* FL_UNBOUNDED_ITERATIONS for None */
+  uint32_t reserved;
+} fl_terrain_desc;
+
+typedef struct fl_ctx fl_ctx;         /* device + stream + scratch */
+typedef struct fl_terrain fl_terrain; /* one terrain resident in HBM */
+
+const char *fl_last_error(void);
+const char *fl_version(void);
+
+/* Context on HIP device `device`. */
+int32_t fl_ctx_create(int32_t device, fl_ctx **out);
+int32_t fl_ctx_destroy(fl_ctx *ctx);
+
+/* Drop-in for generator.rs:118-210: upload, iterate, download.
+ * elevations_out: [n]; iterations_out (may be NULL): outer iterations run. */
+int32_t fl_generate_f64(fl_ctx *ctx, const fl_terrain_desc *desc, double *elevations_out,
+                        uint32_t *iterations_out);
+
+/* Device-resident form: upload once, solve many times. */
+int32_t fl_terrain_create(fl_ctx *ctx, const fl_terrain_desc *desc, fl_terrain **out);
+int32_t fl_terrain_destroy(fl_terrain *t);
+/* Reset elevations to init_elevation and iterate (generator.rs:140-210). */
+int32_t fl_terrain_solve(fl_ctx *ctx, fl_terrain *t, uint32_t max_iteration, uint32_t *iterations_out);
+int32_t fl_terrain_download(fl_ctx *ctx, fl_terrain *t, double *elevations_out);
+/* Device pointer of the current elevations (f64[n]), for callers that keep
+ * the result in HBM. */
+int32_t fl_terrain_device_elevations(fl_terrain *t, double **dev_ptr);
+/* Bytes of HBM the terrain holds. */
+int32_t fl_terrain_device_bytes(const fl_terrain *t, uint64_t *bytes);
+
+/* Forget cached per-terrain host work (the lake-traversal visit order), so
+ * the next solve recomputes it as a fresh generate() call would. */
+int32_t fl_terrain_invalidate_cache(fl_terrain *t);
+
+/* Kernel timing: when enabled, every launch on the context's stream is
+ * bracketed by HIP events and its duration accumulated per kernel name. */
+int32_t fl_ctx_set_profiling(fl_ctx *ctx, int32_t enable);
+/* Accumulated launches and milliseconds of kernel `name` since the last
+ * reset; fl_ctx_reset_profile clears all counters. */
+int32_t fl_ctx_kernel_time(fl_ctx *ctx, const char *name, uint64_t *launches, double *total_ms);
+int32_t fl_ctx_reset_profile(fl_ctx *ctx);
+
+/* Teacher-forced stages (verification entry points): run one stage on the
+ * device from the given state and return its output.
+ *   receivers:   stream_tree.rs:109-137            -> next[n]
+ *   stream tree: stream_tree.rs:72-99 (incl. lakes) -> next[n], subroot[n]
+ *                (subroot = pre-join chain end), returns has_lake in *flag
+ *   iteration:   generator.rs:141-204 from elev     -> elev updated in place,
+ *                next, drainage area, response time; *flag = changed */
+int32_t fl_stage_receivers(fl_ctx *ctx, fl_terrain *t, const double *elev, uint32_t *next_out);
+int32_t fl_stage_stream_tree(fl_ctx *ctx, fl_terrain *t, const double *elev, uint32_t *next_out,
+                             uint32_t *subroot_out, int32_t *flag);
+int32_t fl_stage_iteration(fl_ctx *ctx, fl_terrain *t, double *elev, uint32_t *next_out, double *area_out,
+                           double *response_out, int32_t *flag);
+
+/* ---- host setup helpers (one-time, CPU) ---- */
+
+/* StdRng (rand 0.8.5 ChaCha12) restatement used for the jitter of
+ * generator.rs:134-138: out[i] = base[i] + gen::<f64>() * f64::EPSILON with
+ * StdRng::seed_from_u64(seed). */
+void fl_jitter_elevations(uint64_t seed, size_t n, const double *base, double *out);
+/* from_random_sites (builder.rs:38-52): StdRng::from_seed([0;32]),
+ * gen_range(x) then gen_range(y) per site. */
+void fl_random_sites(size_t n, double xmin, double ymin, double xmax, double ymax, double *xs, double *ys);
+/* ChaCha block function (rounds = 12 for StdRng). */
+void fl_chacha_block(int rounds, const uint32_t in[16], uint32_t out[16]);
+/* Voronoi cells clipped to the box: areas (builder.rs:236-247) and
+ * centroids (the Lloyd step of builder.rs:155-200). */
+int fl_voronoi_cells(size_t n, const double *xs, const double *ys, size_t ntri, const int64_t *tris, double xmin,
+                     double ymin, double xmax, double ymax, double *area_out, double *cx_out, double *cy_out);
+/* Adjacency from triangles with the a<b rule (builder.rs:249-268).  Call with
+ * col == NULL to size: row_ptr[n] = entry count. */
+int fl_build_adjacency(size_t n, const double *xs, const double *ys, size_t ntri, const int64_t *tris,
+                       uint32_t *row_ptr, uint32_t *col, double *len);
+/* Visit order of the lake-removal heap (stream_tree.rs:184-240):
+ * rank_out[i] = first-pop position of site i, UINT32_MAX if never reached. */
+int fl_lake_visit_order(size_t n, const uint32_t *row_ptr, const uint32_t *col, const double *len,
+                        const uint32_t *outlets, size_t n_outlets, uint32_t *rank_out, uint64_t *visited_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTLEM_HIP_H */
