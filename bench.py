@@ -112,20 +112,22 @@ def main() -> int:
         if iters is not None and it != iters:
             log(f"warning: iteration count changed {iters} -> {it}")
         iters = it
-        log(f"step {i}: {it} iterations in {times[-1]:.4f}s")
+        log(f"step {i}: {it} iterations in {times[-1]:.4f}s  stats={t.last_stats()}")
     sync()
     elapsed = time.perf_counter() - t_start
     site_iters = float(n) * iters * args.steps
     value = site_iters / elapsed
 
+    stats = t.last_stats()
     # roofline of the dominant kernel: HIP events around every launch on the
     # library's stream, one extra solve
     L.fl_ctx_set_profiling(ctx, 1)
     L.fl_ctx_reset_profile(ctx)
     t.solve()
     names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_chain_len", "k_bucket_count",
-             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_round", "k_celerity", "k_tau_round",
-             "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
+             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_chain", "k_tau_prep",
+             "k_tau_round", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip",
+             "k_fill_u32", "k_fill_u64", "k_copy_u32"]
     prof = {}
     for nm in names:
         cnt = ctypes.c_uint64()
@@ -176,7 +178,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"fbm-terrain sites={n} box=[0,{args.width:g}]^2 lloyd=1 "
                                f"K=clip(10^fBm,0.1,10) ocean=fBm<-0.2 outlets={len(w.outlets())} nnz={nnz}",
-                   "iterations": iters},
+                   "iterations": iters, "visit_order_ms": stats["visit_order_ms"],
+                   "host_syncs_per_solve": stats["host_syncs"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

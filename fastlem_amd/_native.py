@@ -42,6 +42,14 @@ class TerrainDesc(ctypes.Structure):
     ]
 
 
+class SolveStats(ctypes.Structure):
+    """Mirror of ``fl_solve_stats``."""
+
+    _fields_ = [("iterations", ctypes.c_uint32), ("lake_iterations", ctypes.c_uint32),
+                ("host_syncs", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"fastlem_hip error {code}: {message}")
@@ -68,6 +76,7 @@ def _declare(lib):
         "fl_terrain_device_elevations": (i32, [vp, ctypes.POINTER(vp)]),
         "fl_terrain_device_bytes": (i32, [vp, ctypes.POINTER(u64)]),
         "fl_terrain_invalidate_cache": (i32, [vp]),
+        "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64)]),
         "fl_ctx_reset_profile": (i32, [vp]),

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,8 +60,7 @@ struct fl_ctx {
   int device = 0;
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
-  int* host_flags = nullptr;        // pinned readback of flags
-  uint32_t* host_counters = nullptr;
+  uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
   std::vector<uint32_t> host_boff;
 };
 
@@ -70,11 +70,17 @@ struct fl_terrain {
   fl_dev_work w;
   uint32_t max_iteration = FL_UNBOUNDED_ITERATIONS;
   uint32_t prev_nb = 0;
+  uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
   bool have_order = false;
+  // statistics of the last solve
+  uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0;
+  double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   uint64_t bytes = 0;
-  // host copies needed for the one-time visit order
+  // host copies (relabelled) needed for the one-time visit order
   std::vector<uint32_t> h_row_ptr, h_col, h_outlets;
   std::vector<double> h_len;
+  // locality relabelling: device site u holds caller site old_of_new[u]
+  std::vector<uint32_t> new_of_old, old_of_new;
   std::vector<void*> allocations;
 };
 
@@ -140,19 +146,20 @@ int32_t validate(const fl_terrain_desc* d) {
   return FL_OK;
 }
 
-int32_t read_flags(fl_ctx* c, const fl_terrain* T) {
-  HIPCHK(hipMemcpyAsync(c->host_flags, T->w.flags, FL_NUM_FLAGS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+int32_t read_ctrl(fl_ctx* c, fl_terrain* T) {
+  HIPCHK(hipMemcpyAsync(c->host_ctrl, T->w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  T->stat_syncs++;
   return FL_OK;
 }
 
-int32_t clear_flags(fl_ctx* c, fl_terrain* T) {
-  fl_launch_fill_u32(c->stream, (uint32_t*)T->w.flags, FL_NUM_FLAGS, 0);
-  return FL_OK;
-}
+inline int host_flag(const fl_ctx* c, int f) { return (int)c->host_ctrl[f]; }
+constexpr uint32_t FL_NONE_ROUND = 0xffffffffu;
+inline uint32_t host_counter(const fl_ctx* c, int k) { return c->host_ctrl[FL_NUM_FLAGS + k]; }
 
 int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   if (T->have_order) return FL_OK;
+  auto t0 = std::chrono::steady_clock::now();
   uint32_t n = T->t.n;
   std::vector<uint32_t> rank(n), order(n, 0xffffffffu);
   uint64_t cnt = 0;
@@ -166,88 +173,120 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   HIPCHK(hipMemcpyAsync(T->t.order, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   T->have_order = true;
+  T->stat_order_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return FL_OK;
 }
 
-// Receivers and, if any lake exists, lake joining (StreamTree::construct,
-// stream_tree.rs:72-99).  want_sub forces the chain-end labels (for the
-// stage API) even without lakes.
-int32_t stream_tree(fl_ctx* c, fl_terrain* T, bool want_sub, bool* had_lake) {
+// Chain-end labels of the receiver forest by pointer jumping (the subroot of
+// find_roots_with_lakes, stream_tree.rs:139-173) into w.sub.
+int32_t label_chain_ends(fl_ctx* c, fl_terrain* T) {
   hipStream_t s = c->stream;
-  FLCHK(clear_flags(c, T));
-  fl_launch_receivers(s, T->t, T->w);
-  FLCHK(read_flags(c, T));
-  bool lake = c->host_flags[FL_FLAG_LAKE] != 0;
-  *had_lake = lake;
-  if (!lake && !want_sub) return FL_OK;
   fl_launch_sub_init(s, T->t, T->w);
   bool flip = false;
   for (uint32_t round = 0;; round++) {
     fl_launch_fill_u32(s, (uint32_t*)T->w.flags + FL_FLAG_JUMP, 1, 0);
     fl_launch_sub_jump(s, T->t, T->w, flip);
     flip = !flip;
-    FLCHK(read_flags(c, T));
-    if (!c->host_flags[FL_FLAG_JUMP]) break;
+    FLCHK(read_ctrl(c, T));
+    if (!host_flag(c, FL_FLAG_JUMP)) break;
     if (round > 64) return set_err(FL_ERR_HIP, "chain-end labelling did not converge");
   }
   if (flip) std::swap(T->w.sub, T->w.sub2);
-  if (!lake) return FL_OK;
-  FLCHK(ensure_visit_order(c, T));
-  fl_launch_lake_join(s, T->t, T->w);
   return FL_OK;
 }
 
-// Everything after the stream tree: decomposition and the two sweeps.
-int32_t tree_sweeps(fl_ctx* c, fl_terrain* T, bool* changed) {
+// Lake joining (stream_tree.rs:175-243) on the receivers in w.next.
+int32_t join_lakes(fl_ctx* c, fl_terrain* T) {
+  FLCHK(label_chain_ends(c, T));
+  FLCHK(ensure_visit_order(c, T));
+  fl_launch_lake_join(c->stream, T->t, T->w);
+  return FL_OK;
+}
+
+// Decomposition of the receiver forest into heavy chains laid out by light
+// depth; ends with the control block on the host (bucket offsets, flags).
+int32_t decompose(fl_ctx* c, fl_terrain* T) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
-  fl_launch_fill_u32(s, w.counters, FL_NUM_COUNTERS, 0);
   fl_launch_children(s, T->t, w);
+  uint32_t round = 0;
   bool flip = false;
-  for (uint32_t round = 0;; round++) {
-    fl_launch_fill_u32(s, (uint32_t*)w.flags + FL_FLAG_JUMP, 1, 0);
-    fl_launch_chain_jump(s, T->t, w, flip);
-    flip = !flip;
-    if (round % 4 == 3) {
-      FLCHK(read_flags(c, T));
-      if (!c->host_flags[FL_FLAG_JUMP]) break;
+  uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
+  uint32_t quiet = FL_NONE_ROUND;  // first round in which no pointer moved
+  for (;;) {
+    for (; round < target; round++) {
+      fl_launch_chain_jump(s, T->t, w, flip, round);
+      flip = !flip;
     }
-    if (round > 80) return set_err(FL_ERR_HIP, "tree decomposition did not converge");
+    FLCHK(read_ctrl(c, T));
+    for (uint32_t r = 0; r < round; r++)
+      if (c->host_ctrl[FL_CTRL_JUMPS + r] == 0) {
+        quiet = r;
+        break;
+      }
+    if (quiet != FL_NONE_ROUND) break;
+    target = round + 2;
+    if (target >= FL_MAX_JUMP_ROUNDS) return set_err(FL_ERR_HIP, "tree decomposition did not converge");
   }
+  // quiet + 1 rounds suffice (the quiet round confirms convergence)
+  T->jump_rounds = quiet + 1;
   if (flip) {
     std::swap(w.jc, w.jc2);
     std::swap(w.pc, w.pc2);
     std::swap(w.jn, w.jn2);
     std::swap(w.lw, w.lw2);
   }
-  uint32_t clear = std::min<uint32_t>(T->prev_nb + 1, n + 1);
-  fl_launch_fill_u32(s, w.bcnt, clear, 0);
-  fl_launch_fill_u32(s, w.bnodes, clear, 0);
-  fl_launch_fill_u32(s, w.bcur, clear, 0);
-  fl_launch_fill_u32(s, w.bncur, clear, 0);
   fl_launch_chain_count(s, T->t, w);
-  HIPCHK(hipMemcpyAsync(c->host_counters, w.counters, FL_NUM_COUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  uint32_t nb = c->host_counters[FL_CNT_MAXLD] + 1;
+  fl_launch_chain_layout(s, T->t, w);
+  FLCHK(read_ctrl(c, T));
+  uint32_t nb = host_counter(c, FL_CNT_MAXLD) + 1;
   if (nb > n + 1) return set_err(FL_ERR_HIP, "light depth out of range");
   T->prev_nb = nb;
-  fl_launch_chain_layout(s, T->t, w, nb);
-  c->host_boff.resize(nb + 1);
-  HIPCHK(hipMemcpyAsync(c->host_boff.data(), w.boff, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const std::vector<uint32_t>& bo = c->host_boff;
-  for (uint32_t l = nb; l-- > 0;) fl_launch_da_round(s, T->t, w, bo[l], bo[l + 1]);
-  fl_launch_celerity(s, T->t, w);
-  for (uint32_t l = 0; l < nb; l++) fl_launch_tau_round(s, T->t, w, bo[l], bo[l + 1]);
-  FLCHK(read_flags(c, T));
-  *changed = c->host_flags[FL_FLAG_CHANGED] != 0;
+  c->host_boff.resize(2 * (nb + 1));
+  if (nb <= FL_CTRL_HEAD) {
+    const uint32_t* hc = c->host_ctrl + FL_NUM_FLAGS + FL_NUM_COUNTERS;
+    std::copy(hc, hc + nb + 1, c->host_boff.begin());
+    std::copy(hc + FL_CTRL_HEAD + 1, hc + FL_CTRL_HEAD + 1 + nb + 1, c->host_boff.begin() + nb + 1);
+  } else {
+    HIPCHK(hipMemcpyAsync(c->host_boff.data(), w.boff, (nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(c->host_boff.data() + nb + 1, w.bnoff, (nb + 1) * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    T->stat_syncs++;
+  }
   return FL_OK;
 }
 
+// One outer iteration (generator.rs:141-209).  The decomposition is built
+// speculatively before the lake test: lakes occur in the first iteration
+// only in practice, so the common path costs two host synchronisations.
 int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
-  FLCHK(stream_tree(c, T, false, had_lake));
-  return tree_sweeps(c, T, changed);
+  hipStream_t s = c->stream;
+  fl_dev_work& w = T->w;
+  uint32_t n = T->t.n;
+  fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, n + 1));
+  fl_launch_receivers(s, T->t, w);
+  FLCHK(decompose(c, T));
+  *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
+  if (*had_lake) {
+    FLCHK(join_lakes(c, T));
+    fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, n + 1));
+    FLCHK(decompose(c, T));
+  }
+  uint32_t nb = T->prev_nb;
+  const uint32_t* bo = c->host_boff.data();
+  const uint32_t* bn = bo + nb + 1;
+  uint32_t npos = bn[nb];
+  bool clamp = T->t.any_max_slope;
+  for (uint32_t l = nb; l-- > 0;) fl_launch_da_round(s, T->t, w, bo[l], bo[l + 1], bn[l], bn[l + 1]);
+  fl_launch_tau_prep(s, T->t, w, npos, clamp);
+  for (uint32_t l = 0; l < nb; l++) fl_launch_tau_round(s, T->t, w, bo[l], bo[l + 1], clamp);
+  fl_launch_tau_scatter(s, T->t, w, npos, clamp);
+  FLCHK(read_ctrl(c, T));
+  *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
+  std::swap(w.h, w.h2);
+  return FL_OK;
 }
 
 int32_t reset_state(fl_ctx* c, fl_terrain* T) {
@@ -275,8 +314,7 @@ extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   c->device = device;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(hipHostMalloc((void**)&c->host_flags, 64 * sizeof(int), hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void**)&c->host_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&c->host_ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
   *out = c.release();
   return FL_OK;
 }
@@ -286,8 +324,7 @@ extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
   hipSetDevice(c->device);
   delete c->prof;
   if (c->stream) hipStreamDestroy(c->stream);
-  if (c->host_flags) hipHostFree(c->host_flags);
-  if (c->host_counters) hipHostFree(c->host_counters);
+  if (c->host_ctrl) hipHostFree(c->host_ctrl);
   delete c;
   return FL_OK;
 }
@@ -299,6 +336,42 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   delete T;
   return FL_OK;
 }
+
+namespace {
+
+// Breadth-first (Cuthill-McKee style) site order from the graph alone:
+// neighbours of a site get nearby labels, so the gathers of every kernel
+// touch few cache lines.  Row contents keep their order, so every slot-order
+// tie-break of the reference is preserved.
+void locality_order(uint32_t n, const uint32_t* rp, const uint32_t* col, std::vector<uint32_t>& old_of_new,
+                    std::vector<uint32_t>& new_of_old) {
+  old_of_new.assign(n, 0);
+  new_of_old.assign(n, 0xffffffffu);
+  uint32_t head = 0, tail = 0;
+  for (uint32_t start = 0; start < n; start++) {
+    if (new_of_old[start] != 0xffffffffu) continue;
+    new_of_old[start] = tail;
+    old_of_new[tail++] = start;
+    while (head < tail) {
+      uint32_t u = old_of_new[head++];
+      for (uint32_t k = rp[u]; k < rp[u + 1]; k++) {
+        uint32_t v = col[k];
+        if (new_of_old[v] == 0xffffffffu) {
+          new_of_old[v] = tail;
+          old_of_new[tail++] = v;
+        }
+      }
+    }
+  }
+}
+
+template <typename V>
+void gather_perm(const V* src, const std::vector<uint32_t>& old_of_new, std::vector<V>& dst) {
+  dst.resize(old_of_new.size());
+  for (size_t u = 0; u < old_of_new.size(); u++) dst[u] = src[old_of_new[u]];
+}
+
+}  // namespace
 
 extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out) {
   if (!c || !out) return set_err(FL_ERR_INVALID, "null context or output");
@@ -314,10 +387,36 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
     *out = T.release();
     return FL_OK;
   }
-  T->h_row_ptr.assign(d->row_ptr, d->row_ptr + n + 1);
-  T->h_col.assign(d->col, d->col + nnz);
-  T->h_len.assign(d->length, d->length + nnz);
-  T->h_outlets.assign(d->outlets, d->outlets + d->n_outlets);
+  // relabel for locality
+  locality_order(n, d->row_ptr, d->col, T->old_of_new, T->new_of_old);
+  const std::vector<uint32_t>& o2 = T->old_of_new;
+  const std::vector<uint32_t>& n2 = T->new_of_old;
+  T->h_row_ptr.resize((size_t)n + 1);
+  T->h_col.resize(nnz);
+  T->h_len.resize(nnz);
+  T->h_row_ptr[0] = 0;
+  for (uint32_t u = 0; u < n; u++) {
+    uint32_t i = o2[u], b = d->row_ptr[i], e = d->row_ptr[i + 1], q = T->h_row_ptr[u];
+    for (uint32_t k = b; k < e; k++, q++) {
+      T->h_col[q] = n2[d->col[k]];
+      T->h_len[q] = d->length[k];
+    }
+    T->h_row_ptr[u + 1] = q;
+  }
+  T->h_outlets.resize(d->n_outlets);
+  for (uint64_t k = 0; k < d->n_outlets; k++) T->h_outlets[k] = n2[d->outlets[k]];
+  std::vector<double> area, erod, uplift, init, tan_ms(n);
+  gather_perm(d->area, o2, area);
+  gather_perm(d->erodibility, o2, erod);
+  gather_perm(d->uplift_rate, o2, uplift);
+  gather_perm(d->init_elevation, o2, init);
+  for (uint32_t u = 0; u < n; u++) {
+    uint32_t i = o2[u];
+    tan_ms[u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
+  }
+  std::vector<uint8_t> is_out(n, 0);
+  for (uint32_t o : T->h_outlets) is_out[o] = 1;
+
   fl_dev_terrain& t = T->t;
   fl_dev_work& w = T->w;
   fl_terrain* P = T.get();
@@ -332,31 +431,34 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   FLCHK(dev_alloc(P, &t.is_outlet, n));
   FLCHK(dev_alloc(P, &t.rank, n));
   FLCHK(dev_alloc(P, &t.order, n));
-  for (double** p : {&w.h, &w.A, &w.tau, &w.x, &w.dnext}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.coff, &w.ccnt, &w.clist, &w.hc, &w.jc, &w.pc, &w.jn, &w.lw,
-                       &w.jc2, &w.pc2, &w.jn2, &w.lw2, &w.clen, &w.chain_base, &w.chain_len, &w.head_base, &w.perm})
+  bool clamp = false;
+  for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
+  t.any_max_slope = clamp;
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.pre, &w.post0, &w.post1, &w.xp, &w.taup})
     FLCHK(dev_alloc(P, p, n));
+  if (clamp)
+    for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
+                       &w.jn2, &w.lw2, &w.clen, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
+    FLCHK(dev_alloc(P, p, n));
+  FLCHK(dev_alloc(P, &w.cmask, n));
   for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur}) FLCHK(dev_alloc(P, p, (uint64_t)n + 2));
   FLCHK(dev_alloc(P, &w.lake_key, n));
-  FLCHK(dev_alloc(P, &w.flags, FL_NUM_FLAGS));
-  FLCHK(dev_alloc(P, &w.counters, FL_NUM_COUNTERS));
+  FLCHK(dev_alloc(P, &w.ctrl, FL_CTRL_WORDS));
+  w.flags = (int*)w.ctrl;
+  w.counters = w.ctrl + FL_NUM_FLAGS;
 
-  std::vector<double> tan_ms(n);
-  std::vector<uint8_t> is_out(n, 0);
-  for (uint32_t i = 0; i < n; i++)
-    tan_ms[i] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
-  for (uint64_t k = 0; k < d->n_outlets; k++) is_out[d->outlets[k]] = 1;
   hipStream_t s = c->stream;
-  HIPCHK(hipMemcpyAsync(t.row_ptr, d->row_ptr, ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(t.row_ptr, T->h_row_ptr.data(), ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
   if (nnz) {
-    HIPCHK(hipMemcpyAsync(t.col, d->col, (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(t.len, d->length, (uint64_t)nnz * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.col, T->h_col.data(), (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.len, T->h_len.data(), (uint64_t)nnz * 8, hipMemcpyHostToDevice, s));
   }
-  HIPCHK(hipMemcpyAsync(t.area, d->area, (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.erod, d->erodibility, (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.uplift, d->uplift_rate, (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(t.area, area.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(t.erod, erod.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(t.uplift, uplift.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.tan_ms, tan_ms.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.init_h, d->init_elevation, (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(t.init_h, init.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.is_outlet, is_out.data(), n, hipMemcpyHostToDevice, s));
   fl_launch_fill_u32(s, w.bcnt, (uint64_t)n + 2, 0);
   fl_launch_fill_u32(s, w.bnodes, (uint64_t)n + 2, 0);
@@ -364,6 +466,7 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   fl_launch_fill_u32(s, w.bncur, (uint64_t)n + 2, 0);
   T->prev_nb = 0;
   FLCHK(reset_state(c, P));
+  // the pageable staging vectors above must outlive the copies
   HIPCHK(hipStreamSynchronize(s));
   *out = T.release();
   return FL_OK;
@@ -379,24 +482,60 @@ extern "C" int32_t fl_terrain_solve(fl_ctx* c, fl_terrain* T, uint32_t max_itera
     if (iterations_out) *iterations_out = max_iteration > 0 ? 1 : 0;
     return FL_OK;
   }
+  auto t0 = std::chrono::steady_clock::now();
+  T->stat_syncs = 0;
+  T->stat_order_ms = 0.0;
+  T->stat_lake_iterations = 0;
   FLCHK(reset_state(c, T));
   for (uint32_t step = 0; step < max_iteration; step++) {
     bool changed = false, lake = false;
     FLCHK(one_iteration(c, T, &changed, &lake));
     it++;
+    T->stat_lake_iterations += lake ? 1 : 0;
     if (!changed) break;
   }
+  T->stat_iterations = it;
+  T->stat_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (iterations_out) *iterations_out = it;
   return FL_OK;
 }
+
+namespace {
+// Copy a device f64[n] array (device labels) into caller order.
+int32_t download_f64_old(fl_ctx* c, fl_terrain* T, double* dst, const double* src) {
+  if (!dst) return FL_OK;
+  uint32_t n = T->t.n;
+  std::vector<double> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), src, (uint64_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < n; i++) dst[i] = tmp[T->new_of_old[i]];
+  return FL_OK;
+}
+// Copy a device u32[n] array of site indices into caller order and labels.
+int32_t download_idx_old(fl_ctx* c, fl_terrain* T, uint32_t* dst, const uint32_t* src) {
+  if (!dst) return FL_OK;
+  uint32_t n = T->t.n;
+  std::vector<uint32_t> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), src, (uint64_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < n; i++) dst[i] = T->old_of_new[tmp[T->new_of_old[i]]];
+  return FL_OK;
+}
+int32_t upload_h(fl_ctx* c, fl_terrain* T, const double* elev) {
+  uint32_t n = T->t.n;
+  std::vector<double> tmp(n);
+  for (uint32_t u = 0; u < n; u++) tmp[u] = elev[T->old_of_new[u]];
+  HIPCHK(hipMemcpyAsync(T->w.h, tmp.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FL_OK;
+}
+}  // namespace
 
 extern "C" int32_t fl_terrain_download(fl_ctx* c, fl_terrain* T, double* out) {
   if (!c || !T || (!out && T->t.n)) return set_err(FL_ERR_INVALID, "null argument");
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemcpyAsync(out, T->w.h, (uint64_t)T->t.n * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return FL_OK;
+  return download_f64_old(c, T, out, T->w.h);
 }
 
 extern "C" int32_t fl_terrain_device_elevations(fl_terrain* T, double** dev_ptr) {
@@ -424,34 +563,15 @@ extern "C" int32_t fl_generate_f64(fl_ctx* c, const fl_terrain_desc* d, double* 
 
 // ------------------------------------------------------------- stages ---
 
-namespace {
-int32_t upload_h(fl_ctx* c, fl_terrain* T, const double* elev) {
-  HIPCHK(hipMemcpyAsync(T->w.h, elev, (uint64_t)T->t.n * 8, hipMemcpyHostToDevice, c->stream));
-  return FL_OK;
-}
-int32_t download_u32(fl_ctx* c, uint32_t* dst, const uint32_t* src, uint32_t n) {
-  if (!dst) return FL_OK;
-  HIPCHK(hipMemcpyAsync(dst, src, (uint64_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  return FL_OK;
-}
-int32_t download_f64(fl_ctx* c, double* dst, const double* src, uint32_t n) {
-  if (!dst) return FL_OK;
-  HIPCHK(hipMemcpyAsync(dst, src, (uint64_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-  return FL_OK;
-}
-}  // namespace
-
 extern "C" int32_t fl_stage_receivers(fl_ctx* c, fl_terrain* T, const double* elev, uint32_t* next_out) {
   if (!c || !T || !elev || !next_out) return set_err(FL_ERR_INVALID, "null argument");
   ProfScope prof_scope(c->prof);
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
-  FLCHK(clear_flags(c, T));
+  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, T->t.n + 1));
   fl_launch_receivers(c->stream, T->t, T->w);
-  FLCHK(download_u32(c, next_out, T->w.next, T->t.n));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return FL_OK;
+  return download_idx_old(c, T, next_out, T->w.next);
 }
 
 extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* elev, uint32_t* next_out,
@@ -461,11 +581,17 @@ extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* 
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
-  bool lake = false;
-  FLCHK(stream_tree(c, T, true, &lake));
-  FLCHK(download_u32(c, next_out, T->w.next, T->t.n));
-  FLCHK(download_u32(c, subroot_out, T->w.sub, T->t.n));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, T->t.n + 1));
+  fl_launch_receivers(c->stream, T->t, T->w);
+  FLCHK(read_ctrl(c, T));
+  bool lake = host_flag(c, FL_FLAG_LAKE) != 0;
+  FLCHK(label_chain_ends(c, T));
+  if (lake) {
+    FLCHK(ensure_visit_order(c, T));
+    fl_launch_lake_join(c->stream, T->t, T->w);
+  }
+  FLCHK(download_idx_old(c, T, next_out, T->w.next));
+  FLCHK(download_idx_old(c, T, subroot_out, T->w.sub));
   if (flag) *flag = lake ? 1 : 0;
   return FL_OK;
 }
@@ -479,11 +605,10 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
   FLCHK(upload_h(c, T, elev));
   bool changed = false, lake = false;
   FLCHK(one_iteration(c, T, &changed, &lake));
-  FLCHK(download_f64(c, elev, T->w.h, T->t.n));
-  FLCHK(download_u32(c, next_out, T->w.next, T->t.n));
-  FLCHK(download_f64(c, area_out, T->w.A, T->t.n));
-  FLCHK(download_f64(c, response_out, T->w.tau, T->t.n));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  FLCHK(download_f64_old(c, T, elev, T->w.h));
+  FLCHK(download_idx_old(c, T, next_out, T->w.next));
+  FLCHK(download_f64_old(c, T, area_out, T->w.A));
+  FLCHK(download_f64_old(c, T, response_out, T->w.tau));
   if (flag) *flag = changed ? 1 : 0;
   return FL_OK;
 }
@@ -521,5 +646,16 @@ extern "C" int32_t fl_ctx_kernel_time(fl_ctx* c, const char* name, uint64_t* lau
 extern "C" int32_t fl_ctx_reset_profile(fl_ctx* c) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
   if (c->prof) c->prof->reset();
+  return FL_OK;
+}
+
+extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* out) {
+  if (!T || !out) return set_err(FL_ERR_INVALID, "null argument");
+  out->iterations = T->stat_iterations;
+  out->lake_iterations = T->stat_lake_iterations;
+  out->host_syncs = T->stat_syncs;
+  out->reserved = 0;
+  out->visit_order_ms = T->stat_order_ms;
+  out->solve_ms = T->stat_solve_ms;
   return FL_OK;
 }

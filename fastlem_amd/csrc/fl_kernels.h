@@ -26,8 +26,13 @@ struct fl_profiler {
 // Set by the C ABI for the duration of a call on a profiled context.
 extern thread_local fl_profiler* fl_current_profiler;
 
-enum { FL_FLAG_LAKE = 0, FL_FLAG_JUMP = 1, FL_FLAG_CHANGED = 2, FL_NUM_FLAGS = 4 };
-enum { FL_CNT_CHILDREN = 0, FL_CNT_MAXLD = 1, FL_NUM_COUNTERS = 4 };
+// Control block read back once per phase: [flags | counters | boff head | bnoff head].
+enum { FL_FLAG_LAKE = 0, FL_FLAG_JUMP = 1 /* and 2: alternating */, FL_FLAG_CHANGED = 3, FL_NUM_FLAGS = 4 };
+enum { FL_CNT_MAXLD = 0, FL_CNT_CHAINS = 1, FL_CNT_POSITIONS = 2, FL_NUM_COUNTERS = 4 };
+enum { FL_CTRL_HEAD = 128 };  // bucket offsets mirrored into the control block
+enum { FL_MAX_JUMP_ROUNDS = 96 };  // one "pointers moved" word per doubling round
+enum { FL_CTRL_JUMPS = FL_NUM_FLAGS + FL_NUM_COUNTERS + 2 * (FL_CTRL_HEAD + 1) };
+enum { FL_CTRL_WORDS = FL_CTRL_JUMPS + FL_MAX_JUMP_ROUNDS };
 
 // Read-only inputs of one terrain, resident in HBM.
 struct fl_dev_terrain {
@@ -41,6 +46,7 @@ struct fl_dev_terrain {
   double* uplift = nullptr;      // [n]
   double* tan_ms = nullptr;      // [n] tan(max_slope), NaN when None
   double* init_h = nullptr;      // [n]
+  bool any_max_slope = false;
   uint8_t* is_outlet = nullptr;  // [n]
   uint32_t* rank = nullptr;      // [n] lake-traversal visit rank (UINT32_MAX: never)
   uint32_t* order = nullptr;     // [n] visited sites by rank
@@ -48,7 +54,17 @@ struct fl_dev_terrain {
 
 // Per-iteration state and scratch of one terrain.
 struct fl_dev_work {
-  double* h = nullptr;      // [n] elevations
+  double* h = nullptr;      // [n] elevations (current)
+  double* h2 = nullptr;     // [n] elevations being computed
+  double* xp = nullptr;     // [n] 1/celerity*distance, chain-major
+  double* taup = nullptr;   // [n] response time, chain-major
+  double* hnp = nullptr;    // [n] new elevation, chain-major (max-slope form)
+  double* up = nullptr;     // [n] uplift, chain-major (max-slope form)
+  double* tanp = nullptr;   // [n] tan(max slope), chain-major (max-slope form)
+  double* dp = nullptr;     // [n] receiver distance, chain-major (max-slope form)
+  double* hop = nullptr;    // [n] old elevation, chain-major (max-slope form)
+  uint32_t* ppos = nullptr; // [n] at a chain's base position: position of the head's receiver
+  uint32_t* rpos = nullptr; // [n] at a chain's base position: position of the basin outlet
   double* A = nullptr;      // [n] drainage area (kept: heavy-child hint)
   double* tau = nullptr;    // [n] response time
   double* x = nullptr;      // [n] 1/celerity * distance
@@ -59,10 +75,12 @@ struct fl_dev_work {
   uint32_t* sub2 = nullptr;
   uint64_t* lake_key = nullptr;
   // decomposition
-  uint32_t* coff = nullptr;
-  uint32_t* ccnt = nullptr;
-  uint32_t* clist = nullptr;
-  uint32_t* hc = nullptr;
+  uint64_t* cmask = nullptr;  // [n] child slots of each site
+  uint32_t* hc = nullptr;     // [n] heavy child
+  double* pre = nullptr;      // [n] drainage-area partial folds, chain-major
+  double* post0 = nullptr;
+  double* post1 = nullptr;
+  uint32_t* npost = nullptr;
   uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;
   uint32_t *jc2 = nullptr, *pc2 = nullptr, *jn2 = nullptr, *lw2 = nullptr;
   uint32_t* clen = nullptr;
@@ -71,8 +89,9 @@ struct fl_dev_work {
   uint32_t *chain_base = nullptr, *chain_len = nullptr;      // [n] chains by bucket
   uint32_t* head_base = nullptr;                             // [n]
   uint32_t* perm = nullptr;                                  // [n] chain-major site order
-  int* flags = nullptr;                                      // [FL_NUM_FLAGS]
-  uint32_t* counters = nullptr;                              // [FL_NUM_COUNTERS]
+  uint32_t* ctrl = nullptr;                                  // [FL_CTRL_WORDS]
+  int* flags = nullptr;                                      // = ctrl
+  uint32_t* counters = nullptr;                              // = ctrl + FL_NUM_FLAGS
 };
 
 void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
@@ -80,10 +99,14 @@ void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
 void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
+void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t nb);
-void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1);
-void fl_launch_celerity(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1);
+void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
+                        uint32_t p0, uint32_t p1);
+void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
+void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
+void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
+                         bool clamp);
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);

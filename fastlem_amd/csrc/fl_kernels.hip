@@ -188,54 +188,32 @@ __global__ void k_lake_flip(uint32_t n, const uint32_t* __restrict__ rp, const u
 
 // Children of u are the neighbours v with next[v] == u, in row order (the
 // order DrainageBasin::construct appends them, drainage_basin.rs:23-27).
-// The heavy child (first with the largest previous drainage area) continues
-// u's chain; the choice only shapes the work, never the results.  Child
-// segments are allocated with one atomic per wavefront.
+// cmask[u] bit s marks slot s as a child (rows of up to 64 entries; longer
+// rows are rescanned by the consumers).  The heavy child (first with the
+// largest previous drainage area) continues u's chain; the choice only
+// shapes the work, never the results.
 __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const double* __restrict__ prev_area,
-                           uint32_t* __restrict__ coff, uint32_t* __restrict__ ccnt, uint32_t* __restrict__ clist,
-                           uint32_t* __restrict__ hc, uint32_t* __restrict__ total) {
+                           unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc) {
   uint32_t u = gid();
-  uint32_t cnt = 0, heavy = FL_NONE;
-  uint32_t b = 0, e = 0;
-  if (u < n) {
-    b = rp[u];
-    e = rp[u + 1];
-    double best = 0.0;
-    for (uint32_t k = b; k < e; ++k) {
-      uint32_t v = col[k];
-      if (v != u && next[v] == u) {
-        double av = prev_area[v];
-        if (heavy == FL_NONE || av > best) {
-          best = av;
-          heavy = v;
-        }
-        ++cnt;
-      }
-    }
-  }
-  // wavefront-aggregated allocation of the child segments
-  uint32_t lane = threadIdx.x & 63u;
-  uint32_t incl = cnt;
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t t = __shfl_up(incl, off, 64);
-    if ((int)lane >= off) incl += t;
-  }
-  uint32_t wave_total = __shfl(incl, 63, 64);
-  uint32_t base = 0;
-  if (lane == 63 && wave_total) base = atomicAdd(total, wave_total);
-  base = __shfl(base, 63, 64);
   if (u >= n) return;
-  uint32_t off = base + incl - cnt;
-  coff[u] = off;
-  ccnt[u] = cnt;
-  hc[u] = heavy;
-  if (cnt) {
-    for (uint32_t k = b; k < e; ++k) {
-      uint32_t v = col[k];
-      if (v != u && next[v] == u) clist[off++] = v;
+  uint32_t b = rp[u], e = rp[u + 1];
+  uint32_t heavy = FL_NONE;
+  unsigned long long m = 0;
+  double best = 0.0;
+  for (uint32_t k = b; k < e; ++k) {
+    uint32_t v = col[k];
+    if (next[v] == u) {
+      double av = prev_area[v];
+      if (heavy == FL_NONE || av > best) {
+        best = av;
+        heavy = v;
+      }
+      if (k - b < 64) m |= 1ull << (k - b);
     }
   }
+  cmask[u] = m;
+  hc[u] = heavy;
 }
 
 // Chain and root pointers for pointer jumping.
@@ -258,23 +236,28 @@ __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, cons
 // One doubling round on both pointer systems.  After convergence
 // jc[v] = chain head, pc[v] = distance to it, jn[v] = root of v's tree,
 // lw[v] = light edges between v and its root (= light depth of v's chain).
+// Movement is flagged in the round's own control word (zeroed at the start
+// of the iteration), so rounds need no separate clearing launch.
 __global__ void k_chain_jump(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ pc,
                              const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
                              uint32_t* __restrict__ jc2, uint32_t* __restrict__ pc2, uint32_t* __restrict__ jn2,
-                             uint32_t* __restrict__ lw2, int* __restrict__ flag) {
+                             uint32_t* __restrict__ lw2, uint32_t* __restrict__ flag_set) {
   uint32_t v = gid();
-  if (v >= n) return;
-  uint32_t a = jc[v], b = jn[v];
-  uint32_t aa = jc[a], bb = jn[b];
-  jc2[v] = aa;
-  pc2[v] = pc[v] + pc[a];
-  jn2[v] = bb;
-  lw2[v] = lw[v] + lw[b];
-  if (aa != a || bb != b) *flag = 1;
+  bool moved = false;
+  if (v < n) {
+    uint32_t a = jc[v], b = jn[v];
+    uint32_t aa = jc[a], bb = jn[b];
+    jc2[v] = aa;
+    pc2[v] = pc[v] + pc[a];
+    jn2[v] = bb;
+    lw2[v] = lw[v] + lw[b];
+    moved = (aa != a || bb != b);
+  }
+  if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
 
-// Per chain head inside an outlet's basin: chain length (from its tail, the
-// member without a heavy child) and light-depth histogram.
+// Chain length, stored at the head by the chain's tail (the member without
+// a heavy child), for chains inside an outlet's basin.
 __global__ void k_chain_len(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ jc,
                             const uint32_t* __restrict__ pc, const uint32_t* __restrict__ jn,
                             const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ clen) {
@@ -283,24 +266,63 @@ __global__ void k_chain_len(uint32_t n, const uint32_t* __restrict__ hc, const u
   if (hc[v] == FL_NONE && is_outlet[jn[v]]) clen[jc[v]] = pc[v] + 1;
 }
 
-__global__ void k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
-                               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
-                               const uint32_t* __restrict__ clen, uint32_t* __restrict__ bcnt,
-                               uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  if (jc[v] != v || !is_outlet[jn[v]]) return;
-  uint32_t l = lw[v];
-  atomicAdd(bcnt + l, 1u);
-  atomicAdd(bnodes + l, clen[v]);
-  atomicMax(maxld, l);
+#define FL_LDS_BUCKETS 128
+#define FL_BUCKET_BLOCK 1024
+#define FL_BUCKET_PER_THREAD 4
+
+// Light-depth histogram of the chains (count and total length), aggregated
+// per workgroup in LDS before one global atomic per (workgroup, bucket).
+__global__ void __launch_bounds__(FL_BUCKET_BLOCK)
+k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
+               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
+               const uint32_t* __restrict__ clen, uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes,
+               uint32_t* __restrict__ maxld) {
+  __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
+  __shared__ uint32_t s_max;
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+    s_c[t] = 0;
+    s_n[t] = 0;
+  }
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
+    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
+    if (v < n && jc[v] == v && is_outlet[jn[v]]) {
+      uint32_t l = lw[v], len = clen[v];
+      if (l < FL_LDS_BUCKETS) {
+        atomicAdd(s_c + l, 1u);
+        atomicAdd(s_n + l, len);
+      } else {
+        atomicAdd(bcnt + l, 1u);
+        atomicAdd(bnodes + l, len);
+      }
+      atomicMax(&s_max, l);
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+    if (s_c[t]) {
+      atomicAdd(bcnt + t, s_c[t]);
+      atomicAdd(bnodes + t, s_n[t]);
+    }
+  }
+  if (threadIdx.x == 0) atomicMax(maxld, s_max);
 }
 
-// Exclusive scans of the (small) bucket histograms, one workgroup.
-__global__ void k_bucket_scan(uint32_t nb, const uint32_t* __restrict__ bcnt, const uint32_t* __restrict__ bnodes,
-                              uint32_t* __restrict__ boff, uint32_t* __restrict__ bnoff) {
+// Exclusive scans of the bucket histograms over the light depths actually
+// present (counters[FL_CNT_MAXLD] + 1), one workgroup.  The first
+// FL_CTRL_HEAD + 1 offsets are mirrored into the control block.
+__global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uint32_t* __restrict__ bcnt,
+                              const uint32_t* __restrict__ bnodes, uint32_t* __restrict__ boff,
+                              uint32_t* __restrict__ bnoff) {
   __shared__ uint32_t s_a[1024], s_b[1024];
   __shared__ uint32_t carry_a, carry_b;
+  uint32_t* counters = ctrl + FL_NUM_FLAGS;
+  uint32_t nb = counters[FL_CNT_MAXLD] + 1;
+  if (nb > n + 1) nb = n + 1;
+  uint32_t* head_c = counters + FL_NUM_COUNTERS;
+  uint32_t* head_n = head_c + FL_CTRL_HEAD + 1;
   if (threadIdx.x == 0) {
     carry_a = 0;
     carry_b = 0;
@@ -321,8 +343,13 @@ __global__ void k_bucket_scan(uint32_t nb, const uint32_t* __restrict__ bcnt, co
       __syncthreads();
     }
     if (i < nb) {
-      boff[i] = carry_a + s_a[threadIdx.x] - a;
-      bnoff[i] = carry_b + s_b[threadIdx.x] - b;
+      uint32_t oa = carry_a + s_a[threadIdx.x] - a, ob = carry_b + s_b[threadIdx.x] - b;
+      boff[i] = oa;
+      bnoff[i] = ob;
+      if (i <= FL_CTRL_HEAD) {
+        head_c[i] = oa;
+        head_n[i] = ob;
+      }
     }
     __syncthreads();
     if (threadIdx.x == blockDim.x - 1) {
@@ -334,152 +361,380 @@ __global__ void k_bucket_scan(uint32_t nb, const uint32_t* __restrict__ bcnt, co
   if (threadIdx.x == 0) {
     boff[nb] = carry_a;
     bnoff[nb] = carry_b;
+    if (nb <= FL_CTRL_HEAD) {
+      head_c[nb] = carry_a;
+      head_n[nb] = carry_b;
+    }
+    counters[FL_CNT_CHAINS] = carry_a;
+    counters[FL_CNT_POSITIONS] = carry_b;
   }
 }
 
-// Give every chain a slot in its light-depth bucket and a contiguous node
-// range.  Order inside a bucket is immaterial to the results.
-__global__ void k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
-                               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
-                               const uint32_t* __restrict__ clen, const uint32_t* __restrict__ boff,
-                               const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur,
-                               uint32_t* __restrict__ bncur, uint32_t* __restrict__ chain_base,
-                               uint32_t* __restrict__ chain_len, uint32_t* __restrict__ head_base) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  if (jc[v] != v || !is_outlet[jn[v]]) return;
-  uint32_t l = lw[v];
-  uint32_t len = clen[v];
-  uint32_t c = boff[l] + atomicAdd(bcur + l, 1u);
-  uint32_t base = bnoff[l] + atomicAdd(bncur + l, len);
-  chain_base[c] = base;
-  chain_len[c] = len;
-  head_base[v] = base;
+// Give every chain a slot in its light-depth bucket and a contiguous range
+// of positions: LDS atomics rank the chains of a workgroup, one global
+// atomic per (workgroup, bucket) reserves the ranges.  Order inside a bucket
+// is immaterial to the results.
+__global__ void __launch_bounds__(FL_BUCKET_BLOCK)
+k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
+               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
+               const uint32_t* __restrict__ clen, const uint32_t* __restrict__ boff,
+               const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
+               uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
+               uint32_t* __restrict__ head_base) {
+  __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+    s_c[t] = 0;
+    s_n[t] = 0;
+  }
+  __syncthreads();
+  uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  uint32_t lc[FL_BUCKET_PER_THREAD], ln[FL_BUCKET_PER_THREAD];
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
+    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
+    lc[j] = FL_NONE;
+    ln[j] = 0;
+    if (v < n && jc[v] == v && is_outlet[jn[v]]) {
+      uint32_t l = lw[v], len = clen[v];
+      if (l < FL_LDS_BUCKETS) {
+        lc[j] = atomicAdd(s_c + l, 1u);
+        ln[j] = atomicAdd(s_n + l, len);
+      } else {
+        uint32_t c = boff[l] + atomicAdd(bcur + l, 1u);
+        uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
+        chain_base[c] = pos;
+        chain_len[c] = len;
+        head_base[v] = pos;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+    if (s_c[t]) {
+      s_bc[t] = atomicAdd(bcur + t, s_c[t]);
+      s_bn[t] = atomicAdd(bncur + t, s_n[t]);
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
+    if (lc[j] == FL_NONE) continue;
+    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
+    uint32_t l = lw[v];
+    uint32_t c = boff[l] + s_bc[l] + lc[j];
+    uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
+    chain_base[c] = pos;
+    chain_len[c] = clen[v];
+    head_base[v] = pos;
+  }
 }
 
+// Scatter sites to their chain-major positions.  At each chain's base
+// position record the position of the head's receiver (FL_NONE for an
+// outlet) and of the basin's outlet.  Sites outside every basin keep their
+// elevation, drainage area = own area and response time 0 (generator.rs:
+// 144-145), as the reference leaves them.
 __global__ void k_chain_perm(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ pc,
-                             const uint32_t* __restrict__ jn, const uint8_t* __restrict__ is_outlet,
-                             const uint32_t* __restrict__ head_base, uint32_t* __restrict__ perm) {
+                             const uint32_t* __restrict__ jn, const uint32_t* __restrict__ next,
+                             const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
+                             uint32_t* __restrict__ perm, uint32_t* __restrict__ ppos, uint32_t* __restrict__ rpos,
+                             const double* __restrict__ area, double* __restrict__ A, double* __restrict__ tau,
+                             const double* __restrict__ h, double* __restrict__ h2) {
   uint32_t v = gid();
   if (v >= n) return;
-  if (!is_outlet[jn[v]]) return;
-  perm[head_base[jc[v]] + pc[v]] = v;
+  uint32_t r = jn[v];
+  if (!is_outlet[r]) {
+    A[v] = area[v];
+    tau[v] = 0.0;
+    h2[v] = h[v];
+    return;
+  }
+  uint32_t hd = jc[v];
+  uint32_t base = head_base[hd];
+  perm[base + pc[v]] = v;
+  if (hd == v) {
+    uint32_t p = next[v];
+    ppos[base] = (p == v) ? FL_NONE : head_base[jc[p]] + pc[p];
+    rpos[base] = head_base[r];
+  }
 }
 
 // --------------------------------------------------------------- sweeps ---
 
-// Drainage area for the chains of one light depth, one lane per chain,
-// tail to head.  Light children were finished in the previous (deeper)
-// round; the heavy child's value is the running register.
-__global__ void k_da_round(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
+namespace {
+
+// Is slot s of row k (starting at b) a child of k?
+__device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uint32_t s, uint32_t b,
+                                         const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                         uint32_t k) {
+  return deg <= 64 ? ((m >> s) & 1ull) != 0 : next[col[b + s]] == k;
+}
+
+}  // namespace
+
+// Drainage area, part 1, for the sites of one light-depth round (positions
+// [p0, p1) of the chain-major order): fold the site's own area with the
+// children that precede the heavy child in the reference's accumulation
+// order (reverse row order), and collect the light children that follow it.
+// Light children head chains of the next deeper round, already finished.
+__global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ perm,
+                         const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+                         const uint32_t* __restrict__ hc, const double* __restrict__ area,
+                         const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
+                         double* __restrict__ post1, uint32_t* __restrict__ npost) {
+  uint32_t P = p0 + gid();
+  if (P >= p1) return;
+  uint32_t k = perm[P];
+  uint32_t b = rp[k], deg = rp[k + 1] - b, hk = hc[k];
+  unsigned long long m = cmask[k];
+  double acc = area[k], pa = 0.0, pb = 0.0;
+  uint32_t np = 0;
+  bool after = false;
+  for (uint32_t s = deg; s-- > 0;) {
+    if (!is_child(m, deg, s, b, col, next, k)) continue;
+    uint32_t c = col[b + s];
+    if (c == hk) {
+      after = true;
+      continue;
+    }
+    if (!after) {
+      acc = acc + A[c];
+    } else {
+      if (np == 0) pa = A[c];
+      else if (np == 1) pb = A[c];
+      ++np;
+    }
+  }
+  pre[P] = acc;
+  post0[P] = pa;
+  post1[P] = pb;
+  npost[P] = np;
+}
+
+// The light children after the heavy one beyond the first two (rare).
+__device__ __noinline__ double da_post_rest(double r, uint32_t k, const uint32_t* __restrict__ rp,
+                                            const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                            const unsigned long long* __restrict__ cmask,
+                                            const uint32_t* __restrict__ hc, const double* __restrict__ A) {
+  uint32_t b = rp[k], deg = rp[k + 1] - b, hk = hc[k];
+  unsigned long long m = cmask[k];
+  bool after = false;
+  uint32_t seen = 0;
+  for (uint32_t s = deg; s-- > 0;) {
+    if (!is_child(m, deg, s, b, col, next, k)) continue;
+    uint32_t c = col[b + s];
+    if (c == hk) {
+      after = true;
+      continue;
+    }
+    if (after) {
+      if (seen >= 2) r = r + A[c];
+      ++seen;
+    }
+  }
+  return r;
+}
+
+#define FL_BATCH 8
+
+// Drainage area, part 2: one lane per chain of the round, tail to head:
+// A = pre (tail), A = (pre + A_heavy) + post0 + post1 + ... (others).
+// Inputs for FL_BATCH positions are loaded before the dependent adds.
+__global__ void k_da_chain(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
-                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ ccnt,
-                           const uint32_t* __restrict__ clist, const uint32_t* __restrict__ hc,
-                           const double* __restrict__ area, double* __restrict__ A) {
+                           const double* __restrict__ pre, const double* __restrict__ post0,
+                           const double* __restrict__ post1, const uint32_t* __restrict__ npost,
+                           const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                           const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+                           const uint32_t* __restrict__ hc, double* __restrict__ A) {
   uint32_t c = c0 + gid();
   if (c >= c1) return;
   uint32_t base = chain_base[c], len = chain_len[c];
-  double running = 0.0;
-  for (uint32_t t = len; t-- > 0;) {
-    uint32_t k = perm[base + t];
-    double acc = area[k];
-    uint32_t o = coff[k], m = ccnt[k], hk = hc[k];
-    for (uint32_t q = m; q-- > 0;) {
-      uint32_t ch = clist[o + q];
-      double val = (ch == hk) ? running : A[ch];
-      acc = acc + val;
+  double r = 0.0;
+  uint32_t t = len;  // positions still to do: [base, base + t)
+  bool first = true;
+  while (t > 0) {
+    uint32_t cnt = t < FL_BATCH ? t : FL_BATCH;
+    double vp[FL_BATCH], va[FL_BATCH], vb[FL_BATCH];
+    uint32_t vn[FL_BATCH], vk[FL_BATCH];
+#pragma unroll
+    for (uint32_t q = 0; q < FL_BATCH; ++q) {
+      if (q < cnt) {
+        uint32_t P = base + t - 1 - q;
+        vp[q] = pre[P];
+        va[q] = post0[P];
+        vb[q] = post1[P];
+        vn[q] = npost[P];
+        vk[q] = perm[P];
+      }
     }
-    A[k] = acc;
-    running = acc;
+#pragma unroll
+    for (uint32_t q = 0; q < FL_BATCH; ++q) {
+      if (q < cnt) {
+        r = first ? vp[q] : (vp[q] + r);
+        first = false;
+        if (vn[q] >= 1) r = r + va[q];
+        if (vn[q] >= 2) r = r + vb[q];
+        if (vn[q] >= 3) r = da_post_rest(r, vk[q], rp, col, next, cmask, hc, A);
+        A[vk[q]] = r;
+      }
+    }
+    t -= cnt;
   }
 }
 
-// x_k = 1.0 / (K_k * A_k^0.5) * d_k for every site in an outlet's basin
-// (generator.rs:172-173), evaluated in the reference's association.  Sites
-// outside every basin keep drainage area = own area and response time 0
-// (generator.rs:144-145), as the reference leaves them.
-__global__ void k_celerity(uint32_t n, const uint32_t* __restrict__ jn, const uint8_t* __restrict__ is_outlet,
-                           const double* __restrict__ K, const double* __restrict__ area, double* __restrict__ A,
-                           const double* __restrict__ dnext, double* __restrict__ x, double* __restrict__ tau) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  if (!is_outlet[jn[v]]) {
-    A[v] = area[v];
-    tau[v] = 0.0;
-    return;
+// Per chain-major position: x = 1.0 / (K * A^0.5) * d (generator.rs:172-173,
+// the reference's association); with max slopes also gather the inputs of
+// the elevation recurrence so the chain lanes read contiguous memory.
+__global__ void k_tau_prep(uint32_t npos, const uint32_t* __restrict__ perm, const double* __restrict__ K,
+                           const double* __restrict__ A, const double* __restrict__ dnext,
+                           const double* __restrict__ U, const double* __restrict__ tan_ms,
+                           const double* __restrict__ h, double* __restrict__ xp, double* __restrict__ up,
+                           double* __restrict__ tanp, double* __restrict__ dp, double* __restrict__ hop, int clamp) {
+  uint32_t P = gid();
+  if (P >= npos) return;
+  uint32_t k = perm[P];
+  double d = dnext[k];
+  double celerity = K[k] * rust_pow_half(A[k]);
+  xp[P] = 1.0 / celerity * d;
+  if (clamp) {
+    up[P] = U[k];
+    tanp[P] = tan_ms[k];
+    dp[P] = d;
+    hop[P] = h[k];
   }
-  double celerity = K[v] * rust_pow_half(A[v]);
-  x[v] = 1.0 / celerity * dnext[v];
 }
 
-// Response time and elevation for the chains of one light depth, head to
-// tail (generator.rs:162-203).  A head's receiver lies on a chain of a
-// smaller light depth, finished in an earlier round; the basin's outlet
-// (jn = root) was finished in round 0.
+// Response time along the chains of one light depth, head to tail:
+// rt_i = 0.0 + (rt_next + x_i) (generator.rs:173 with rt_i initially 0),
+// the basin outlet first (its receiver is itself).  A head's receiver lies
+// on a chain finished in an earlier round.  With clamp, also the elevation
+// recurrence of generator.rs:177-202, whose max-slope limit depends on the
+// receiver's new elevation.
 __global__ void k_tau_round(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
-                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ next, const uint32_t* __restrict__ jn,
-                            const double* __restrict__ x, const double* __restrict__ U,
-                            const double* __restrict__ tan_ms, const double* __restrict__ dnext,
-                            double* __restrict__ tau, double* __restrict__ h, int* __restrict__ flags) {
+                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
+                            const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+                            double* __restrict__ taup, const double* __restrict__ up,
+                            const double* __restrict__ tanp, const double* __restrict__ dp,
+                            const double* __restrict__ hop, double* __restrict__ hnp, int clamp) {
   uint32_t c = c0 + gid();
-  bool changed = false;
-  if (c < c1) {
-    uint32_t base = chain_base[c], len = chain_len[c];
-    uint32_t k = perm[base];
-    uint32_t p = next[k];
-    uint32_t o = jn[k];
-    double tau_p, h_p, tau_o, h_o;
-    if (p == k) {
-      // the outlet: rt[o] += rt[o] + x_o with rt[o] = 0; h_o' = h_o + U_o*max(0,0)
-      double t0 = 0.0 + (0.0 + x[k]);
-      double ho = h[k];
-      double nh = ho + U[k] * rust_max(t0 - t0, 0.0);
-      double tm = tan_ms[k];
-      double slope = (nh - ho) / dnext[k];  // receiver is itself, not yet updated
-      if (slope > tm) nh = ho + tm * dnext[k];
-      changed |= (nh != ho);
-      tau[k] = t0;
-      h[k] = nh;
-      tau_p = t0;
+  if (c >= c1) return;
+  uint32_t base = chain_base[c], len = chain_len[c];
+  uint32_t pp = ppos[base];
+  double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
+  uint32_t t = 0;
+  if (pp == FL_NONE) {
+    tau = 0.0 + (0.0 + xp[base]);
+    taup[base] = tau;
+    if (clamp) {
+      double ho = hop[base];
+      double nh = ho + up[base] * rust_max(tau - tau, 0.0);
+      double tm = tanp[base], d = dp[base];
+      double slope = (nh - ho) / d;  // the receiver is the outlet itself, not yet updated
+      if (slope > tm) nh = ho + tm * d;
+      hnp[base] = nh;
       h_p = nh;
-      tau_o = t0;
       h_o = nh;
+      tau_o = tau;
+    }
+    t = 1;
+  } else {
+    tau = taup[pp];
+    if (clamp) {
+      uint32_t ro = rpos[base];
+      tau_o = taup[ro];
+      h_o = hnp[ro];
+      h_p = hnp[pp];
+    }
+  }
+  if (!clamp) {
+    while (t < len) {
+      uint32_t cnt = (len - t) < 16 ? (len - t) : 16;
+      double vx[16];
+#pragma unroll
+      for (uint32_t q = 0; q < 16; ++q)
+        if (q < cnt) vx[q] = xp[base + t + q];
+#pragma unroll
+      for (uint32_t q = 0; q < 16; ++q) {
+        if (q < cnt) {
+          tau = 0.0 + (tau + vx[q]);
+          taup[base + t + q] = tau;
+        }
+      }
+      t += cnt;
+    }
+  } else {
+    while (t < len) {
+      uint32_t cnt = (len - t) < FL_BATCH ? (len - t) : FL_BATCH;
+      double vx[FL_BATCH], vu[FL_BATCH], vt[FL_BATCH], vd[FL_BATCH];
+#pragma unroll
+      for (uint32_t q = 0; q < FL_BATCH; ++q) {
+        if (q < cnt) {
+          uint32_t P = base + t + q;
+          vx[q] = xp[P];
+          vu[q] = up[P];
+          vt[q] = tanp[P];
+          vd[q] = dp[P];
+        }
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < FL_BATCH; ++q) {
+        if (q < cnt) {
+          tau = 0.0 + (tau + vx[q]);
+          double nh = h_o + vu[q] * rust_max(tau - tau_o, 0.0);
+          double slope = (nh - h_p) / vd[q];
+          if (slope > vt[q]) nh = h_p + vt[q] * vd[q];
+          taup[base + t + q] = tau;
+          hnp[base + t + q] = nh;
+          h_p = nh;
+        }
+      }
+      t += cnt;
+    }
+  }
+}
+
+// Back to site order: response time, new elevation (computed here when no
+// site has a max slope: h_i' = h_o' + U_i * max(rt_i - rt_o, 0) with
+// h_o' = h_o + U_o * max(0, 0), generator.rs:178-180) and the changed flag
+// (generator.rs:201).
+__global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ jn,
+                              const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
+                              const double* __restrict__ hnp, const double* __restrict__ U,
+                              const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
+                              int* __restrict__ flags, int clamp) {
+  uint32_t P = gid();
+  bool changed = false;
+  if (P < npos) {
+    uint32_t k = perm[P];
+    double tk = taup[P];
+    double nh;
+    if (clamp) {
+      nh = hnp[P];
     } else {
-      tau_p = tau[p];
-      h_p = h[p];
-      tau_o = tau[o];
-      h_o = h[o];
-      double t = 0.0 + (tau_p + x[k]);
-      double nh = h_o + U[k] * rust_max(t - tau_o, 0.0);
-      double tm = tan_ms[k];
-      double d = dnext[k];
-      double slope = (nh - h_p) / d;
-      if (slope > tm) nh = h_p + tm * d;
-      double old = h[k];
-      changed |= (nh != old);
-      tau[k] = t;
-      h[k] = nh;
-      tau_p = t;
-      h_p = nh;
+      uint32_t o = jn[k];
+      double to = taup[head_base[o]];
+      double ho = h[o] + U[o] * rust_max(to - to, 0.0);
+      nh = (k == o) ? ho : ho + U[k] * rust_max(tk - to, 0.0);
     }
-    for (uint32_t t = 1; t < len; ++t) {
-      k = perm[base + t];
-      double tk = 0.0 + (tau_p + x[k]);
-      double nh = h_o + U[k] * rust_max(tk - tau_o, 0.0);
-      double tm = tan_ms[k];
-      double d = dnext[k];
-      double slope = (nh - h_p) / d;
-      if (slope > tm) nh = h_p + tm * d;
-      double old = h[k];
-      changed |= (nh != old);
-      tau[k] = tk;
-      h[k] = nh;
-      tau_p = tk;
-      h_p = nh;
-    }
+    tau[k] = tk;
+    h2[k] = nh;
+    changed = nh != h[k];
   }
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
+}
+
+// Zero the control block and the first `nb` bucket entries.
+__global__ void k_clear_iter(uint32_t nb, uint32_t* __restrict__ ctrl, uint32_t* __restrict__ bcnt,
+                             uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
+                             uint32_t* __restrict__ bncur) {
+  uint32_t i = gid();
+  if (i < FL_CTRL_WORDS) ctrl[i] = 0;
+  if (i < nb) {
+    bcnt[i] = 0;
+    bnodes[i] = 0;
+    bcur[i] = 0;
+    bncur[i] = 0;
+  }
 }
 
 // --------------------------------------------------------- launchers -----
@@ -572,43 +827,62 @@ void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 }
 
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, w.coff, w.ccnt, w.clist, w.hc,
-            w.counters + FL_CNT_CHILDREN);
+  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc);
   FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, w.jc, w.pc, w.jn, w.lw);
 }
 
-void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip) {
+void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip, uint32_t round) {
+  uint32_t* fset = w.ctrl + FL_CTRL_JUMPS + (round < FL_MAX_JUMP_ROUNDS ? round : FL_MAX_JUMP_ROUNDS - 1);
   if (!flip)
-    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc, w.pc, w.jn, w.lw, w.jc2, w.pc2, w.jn2, w.lw2, w.flags + FL_FLAG_JUMP);
+    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc, w.pc, w.jn, w.lw, w.jc2, w.pc2, w.jn2, w.lw2, fset);
   else
-    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc2, w.pc2, w.jn2, w.lw2, w.jc, w.pc, w.jn, w.lw, w.flags + FL_FLAG_JUMP);
+    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc2, w.pc2, w.jn2, w.lw2, w.jc, w.pc, w.jn, w.lw, fset);
 }
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   FL_LAUNCH(k_chain_len, t.n, s, t.n, w.hc, w.jc, w.pc, w.jn, t.is_outlet, w.clen);
-  FL_LAUNCH(k_bucket_count, t.n, s, t.n, w.jc, w.jn, w.lw, t.is_outlet, w.clen, w.bcnt, w.bnodes,
-            w.counters + FL_CNT_MAXLD);
+  uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
+  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.jc, w.jn, w.lw,
+                 t.is_outlet, w.clen, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
 }
 
-void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t nb) {
-  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), s, nb, w.bcnt, w.bnodes, w.boff, w.bnoff);
-  FL_LAUNCH(k_bucket_place, t.n, s, t.n, w.jc, w.jn, w.lw, t.is_outlet, w.clen, w.boff, w.bnoff, w.bcur, w.bncur,
-            w.chain_base, w.chain_len, w.head_base);
-  FL_LAUNCH(k_chain_perm, t.n, s, t.n, w.jc, w.pc, w.jn, t.is_outlet, w.head_base, w.perm);
+void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
+  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
+  uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
+  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.jc, w.jn, w.lw,
+                 t.is_outlet, w.clen, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
+  FL_LAUNCH(k_chain_perm, t.n, s, t.n, w.jc, w.pc, w.jn, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
+            t.area, w.A, w.tau, w.h, w.h2);
 }
 
-void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1) {
-  FL_LAUNCH(k_da_round, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.coff, w.ccnt, w.clist, w.hc,
-            t.area, w.A);
+void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
+                        uint32_t p0, uint32_t p1) {
+  FL_LAUNCH(k_da_pre, p1 - p0, s, p0, p1, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
+            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
+  FL_LAUNCH(k_da_chain, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
+            t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, w.A);
 }
 
-void fl_launch_celerity(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_celerity, t.n, s, t.n, w.jn, t.is_outlet, t.erod, t.area, w.A, w.dnext, w.x, w.tau);
+void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
+  FL_LAUNCH(k_tau_prep, npos, s, npos, w.perm, t.erod, w.A, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp,
+            w.dp, w.hop, clamp ? 1 : 0);
 }
 
-void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1) {
-  FL_LAUNCH(k_tau_round, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.next, w.jn, w.x, t.uplift,
-            t.tan_ms, w.dnext, w.tau, w.h, w.flags);
+void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
+                         bool clamp) {
+  (void)t;
+  FL_LAUNCH(k_tau_round, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp,
+            w.dp, w.hop, w.hnp, clamp ? 1 : 0);
+}
+
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
+  FL_LAUNCH(k_tau_scatter, npos, s, npos, w.perm, w.jn, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau,
+            w.flags, clamp ? 1 : 0);
+}
+
+void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {
+  uint32_t cnt = nb > FL_CTRL_WORDS ? nb : FL_CTRL_WORDS;
+  FL_LAUNCH(k_clear_iter, cnt, s, nb, w.ctrl, w.bcnt, w.bnodes, w.bcur, w.bncur);
 }
 
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v) {

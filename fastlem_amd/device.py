@@ -72,6 +72,11 @@ class DeviceTerrain:
         check(_native.lib().fl_terrain_download(self._ctx, self._h, ptr(out, ctypes.c_double)))
         return out
 
+    def last_stats(self) -> dict:
+        st = _native.SolveStats()
+        check(_native.lib().fl_terrain_last_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_ if f != "reserved"}
+
     def device_bytes(self) -> int:
         b = ctypes.c_uint64()
         check(_native.lib().fl_terrain_device_bytes(self._h, ctypes.byref(b)))

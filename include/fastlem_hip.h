@@ -93,6 +93,17 @@ int32_t fl_terrain_device_elevations(fl_terrain *t, double **dev_ptr);
 /* Bytes of HBM the terrain holds. */
 int32_t fl_terrain_device_bytes(const fl_terrain *t, uint64_t *bytes);
 
+/* Statistics of the last fl_terrain_solve on a terrain. */
+typedef struct fl_solve_stats {
+  uint32_t iterations;      /* outer iterations run */
+  uint32_t lake_iterations; /* iterations that joined lakes */
+  uint32_t host_syncs;      /* host-device synchronisations */
+  uint32_t reserved;
+  double visit_order_ms;    /* host time of the lake-traversal visit order */
+  double solve_ms;          /* wall time of the solve */
+} fl_solve_stats;
+int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
+
 /* Forget cached per-terrain host work (the lake-traversal visit order), so
  * the next solve recomputes it as a fresh generate() call would. */
 int32_t fl_terrain_invalidate_cache(fl_terrain *t);
