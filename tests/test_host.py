@@ -1,0 +1,237 @@
+"""Host-side product checks that need no GPU: the C ABI library loads and
+exports every declared symbol, the one-time setup helpers (StdRng restatement,
+Voronoi cells, adjacency rule, lake visit order) behave as specified, and the
+Python mirror of TerrainGenerator raises the reference's errors."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "fastlem_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(fl_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    from fastlem_amd import _native
+
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_declares_every_symbol():
+    from fastlem_amd import _native
+
+    L = _native.lib()
+    for n in declared_functions():
+        assert getattr(L, n).argtypes is not None or n in ("fl_version",), n
+
+
+def test_context_without_gpu_is_an_error():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from fastlem_amd import _native
+
+    h = ctypes.c_void_p()
+    rc = _native.lib().fl_ctx_create(0, ctypes.byref(h))
+    assert rc == -6  # FL_ERR_NO_DEVICE
+    assert b"no HIP device" in _native.lib().fl_last_error()
+
+
+# ------------------------------------------------------------ StdRng ------
+
+def test_chacha_block_rfc8439():
+    """RFC 8439 section 2.3.2 block-function test vector (20 rounds)."""
+    from fastlem_amd import _native
+
+    st = np.zeros(16, np.uint32)
+    st[:4] = [0x61707865, 0x3320646E, 0x79622D32, 0x6B206574]
+    st[4:12] = np.frombuffer(bytes(range(32)), "<u4")
+    st[12] = 1
+    st[13:16] = np.frombuffer(bytes.fromhex("000000090000004a00000000"), "<u4")
+    out = np.zeros(16, np.uint32)
+    _native.lib().fl_chacha_block(20, _native.ptr(st, ctypes.c_uint32), _native.ptr(out, ctypes.c_uint32))
+    assert out.astype("<u4").tobytes().hex() == (
+        "10f1e7e4d13b5915500fdd1fa32071c4c7d1f4c733c068030422aa9ac3d46c4e"
+        "d2826446079faa0914c2d705d98b02a2b5129cd1de164eb9cbd083e8a2503c4e")
+
+
+def test_chacha20_zero_key_keystream():
+    """Well-known first keystream block of ChaCha20 with zero key and nonce."""
+    from fastlem_amd import _native
+
+    st = np.zeros(16, np.uint32)
+    st[:4] = [0x61707865, 0x3320646E, 0x79622D32, 0x6B206574]
+    out = np.zeros(16, np.uint32)
+    _native.lib().fl_chacha_block(20, _native.ptr(st, ctypes.c_uint32), _native.ptr(out, ctypes.c_uint32))
+    assert out.astype("<u4").tobytes().hex().startswith("76b8e0ada0f13d90405d6ae55386bd28")
+
+
+def test_jitter_range_and_determinism():
+    from fastlem_amd.generator import jitter_elevations
+
+    base = np.linspace(0, 5, 1000)
+    a = jitter_elevations(base, 0)
+    b = jitter_elevations(base, 0)
+    assert np.array_equal(a, b)
+    d = a - base
+    eps = np.finfo(np.float64).eps
+    assert np.all(d >= 0) and np.all(d < eps * 1.0000001)
+    # with base 0 the value is an exact multiple of 2^-53 * eps
+    z = jitter_elevations(np.zeros(64), 0)
+    k = z / (eps * 2.0 ** -53)
+    assert np.array_equal(k, np.round(k))
+    assert not np.array_equal(jitter_elevations(np.zeros(64), 1), z)
+
+
+def test_random_sites_in_bounds():
+    from fastlem_amd.surface import Site2D, TerrainModel2DBulider
+
+    b = TerrainModel2DBulider.from_random_sites(500, Site2D(10.0, -5.0), Site2D(20.0, 5.0))
+    assert np.all((b._xs >= 10) & (b._xs < 20) & (b._ys >= -5) & (b._ys < 5))
+    b2 = TerrainModel2DBulider.from_random_sites(500, Site2D(10.0, -5.0), Site2D(20.0, 5.0))
+    assert np.array_equal(b._xs, b2._xs)
+
+
+# -------------------------------------------------------- model building ---
+
+@pytest.fixture(scope="module")
+def small_model():
+    from fastlem_amd.surface import Site2D, TerrainModel2DBulider
+
+    return TerrainModel2DBulider.from_random_sites(2000, Site2D(0, 0), Site2D(40, 20)).relaxate_sites(1).build()
+
+
+def test_voronoi_areas_tile_the_box(small_model):
+    assert np.all(small_model.areas() > 0)
+    assert small_model.areas().sum() == pytest.approx(40 * 20, rel=1e-9)
+
+
+def test_adjacency_rule(small_model):
+    """Each interior Delaunay edge appears in two triangles with opposite
+    orientation, so the a<b rule (builder.rs:257-265) adds it exactly once;
+    rows are symmetric, duplicate-free and carry the Euclidean length."""
+    m = small_model
+    rp, col, ln = m.graph()
+    n = m.num()
+    pairs = set()
+    for i in range(n):
+        row = col[rp[i]:rp[i + 1]].tolist()
+        assert len(row) == len(set(row)) and i not in row
+        for k in range(rp[i], rp[i + 1]):
+            j = int(col[k])
+            pairs.add((i, j))
+            assert ln[k] == np.sqrt((m.xs[i] - m.xs[j]) ** 2 + (m.ys[i] - m.ys[j]) ** 2)
+    assert all((j, i) in pairs for i, j in pairs)
+    from fastlem_amd.surface import delaunay_triangles
+
+    tris, hull = delaunay_triangles(m.xs, m.ys)
+    all_edges = set()
+    for a, b, c in tris.tolist():
+        for u, v in ((a, b), (b, c), (c, a)):
+            all_edges.add((min(u, v), max(u, v)))
+    # every kept edge is a Delaunay edge; only hull edges may be missing
+    assert {(min(i, j), max(i, j)) for i, j in pairs} <= all_edges
+    assert len(all_edges) - len(pairs) // 2 <= len(hull)
+
+
+def test_product_visit_order_matches_oracle(small_model):
+    from oracle import flo
+    from fastlem_amd import _native
+
+    m = small_model
+    rp, col, ln = m.graph()
+    rng = np.random.default_rng(5)
+    for k in (1, 7, 200):
+        outlets = rng.choice(m.num(), k, replace=False).astype(np.uint32)
+        ref = flo.visit_order(flo.Graph(rp, col, ln), outlets)
+        rank = np.empty(m.num(), np.uint32)
+        cnt = ctypes.c_uint64()
+        rc = _native.lib().fl_lake_visit_order(m.num(), _native.ptr(rp, ctypes.c_uint32),
+                                               _native.ptr(col, ctypes.c_uint32), _native.ptr(ln, ctypes.c_double),
+                                               _native.ptr(outlets, ctypes.c_uint32), len(outlets),
+                                               _native.ptr(rank, ctypes.c_uint32), ctypes.byref(cnt))
+        assert rc == 0 and cnt.value == len(ref)
+        assert np.array_equal(rank[ref], np.arange(len(ref)))
+
+
+def test_product_visit_order_with_ties_matches_oracle():
+    """A regular grid: many equal edge lengths, so the heap's tie order matters."""
+    from oracle import flo
+    from fastlem_amd import _native
+    from test_oracle_pinned import random_delaunay_graph
+
+    g, _ = random_delaunay_graph(400, 1, grid=True)
+    outlets = np.array([0, 37, 399, 123], np.uint32)
+    ref = flo.visit_order(g, outlets)
+    rank = np.empty(g.n, np.uint32)
+    cnt = ctypes.c_uint64()
+    rc = _native.lib().fl_lake_visit_order(g.n, _native.ptr(g.row_ptr, ctypes.c_uint32),
+                                           _native.ptr(g.col, ctypes.c_uint32),
+                                           _native.ptr(g.length, ctypes.c_double),
+                                           _native.ptr(outlets, ctypes.c_uint32), len(outlets),
+                                           _native.ptr(rank, ctypes.c_uint32), ctypes.byref(cnt))
+    assert rc == 0
+    assert np.array_equal(rank[ref], np.arange(len(ref)))
+
+
+# ------------------------------------------------------- Python mirror ----
+
+def test_generator_errors_match_reference():
+    from fastlem_amd import (InvalidNumberOfParameters, ModelNotSet, ParametersNotSet, TerrainGenerator,
+                             TopographicalParameters)
+    from fastlem_amd.workloads import config_plain
+
+    with pytest.raises(ModelNotSet):
+        TerrainGenerator().generate()
+    w = config_plain(300, 20.0, 1.0)
+    with pytest.raises(ParametersNotSet):
+        TerrainGenerator().set_model(w.model).generate()
+    with pytest.raises(InvalidNumberOfParameters):
+        TerrainGenerator().set_model(w.model).set_parameters([TopographicalParameters()] * 3).generate()
+
+
+def test_parameter_defaults():
+    from fastlem_amd import TopographicalParameters
+
+    p = TopographicalParameters()
+    assert (p.base_elevation, p.erodibility, p.uplift_rate, p.is_outlet, p.max_slope) == (0.0, 1.0, 1.0, False, None)
+    q = p.set_erodibility(2.0).set_is_outlet(True).set_max_slope(0.3)
+    assert (q.erodibility, q.is_outlet, q.max_slope) == (2.0, True, 0.3) and p.erodibility == 1.0
+
+
+def test_outlet_resolution_falls_back_to_hull():
+    from fastlem_amd import ParameterArrays, TerrainGenerator
+    from fastlem_amd.workloads import config_plain
+
+    w = config_plain(300, 20.0, 1.0)
+    n = w.n
+    g = TerrainGenerator().set_model(w.model).set_parameters(
+        ParameterArrays(np.zeros(n), np.ones(n), np.ones(n), np.zeros(n, bool)))
+    assert np.array_equal(g.resolve_outlets(), w.model.default_outlets())
+    flags = np.zeros(n, bool)
+    flags[[5, 2, 9]] = True
+    g.set_parameters(ParameterArrays(np.zeros(n), np.ones(n), np.ones(n), flags))
+    assert g.resolve_outlets().tolist() == [2, 5, 9]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stdout
