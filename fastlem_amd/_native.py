@@ -46,7 +46,7 @@ class SolveStats(ctypes.Structure):
     """Mirror of ``fl_solve_stats``."""
 
     _fields_ = [("iterations", ctypes.c_uint32), ("lake_iterations", ctypes.c_uint32),
-                ("host_syncs", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("host_syncs", ctypes.c_uint32), ("order_from_forest", ctypes.c_uint32),
                 ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
 
 
@@ -91,6 +91,8 @@ def _declare(lib):
         "fl_build_adjacency": (ctypes.c_int, [sz, c_f64p, c_f64p, sz, c_i64p, c_u32p, c_u32p, c_f64p]),
         "fl_lake_visit_order": (ctypes.c_int, [sz, c_u32p, c_u32p, c_f64p, c_u32p, sz, c_u32p,
                                                ctypes.POINTER(u64)]),
+        "fl_outlet_pop_order": (ctypes.c_int, [sz, c_u32p, c_u32p, c_u32p, sz, c_u32p]),
+        "fl_ranks_from_msf": (ctypes.c_int, [sz, c_u32p, sz, c_u32p, c_u32p, sz, c_u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

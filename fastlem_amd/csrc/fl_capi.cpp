@@ -73,7 +73,7 @@ struct fl_terrain {
   uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
   bool have_order = false;
   // statistics of the last solve
-  uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0;
+  uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0;
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   uint64_t bytes = 0;
   // host copies (relabelled) needed for the one-time visit order
@@ -161,18 +161,38 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   if (T->have_order) return FL_OK;
   auto t0 = std::chrono::steady_clock::now();
   uint32_t n = T->t.n;
+  size_t k = T->h_outlets.size();
   std::vector<uint32_t> rank(n), order(n, 0xffffffffu);
-  uint64_t cnt = 0;
-  int rc = fl_lake_visit_order(n, T->h_row_ptr.data(), T->h_col.data(), T->h_len.data(), T->h_outlets.data(),
-                               T->h_outlets.size(), rank.data(), &cnt);
-  if (rc == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
-  if (rc != FL_OK) return set_err(rc, "visit order failed");
+  bool fast = false;
+  if (k > 0) {
+    // outlet phase (exact heap tie handling) + Prim phase from the spanning forest
+    std::vector<uint32_t> first(k), a, b;
+    int rc = fl_outlet_pop_order(n, T->h_row_ptr.data(), T->h_col.data(), T->h_outlets.data(), k, first.data());
+    if (rc != FL_OK) return set_err(rc, "outlet phase failed");
+    std::string err;
+    rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, T->t.is_outlet, first[0], a, b,
+                       &fast, &err);
+    if (rc != FL_OK) return set_err(rc, "spanning forest: %s", err.c_str());
+    if (fast) {
+      rc = fl_ranks_from_msf(n, first.data(), k, a.data(), b.data(), a.size(), rank.data());
+      if (rc != FL_OK) return set_err(rc, "ranks from spanning forest failed");
+    }
+  }
+  if (!fast) {
+    // equal or non-positive lengths: simulate the heap exactly
+    uint64_t cnt = 0;
+    int rc = fl_lake_visit_order(n, T->h_row_ptr.data(), T->h_col.data(), T->h_len.data(), T->h_outlets.data(), k,
+                                 rank.data(), &cnt);
+    if (rc == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
+    if (rc != FL_OK) return set_err(rc, "visit order failed");
+  }
   for (uint32_t i = 0; i < n; i++)
     if (rank[i] != 0xffffffffu) order[rank[i]] = i;
   HIPCHK(hipMemcpyAsync(T->t.rank, rank.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(T->t.order, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   T->have_order = true;
+  T->stat_order_fast = fast ? 1 : 0;
   T->stat_order_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return FL_OK;
 }
@@ -654,7 +674,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->iterations = T->stat_iterations;
   out->lake_iterations = T->stat_lake_iterations;
   out->host_syncs = T->stat_syncs;
-  out->reserved = 0;
+  out->order_from_forest = T->stat_order_fast;
   out->visit_order_ms = T->stat_order_ms;
   out->solve_ms = T->stat_solve_ms;
   return FL_OK;

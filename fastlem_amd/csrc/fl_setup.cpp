@@ -380,3 +380,161 @@ extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint
   if (visited_out) *visited_out = cnt;
   return FL_OK;
 }
+
+// ------------------------------------------- visit order, fast form ----
+//
+// The lake traversal (stream_tree.rs:184-240) first pops every outlet (all
+// pushed with key 0.0, so they share the maximum priority), then behaves as
+// Prim's algorithm from the outlet set.  With positive, pairwise distinct
+// edge lengths the second phase has a unique order, computable from a
+// minimum spanning forest; the first phase depends on the heap's handling of
+// equal keys, which is simulated exactly below.
+
+// Outlet phase.  Elements are either Z (key 0: an outlet) or P (positive
+// key: a pushed edge).  Z outranks P, so Z elements occupy a root-connected
+// set of heap positions; pushes append P elements that never enter it, and a
+// pop shifts the Z elements on the sift-down path up by one level (on Z|Z
+// the path takes the right child, on Z|P the Z child, and it leaves the Z
+// set at a P|P pair).  The element moved from the end, if it is Z, settles in
+// the vacated lowest Z position of that path.  None of this depends on the
+// values of P elements, so only their number (the unvisited neighbours of
+// each popped outlet) is needed.
+extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* outlets,
+                                   size_t k, uint32_t* order_out) {
+  const uint32_t NONE = 0xffffffffu;
+  std::vector<uint32_t> zid(outlets, outlets + k);
+  std::vector<uint8_t> visited(n, 0);
+  for (size_t q = 0; q < k; q++)
+    if (outlets[q] >= n) return FL_ERR_INVALID;
+  uint64_t len = k;
+  for (size_t step = 0; step < k; step++) {
+    uint64_t L = len;
+    uint32_t result = zid[0];
+    uint64_t last = L - 1;
+    uint32_t last_z = last < k ? zid[last] : NONE;
+    if (last < k) zid[last] = NONE;
+    len = L - 1;
+    if (len > 0) {
+      uint64_t end = L - 1, h = 0;
+      for (;;) {
+        uint64_t c1 = 2 * h + 1, c2 = c1 + 1, c;
+        if (c1 >= end) break;
+        bool z1 = c1 < k && zid[c1] != NONE;
+        if (c2 < end) {
+          bool z2 = c2 < k && zid[c2] != NONE;
+          if (z2) c = c2;
+          else if (z1) c = c1;
+          else break;
+        } else {
+          if (!z1) break;
+          c = c1;
+        }
+        zid[h] = zid[c];
+        h = c;
+      }
+      zid[h] = last_z;
+    }
+    if (result == NONE) return FL_ERR_INVALID;  // cannot happen with distinct outlets
+    order_out[step] = result;
+    visited[result] = 1;
+    uint64_t pushes = 0;
+    for (uint32_t e = row_ptr[result]; e < row_ptr[result + 1]; e++) pushes += visited[col[e]] ? 0 : 1;
+    len += pushes;
+  }
+  return FL_OK;
+}
+
+// Prim phase from a minimum spanning forest of the graph with all outlets
+// contracted into one vertex S.  msf_a/msf_b: the forest's edges sorted by
+// increasing length.  Kruskal's merges build the reconstruction tree; Prim
+// from S completes the side of each merge edge that contains the entry
+// point before crossing to the other side, which it enters at the edge's
+// endpoint there.  Ranks of the outlets are 0..k-1 in outlet_order; other
+// reached sites get k.. in Prim order; unreached sites UINT32_MAX.
+extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t k, const uint32_t* msf_a,
+                                 const uint32_t* msf_b, size_t m, uint32_t* rank_out) {
+  const uint32_t NONE = 0xffffffffu;
+  for (size_t i = 0; i < n; i++) rank_out[i] = NONE;
+  if (k == 0) return FL_OK;
+  std::vector<uint8_t> is_out(n, 0);
+  for (size_t q = 0; q < k; q++) {
+    if (outlet_order[q] >= n) return FL_ERR_INVALID;
+    is_out[outlet_order[q]] = 1;
+    rank_out[outlet_order[q]] = (uint32_t)q;
+  }
+  const uint32_t s_leaf = outlet_order[0];
+  auto leaf_of = [&](uint32_t v) { return is_out[v] ? s_leaf : v; };
+  // union-find over sites (outlets pre-united), nodes: leaves 0..n-1, merges n..n+m-1
+  std::vector<uint32_t> uf(n), top(n);
+  for (size_t v = 0; v < n; v++) {
+    uf[v] = (uint32_t)v;
+    top[v] = (uint32_t)v;
+  }
+  for (size_t q = 0; q < k; q++) uf[outlet_order[q]] = s_leaf;
+  auto find = [&](uint32_t v) {
+    uint32_t r = v;
+    while (uf[r] != r) r = uf[r];
+    while (uf[v] != r) {
+      uint32_t nx = uf[v];
+      uf[v] = r;
+      v = nx;
+    }
+    return r;
+  };
+  size_t total = n + m;
+  std::vector<uint32_t> ch0(m), ch1(m), ea(m), eb(m);
+  std::vector<uint32_t> cnt(total), size(total), lo(total, NONE), entry(total, NONE), off(total, 0);
+  for (size_t v = 0; v < n; v++) {
+    cnt[v] = 1;
+    size[v] = is_out[v] ? 0 : 1;
+  }
+  std::vector<uint8_t> is_child(total, 0);
+  for (size_t i = 0; i < m; i++) {
+    uint32_t a = msf_a[i], b = msf_b[i];
+    if (a >= n || b >= n) return FL_ERR_INVALID;
+    uint32_t ra = find(a), rb = find(b);
+    if (ra == rb) return FL_ERR_INVALID;  // not a forest
+    uint32_t x = (uint32_t)(n + i);
+    ch0[i] = top[ra];
+    ch1[i] = top[rb];
+    ea[i] = a;
+    eb[i] = b;
+    is_child[ch0[i]] = 1;
+    is_child[ch1[i]] = 1;
+    cnt[x] = cnt[ch0[i]] + cnt[ch1[i]];
+    size[x] = size[ch0[i]] + size[ch1[i]];
+    uf[rb] = ra;
+    top[ra] = x;
+  }
+  uint32_t root = top[find(s_leaf)];
+  // leaf intervals (top-down; parents have larger indices than children)
+  lo[root] = 0;
+  for (size_t x = total; x-- > n;) {
+    if (lo[x] == NONE) continue;
+    size_t i = x - n;
+    lo[ch0[i]] = lo[x];
+    lo[ch1[i]] = lo[x] + cnt[ch0[i]];
+  }
+  auto inside = [&](uint32_t node, uint32_t v) {
+    uint32_t l = lo[leaf_of(v)];
+    return l >= lo[node] && l < lo[node] + cnt[node];
+  };
+  // entries and rank offsets
+  entry[root] = s_leaf;
+  off[root] = (uint32_t)k;
+  for (size_t x = total; x-- > n;) {
+    if (entry[x] == NONE) continue;
+    size_t i = x - n;
+    uint32_t p = entry[x];
+    uint32_t first = inside(ch0[i], p) ? ch0[i] : ch1[i];
+    uint32_t second = first == ch0[i] ? ch1[i] : ch0[i];
+    uint32_t enter = inside(second, ea[i]) ? ea[i] : eb[i];
+    entry[first] = p;
+    off[first] = off[x];
+    entry[second] = leaf_of(enter);
+    off[second] = off[x] + size[first];
+  }
+  for (size_t v = 0; v < n; v++)
+    if (!is_out[v] && entry[v] != NONE) rank_out[v] = off[v];
+  return FL_OK;
+}

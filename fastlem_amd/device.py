@@ -75,7 +75,7 @@ class DeviceTerrain:
     def last_stats(self) -> dict:
         st = _native.SolveStats()
         check(_native.lib().fl_terrain_last_stats(self._h, ctypes.byref(st)))
-        return {f: getattr(st, f) for f, _ in st._fields_ if f != "reserved"}
+        return {f: getattr(st, f) for f, _ in st._fields_}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64()

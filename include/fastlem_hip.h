@@ -98,7 +98,8 @@ typedef struct fl_solve_stats {
   uint32_t iterations;      /* outer iterations run */
   uint32_t lake_iterations; /* iterations that joined lakes */
   uint32_t host_syncs;      /* host-device synchronisations */
-  uint32_t reserved;
+  uint32_t order_from_forest; /* 1: visit order from the spanning forest,
+                                 0: exact heap simulation (ties present) */
   double visit_order_ms;    /* host time of the lake-traversal visit order */
   double solve_ms;          /* wall time of the solve */
 } fl_solve_stats;
@@ -152,6 +153,17 @@ int fl_build_adjacency(size_t n, const double *xs, const double *ys, size_t ntri
  * rank_out[i] = first-pop position of site i, UINT32_MAX if never reached. */
 int fl_lake_visit_order(size_t n, const uint32_t *row_ptr, const uint32_t *col, const double *len,
                         const uint32_t *outlets, size_t n_outlets, uint32_t *rank_out, uint64_t *visited_out);
+
+/* Pop order of the outlets in the traversal's first phase (all keys 0):
+ * exact std BinaryHeap tie handling, driven only by how many unvisited
+ * neighbours each popped outlet pushes (stream_tree.rs:185-237). */
+int fl_outlet_pop_order(size_t n, const uint32_t *row_ptr, const uint32_t *col, const uint32_t *outlets, size_t k,
+                        uint32_t *order_out);
+/* Visit ranks from a minimum spanning forest of the graph with the outlets
+ * contracted (edges sorted by increasing length; lengths positive and
+ * pairwise distinct): outlets 0..k-1 in outlet_order, then Prim order. */
+int fl_ranks_from_msf(size_t n, const uint32_t *outlet_order, size_t k, const uint32_t *msf_a, const uint32_t *msf_b,
+                      size_t m, uint32_t *rank_out);
 
 #ifdef __cplusplus
 }

@@ -117,3 +117,102 @@ def test_solve_is_deterministic(fbm20k):
     a = t.elevations()
     t.solve()
     assert _bits_equal(a, t.elevations())
+
+
+def test_visit_order_from_forest_is_used_and_exact(fbm20k):
+    """Random geometry: lengths distinct, the spanning-forest form applies."""
+    t = _dev(fbm20k)
+    t.solve()
+    st = t.last_stats()
+    assert st["order_from_forest"] == 1 and st["lake_iterations"] >= 1
+
+
+def _grid_terrain(side, outlet_band, jitter=0.0, seed=0):
+    from test_oracle_pinned import random_delaunay_graph
+
+    g, pts = random_delaunay_graph(side * side, seed, grid=True)
+    n = g.n
+    xs, ys = pts[:, 0], pts[:, 1]
+    out = np.nonzero((xs < outlet_band) | (ys < outlet_band))[0].astype(np.uint32)
+    return g, out
+
+
+def test_generate_grid_with_equal_lengths():
+    """A regular grid: many equal edge lengths, so the exact heap simulation
+    decides the lake joins (ties beyond the outlet phase)."""
+    from oracle import flo
+    from fastlem_amd.device import DeviceTerrain
+    from fastlem_amd.generator import jitter_elevations
+
+    g, out = _grid_terrain(40, 1.0)
+    n = g.n
+    init = jitter_elevations(np.zeros(n), 0)
+    area = np.ones(n)
+    K = np.ones(n)
+    ref, iters, _ = flo.generate(g, area, K, K, init, out)
+    t = DeviceTerrain(g.row_ptr, g.col, g.length, area, K, K, init, out)
+    assert t.solve() == iters
+    assert t.last_stats()["order_from_forest"] == 0
+    assert _bits_equal(ref, t.elevations())
+
+
+def test_generate_drop_in_default_outlets():
+    """TerrainGenerator.generate() (fl_generate_f64) with no flagged outlet:
+    the hull becomes the outlet set (generator.rs:127-129)."""
+    from oracle import flo
+    from fastlem_amd import ParameterArrays, TerrainGenerator
+    from fastlem_amd.generator import jitter_elevations
+    from fastlem_amd.workloads import config_plain
+
+    w = config_plain(3000, 55.0, 1.0)
+    n = w.n
+    gen = TerrainGenerator().set_model(w.model).set_parameters(
+        ParameterArrays(np.zeros(n), np.full(n, 2.0), np.full(n, 0.5), np.zeros(n, bool)))
+    terrain = gen.generate()
+    hull = w.model.default_outlets()
+    ref, iters, _ = flo.generate(graph_of(w), w.model.areas(), np.full(n, 2.0), np.full(n, 0.5),
+                                 jitter_elevations(np.zeros(n), 0), hull)
+    assert gen.last_iterations == iters
+    assert _bits_equal(ref, terrain.elevations())
+
+
+def test_unreachable_component_keeps_elevation():
+    from oracle import flo
+    from fastlem_amd.device import DeviceTerrain
+    from test_oracle_pinned import csr_from_edges, random_delaunay_graph
+
+    g1, _ = random_delaunay_graph(300, 21)
+    g2, _ = random_delaunay_graph(200, 22)
+    # disjoint union: the second component has no outlet
+    edges = []
+    for g, off in ((g1, 0), (g2, g1.n)):
+        for i in range(g.n):
+            for k in range(g.row_ptr[i], g.row_ptr[i + 1]):
+                j = int(g.col[k])
+                if i < j:
+                    edges.append((i + off, j + off, float(g.length[k])))
+    g = csr_from_edges(g1.n + g2.n, edges)
+    n = g.n
+    rng = np.random.default_rng(4)
+    init = rng.uniform(0, 1e-9, n)
+    out = np.array([0, 7, 19], np.uint32)
+    area, K = np.ones(n), np.ones(n)
+    ref, iters, _ = flo.generate(g, area, K, K, init, out)
+    t = DeviceTerrain(g.row_ptr, g.col, g.length, area, K, K, init, out)
+    assert t.solve() == iters
+    got = t.elevations()
+    assert _bits_equal(ref, got)
+    assert np.array_equal(got[g1.n:], init[g1.n:])
+
+
+def test_max_iteration_is_honoured(plain10k):
+    from oracle import flo
+
+    w = plain10k
+    init = init_of(w)
+    for cap in (1, 3):
+        ref, iters, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init, w.outlets(),
+                                     max_iteration=cap)
+        t = _dev(w, max_iteration=cap)
+        assert t.solve() == iters == cap
+        assert _bits_equal(ref, t.elevations())

@@ -235,3 +235,76 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+# ------------------------------------------- visit order, fast form ------
+
+def _outlet_order(g, outlets):
+    from fastlem_amd import _native
+
+    out = np.empty(len(outlets), np.uint32)
+    rc = _native.lib().fl_outlet_pop_order(g.n, _native.ptr(g.row_ptr, ctypes.c_uint32),
+                                           _native.ptr(g.col, ctypes.c_uint32),
+                                           _native.ptr(np.ascontiguousarray(outlets, np.uint32), ctypes.c_uint32),
+                                           len(outlets), _native.ptr(out, ctypes.c_uint32))
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("seed,k,grid", [(0, 1, False), (1, 5, False), (2, 40, False), (3, 150, False),
+                                         (4, 60, True), (5, 399, False)])
+def test_outlet_phase_matches_heap_simulation(seed, k, grid):
+    """The first k pops of the full Rust-heap simulation are the outlets; the
+    shape-only simulation must give the same order (outlets adjacent to each
+    other, many pushes, grids with equal lengths included)."""
+    from oracle import flo
+    from test_oracle_pinned import random_delaunay_graph
+
+    g, _ = random_delaunay_graph(400, 300 + seed, grid=grid)
+    rng = np.random.default_rng(seed)
+    outlets = rng.choice(g.n, k, replace=False).astype(np.uint32)
+    ref = flo.visit_order(g, outlets)[:k]
+    assert np.array_equal(_outlet_order(g, outlets), ref)
+
+
+def _contracted_msf(g, outlets):
+    """Minimum spanning forest of the graph with the outlets merged (scipy)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    r0 = int(outlets[0])
+    is_out = np.zeros(g.n, bool)
+    is_out[outlets] = True
+    owner = np.repeat(np.arange(g.n), np.diff(g.row_ptr.astype(np.int64)))
+    a = np.where(is_out[owner], r0, owner)
+    b = np.where(is_out[g.col], r0, g.col.astype(np.int64))
+    keep = a < b
+    best = {}
+    for x, y, w in zip(a[keep], b[keep], g.length[keep]):
+        if (x, y) not in best or w < best[(x, y)]:
+            best[(x, y)] = w
+    xs = np.array([k[0] for k in best]), np.array([k[1] for k in best])
+    mst = minimum_spanning_tree(coo_matrix((list(best.values()), xs), shape=(g.n, g.n))).tocoo()
+    order = np.argsort(mst.data, kind="stable")
+    return mst.row[order].astype(np.uint32), mst.col[order].astype(np.uint32)
+
+
+@pytest.mark.parametrize("seed,k", [(0, 1), (1, 4), (2, 30), (3, 120)])
+def test_ranks_from_msf_match_heap_simulation(seed, k):
+    from oracle import flo
+    from fastlem_amd import _native
+    from test_oracle_pinned import random_delaunay_graph
+
+    g, _ = random_delaunay_graph(500, 400 + seed)
+    rng = np.random.default_rng(seed)
+    outlets = rng.choice(g.n, k, replace=False).astype(np.uint32)
+    ref = flo.visit_order(g, outlets)
+    ma, mb = _contracted_msf(g, outlets)
+    first = _outlet_order(g, outlets)
+    rank = np.empty(g.n, np.uint32)
+    rc = _native.lib().fl_ranks_from_msf(g.n, _native.ptr(first, ctypes.c_uint32), k,
+                                         _native.ptr(ma, ctypes.c_uint32), _native.ptr(mb, ctypes.c_uint32),
+                                         len(ma), _native.ptr(rank, ctypes.c_uint32))
+    assert rc == 0
+    assert np.array_equal(rank[ref], np.arange(len(ref)))
+    assert len(ref) == g.n
