@@ -260,8 +260,8 @@ int32_t decompose(fl_ctx* c, fl_terrain* T) {
   fl_launch_chain_count(s, T->t, w);
   fl_launch_chain_layout(s, T->t, w);
   FLCHK(read_ctrl(c, T));
-  uint32_t nb = host_counter(c, FL_CNT_MAXLD) + 1;
-  if (nb > n + 1) return set_err(FL_ERR_HIP, "light depth out of range");
+  uint32_t nb = 2 * (host_counter(c, FL_CNT_MAXLD) + 1);  // short and long bucket per light depth
+  if (nb > 2 * n + 2) return set_err(FL_ERR_HIP, "light depth out of range");
   T->prev_nb = nb;
   c->host_boff.resize(2 * (nb + 1));
   if (nb <= FL_CTRL_HEAD) {
@@ -285,13 +285,13 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
-  fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, n + 1));
+  fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
   fl_launch_receivers(s, T->t, w);
   FLCHK(decompose(c, T));
   *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
   if (*had_lake) {
     FLCHK(join_lakes(c, T));
-    fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, n + 1));
+    fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
     FLCHK(decompose(c, T));
   }
   uint32_t nb = T->prev_nb;
@@ -299,9 +299,10 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   const uint32_t* bn = bo + nb + 1;
   uint32_t npos = bn[nb];
   bool clamp = T->t.any_max_slope;
-  for (uint32_t l = nb; l-- > 0;) fl_launch_da_round(s, T->t, w, bo[l], bo[l + 1], bn[l], bn[l + 1]);
+  uint32_t rounds = nb / 2;
+  for (uint32_t l = rounds; l-- > 0;) fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2]);
   fl_launch_tau_prep(s, T->t, w, npos, clamp);
-  for (uint32_t l = 0; l < nb; l++) fl_launch_tau_round(s, T->t, w, bo[l], bo[l + 1], clamp);
+  for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], clamp);
   fl_launch_tau_scatter(s, T->t, w, npos, clamp);
   FLCHK(read_ctrl(c, T));
   *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
@@ -454,15 +455,16 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   bool clamp = false;
   for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.pre, &w.post0, &w.post1, &w.xp, &w.taup})
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup})
     FLCHK(dev_alloc(P, p, n));
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
                        &w.jn2, &w.lw2, &w.clen, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
   FLCHK(dev_alloc(P, &w.cmask, n));
-  for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur}) FLCHK(dev_alloc(P, p, (uint64_t)n + 2));
+  for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur})
+    FLCHK(dev_alloc(P, p, 2 * (uint64_t)n + 4));
   FLCHK(dev_alloc(P, &w.lake_key, n));
   FLCHK(dev_alloc(P, &w.ctrl, FL_CTRL_WORDS));
   w.flags = (int*)w.ctrl;
@@ -480,10 +482,10 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   HIPCHK(hipMemcpyAsync(t.tan_ms, tan_ms.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.init_h, init.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.is_outlet, is_out.data(), n, hipMemcpyHostToDevice, s));
-  fl_launch_fill_u32(s, w.bcnt, (uint64_t)n + 2, 0);
-  fl_launch_fill_u32(s, w.bnodes, (uint64_t)n + 2, 0);
-  fl_launch_fill_u32(s, w.bcur, (uint64_t)n + 2, 0);
-  fl_launch_fill_u32(s, w.bncur, (uint64_t)n + 2, 0);
+  fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bnodes, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bcur, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bncur, 2 * (uint64_t)n + 4, 0);
   T->prev_nb = 0;
   FLCHK(reset_state(c, P));
   // the pageable staging vectors above must outlive the copies
@@ -589,7 +591,7 @@ extern "C" int32_t fl_stage_receivers(fl_ctx* c, fl_terrain* T, const double* el
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
-  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, T->t.n + 1));
+  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
   fl_launch_receivers(c->stream, T->t, T->w);
   return download_idx_old(c, T, next_out, T->w.next);
 }
@@ -601,7 +603,7 @@ extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* 
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
-  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, T->t.n + 1));
+  fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
   fl_launch_receivers(c->stream, T->t, T->w);
   FLCHK(read_ctrl(c, T));
   bool lake = host_flag(c, FL_FLAG_LAKE) != 0;

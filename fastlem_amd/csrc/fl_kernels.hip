@@ -14,11 +14,12 @@
 //     k_children        children of each site in row (slot) order
 //     k_chain_init/jump chain head, position in chain, light depth, root
 //     k_chain_*         chains bucketed by light depth, laid out contiguously
-//   sweeps
-//     k_da_round        generator.rs:154-159  drainage area, exact fold order
-//     k_celerity        generator.rs:172      1/(K*A^0.5) * distance
-//     k_tau_round       generator.rs:162-203  response time, elevation,
-//                                             max-slope clamp, changed flag
+//   sweeps, one round per light depth (deepest first for drainage area)
+//     k_da_short/long   generator.rs:154-159  drainage area, exact fold order
+//     k_tau_prep        generator.rs:172      1/(K*A^0.5) * distance
+//     k_tau_round/long  generator.rs:162-203  response time (+ clamped
+//                                             elevation), head to tail
+//     k_tau_scatter     generator.rs:177-202  elevation, changed flag
 //
 // Exactness.  Every floating-point expression is evaluated in the order the
 // reference writes it; the library is built with -ffp-contract=off so no
@@ -266,6 +267,10 @@ __global__ void k_chain_len(uint32_t n, const uint32_t* __restrict__ hc, const u
   if (hc[v] == FL_NONE && is_outlet[jn[v]]) clen[jc[v]] = pc[v] + 1;
 }
 
+// Buckets: 2 * light depth + (chain length >= FL_LONG), so every round has a
+// range of short chains (one lane each) followed by long ones (one
+// wavefront each).
+#define FL_LONG 16
 #define FL_LDS_BUCKETS 128
 #define FL_BUCKET_BLOCK 1024
 #define FL_BUCKET_PER_THREAD 4
@@ -289,7 +294,7 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     if (v < n && jc[v] == v && is_outlet[jn[v]]) {
-      uint32_t l = lw[v], len = clen[v];
+      uint32_t len = clen[v], l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
       if (l < FL_LDS_BUCKETS) {
         atomicAdd(s_c + l, 1u);
         atomicAdd(s_n + l, len);
@@ -297,7 +302,7 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
         atomicAdd(bcnt + l, 1u);
         atomicAdd(bnodes + l, len);
       }
-      atomicMax(&s_max, l);
+      atomicMax(&s_max, lw[v]);
     }
   }
   __syncthreads();
@@ -311,7 +316,7 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
 }
 
 // Exclusive scans of the bucket histograms over the light depths actually
-// present (counters[FL_CNT_MAXLD] + 1), one workgroup.  The first
+// present (2 * (counters[FL_CNT_MAXLD] + 1) buckets), one workgroup.  The first
 // FL_CTRL_HEAD + 1 offsets are mirrored into the control block.
 __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uint32_t* __restrict__ bcnt,
                               const uint32_t* __restrict__ bnodes, uint32_t* __restrict__ boff,
@@ -319,8 +324,8 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uin
   __shared__ uint32_t s_a[1024], s_b[1024];
   __shared__ uint32_t carry_a, carry_b;
   uint32_t* counters = ctrl + FL_NUM_FLAGS;
-  uint32_t nb = counters[FL_CNT_MAXLD] + 1;
-  if (nb > n + 1) nb = n + 1;
+  uint32_t nb = 2 * (counters[FL_CNT_MAXLD] + 1);
+  if (nb > 2 * n + 2) nb = 2 * n + 2;
   uint32_t* head_c = counters + FL_NUM_COUNTERS;
   uint32_t* head_n = head_c + FL_CTRL_HEAD + 1;
   if (threadIdx.x == 0) {
@@ -394,7 +399,7 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
     lc[j] = FL_NONE;
     ln[j] = 0;
     if (v < n && jc[v] == v && is_outlet[jn[v]]) {
-      uint32_t l = lw[v], len = clen[v];
+      uint32_t len = clen[v], l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
       if (l < FL_LDS_BUCKETS) {
         lc[j] = atomicAdd(s_c + l, 1u);
         ln[j] = atomicAdd(s_n + l, len);
@@ -418,7 +423,7 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     if (lc[j] == FL_NONE) continue;
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    uint32_t l = lw[v];
+    uint32_t l = 2u * lw[v] + (clen[v] >= FL_LONG ? 1u : 0u);
     uint32_t c = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
     chain_base[c] = pos;
@@ -470,24 +475,27 @@ __device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uin
 
 }  // namespace
 
-// Drainage area, part 1, for the sites of one light-depth round (positions
-// [p0, p1) of the chain-major order): fold the site's own area with the
-// children that precede the heavy child in the reference's accumulation
-// order (reverse row order), and collect the light children that follow it.
-// Light children head chains of the next deeper round, already finished.
-__global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ perm,
-                         const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-                         const uint32_t* __restrict__ hc, const double* __restrict__ area,
-                         const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
-                         double* __restrict__ post1, uint32_t* __restrict__ npost) {
-  uint32_t P = p0 + gid();
-  if (P >= p1) return;
-  uint32_t k = perm[P];
+#define FL_BATCH 8
+
+// Drainage area of the site at chain position P, folded in the reference's
+// order (generator.rs:154-159 accumulates children in reverse BFS order, so
+// A_k = ((a_k + A_{c_m}) + A_{c_{m-1}}) + ... + A_{c_1} over the children in
+// row order).  The heavy child's value is not known here: the fold is split
+// into `pre` (the part before it) and the light children after it
+// (post0, post1, npost; more than two are re-read by da_post_rest).  Light
+// children head chains of a deeper round, already finished.
+__device__ __forceinline__ void da_partial(uint32_t k, const uint32_t* __restrict__ rp,
+                                           const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                           const unsigned long long* __restrict__ cmask,
+                                           const uint32_t* __restrict__ hc, const double* __restrict__ area,
+                                           const double* __restrict__ A, double& pre, double& pa, double& pb,
+                                           uint32_t& np) {
   uint32_t b = rp[k], deg = rp[k + 1] - b, hk = hc[k];
   unsigned long long m = cmask[k];
-  double acc = area[k], pa = 0.0, pb = 0.0;
-  uint32_t np = 0;
+  double acc = area[k];
+  pa = 0.0;
+  pb = 0.0;
+  np = 0;
   bool after = false;
   for (uint32_t s = deg; s-- > 0;) {
     if (!is_child(m, deg, s, b, col, next, k)) continue;
@@ -496,18 +504,16 @@ __global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ 
       after = true;
       continue;
     }
+    double v = A[c];
     if (!after) {
-      acc = acc + A[c];
+      acc = acc + v;
     } else {
-      if (np == 0) pa = A[c];
-      else if (np == 1) pb = A[c];
+      if (np == 0) pa = v;
+      else if (np == 1) pb = v;
       ++np;
     }
   }
-  pre[P] = acc;
-  post0[P] = pa;
-  post1[P] = pb;
-  npost[P] = np;
+  pre = acc;
 }
 
 // The light children after the heavy one beyond the first two (rare).
@@ -534,51 +540,87 @@ __device__ __noinline__ double da_post_rest(double r, uint32_t k, const uint32_t
   return r;
 }
 
-#define FL_BATCH 8
-
-// Drainage area, part 2: one lane per chain of the round, tail to head:
-// A = pre (tail), A = (pre + A_heavy) + post0 + post1 + ... (others).
-// Inputs for FL_BATCH positions are loaded before the dependent adds.
-__global__ void k_da_chain(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
+// Drainage area of the short chains (length < FL_LONG) of one round, one
+// lane per chain, tail to head; results stay in registers until the chain
+// is done, so the loads of all its sites can be in flight together.
+__global__ void k_da_short(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
-                           const double* __restrict__ pre, const double* __restrict__ post0,
-                           const double* __restrict__ post1, const uint32_t* __restrict__ npost,
                            const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-                           const uint32_t* __restrict__ hc, double* __restrict__ A) {
+                           const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A) {
   uint32_t c = c0 + gid();
   if (c >= c1) return;
   uint32_t base = chain_base[c], len = chain_len[c];
+  uint32_t ks[FL_LONG];
+  double out[FL_LONG];
+#pragma unroll
+  for (uint32_t q = 0; q < FL_LONG; ++q)
+    if (q < len) ks[q] = perm[base + q];
   double r = 0.0;
-  uint32_t t = len;  // positions still to do: [base, base + t)
+#pragma unroll
+  for (uint32_t qq = 0; qq < FL_LONG; ++qq) {
+    uint32_t q = FL_LONG - 1 - qq;
+    if (q < len) {
+      double pre, pa, pb;
+      uint32_t np;
+      da_partial(ks[q], rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
+      r = (q == len - 1) ? pre : (pre + r);  // the tail has no heavy child
+      if (np >= 1) r = r + pa;
+      if (np >= 2) r = r + pb;
+      if (np >= 3) r = da_post_rest(r, ks[q], rp, col, next, cmask, hc, A);
+      out[q] = r;
+    }
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < FL_LONG; ++q)
+    if (q < len) A[ks[q]] = out[q];
+}
+
+// Drainage area of one long chain per 64-lane workgroup: for each block of
+// 64 positions (from the tail) every lane folds its own site's light
+// children, then one lane runs the exact sequential fold along the chain
+// from LDS, and the lanes store the results.
+__global__ void __launch_bounds__(64)
+k_da_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A) {
+  __shared__ double s_pre[64], s_pa[64], s_pb[64], s_out[64];
+  __shared__ uint32_t s_np[64], s_k[64];
+  uint32_t c = c0 + blockIdx.x;
+  uint32_t lane = threadIdx.x;
+  uint32_t base = chain_base[c], len = chain_len[c];
+  double r = 0.0;
   bool first = true;
-  while (t > 0) {
-    uint32_t cnt = t < FL_BATCH ? t : FL_BATCH;
-    double vp[FL_BATCH], va[FL_BATCH], vb[FL_BATCH];
-    uint32_t vn[FL_BATCH], vk[FL_BATCH];
-#pragma unroll
-    for (uint32_t q = 0; q < FL_BATCH; ++q) {
-      if (q < cnt) {
-        uint32_t P = base + t - 1 - q;
-        vp[q] = pre[P];
-        va[q] = post0[P];
-        vb[q] = post1[P];
-        vn[q] = npost[P];
-        vk[q] = perm[P];
-      }
+  for (uint32_t end = len; end > 0;) {
+    uint32_t cs = end > 64 ? end - 64 : 0, cnt = end - cs;
+    if (lane < cnt) {
+      uint32_t k = perm[base + cs + lane];
+      double pre, pa, pb;
+      uint32_t np;
+      da_partial(k, rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
+      s_pre[lane] = pre;
+      s_pa[lane] = pa;
+      s_pb[lane] = pb;
+      s_np[lane] = np;
+      s_k[lane] = k;
     }
-#pragma unroll
-    for (uint32_t q = 0; q < FL_BATCH; ++q) {
-      if (q < cnt) {
-        r = first ? vp[q] : (vp[q] + r);
+    __syncthreads();
+    if (lane == 0) {
+      for (uint32_t j = cnt; j-- > 0;) {
+        r = first ? s_pre[j] : (s_pre[j] + r);
         first = false;
-        if (vn[q] >= 1) r = r + va[q];
-        if (vn[q] >= 2) r = r + vb[q];
-        if (vn[q] >= 3) r = da_post_rest(r, vk[q], rp, col, next, cmask, hc, A);
-        A[vk[q]] = r;
+        uint32_t np = s_np[j];
+        if (np >= 1) r = r + s_pa[j];
+        if (np >= 2) r = r + s_pb[j];
+        if (np >= 3) r = da_post_rest(r, s_k[j], rp, col, next, cmask, hc, A);
+        s_out[j] = r;
       }
     }
-    t -= cnt;
+    __syncthreads();
+    if (lane < cnt) A[s_k[lane]] = s_out[lane];
+    __syncthreads();
+    end = cs;
   }
 }
 
@@ -690,6 +732,85 @@ __global__ void k_tau_round(uint32_t c0, uint32_t c1, const uint32_t* __restrict
       }
       t += cnt;
     }
+  }
+}
+
+// Response time (and, with clamp, elevation) of one long chain per 64-lane
+// workgroup: lanes stage 64 positions in LDS, one lane runs the sequential
+// recurrence of k_tau_round, lanes store.
+__global__ void __launch_bounds__(64)
+k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+           const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+           double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
+           const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp, int clamp) {
+  __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
+  uint32_t c = c0 + blockIdx.x;
+  uint32_t lane = threadIdx.x;
+  uint32_t base = chain_base[c], len = chain_len[c];
+  uint32_t pp = ppos[base];
+  double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
+  if (lane == 0 && pp != FL_NONE) {
+    tau = taup[pp];
+    if (clamp) {
+      uint32_t ro = rpos[base];
+      tau_o = taup[ro];
+      h_o = hnp[ro];
+      h_p = hnp[pp];
+    }
+  }
+  for (uint32_t t = 0; t < len; t += 64) {
+    uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
+    if (lane < cnt) {
+      uint32_t P = base + t + lane;
+      s_x[lane] = xp[P];
+      if (clamp) {
+        s_u[lane] = up[P];
+        s_t[lane] = tanp[P];
+        s_d[lane] = dp[P];
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+      uint32_t j = 0;
+      if (t == 0 && pp == FL_NONE) {
+        // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
+        tau = 0.0 + (0.0 + s_x[0]);
+        s_tau[0] = tau;
+        if (clamp) {
+          double ho = hop[base];
+          double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
+          double slope = (nh - ho) / s_d[0];
+          if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
+          s_h[0] = nh;
+          h_p = nh;
+          h_o = nh;
+          tau_o = tau;
+        }
+        j = 1;
+      }
+      if (!clamp) {
+        for (; j < cnt; ++j) {
+          tau = 0.0 + (tau + s_x[j]);
+          s_tau[j] = tau;
+        }
+      } else {
+        for (; j < cnt; ++j) {
+          tau = 0.0 + (tau + s_x[j]);
+          double nh = h_o + s_u[j] * rust_max(tau - tau_o, 0.0);
+          double slope = (nh - h_p) / s_d[j];
+          if (slope > s_t[j]) nh = h_p + s_t[j] * s_d[j];
+          s_tau[j] = tau;
+          s_h[j] = nh;
+          h_p = nh;
+        }
+      }
+    }
+    __syncthreads();
+    if (lane < cnt) {
+      taup[base + t + lane] = s_tau[lane];
+      if (clamp) hnp[base + t + lane] = s_h[lane];
+    }
+    __syncthreads();
   }
 }
 
@@ -856,11 +977,12 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t p0, uint32_t p1) {
-  FL_LAUNCH(k_da_pre, p1 - p0, s, p0, p1, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
-            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
-  FL_LAUNCH(k_da_chain, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
-            t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, w.A);
+                        uint32_t c2) {
+  FL_LAUNCH(k_da_short, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col, w.next,
+            (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
+  if (c2 > c1)
+    FL_LAUNCH_GRID(k_da_long, dim3(c2 - c1), dim3(64), s, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col,
+                   w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
 }
 
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
@@ -869,10 +991,13 @@ void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                         bool clamp) {
+                         uint32_t c2, bool clamp) {
   (void)t;
   FL_LAUNCH(k_tau_round, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp,
             w.dp, w.hop, w.hnp, clamp ? 1 : 0);
+  if (c2 > c1)
+    FL_LAUNCH_GRID(k_tau_long, dim3(c2 - c1), dim3(64), s, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp,
+                   w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp, clamp ? 1 : 0);
 }
 
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
