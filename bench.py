@@ -125,9 +125,9 @@ def main() -> int:
     L.fl_ctx_reset_profile(ctx)
     t.solve()
     names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_chain_len", "k_bucket_count",
-             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_chain", "k_tau_prep",
-             "k_tau_round", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip",
-             "k_fill_u32", "k_fill_u64", "k_copy_u32"]
+             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_chain", "k_da_long", "k_tau_prep",
+             "k_tau_round", "k_tau_long", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys",
+             "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
     prof = {}
     for nm in names:
         cnt = ctypes.c_uint64()

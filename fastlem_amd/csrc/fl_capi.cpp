@@ -300,7 +300,8 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   uint32_t npos = bn[nb];
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
-  for (uint32_t l = rounds; l-- > 0;) fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2]);
+  for (uint32_t l = rounds; l-- > 0;)
+    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1]);
   fl_launch_tau_prep(s, T->t, w, npos, clamp);
   for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], clamp);
   fl_launch_tau_scatter(s, T->t, w, npos, clamp);
@@ -455,11 +456,11 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   bool clamp = false;
   for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup})
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1})
     FLCHK(dev_alloc(P, p, n));
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
                        &w.jn2, &w.lw2, &w.clen, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
   FLCHK(dev_alloc(P, &w.cmask, n));

@@ -77,6 +77,10 @@ struct fl_dev_work {
   // decomposition
   uint64_t* cmask = nullptr;  // [n] child slots of each site
   uint32_t* hc = nullptr;     // [n] heavy child
+  double* pre = nullptr;      // [n] drainage-area partial folds, chain-major
+  double* post0 = nullptr;
+  double* post1 = nullptr;
+  uint32_t* npost = nullptr;
 
   uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;
   uint32_t *jc2 = nullptr, *pc2 = nullptr, *jn2 = nullptr, *lw2 = nullptr;
@@ -99,9 +103,10 @@ void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-// Round l of a sweep: short chains [c0, c1), long chains [c1, c2).
+// Round l of a sweep: short chains [c0, c1) at positions [p0, p1), long
+// chains [c1, c2).
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2);
+                        uint32_t c2, uint32_t p0, uint32_t p1);
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,

@@ -540,87 +540,129 @@ __device__ __noinline__ double da_post_rest(double r, uint32_t k, const uint32_t
   return r;
 }
 
-// Drainage area of the short chains (length < FL_LONG) of one round, one
-// lane per chain, tail to head; results stay in registers until the chain
-// is done, so the loads of all its sites can be in flight together.
-__global__ void k_da_short(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
+// Drainage area, short chains, part 1: the partial fold of every site of
+// the round's short chains (positions [p0, p1)), one thread per site.
+__global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ perm,
+                         const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+                         const uint32_t* __restrict__ hc, const double* __restrict__ area,
+                         const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
+                         double* __restrict__ post1, uint32_t* __restrict__ npost) {
+  uint32_t P = p0 + gid();
+  if (P >= p1) return;
+  double pr, pa, pb;
+  uint32_t np;
+  da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, pa, pb, np);
+  pre[P] = pr;
+  post0[P] = pa;
+  post1[P] = pb;
+  npost[P] = np;
+}
+
+// Drainage area, short chains, part 2: one lane per chain, tail to head:
+// A = pre (tail), A = (pre + A_heavy) + post0 + post1 + ... (others), with
+// the chain's inputs (contiguous) loaded before the dependent adds.
+__global__ void k_da_chain(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
+                           const double* __restrict__ pre, const double* __restrict__ post0,
+                           const double* __restrict__ post1, const uint32_t* __restrict__ npost,
                            const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-                           const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A) {
+                           const uint32_t* __restrict__ hc, double* __restrict__ A) {
   uint32_t c = c0 + gid();
   if (c >= c1) return;
   uint32_t base = chain_base[c], len = chain_len[c];
-  uint32_t ks[FL_LONG];
-  double out[FL_LONG];
+  double vp[FL_LONG], va[FL_LONG], vb[FL_LONG];
+  uint32_t vn[FL_LONG], vk[FL_LONG];
 #pragma unroll
-  for (uint32_t q = 0; q < FL_LONG; ++q)
-    if (q < len) ks[q] = perm[base + q];
+  for (uint32_t q = 0; q < FL_LONG; ++q) {
+    if (q < len) {
+      uint32_t P = base + q;
+      vp[q] = pre[P];
+      va[q] = post0[P];
+      vb[q] = post1[P];
+      vn[q] = npost[P];
+      vk[q] = perm[P];
+    }
+  }
   double r = 0.0;
 #pragma unroll
   for (uint32_t qq = 0; qq < FL_LONG; ++qq) {
     uint32_t q = FL_LONG - 1 - qq;
     if (q < len) {
-      double pre, pa, pb;
-      uint32_t np;
-      da_partial(ks[q], rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
-      r = (q == len - 1) ? pre : (pre + r);  // the tail has no heavy child
-      if (np >= 1) r = r + pa;
-      if (np >= 2) r = r + pb;
-      if (np >= 3) r = da_post_rest(r, ks[q], rp, col, next, cmask, hc, A);
-      out[q] = r;
+      r = (q == len - 1) ? vp[q] : (vp[q] + r);  // the tail has no heavy child
+      if (vn[q] >= 1) r = r + va[q];
+      if (vn[q] >= 2) r = r + vb[q];
+      if (vn[q] >= 3) r = da_post_rest(r, vk[q], rp, col, next, cmask, hc, A);
+      A[vk[q]] = r;
     }
   }
-#pragma unroll
-  for (uint32_t q = 0; q < FL_LONG; ++q)
-    if (q < len) A[ks[q]] = out[q];
 }
 
-// Drainage area of one long chain per 64-lane workgroup: for each block of
-// 64 positions (from the tail) every lane folds its own site's light
-// children, then one lane runs the exact sequential fold along the chain
-// from LDS, and the lanes store the results.
+namespace {
+__device__ __forceinline__ double bcast_f64(double v, int lane) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+}  // namespace
+
+// Drainage area of one long chain per 64-lane wavefront.  Per block of 64
+// positions (from the tail) every lane folds its own site's light children
+// (da_partial); the sequential fold along the chain then runs with lane
+// values broadcast by readlane, each lane keeping its own site's result.
+// The next block's partial folds are computed before the current block's
+// sequential fold, so their memory latency overlaps it.
 __global__ void __launch_bounds__(64)
 k_da_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
           const uint32_t* __restrict__ perm, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
           const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
           const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A) {
-  __shared__ double s_pre[64], s_pa[64], s_pb[64], s_out[64];
-  __shared__ uint32_t s_np[64], s_k[64];
   uint32_t c = c0 + blockIdx.x;
   uint32_t lane = threadIdx.x;
   uint32_t base = chain_base[c], len = chain_len[c];
+  // current block [cs, end)
+  uint32_t end = len, cs = end > 64 ? end - 64 : 0;
+  uint32_t k = FL_NONE, np = 0;
+  double pre = 0.0, pa = 0.0, pb = 0.0;
+  if (lane < end - cs) {
+    k = perm[base + cs + lane];
+    da_partial(k, rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
+  }
   double r = 0.0;
   bool first = true;
-  for (uint32_t end = len; end > 0;) {
-    uint32_t cs = end > 64 ? end - 64 : 0, cnt = end - cs;
-    if (lane < cnt) {
-      uint32_t k = perm[base + cs + lane];
-      double pre, pa, pb;
-      uint32_t np;
-      da_partial(k, rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
-      s_pre[lane] = pre;
-      s_pa[lane] = pa;
-      s_pb[lane] = pb;
-      s_np[lane] = np;
-      s_k[lane] = k;
+  while (end > 0) {
+    uint32_t cnt = end - cs;
+    // prefetch: partial folds of the next block down the chain
+    uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
+    uint32_t nk = FL_NONE, nnp = 0;
+    double npre = 0.0, npa = 0.0, npb = 0.0;
+    if (nend > 0 && lane < nend - ncs) {
+      nk = perm[base + ncs + lane];
+      da_partial(nk, rp, col, next, cmask, hc, area, A, npre, npa, npb, nnp);
     }
-    __syncthreads();
-    if (lane == 0) {
-      for (uint32_t j = cnt; j-- > 0;) {
-        r = first ? s_pre[j] : (s_pre[j] + r);
-        first = false;
-        uint32_t np = s_np[j];
-        if (np >= 1) r = r + s_pa[j];
-        if (np >= 2) r = r + s_pb[j];
-        if (np >= 3) r = da_post_rest(r, s_k[j], rp, col, next, cmask, hc, A);
-        s_out[j] = r;
-      }
+    double mine = 0.0;
+    for (int j = (int)cnt - 1; j >= 0; --j) {
+      double pj = bcast_f64(pre, j);
+      r = first ? pj : (pj + r);
+      first = false;
+      uint32_t npj = bcast_u32(np, j);
+      if (npj >= 1) r = r + bcast_f64(pa, j);
+      if (npj >= 2) r = r + bcast_f64(pb, j);
+      if (npj >= 3) r = da_post_rest(r, bcast_u32(k, j), rp, col, next, cmask, hc, A);
+      if ((int)lane == j) mine = r;
     }
-    __syncthreads();
-    if (lane < cnt) A[s_k[lane]] = s_out[lane];
-    __syncthreads();
-    end = cs;
+    if (lane < cnt) A[k] = mine;
+    end = nend;
+    cs = ncs;
+    k = nk;
+    np = nnp;
+    pre = npre;
+    pa = npa;
+    pb = npb;
   }
 }
 
@@ -736,20 +778,20 @@ __global__ void k_tau_round(uint32_t c0, uint32_t c1, const uint32_t* __restrict
 }
 
 // Response time (and, with clamp, elevation) of one long chain per 64-lane
-// workgroup: lanes stage 64 positions in LDS, one lane runs the sequential
-// recurrence of k_tau_round, lanes store.
+// wavefront, head to tail: lanes load 64 contiguous positions, the
+// sequential recurrence of k_tau_round runs on readlane broadcasts, each
+// lane keeps its own position's result; the next block is loaded first.
 __global__ void __launch_bounds__(64)
 k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
            const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
            double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
            const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp, int clamp) {
-  __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
   uint32_t c = c0 + blockIdx.x;
   uint32_t lane = threadIdx.x;
   uint32_t base = chain_base[c], len = chain_len[c];
   uint32_t pp = ppos[base];
   double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
-  if (lane == 0 && pp != FL_NONE) {
+  if (pp != FL_NONE) {
     tau = taup[pp];
     if (clamp) {
       uint32_t ro = rpos[base];
@@ -758,59 +800,74 @@ k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t*
       h_p = hnp[pp];
     }
   }
+  double x = 0.0, u = 0.0, tn = 0.0, d = 0.0;
+  if (lane < len) {
+    x = xp[base + lane];
+    if (clamp) {
+      u = up[base + lane];
+      tn = tanp[base + lane];
+      d = dp[base + lane];
+    }
+  }
   for (uint32_t t = 0; t < len; t += 64) {
     uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
-    if (lane < cnt) {
-      uint32_t P = base + t + lane;
-      s_x[lane] = xp[P];
+    double nx = 0.0, nu = 0.0, ntn = 0.0, nd = 0.0;
+    if (t + 64 + lane < len) {
+      uint32_t P = base + t + 64 + lane;
+      nx = xp[P];
       if (clamp) {
-        s_u[lane] = up[P];
-        s_t[lane] = tanp[P];
-        s_d[lane] = dp[P];
+        nu = up[P];
+        ntn = tanp[P];
+        nd = dp[P];
       }
     }
-    __syncthreads();
-    if (lane == 0) {
-      uint32_t j = 0;
-      if (t == 0 && pp == FL_NONE) {
-        // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
-        tau = 0.0 + (0.0 + s_x[0]);
-        s_tau[0] = tau;
-        if (clamp) {
-          double ho = hop[base];
-          double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
-          double slope = (nh - ho) / s_d[0];
-          if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
-          s_h[0] = nh;
-          h_p = nh;
-          h_o = nh;
-          tau_o = tau;
-        }
-        j = 1;
+    double my_tau = 0.0, my_h = 0.0;
+    int j = 0;
+    if (t == 0 && pp == FL_NONE) {
+      // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
+      double x0 = bcast_f64(x, 0);
+      tau = 0.0 + (0.0 + x0);
+      if (clamp) {
+        double ho = hop[base];
+        double u0 = bcast_f64(u, 0), d0 = bcast_f64(d, 0), t0 = bcast_f64(tn, 0);
+        double nh = ho + u0 * rust_max(tau - tau, 0.0);
+        double slope = (nh - ho) / d0;
+        if (slope > t0) nh = ho + t0 * d0;
+        h_p = nh;
+        h_o = nh;
+        tau_o = tau;
+        if (lane == 0) my_h = nh;
       }
-      if (!clamp) {
-        for (; j < cnt; ++j) {
-          tau = 0.0 + (tau + s_x[j]);
-          s_tau[j] = tau;
-        }
-      } else {
-        for (; j < cnt; ++j) {
-          tau = 0.0 + (tau + s_x[j]);
-          double nh = h_o + s_u[j] * rust_max(tau - tau_o, 0.0);
-          double slope = (nh - h_p) / s_d[j];
-          if (slope > s_t[j]) nh = h_p + s_t[j] * s_d[j];
-          s_tau[j] = tau;
-          s_h[j] = nh;
-          h_p = nh;
+      if (lane == 0) my_tau = tau;
+      j = 1;
+    }
+    if (!clamp) {
+      for (; j < (int)cnt; ++j) {
+        tau = 0.0 + (tau + bcast_f64(x, j));
+        if ((int)lane == j) my_tau = tau;
+      }
+    } else {
+      for (; j < (int)cnt; ++j) {
+        tau = 0.0 + (tau + bcast_f64(x, j));
+        double uj = bcast_f64(u, j), dj = bcast_f64(d, j), tj = bcast_f64(tn, j);
+        double nh = h_o + uj * rust_max(tau - tau_o, 0.0);
+        double slope = (nh - h_p) / dj;
+        if (slope > tj) nh = h_p + tj * dj;
+        h_p = nh;
+        if ((int)lane == j) {
+          my_tau = tau;
+          my_h = nh;
         }
       }
     }
-    __syncthreads();
     if (lane < cnt) {
-      taup[base + t + lane] = s_tau[lane];
-      if (clamp) hnp[base + t + lane] = s_h[lane];
+      taup[base + t + lane] = my_tau;
+      if (clamp) hnp[base + t + lane] = my_h;
     }
-    __syncthreads();
+    x = nx;
+    u = nu;
+    tn = ntn;
+    d = nd;
   }
 }
 
@@ -977,12 +1034,14 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2) {
-  FL_LAUNCH(k_da_short, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col, w.next,
-            (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
+                        uint32_t c2, uint32_t p0, uint32_t p1) {
+  FL_LAUNCH(k_da_pre, p1 - p0, s, p0, p1, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
+            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
   if (c2 > c1)
     FL_LAUNCH_GRID(k_da_long, dim3(c2 - c1), dim3(64), s, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col,
                    w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
+  FL_LAUNCH(k_da_chain, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
+            t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, w.A);
 }
 
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
