@@ -147,3 +147,49 @@ def config_fbm(n: int = 1_000_000, width: float = 1000.0, seed: int = 7, ocean: 
     is_out = sea < ocean
     cnt = m.num()
     return Workload("fbm-%d" % n, m, k, np.ones(cnt), is_out, np.zeros(cnt))
+
+
+def grid_triangles(nx: int, ny: int, xs: np.ndarray, ys: np.ndarray) -> np.ndarray:
+    """Counter-clockwise triangles of a jittered nx-by-ny grid (site id =
+    iy * nx + ix): each cell quad is split along the diagonal that satisfies
+    the empty-circumcircle test, i.e. the Delaunay choice for the quad."""
+    ix, iy = np.meshgrid(np.arange(nx - 1, dtype=np.int64), np.arange(ny - 1, dtype=np.int64))
+    a = (iy * nx + ix).ravel()      # (x, y)
+    b = a + 1                        # (x+1, y)
+    c = a + nx + 1                   # (x+1, y+1)
+    d = a + nx                       # (x, y+1)
+    # is d inside the circumcircle of (a, b, c)?  (a, b, c counter-clockwise)
+    ax, ay = xs[a] - xs[d], ys[a] - ys[d]
+    bx, by = xs[b] - xs[d], ys[b] - ys[d]
+    cx, cy = xs[c] - xs[d], ys[c] - ys[d]
+    det = (ax * ax + ay * ay) * (bx * cy - cx * by) - (bx * bx + by * by) * (ax * cy - cx * ay) \
+        + (cx * cx + cy * cy) * (ax * by - bx * ay)
+    flip = det > 0  # d inside: use diagonal b-d instead of a-c
+    t1 = np.where(flip[:, None], np.stack([a, b, d], 1), np.stack([a, b, c], 1))
+    t2 = np.where(flip[:, None], np.stack([b, c, d], 1), np.stack([a, c, d], 1))
+    return np.ascontiguousarray(np.concatenate([t1, t2]))
+
+
+def config_strip(nx: int = 16384, ny: int = 1024, jitter: float = 0.4, seed: int = 11) -> Workload:
+    """Deep-tree stress: one site per unit cell of an nx-by-ny strip at
+    (cx + 0.5 + U(-j, j), cy + 0.5 + U(-j, j)), outlets = the x = 0 column,
+    erodibility 1.0 * (1 + U(-0.05, 0.05)), no Lloyd step.  Rivers run the
+    whole strip (depth about nx)."""
+    from .surface import TerrainModel2D, build_adjacency, voronoi_cells
+
+    rng = np.random.default_rng(seed)
+    n = nx * ny
+    gx, gy = np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64))
+    xs = (gx.ravel() + 0.5 + rng.uniform(-jitter, jitter, n))
+    ys = (gy.ravel() + 0.5 + rng.uniform(-jitter, jitter, n))
+    tris = grid_triangles(nx, ny, xs, ys)
+    area, _, _ = voronoi_cells(xs, ys, tris, Site2D(0.0, 0.0), Site2D(float(nx), float(ny)))
+    rp, col, ln = build_adjacency(xs, ys, tris)
+    del tris
+    hull = np.concatenate([np.arange(nx), np.arange(1, ny) * nx + nx - 1, (ny - 1) * nx + np.arange(nx - 2, -1, -1),
+                           np.arange(ny - 2, 0, -1) * nx]).astype(np.uint32)
+    m = TerrainModel2D(xs, ys, area, rp, col, ln, hull)
+    is_out = np.zeros(n, bool)
+    is_out[np.arange(ny) * nx] = True
+    k = 1.0 * (1.0 + rng.uniform(-0.05, 0.05, n))
+    return Workload("strip-%dx%d" % (nx, ny), m, k, np.ones(n), is_out, np.zeros(n))

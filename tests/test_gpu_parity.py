@@ -216,3 +216,43 @@ def test_max_iteration_is_honoured(plain10k):
         t = _dev(w, max_iteration=cap)
         assert t.solve() == iters == cap
         assert _bits_equal(ref, t.elevations())
+
+
+@pytest.fixture(scope="module")
+def strip():
+    from fastlem_amd.workloads import config_strip
+
+    return config_strip(512, 64)
+
+
+def test_deep_strip_end_to_end(strip):
+    """Outlets on one short side only: rivers ~900 sites deep, chains that
+    span many 64-site blocks of the wavefront kernels."""
+    from oracle import flo
+
+    w = strip
+    ref, iters, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+    t = _dev(w)
+    assert t.solve() == iters
+    assert _bits_equal(ref, t.elevations())
+
+
+def test_deep_strip_teacher_forced_with_max_slope(strip):
+    from oracle import flo
+    from fastlem_amd.workloads import Workload
+
+    w0 = strip
+    ms = np.where(np.arange(w0.n) % 3 == 0, 0.05, np.nan)  # a third of the sites clamp
+    w = Workload(w0.name, w0.model, w0.erodibility, w0.uplift, w0.is_outlet, w0.base, ms)
+    g = graph_of(w)
+    t = _dev(w)
+    elev = init_of(w)
+    for it in range(8):
+        h_ref, n_ref, _, da_ref, rt_ref, ch_ref, _ = flo.iteration(g, w.model.areas(), w.erodibility, w.uplift,
+                                                                   w.outlets(), elev, ms)
+        h_got, n_got, da_got, rt_got, ch_got = t.stage_iteration(elev)
+        assert np.array_equal(n_ref, n_got), it
+        assert _bits_equal(da_ref, da_got), it
+        assert _bits_equal(rt_ref, rt_got), it
+        assert _bits_equal(h_ref, h_got), it
+        elev = h_ref
