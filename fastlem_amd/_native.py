@@ -76,6 +76,10 @@ def _declare(lib):
         "fl_terrain_device_elevations": (i32, [vp, ctypes.POINTER(vp)]),
         "fl_terrain_device_bytes": (i32, [vp, ctypes.POINTER(u64)]),
         "fl_terrain_invalidate_cache": (i32, [vp]),
+        "fl_batch_create": (i32, [vp, ctypes.POINTER(TerrainDesc), u32, ctypes.POINTER(vp)]),
+        "fl_batch_solve": (i32, [vp, vp, c_u32p]),
+        "fl_batch_download": (i32, [vp, vp, u32, c_f64p]),
+        "fl_batch_count": (i32, [vp, c_u32p]),
         "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64)]),

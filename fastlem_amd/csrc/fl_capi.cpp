@@ -82,6 +82,15 @@ struct fl_terrain {
   // locality relabelling: device site u holds caller site old_of_new[u]
   std::vector<uint32_t> new_of_old, old_of_new;
   std::vector<void*> allocations;
+  // terrains packed into this object (one for fl_terrain_create); terrain g
+  // owns sites [site_off[g], site_off[g+1]) and outlets [outlet_off[g], ...)
+  uint32_t nterr = 1;
+  std::vector<uint32_t> site_off, outlet_off, t_max_it, t_iters;
+  std::vector<uint8_t> t_active;
+  uint32_t* d_tid = nullptr;      // [n] terrain of each site (batches)
+  uint8_t* d_active = nullptr;    // [nterr] terrains still iterating
+  uint32_t* d_tchanged = nullptr; // [nterr] per-terrain changed flags
+  uint32_t* h_tchanged = nullptr; // pinned copy
 };
 
 namespace {
@@ -160,31 +169,43 @@ inline uint32_t host_counter(const fl_ctx* c, int k) { return c->host_ctrl[FL_NU
 int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   if (T->have_order) return FL_OK;
   auto t0 = std::chrono::steady_clock::now();
-  uint32_t n = T->t.n;
+  uint32_t n = T->t.n, G = T->nterr;
   size_t k = T->h_outlets.size();
   std::vector<uint32_t> rank(n), order(n, 0xffffffffu);
   bool fast = false;
   if (k > 0) {
-    // outlet phase (exact heap tie handling) + Prim phase from the spanning forest
-    std::vector<uint32_t> first(k), a, b;
-    int rc = fl_outlet_pop_order(n, T->h_row_ptr.data(), T->h_col.data(), T->h_outlets.data(), k, first.data());
-    if (rc != FL_OK) return set_err(rc, "outlet phase failed");
+    // per terrain: outlet phase (exact heap tie handling); then the Prim
+    // phase of all terrains from one spanning forest with each terrain's
+    // outlets merged into its own component
+    std::vector<uint32_t> first(k), a, b, init(n);
+    for (uint32_t g = 0; g < G; g++) {
+      uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
+      int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
+                                         T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
+      if (rc != FL_OK) return set_err(rc, "outlet phase failed");
+    }
+    for (uint32_t v = 0; v < n; v++) init[v] = v;
+    for (uint32_t g = 0; g < G; g++)
+      for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++) init[first[q]] = first[T->outlet_off[g]];
     std::string err;
-    rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, T->t.is_outlet, first[0], a, b,
-                       &fast, &err);
+    int rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, init, a, b, &fast, &err);
     if (rc != FL_OK) return set_err(rc, "spanning forest: %s", err.c_str());
     if (fast) {
-      rc = fl_ranks_from_msf(n, first.data(), k, a.data(), b.data(), a.size(), rank.data());
+      rc = fl_ranks_from_msf_groups(n, G, T->outlet_off.data(), first.data(), T->site_off.data(), a.data(), b.data(),
+                                    a.size(), rank.data());
       if (rc != FL_OK) return set_err(rc, "ranks from spanning forest failed");
     }
   }
   if (!fast) {
-    // equal or non-positive lengths: simulate the heap exactly
-    uint64_t cnt = 0;
-    int rc = fl_lake_visit_order(n, T->h_row_ptr.data(), T->h_col.data(), T->h_len.data(), T->h_outlets.data(), k,
-                                 rank.data(), &cnt);
-    if (rc == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
-    if (rc != FL_OK) return set_err(rc, "visit order failed");
+    // equal or non-positive lengths: simulate each terrain's heap exactly
+    for (uint32_t g = 0; g < G; g++) {
+      uint32_t s0 = T->site_off[g], o0 = T->outlet_off[g];
+      int rc = fl_lake_visit_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
+                                         T->h_len.data(), T->h_outlets.data() + o0, T->outlet_off[g + 1] - o0, s0,
+                                         rank.data());
+      if (rc == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
+      if (rc != FL_OK) return set_err(rc, "visit order failed");
+    }
   }
   for (uint32_t i = 0; i < n; i++)
     if (rank[i] != 0xffffffffu) order[rank[i]] = i;
@@ -304,7 +325,10 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
     fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1]);
   fl_launch_tau_prep(s, T->t, w, npos, clamp);
   for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], clamp);
-  fl_launch_tau_scatter(s, T->t, w, npos, clamp);
+  if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
+  fl_launch_tau_scatter(s, T->t, w, npos, clamp, T->d_tid, T->d_active, T->d_tchanged);
+  if (T->d_tchanged)
+    HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   FLCHK(read_ctrl(c, T));
   *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
   std::swap(w.h, w.h2);
@@ -355,6 +379,7 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   if (!T) return FL_OK;
   if (T->ctx) hipSetDevice(T->ctx->device);
   for (void* p : T->allocations) hipFree(p);
+  if (T->h_tchanged) hipHostFree(T->h_tchanged);
   delete T;
   return FL_OK;
 }
@@ -395,49 +420,86 @@ void gather_perm(const V* src, const std::vector<uint32_t>& old_of_new, std::vec
 
 }  // namespace
 
-extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out) {
-  if (!c || !out) return set_err(FL_ERR_INVALID, "null context or output");
-  FLCHK(validate(d));
+namespace {
+
+// Upload `count` terrains as one union graph: terrain g's sites occupy
+// [site_off[g], site_off[g+1]), each relabelled for locality inside its range;
+// no edge joins two terrains.
+int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl_terrain** out) {
+  if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
+  if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
+  uint64_t N = 0, NNZ = 0, K = 0;
+  for (uint32_t g = 0; g < count; g++) {
+    int32_t rc = validate(&descs[g]);
+    if (rc != FL_OK) {
+      if (count > 1) g_err = "terrain " + std::to_string(g) + ": " + g_err;
+      return rc;
+    }
+    N += descs[g].n;
+    NNZ += descs[g].nnz;
+    K += descs[g].n_outlets;
+  }
+  if (N >= 0xffffffffull || NNZ >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "batch exceeds 2^32-2 sites/entries");
   HIPCHK(hipSetDevice(c->device));
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> T(new fl_terrain(), fl_terrain_destroy);
   T->ctx = c;
-  T->max_iteration = d->max_iteration;
-  uint32_t n = (uint32_t)d->n, nnz = (uint32_t)d->nnz;
+  T->nterr = count;
+  T->max_iteration = descs[0].max_iteration;
+  uint32_t n = (uint32_t)N, nnz = (uint32_t)NNZ;
   T->t.n = n;
   T->t.nnz = nnz;
+  T->site_off.assign(count + 1, 0);
+  T->outlet_off.assign(count + 1, 0);
+  T->t_max_it.resize(count);
+  T->t_iters.assign(count, 0);
+  T->t_active.assign(count, 0);
+  for (uint32_t g = 0; g < count; g++) {
+    T->site_off[g + 1] = T->site_off[g] + (uint32_t)descs[g].n;
+    T->outlet_off[g + 1] = T->outlet_off[g] + (uint32_t)descs[g].n_outlets;
+    T->t_max_it[g] = descs[g].max_iteration;
+  }
   if (n == 0) {
     *out = T.release();
     return FL_OK;
   }
-  // relabel for locality
-  locality_order(n, d->row_ptr, d->col, T->old_of_new, T->new_of_old);
-  const std::vector<uint32_t>& o2 = T->old_of_new;
-  const std::vector<uint32_t>& n2 = T->new_of_old;
-  T->h_row_ptr.resize((size_t)n + 1);
+  T->old_of_new.resize(n);
+  T->new_of_old.resize(n);
+  T->h_row_ptr.assign((size_t)n + 1, 0);
   T->h_col.resize(nnz);
   T->h_len.resize(nnz);
-  T->h_row_ptr[0] = 0;
-  for (uint32_t u = 0; u < n; u++) {
-    uint32_t i = o2[u], b = d->row_ptr[i], e = d->row_ptr[i + 1], q = T->h_row_ptr[u];
-    for (uint32_t k = b; k < e; k++, q++) {
-      T->h_col[q] = n2[d->col[k]];
-      T->h_len[q] = d->length[k];
-    }
-    T->h_row_ptr[u + 1] = q;
-  }
-  T->h_outlets.resize(d->n_outlets);
-  for (uint64_t k = 0; k < d->n_outlets; k++) T->h_outlets[k] = n2[d->outlets[k]];
-  std::vector<double> area, erod, uplift, init, tan_ms(n);
-  gather_perm(d->area, o2, area);
-  gather_perm(d->erodibility, o2, erod);
-  gather_perm(d->uplift_rate, o2, uplift);
-  gather_perm(d->init_elevation, o2, init);
-  for (uint32_t u = 0; u < n; u++) {
-    uint32_t i = o2[u];
-    tan_ms[u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
-  }
+  T->h_outlets.resize(K);
+  std::vector<double> area(n), erod(n), uplift(n), init(n), tan_ms(n);
   std::vector<uint8_t> is_out(n, 0);
-  for (uint32_t o : T->h_outlets) is_out[o] = 1;
+  std::vector<uint32_t> tid(count > 1 ? n : 0);
+  std::vector<uint32_t> o2, n2;
+  uint32_t q = 0;
+  for (uint32_t g = 0; g < count; g++) {
+    const fl_terrain_desc* d = &descs[g];
+    uint32_t s0 = T->site_off[g], ng = (uint32_t)d->n;
+    if (ng == 0) continue;
+    locality_order(ng, d->row_ptr, d->col, o2, n2);
+    for (uint32_t u = 0; u < ng; u++) {
+      uint32_t i = o2[u];
+      T->old_of_new[s0 + u] = s0 + i;
+      T->new_of_old[s0 + i] = s0 + u;
+      for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++, q++) {
+        T->h_col[q] = s0 + n2[d->col[k]];
+        T->h_len[q] = d->length[k];
+      }
+      T->h_row_ptr[s0 + u + 1] = q;
+      area[s0 + u] = d->area[i];
+      erod[s0 + u] = d->erodibility[i];
+      uplift[s0 + u] = d->uplift_rate[i];
+      init[s0 + u] = d->init_elevation[i];
+      tan_ms[s0 + u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
+      if (count > 1) tid[s0 + u] = g;
+    }
+    for (uint64_t k = 0; k < d->n_outlets; k++) {
+      uint32_t o = s0 + n2[d->outlets[k]];
+      T->h_outlets[T->outlet_off[g] + k] = o;
+      is_out[o] = 1;
+    }
+  }
 
   fl_dev_terrain& t = T->t;
   fl_dev_work& w = T->w;
@@ -470,8 +532,14 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   FLCHK(dev_alloc(P, &w.ctrl, FL_CTRL_WORDS));
   w.flags = (int*)w.ctrl;
   w.counters = w.ctrl + FL_NUM_FLAGS;
-
   hipStream_t s = c->stream;
+  if (count > 1) {
+    FLCHK(dev_alloc(P, &T->d_tid, n));
+    FLCHK(dev_alloc(P, &T->d_active, count));
+    FLCHK(dev_alloc(P, &T->d_tchanged, count));
+    HIPCHK(hipHostMalloc((void**)&T->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipMemcpyAsync(T->d_tid, tid.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
+  }
   HIPCHK(hipMemcpyAsync(t.row_ptr, T->h_row_ptr.data(), ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
   if (nnz) {
     HIPCHK(hipMemcpyAsync(t.col, T->h_col.data(), (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
@@ -495,31 +563,87 @@ extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_ter
   return FL_OK;
 }
 
-extern "C" int32_t fl_terrain_solve(fl_ctx* c, fl_terrain* T, uint32_t max_iteration, uint32_t* iterations_out) {
-  if (!c || !T) return set_err(FL_ERR_INVALID, "null context or terrain");
-  ProfScope prof_scope(c->prof);
-  HIPCHK(hipSetDevice(c->device));
-  uint32_t it = 0;
-  if (T->t.n == 0) {
-    // the loop body runs once and finds nothing changed (generator.rs:140-209)
-    if (iterations_out) *iterations_out = max_iteration > 0 ? 1 : 0;
-    return FL_OK;
-  }
+// Iterate every terrain of T from its initial elevations: terrain g runs
+// until an iteration changes none of its elevations or it reached max_it[g];
+// it is then frozen (its elevations no longer change) while the others go
+// on, so each terrain's result and count equal its solo run.
+int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
+  uint32_t G = T->nterr;
   auto t0 = std::chrono::steady_clock::now();
   T->stat_syncs = 0;
   T->stat_order_ms = 0.0;
   T->stat_lake_iterations = 0;
+  uint32_t active = 0;
+  for (uint32_t g = 0; g < G; g++) {
+    T->t_iters[g] = 0;
+    bool empty = T->site_off[g + 1] == T->site_off[g];
+    // an empty terrain runs the loop body once and finds nothing changed
+    if (empty && max_it[g] > 0) T->t_iters[g] = 1;
+    T->t_active[g] = (!empty && max_it[g] > 0) ? 1 : 0;
+    active += T->t_active[g];
+  }
+  if (T->t.n == 0 || active == 0) return FL_OK;
   FLCHK(reset_state(c, T));
-  for (uint32_t step = 0; step < max_iteration; step++) {
+  if (T->d_active)
+    HIPCHK(hipMemcpyAsync(T->d_active, T->t_active.data(), G, hipMemcpyHostToDevice, c->stream));
+  uint32_t it = 0;
+  while (active) {
     bool changed = false, lake = false;
     FLCHK(one_iteration(c, T, &changed, &lake));
     it++;
     T->stat_lake_iterations += lake ? 1 : 0;
-    if (!changed) break;
+    bool mask_changed = false;
+    for (uint32_t g = 0; g < G; g++) {
+      if (!T->t_active[g]) continue;
+      T->t_iters[g]++;
+      bool ch = T->d_tchanged ? T->h_tchanged[g] != 0 : changed;
+      if (!ch || T->t_iters[g] >= max_it[g]) {
+        T->t_active[g] = 0;
+        active--;
+        mask_changed = true;
+      }
+    }
+    if (mask_changed && active && T->d_active)
+      HIPCHK(hipMemcpyAsync(T->d_active, T->t_active.data(), G, hipMemcpyHostToDevice, c->stream));
   }
   T->stat_iterations = it;
   T->stat_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (iterations_out) *iterations_out = it;
+  return FL_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out) {
+  if (!d) return set_err(FL_ERR_INVALID, "null terrain description");
+  return create_union(c, d, 1, out);
+}
+
+extern "C" int32_t fl_batch_create(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl_terrain** out) {
+  return create_union(c, descs, count, out);
+}
+
+extern "C" int32_t fl_batch_solve(fl_ctx* c, fl_terrain* T, uint32_t* iterations_out) {
+  if (!c || !T) return set_err(FL_ERR_INVALID, "null context or batch");
+  ProfScope prof_scope(c->prof);
+  HIPCHK(hipSetDevice(c->device));
+  FLCHK(solve_union(c, T, T->t_max_it.data()));
+  if (iterations_out) std::copy(T->t_iters.begin(), T->t_iters.end(), iterations_out);
+  return FL_OK;
+}
+
+extern "C" int32_t fl_batch_count(const fl_terrain* T, uint32_t* count) {
+  if (!T || !count) return set_err(FL_ERR_INVALID, "null argument");
+  *count = T->nterr;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_terrain_solve(fl_ctx* c, fl_terrain* T, uint32_t max_iteration, uint32_t* iterations_out) {
+  if (!c || !T) return set_err(FL_ERR_INVALID, "null context or terrain");
+  ProfScope prof_scope(c->prof);
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<uint32_t> caps(T->nterr, max_iteration);
+  FLCHK(solve_union(c, T, caps.data()));
+  if (iterations_out) *iterations_out = T->t_iters[0];
   return FL_OK;
 }
 
@@ -559,6 +683,20 @@ extern "C" int32_t fl_terrain_download(fl_ctx* c, fl_terrain* T, double* out) {
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   return download_f64_old(c, T, out, T->w.h);
+}
+
+extern "C" int32_t fl_batch_download(fl_ctx* c, fl_terrain* T, uint32_t index, double* out) {
+  if (!c || !T || index >= T->nterr) return set_err(FL_ERR_INVALID, "bad batch index or null argument");
+  uint32_t s0 = T->site_off[index], ng = T->site_off[index + 1] - s0;
+  if (ng == 0) return FL_OK;
+  if (!out) return set_err(FL_ERR_INVALID, "null output");
+  HIPCHK(hipSetDevice(c->device));
+  // a terrain's relabelled sites stay inside its own range
+  std::vector<double> tmp(ng);
+  HIPCHK(hipMemcpyAsync(tmp.data(), T->w.h + s0, (uint64_t)ng * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < ng; i++) out[i] = tmp[T->new_of_old[s0 + i] - s0];
+  return FL_OK;
 }
 
 extern "C" int32_t fl_terrain_device_elevations(fl_terrain* T, double** dev_ptr) {

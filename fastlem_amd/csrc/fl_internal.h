@@ -12,6 +12,18 @@ struct fl_rng;
 
 // fl_order.hip: minimum spanning forest with the outlets merged, sorted edges.
 int fl_device_msf(hipStream_t s, uint32_t n, uint32_t nnz, const uint32_t* rp, const uint32_t* col, const double* len,
-                  const uint8_t* is_outlet, uint32_t r0, std::vector<uint32_t>& a_sorted,
+                  const std::vector<uint32_t>& init_comp, std::vector<uint32_t>& a_sorted,
                   std::vector<uint32_t>& b_sorted, bool* applies, std::string* err);
+
+// fl_setup.cpp: per-terrain forms on a union graph whose terrain occupies
+// sites [s0, s0 + nl) (its rows reference only those sites).
+int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
+                              const uint32_t* outlets, size_t k, uint32_t* order_out);
+int fl_lake_visit_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
+                              const double* len, const uint32_t* outlets, size_t k, uint32_t rank_base,
+                              uint32_t* rank_out);
+extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,
+                                        const uint32_t* outlet_order, const uint32_t* rank_off,
+                                        const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                                        uint32_t* rank_out);
 #endif

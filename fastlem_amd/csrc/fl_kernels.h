@@ -111,5 +111,6 @@ void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
                          uint32_t c2, bool clamp);
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
+                           const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);

@@ -874,29 +874,37 @@ k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t*
 // Back to site order: response time, new elevation (computed here when no
 // site has a max slope: h_i' = h_o' + U_i * max(rt_i - rt_o, 0) with
 // h_o' = h_o + U_o * max(0, 0), generator.rs:178-180) and the changed flag
-// (generator.rs:201).
+// (generator.rs:201).  In a batch (tid != null) a terrain that stopped
+// iterating keeps its elevations and each terrain has its own flag.
 __global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ jn,
                               const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
-                              int* __restrict__ flags, int clamp) {
+                              int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
+                              const uint8_t* __restrict__ active, uint32_t* __restrict__ tchanged) {
   uint32_t P = gid();
   bool changed = false;
   if (P < npos) {
     uint32_t k = perm[P];
-    double tk = taup[P];
-    double nh;
-    if (clamp) {
-      nh = hnp[P];
+    uint32_t g = tid ? tid[k] : 0;
+    if (tid && !active[g]) {
+      h2[k] = h[k];
     } else {
-      uint32_t o = jn[k];
-      double to = taup[head_base[o]];
-      double ho = h[o] + U[o] * rust_max(to - to, 0.0);
-      nh = (k == o) ? ho : ho + U[k] * rust_max(tk - to, 0.0);
+      double tk = taup[P];
+      double nh;
+      if (clamp) {
+        nh = hnp[P];
+      } else {
+        uint32_t o = jn[k];
+        double to = taup[head_base[o]];
+        double ho = h[o] + U[o] * rust_max(to - to, 0.0);
+        nh = (k == o) ? ho : ho + U[k] * rust_max(tk - to, 0.0);
+      }
+      tau[k] = tk;
+      h2[k] = nh;
+      changed = nh != h[k];
+      if (changed && tchanged) tchanged[g] = 1;
     }
-    tau[k] = tk;
-    h2[k] = nh;
-    changed = nh != h[k];
   }
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
@@ -1059,9 +1067,10 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
                    w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp, clamp ? 1 : 0);
 }
 
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
+                           const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
   FL_LAUNCH(k_tau_scatter, npos, s, npos, w.perm, w.jn, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau,
-            w.flags, clamp ? 1 : 0);
+            w.flags, clamp ? 1 : 0, tid, active, tchanged);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {

@@ -66,11 +66,11 @@ __global__ void k_len_check(uint32_t n, const uint32_t* __restrict__ rp, const u
   }
 }
 
-__global__ void k_bv_init(uint32_t n, const uint8_t* __restrict__ is_outlet, uint32_t r0, uint32_t* __restrict__ comp,
+__global__ void k_bv_init(uint32_t n, const uint32_t* __restrict__ init_comp, uint32_t* __restrict__ comp,
                           unsigned long long* __restrict__ cbest) {
   uint32_t v = gid();
   if (v >= n) return;
-  comp[v] = is_outlet[v] ? r0 : v;
+  comp[v] = init_comp[v];
   cbest[v] = ~0ull;
 }
 
@@ -188,14 +188,16 @@ struct DevBuf {
 
 }  // namespace
 
-// Minimum spanning forest of the graph with the outlets merged into one
-// component, edges sorted by increasing length, into (a_sorted, b_sorted).
+// Minimum spanning forest of the graph whose sites start in the components
+// init_comp (host array: each terrain's outlets share one label, every other
+// site is its own), edges sorted by increasing length, into
+// (a_sorted, b_sorted).
 // *applies is false (and nothing else is produced) when some edge length is
 // not positive or two lengths are equal: then the traversal order depends on
 // the heap's tie handling beyond the outlet phase, and the caller simulates
 // it exactly instead.
 int fl_device_msf(hipStream_t s, uint32_t n, uint32_t nnz, const uint32_t* rp, const uint32_t* col, const double* len,
-                  const uint8_t* is_outlet, uint32_t r0, std::vector<uint32_t>& a_sorted,
+                  const std::vector<uint32_t>& init_comp, std::vector<uint32_t>& a_sorted,
                   std::vector<uint32_t>& b_sorted, bool* applies, std::string* err) {
   *applies = false;
   DevBuf buf;
@@ -218,7 +220,8 @@ int fl_device_msf(hipStream_t s, uint32_t n, uint32_t nnz, const uint32_t* rp, c
   for (uint32_t** p : {&comp, &parent, &parent2, &vbest, &cedge, &csrc, &ma, &mb, &idx, &idx2, &a2, &b2})
     HIPTRY(buf.alloc((void**)p, (size_t)n * 4));
   for (unsigned long long** p : {&cbest, &mw, &mw2}) HIPTRY(buf.alloc((void**)p, (size_t)n * 8));
-  hipLaunchKernelGGL(k_bv_init, grid_for(n), dim3(256), 0, s, n, is_outlet, r0, comp, cbest);
+  HIPTRY(hipMemcpyAsync(parent, init_comp.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_bv_init, grid_for(n), dim3(256), 0, s, n, parent, comp, cbest);
   uint32_t m = 0;
   for (uint32_t round = 0;; ++round) {
     if (round > 64) {

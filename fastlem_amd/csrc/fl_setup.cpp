@@ -347,6 +347,36 @@ struct RustHeap {
 
 }  // namespace
 
+// Exact simulation of the traversal for the terrain occupying sites
+// [s0, s0 + nl); its ranks start at rank_base.
+int fl_lake_visit_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
+                              const double* len, const uint32_t* outlets, size_t k, uint32_t rank_base,
+                              uint32_t* rank_out) {
+  RustHeap h;
+  h.d.reserve(nl);
+  std::vector<uint8_t> visited(nl, 0);
+  for (uint32_t i = 0; i < nl; i++) rank_out[s0 + i] = UINT32_MAX;
+  for (size_t q = 0; q < k; q++) {
+    if (outlets[q] < s0 || outlets[q] - s0 >= nl) return FL_ERR_INVALID;
+    h.push(Ridge{outlets[q], 0.0});
+  }
+  uint32_t cnt = 0;
+  Ridge el;
+  while (h.pop(&el)) {
+    uint32_t i = el.index;
+    if (visited[i - s0]) continue;
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; e++) {
+      uint32_t j = col[e];
+      if (visited[j - s0]) continue;
+      if (std::isnan(len[e])) return FL_ERR_NAN_LENGTH;
+      h.push(Ridge{j, len[e]});
+    }
+    visited[i - s0] = 1;
+    rank_out[i] = rank_base + cnt++;
+  }
+  return FL_OK;
+}
+
 // Visit order of the lake-removal traversal: sites in the order in which the
 // heap first pops them.  rank_out[i] = position of i in that order, or
 // UINT32_MAX when i is never reached (no path to an outlet).  The flips the
@@ -355,29 +385,13 @@ struct RustHeap {
 extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const double* len,
                                    const uint32_t* outlets, size_t n_outlets, uint32_t* rank_out,
                                    uint64_t* visited_out) {
-  RustHeap h;
-  h.d.reserve(n);
-  std::vector<uint8_t> visited(n, 0);
-  for (size_t i = 0; i < n; i++) rank_out[i] = UINT32_MAX;
-  for (size_t k = 0; k < n_outlets; k++) {
-    if (outlets[k] >= n) return FL_ERR_INVALID;
-    h.push(Ridge{outlets[k], 0.0});
+  int rc = fl_lake_visit_order_range(0, (uint32_t)n, row_ptr, col, len, outlets, n_outlets, 0, rank_out);
+  if (rc != FL_OK) return rc;
+  if (visited_out) {
+    uint64_t cnt = 0;
+    for (size_t i = 0; i < n; i++) cnt += rank_out[i] != UINT32_MAX;
+    *visited_out = cnt;
   }
-  uint32_t cnt = 0;
-  Ridge el;
-  while (h.pop(&el)) {
-    uint32_t i = el.index;
-    if (visited[i]) continue;
-    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; e++) {
-      uint32_t j = col[e];
-      if (visited[j]) continue;
-      if (std::isnan(len[e])) return FL_ERR_NAN_LENGTH;
-      h.push(Ridge{j, len[e]});
-    }
-    visited[i] = 1;
-    rank_out[i] = cnt++;
-  }
-  if (visited_out) *visited_out = cnt;
   return FL_OK;
 }
 
@@ -399,13 +413,13 @@ extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint
 // the vacated lowest Z position of that path.  None of this depends on the
 // values of P elements, so only their number (the unvisited neighbours of
 // each popped outlet) is needed.
-extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* outlets,
-                                   size_t k, uint32_t* order_out) {
+int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
+                              const uint32_t* outlets, size_t k, uint32_t* order_out) {
   const uint32_t NONE = 0xffffffffu;
   std::vector<uint32_t> zid(outlets, outlets + k);
-  std::vector<uint8_t> visited(n, 0);
+  std::vector<uint8_t> visited(nl, 0);
   for (size_t q = 0; q < k; q++)
-    if (outlets[q] >= n) return FL_ERR_INVALID;
+    if (outlets[q] < s0 || outlets[q] - s0 >= nl) return FL_ERR_INVALID;
   uint64_t len = k;
   for (size_t step = 0; step < k; step++) {
     uint64_t L = len;
@@ -436,41 +450,51 @@ extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint
     }
     if (result == NONE) return FL_ERR_INVALID;  // cannot happen with distinct outlets
     order_out[step] = result;
-    visited[result] = 1;
+    visited[result - s0] = 1;
     uint64_t pushes = 0;
-    for (uint32_t e = row_ptr[result]; e < row_ptr[result + 1]; e++) pushes += visited[col[e]] ? 0 : 1;
+    for (uint32_t e = row_ptr[result]; e < row_ptr[result + 1]; e++) pushes += visited[col[e] - s0] ? 0 : 1;
     len += pushes;
   }
   return FL_OK;
 }
 
-// Prim phase from a minimum spanning forest of the graph with all outlets
-// contracted into one vertex S.  msf_a/msf_b: the forest's edges sorted by
-// increasing length.  Kruskal's merges build the reconstruction tree; Prim
-// from S completes the side of each merge edge that contains the entry
-// point before crossing to the other side, which it enters at the edge's
-// endpoint there.  Ranks of the outlets are 0..k-1 in outlet_order; other
-// reached sites get k.. in Prim order; unreached sites UINT32_MAX.
-extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t k, const uint32_t* msf_a,
-                                 const uint32_t* msf_b, size_t m, uint32_t* rank_out) {
+extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* outlets,
+                                   size_t k, uint32_t* order_out) {
+  return fl_outlet_pop_order_range(0, (uint32_t)n, row_ptr, col, outlets, k, order_out);
+}
+
+// Prim phase from a minimum spanning forest of the graph in which each
+// group's outlets are contracted into one vertex S_g (groups are independent
+// terrains: no edge joins two groups).  msf_a/msf_b: the forest's edges
+// sorted by increasing length.  Kruskal's merges build the reconstruction
+// tree; Prim from S_g completes the side of each merge edge that contains
+// the entry point before crossing to the other side, which it enters at the
+// edge's endpoint there.  Group g's outlets get ranks rank_off[g] + 0..k_g-1
+// in pop order, its other reached sites rank_off[g] + k_g.. in Prim order;
+// unreached sites UINT32_MAX.
+extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,
+                                        const uint32_t* outlet_order, const uint32_t* rank_off,
+                                        const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                                        uint32_t* rank_out) {
   const uint32_t NONE = 0xffffffffu;
   for (size_t i = 0; i < n; i++) rank_out[i] = NONE;
-  if (k == 0) return FL_OK;
+  std::vector<uint32_t> sleaf(n, NONE);  // outlet -> its group's S leaf
   std::vector<uint8_t> is_out(n, 0);
-  for (size_t q = 0; q < k; q++) {
-    if (outlet_order[q] >= n) return FL_ERR_INVALID;
-    is_out[outlet_order[q]] = 1;
-    rank_out[outlet_order[q]] = (uint32_t)q;
+  for (size_t g = 0; g < groups; g++) {
+    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) {
+      uint32_t o = outlet_order[q];
+      if (o >= n) return FL_ERR_INVALID;
+      is_out[o] = 1;
+      sleaf[o] = outlet_order[outlet_off[g]];
+      rank_out[o] = rank_off[g] + (q - outlet_off[g]);
+    }
   }
-  const uint32_t s_leaf = outlet_order[0];
-  auto leaf_of = [&](uint32_t v) { return is_out[v] ? s_leaf : v; };
-  // union-find over sites (outlets pre-united), nodes: leaves 0..n-1, merges n..n+m-1
+  auto leaf_of = [&](uint32_t v) { return is_out[v] ? sleaf[v] : v; };
   std::vector<uint32_t> uf(n), top(n);
   for (size_t v = 0; v < n; v++) {
-    uf[v] = (uint32_t)v;
+    uf[v] = is_out[v] ? sleaf[v] : (uint32_t)v;
     top[v] = (uint32_t)v;
   }
-  for (size_t q = 0; q < k; q++) uf[outlet_order[q]] = s_leaf;
   auto find = [&](uint32_t v) {
     uint32_t r = v;
     while (uf[r] != r) r = uf[r];
@@ -488,7 +512,6 @@ extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t 
     cnt[v] = 1;
     size[v] = is_out[v] ? 0 : 1;
   }
-  std::vector<uint8_t> is_child(total, 0);
   for (size_t i = 0; i < m; i++) {
     uint32_t a = msf_a[i], b = msf_b[i];
     if (a >= n || b >= n) return FL_ERR_INVALID;
@@ -499,16 +522,20 @@ extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t 
     ch1[i] = top[rb];
     ea[i] = a;
     eb[i] = b;
-    is_child[ch0[i]] = 1;
-    is_child[ch1[i]] = 1;
     cnt[x] = cnt[ch0[i]] + cnt[ch1[i]];
     size[x] = size[ch0[i]] + size[ch1[i]];
     uf[rb] = ra;
     top[ra] = x;
   }
-  uint32_t root = top[find(s_leaf)];
+  for (size_t g = 0; g < groups; g++) {
+    if (outlet_off[g + 1] == outlet_off[g]) continue;
+    uint32_t s_leaf = outlet_order[outlet_off[g]];
+    uint32_t root = top[find(s_leaf)];
+    lo[root] = 0;
+    entry[root] = s_leaf;
+    off[root] = rank_off[g] + (outlet_off[g + 1] - outlet_off[g]);
+  }
   // leaf intervals (top-down; parents have larger indices than children)
-  lo[root] = 0;
   for (size_t x = total; x-- > n;) {
     if (lo[x] == NONE) continue;
     size_t i = x - n;
@@ -519,9 +546,6 @@ extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t 
     uint32_t l = lo[leaf_of(v)];
     return l >= lo[node] && l < lo[node] + cnt[node];
   };
-  // entries and rank offsets
-  entry[root] = s_leaf;
-  off[root] = (uint32_t)k;
   for (size_t x = total; x-- > n;) {
     if (entry[x] == NONE) continue;
     size_t i = x - n;
@@ -537,4 +561,11 @@ extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t 
   for (size_t v = 0; v < n; v++)
     if (!is_out[v] && entry[v] != NONE) rank_out[v] = off[v];
   return FL_OK;
+}
+
+// One group: all outlets contracted into one vertex.
+extern "C" int fl_ranks_from_msf(size_t n, const uint32_t* outlet_order, size_t k, const uint32_t* msf_a,
+                                 const uint32_t* msf_b, size_t m, uint32_t* rank_out) {
+  uint32_t off[2] = {0, (uint32_t)k}, roff[1] = {0};
+  return fl_ranks_from_msf_groups(n, 1, off, outlet_order, roff, msf_a, msf_b, m, rank_out);
 }

@@ -110,3 +110,76 @@ class DeviceTerrain:
                                                ptr(nxt, ctypes.c_uint32), ptr(da, ctypes.c_double),
                                                ptr(rt, ctypes.c_double), ctypes.byref(flag)))
         return h, nxt, da, rt, bool(flag.value)
+
+
+class DeviceBatch:
+    """Several terrains iterated together (C ABI ``fl_batch_*``): one launch
+    sequence per iteration for all of them; each terrain stops at its own
+    convergence or ``max_iteration`` and keeps the result of its solo run."""
+
+    def __init__(self, terrains):
+        """terrains: list of dicts with keys row_ptr, col, length, area,
+        erodibility, uplift, init_elevation, outlets and optionally max_slope,
+        max_iteration."""
+        self._keep: list = []
+        descs = (TerrainDesc * len(terrains))()
+        self.sizes = []
+        for i, tr in enumerate(terrains):
+            class _M:
+                pass
+
+            m = _M()
+            g = (np.ascontiguousarray(tr["row_ptr"], np.uint32), np.ascontiguousarray(tr["col"], np.uint32),
+                 np.ascontiguousarray(tr["length"], np.float64))
+            a = np.ascontiguousarray(tr["area"], np.float64)
+            m.graph = (lambda g=g: g)
+            m.num = (lambda g=g: len(g[0]) - 1)
+            m.areas = (lambda a=a: a)
+            n = m.num()
+            params = ParameterArrays(np.zeros(n), tr["erodibility"], tr["uplift"], np.zeros(n, bool),
+                                     tr.get("max_slope"))
+            init = np.ascontiguousarray(tr["init_elevation"], np.float64)
+            outlets = np.ascontiguousarray(tr["outlets"], np.uint32)
+            self._keep.extend([g, a, init, outlets, params])
+            descs[i] = make_desc(m, params, init, outlets, tr.get("max_iteration", UNBOUNDED), self._keep)
+            self.sizes.append(n)
+        self._descs = descs
+        self._ctx = context()
+        h = ctypes.c_void_p()
+        check(_native.lib().fl_batch_create(self._ctx, descs, len(terrains), ctypes.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def terrain_of(w, init_elevation=None, max_iteration: int = UNBOUNDED) -> dict:
+        from .generator import jitter_elevations
+
+        rp, col, ln = w.model.graph()
+        return dict(row_ptr=rp, col=col, length=ln, area=w.model.areas(), erodibility=w.erodibility,
+                    uplift=w.uplift, outlets=w.outlets(), max_slope=w.max_slope, max_iteration=max_iteration,
+                    init_elevation=jitter_elevations(w.base, 0) if init_elevation is None else init_elevation)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().fl_terrain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self) -> np.ndarray:
+        its = np.zeros(len(self.sizes), np.uint32)
+        check(_native.lib().fl_batch_solve(self._ctx, self._h, ptr(its, ctypes.c_uint32)))
+        return its
+
+    def elevations(self, index: int) -> np.ndarray:
+        out = np.empty(self.sizes[index], np.float64)
+        check(_native.lib().fl_batch_download(self._ctx, self._h, index, ptr(out, ctypes.c_double)))
+        return out
+
+    def last_stats(self) -> dict:
+        st = _native.SolveStats()
+        check(_native.lib().fl_terrain_last_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}

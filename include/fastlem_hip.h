@@ -93,6 +93,17 @@ int32_t fl_terrain_device_elevations(fl_terrain *t, double **dev_ptr);
 /* Bytes of HBM the terrain holds. */
 int32_t fl_terrain_device_bytes(const fl_terrain *t, uint64_t *bytes);
 
+/* Batched form: `count` independent terrains packed into one object and
+ * iterated together, one launch sequence per iteration for all of them.
+ * Terrain g stops when an iteration changes none of its elevations or after
+ * descs[g].max_iteration iterations, and is then frozen while the others go
+ * on, so every terrain's elevations and iteration count equal its solo run.
+ * The handle is an fl_terrain (destroy with fl_terrain_destroy). */
+int32_t fl_batch_create(fl_ctx *ctx, const fl_terrain_desc *descs, uint32_t count, fl_terrain **out);
+int32_t fl_batch_solve(fl_ctx *ctx, fl_terrain *batch, uint32_t *iterations_out /* [count] or NULL */);
+int32_t fl_batch_download(fl_ctx *ctx, fl_terrain *batch, uint32_t index, double *elevations_out);
+int32_t fl_batch_count(const fl_terrain *batch, uint32_t *count);
+
 /* Statistics of the last fl_terrain_solve on a terrain. */
 typedef struct fl_solve_stats {
   uint32_t iterations;      /* outer iterations run */

@@ -256,3 +256,33 @@ def test_deep_strip_teacher_forced_with_max_slope(strip):
         assert _bits_equal(rt_ref, rt_got), it
         assert _bits_equal(h_ref, h_got), it
         elev = h_ref
+
+
+def test_batch_matches_solo_runs(plain10k, fbm20k, strip):
+    """A mixed batch: every terrain equals its own oracle run bit for bit,
+    with its own iteration count (caps and convergence per terrain)."""
+    from oracle import flo
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import Workload, config_fbm, config_plain
+
+    small = config_plain(800, 28.0, 1.0)
+    ms = np.full(fbm20k.n, 0.2)
+    fbm_ms = Workload("fbm-ms", fbm20k.model, fbm20k.erodibility, fbm20k.uplift, fbm20k.is_outlet, fbm20k.base, ms)
+    g_grid, out_grid = _grid_terrain(30, 1.0)
+    n_grid = g_grid.n
+    grid = dict(row_ptr=g_grid.row_ptr, col=g_grid.col, length=g_grid.length, area=np.ones(n_grid),
+                erodibility=np.ones(n_grid), uplift=np.ones(n_grid), outlets=out_grid,
+                init_elevation=np.random.default_rng(3).uniform(0, 1e-12, n_grid))
+    terrains = [DeviceBatch.terrain_of(plain10k), DeviceBatch.terrain_of(small, max_iteration=4),
+                DeviceBatch.terrain_of(fbm_ms), grid, DeviceBatch.terrain_of(strip),
+                DeviceBatch.terrain_of(config_fbm(5000, 70.0, seed=9)),
+                dict(DeviceBatch.terrain_of(small), outlets=np.zeros(0, np.uint32))]
+    b = DeviceBatch(terrains)
+    its = b.solve()
+    for i, tr in enumerate(terrains):
+        g = flo.Graph(tr["row_ptr"], tr["col"], tr["length"])
+        ref, ref_it, _ = flo.generate(g, tr["area"], tr["erodibility"], tr["uplift"], tr["init_elevation"],
+                                      tr["outlets"], max_iteration=tr.get("max_iteration", 0xFFFFFFFF),
+                                      max_slope=tr.get("max_slope"))
+        assert its[i] == ref_it, (i, its[i], ref_it)
+        assert _bits_equal(ref, b.elevations(i)), i
