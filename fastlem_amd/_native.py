@@ -80,6 +80,11 @@ def _declare(lib):
         "fl_batch_solve": (i32, [vp, vp, c_u32p]),
         "fl_batch_download": (i32, [vp, vp, u32, c_f64p]),
         "fl_batch_count": (i32, [vp, c_u32p]),
+        "fl_server_create": (i32, [i32, u32, u64, ctypes.POINTER(vp)]),
+        "fl_server_submit": (i32, [vp, ctypes.POINTER(TerrainDesc), c_f64p, ctypes.POINTER(u64)]),
+        "fl_server_wait": (i32, [vp, u64, c_u32p, ctypes.POINTER(f64), ctypes.POINTER(f64)]),
+        "fl_server_stats": (i32, [vp, ctypes.POINTER(u64), c_u32p]),
+        "fl_server_destroy": (i32, [vp]),
         "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64)]),

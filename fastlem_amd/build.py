@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OUT = os.path.join(LIBDIR, "libfastlem_hip.so")
-SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_capi.cpp", "fl_setup.cpp"]
+SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_capi.cpp", "fl_setup.cpp", "fl_server.cpp"]
 ARCH = os.environ.get("FASTLEM_ARCH", "gfx950")
 
 
@@ -34,7 +34,7 @@ def build(verbose: bool = False) -> str:
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I", os.path.join(HERE, "..", "include"),
+           "-I", os.path.join(HERE, "..", "include"), "-pthread",
            "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -348,6 +348,8 @@ int32_t reset_state(fl_ctx* c, fl_terrain* T) {
 
 extern "C" const char* fl_last_error(void) { return g_err.c_str(); }
 
+void fl_internal_set_error(const char* msg) { g_err = msg ? msg : ""; }
+
 extern "C" const char* fl_version(void) { return "fastlem_hip 0.1.0 (gfx950, f64)"; }
 
 extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {

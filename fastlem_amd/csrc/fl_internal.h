@@ -7,6 +7,9 @@ struct fl_rng;
 #ifdef __cplusplus
 #include <hip/hip_runtime.h>
 
+// Set the calling thread's fl_last_error() message.
+void fl_internal_set_error(const char* msg);
+
 #include <string>
 #include <vector>
 

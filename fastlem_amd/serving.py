@@ -1,0 +1,79 @@
+"""Serving: generate() requests through the library's continuous-batching
+request loop (C ABI ``fl_server_*``).
+
+``Server(slots, hbm_budget_bytes)`` starts ``slots`` native worker threads,
+each with its own HIP stream.  ``submit`` enqueues a request and returns a
+ticket at once; a request is admitted as soon as a slot is free and its HBM
+estimate fits the budget, so work of requests arriving while others run
+overlaps on the device.  ``wait`` returns (elevations, iterations, queue_ms,
+latency_ms).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import check, ptr
+from .generator import ParameterArrays, make_desc
+
+
+class Server:
+    def __init__(self, slots: int = 8, hbm_budget_bytes: int = 64 << 30, device: int = 0):
+        h = ctypes.c_void_p()
+        check(_native.lib().fl_server_create(device, slots, hbm_budget_bytes, ctypes.byref(h)))
+        self._h = h
+        self._pending: dict = {}
+
+    def submit(self, terrain: dict) -> int:
+        """terrain: dict as for fastlem_amd.device.DeviceBatch."""
+        class _M:
+            pass
+
+        m = _M()
+        g = (np.ascontiguousarray(terrain["row_ptr"], np.uint32), np.ascontiguousarray(terrain["col"], np.uint32),
+             np.ascontiguousarray(terrain["length"], np.float64))
+        a = np.ascontiguousarray(terrain["area"], np.float64)
+        m.graph = lambda: g
+        m.num = lambda: len(g[0]) - 1
+        m.areas = lambda: a
+        n = m.num()
+        params = ParameterArrays(np.zeros(n), terrain["erodibility"], terrain["uplift"], np.zeros(n, bool),
+                                 terrain.get("max_slope"))
+        init = np.ascontiguousarray(terrain["init_elevation"], np.float64)
+        outlets = np.ascontiguousarray(terrain["outlets"], np.uint32)
+        keep = [g, a, params, init, outlets]
+        desc = make_desc(m, params, init, outlets, terrain.get("max_iteration", 0xFFFFFFFF), keep)
+        out = np.empty(n, np.float64)
+        ticket = ctypes.c_uint64()
+        check(_native.lib().fl_server_submit(self._h, ctypes.byref(desc), ptr(out, ctypes.c_double),
+                                             ctypes.byref(ticket)))
+        self._pending[ticket.value] = (desc, keep, out)
+        return ticket.value
+
+    def wait(self, ticket: int):
+        its = ctypes.c_uint32()
+        q = ctypes.c_double()
+        lat = ctypes.c_double()
+        rc = _native.lib().fl_server_wait(self._h, ticket, ctypes.byref(its), ctypes.byref(q), ctypes.byref(lat))
+        desc, keep, out = self._pending.pop(ticket)
+        check(rc)
+        return out, its.value, q.value, lat.value
+
+    def stats(self):
+        b = ctypes.c_uint64()
+        r = ctypes.c_uint32()
+        check(_native.lib().fl_server_stats(self._h, ctypes.byref(b), ctypes.byref(r)))
+        return {"peak_bytes": b.value, "peak_running": r.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().fl_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -104,6 +104,23 @@ int32_t fl_batch_solve(fl_ctx *ctx, fl_terrain *batch, uint32_t *iterations_out 
 int32_t fl_batch_download(fl_ctx *ctx, fl_terrain *batch, uint32_t index, double *elevations_out);
 int32_t fl_batch_count(const fl_terrain *batch, uint32_t *count);
 
+/* Serving: a request loop on one GPU with continuous batching.  `slots`
+ * worker threads each own a context (stream); a request is admitted to a free
+ * slot as soon as the HBM it needs fits in hbm_budget_bytes (first come,
+ * first served), so requests arriving while others run overlap with them on
+ * the device.  Each request runs exactly fl_generate_f64.  The caller keeps
+ * the description's arrays and elevations_out alive until fl_server_wait
+ * returns for that ticket.  fl_server_destroy drains the queue. */
+typedef struct fl_server fl_server;
+int32_t fl_server_create(int32_t device, uint32_t slots, uint64_t hbm_budget_bytes, fl_server **out);
+int32_t fl_server_submit(fl_server *server, const fl_terrain_desc *desc, double *elevations_out, uint64_t *ticket);
+/* Blocks until the request finished; returns its status.  queue_ms: time
+ * from submission to admission; latency_ms: submission to completion. */
+int32_t fl_server_wait(fl_server *server, uint64_t ticket, uint32_t *iterations_out, double *queue_ms,
+                       double *latency_ms);
+int32_t fl_server_stats(fl_server *server, uint64_t *peak_bytes, uint32_t *peak_running);
+int32_t fl_server_destroy(fl_server *server);
+
 /* Statistics of the last fl_terrain_solve on a terrain. */
 typedef struct fl_solve_stats {
   uint32_t iterations;      /* outer iterations run */

@@ -286,3 +286,29 @@ def test_batch_matches_solo_runs(plain10k, fbm20k, strip):
                                       max_slope=tr.get("max_slope"))
         assert its[i] == ref_it, (i, its[i], ref_it)
         assert _bits_equal(ref, b.elevations(i)), i
+
+
+def test_served_results_equal_solo_results(plain10k, fbm20k, strip):
+    """Continuous batching: requests of mixed sizes admitted as slots and HBM
+    free up; every result is bit-identical to its solo oracle run."""
+    from oracle import flo
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.serving import Server
+    from fastlem_amd.workloads import config_fbm, config_plain
+
+    terrains = [DeviceBatch.terrain_of(w) for w in
+                (plain10k, fbm20k, strip, config_plain(800, 28.0, 1.0), config_fbm(5000, 70.0, seed=9))]
+    terrains += [DeviceBatch.terrain_of(config_fbm(3000, 55.0, seed=s)) for s in (21, 22, 23)]
+    budget = sum(len(t["row_ptr"]) * 400 + len(t["col"]) * 24 + (1 << 20) for t in terrains[:3])
+    srv = Server(slots=4, hbm_budget_bytes=budget)  # not everything fits at once
+    tickets = [srv.submit(t) for t in terrains]
+    results = [srv.wait(k) for k in tickets]
+    assert srv.stats()["peak_running"] >= 2
+    srv.close()
+    for tr, (h, its, q_ms, lat_ms) in zip(terrains, results):
+        g = flo.Graph(tr["row_ptr"], tr["col"], tr["length"])
+        ref, ref_it, _ = flo.generate(g, tr["area"], tr["erodibility"], tr["uplift"], tr["init_elevation"],
+                                      tr["outlets"])
+        assert its == ref_it
+        assert _bits_equal(ref, h)
+        assert lat_ms >= q_ms >= 0

@@ -308,3 +308,30 @@ def test_ranks_from_msf_match_heap_simulation(seed, k):
     assert rc == 0
     assert np.array_equal(rank[ref], np.arange(len(ref)))
     assert len(ref) == g.n
+
+
+def test_server_budget_and_errors_without_gpu():
+    """The request loop's admission control needs no GPU: a request larger
+    than the budget is refused at submission; without a device the admitted
+    request completes with FL_ERR_NO_DEVICE instead of hanging."""
+    import torch
+
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.serving import Server
+    from fastlem_amd.workloads import config_plain
+
+    w = config_plain(500, 22.0, 1.0)
+    small = Server(slots=2, hbm_budget_bytes=1 << 20)  # 1 MiB: too small for any terrain
+    with pytest.raises(_native.NativeError) as e:
+        small.submit(DeviceBatch.terrain_of(w))
+    assert e.value.code == -7  # FL_ERR_BUDGET
+    small.close()
+    if torch.cuda.is_available():
+        return
+    srv = Server(slots=2, hbm_budget_bytes=1 << 30)
+    t = srv.submit(DeviceBatch.terrain_of(w))
+    with pytest.raises(_native.NativeError) as e:
+        srv.wait(t)
+    assert e.value.code == -6  # FL_ERR_NO_DEVICE
+    srv.close()
