@@ -64,11 +64,21 @@ def main() -> int:
     ap.add_argument("--width", type=float, default=1000.0)
     ap.add_argument("--cpu-iters", type=int, default=16, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["single", "batched", "served"], default="single",
+                    help="single: the metric line; batched / served: the multi-terrain forms")
+    ap.add_argument("--terrains", type=int, default=128, help="batched/served: number of terrains")
+    ap.add_argument("--min-sites", type=int, default=1000)
+    ap.add_argument("--max-sites", type=int, default=65536)
+    ap.add_argument("--slots", type=int, default=8, help="served: worker slots (streams)")
+    ap.add_argument("--workers", type=int, default=8, help="host processes building the terrains")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solve runs on one GPU; --gpus > 1 is not supported "
                                    "(terrains shard across GPUs without communication, see DESIGN.md)"}))
         return 2
+
+    if args.mode != "single":
+        return multi_terrain(args)
 
     import ctypes
 
@@ -183,6 +193,78 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def multi_terrain(args) -> int:
+    """Batched (one union solve) or served (continuous-batching request loop)
+    against the plain path (one terrain at a time), on the same terrains."""
+    from fastlem_amd.workloads import terrain_stream
+
+    kind = "boundary" if args.mode == "batched" else "ocean"
+    t0 = time.time()
+    terrains = terrain_stream(args.terrains, args.min_sites, args.max_sites, 4242, kind, args.workers)
+    sites = sum(len(t["row_ptr"]) - 1 for t in terrains)
+    log(f"{len(terrains)} terrains, {sites} sites built in {time.time() - t0:.1f}s")
+    from fastlem_amd.device import DeviceBatch, DeviceTerrain
+
+    # plain path: one terrain per solve, inputs resident
+    devs = [DeviceTerrain(t["row_ptr"], t["col"], t["length"], t["area"], t["erodibility"], t["uplift"],
+                          t["init_elevation"], t["outlets"]) for t in terrains]
+    devs[0].solve()  # warm-up
+    ts = time.perf_counter()
+    iters = [d.solve() for d in devs]
+    plain_s = time.perf_counter() - ts
+    work = float(sum((len(t["row_ptr"]) - 1) * i for t, i in zip(terrains, iters)))
+    plain = work / plain_s
+    log(f"plain: {plain_s:.3f}s, {plain:.3e} site-iterations/s")
+    for d in devs:
+        d.close()
+    line = {"mode": args.mode, "terrains": len(terrains), "sites": sites,
+            "site_range": [args.min_sites, args.max_sites], "plain_site_iter_per_s": plain,
+            "plain_seconds": plain_s, "unit": "site-iterations/s"}
+    if args.mode == "batched":
+        b = DeviceBatch(terrains)
+        b.solve()  # warm-up (and the visit orders)
+        ts = time.perf_counter()
+        its = b.solve()
+        batch_s = time.perf_counter() - ts
+        assert list(its) == iters, "batched iteration counts differ from the plain runs"
+        line.update({"batched_site_iter_per_s": work / batch_s, "batched_seconds": batch_s,
+                     "speedup_vs_plain": plain_s / batch_s, "stats": b.last_stats()})
+        log(f"batched: {batch_s:.3f}s, {work / batch_s:.3e} site-iterations/s, x{plain_s / batch_s:.2f}")
+    else:
+        from fastlem_amd.serving import Server
+
+        rng = np.random.default_rng(7)
+        mean_service = plain_s / len(terrains)
+        results = []
+        for load in (0.5, 1.0, 2.0, 4.0, float("inf")):
+            srv = Server(slots=args.slots, hbm_budget_bytes=32 << 30)
+            gaps = np.zeros(len(terrains)) if load == float("inf") else \
+                rng.exponential(mean_service / load, len(terrains))
+            ts = time.perf_counter()
+            tickets = []
+            for t, gap in zip(terrains, gaps):
+                if gap > 0:
+                    time.sleep(gap)
+                tickets.append(srv.submit(t))
+            lat = []
+            for k in tickets:
+                _, _, _, ms = srv.wait(k)
+                lat.append(ms)
+            span = time.perf_counter() - ts
+            peak = srv.stats()["peak_running"]
+            srv.close()
+            r = {"offered_load": load if load != float("inf") else "saturation",
+                 "site_iter_per_s": work / span, "p50_ms": float(np.percentile(lat, 50)),
+                 "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span, "peak_running": peak}
+            results.append(r)
+            log(f"served load={r['offered_load']}: {r['site_iter_per_s']:.3e} site-it/s "
+                f"p50={r['p50_ms']:.1f}ms p99={r['p99_ms']:.1f}ms peak={peak}")
+        line.update({"served": results, "slots": args.slots,
+                     "speedup_vs_plain_at_saturation": results[-1]["site_iter_per_s"] / plain})
     print(json.dumps(line), flush=True)
     return 0
 

@@ -193,3 +193,51 @@ def config_strip(nx: int = 16384, ny: int = 1024, jitter: float = 0.4, seed: int
     is_out[np.arange(ny) * nx] = True
     k = 1.0 * (1.0 + rng.uniform(-0.05, 0.05, n))
     return Workload("strip-%dx%d" % (nx, ny), m, k, np.ones(n), is_out, np.zeros(n))
+
+
+def _terrain_dict(args):
+    """Worker: one terrain of the batched / served streams (no GPU use)."""
+    kind, n, seed = args
+    import numpy as _np
+
+    from .generator import jitter_elevations
+    from .surface import Site2D, TerrainModel2DBulider
+
+    rng = _np.random.default_rng(seed)
+    width = float(_np.sqrt(n))
+    xs = rng.uniform(0.0, width, n)
+    ys = rng.uniform(0.0, width, n)
+    m = TerrainModel2DBulider().set_sites((xs, ys)).set_bounding_box(Site2D(0.0, 0.0), Site2D(width, width)).build()
+    lam = max(width / 4.0, 1.0)
+    k = _np.clip(10.0 ** fbm(m.xs / lam, m.ys / lam, seed * 2 + 1), 0.1, 10.0)
+    if kind == "boundary":
+        band = 2.0
+        out = (m.xs < band) | (m.ys < band) | (m.xs > width - band) | (m.ys > width - band)
+    else:  # ocean mask
+        out = fbm(m.xs / lam + 17.31, m.ys / lam + 5.77, seed * 2 + 2) < -0.2
+        if not out.any():
+            out[_np.argmin(m.xs)] = True
+    outlets = _np.nonzero(out)[0].astype(_np.uint32)
+    rp, col, ln = m.graph()
+    cnt = m.num()
+    return dict(row_ptr=rp, col=col, length=ln, area=m.areas(), erodibility=k, uplift=_np.ones(cnt),
+                outlets=outlets, init_elevation=jitter_elevations(_np.zeros(cnt), 0))
+
+
+def terrain_stream(count: int, lo: int, hi: int, seed: int, kind: str, workers: int = 8) -> list:
+    """`count` independent terrains with site counts log-uniform in [lo, hi]
+    (seeded): uniform-random sites in [0, sqrt(N)]^2, Delaunay graph,
+    erodibility = clip(10^fBm, 0.1, 10), outlets on the boundary band
+    (kind="boundary", the batched workload) or where a second fBm < -0.2
+    (kind="ocean", the served workload).  Built in a spawn process pool; call
+    before the GPU is initialised."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    jobs = [(kind, int(n), seed * 100003 + i) for i, n in enumerate(sizes)]
+    if workers <= 1:
+        return [_terrain_dict(j) for j in jobs]
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_terrain_dict, jobs, chunksize=4))
