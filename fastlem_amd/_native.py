@@ -47,7 +47,9 @@ class SolveStats(ctypes.Structure):
 
     _fields_ = [("iterations", ctypes.c_uint32), ("lake_iterations", ctypes.c_uint32),
                 ("host_syncs", ctypes.c_uint32), ("order_from_forest", ctypes.c_uint32),
-                ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+                ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("outlet_phase_ms", ctypes.c_double), ("forest_ms", ctypes.c_double),
+                ("ranks_ms", ctypes.c_double)]
 
 
 class NativeError(RuntimeError):

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -75,6 +76,7 @@ struct fl_terrain {
   // statistics of the last solve
   uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0;
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
+  double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   uint64_t bytes = 0;
   // host copies (relabelled) needed for the one-time visit order
   std::vector<uint32_t> h_row_ptr, h_col, h_outlets;
@@ -174,25 +176,40 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   std::vector<uint32_t> rank(n), order(n, 0xffffffffu);
   bool fast = false;
   if (k > 0) {
-    // per terrain: outlet phase (exact heap tie handling); then the Prim
-    // phase of all terrains from one spanning forest with each terrain's
-    // outlets merged into its own component
+    // per terrain: outlet phase (exact heap tie handling, host thread) while
+    // the GPU builds one spanning forest of all terrains with each terrain's
+    // outlets merged into its own component (labelled by its first listed
+    // outlet; the ranks only need the component, not the label)
     std::vector<uint32_t> first(k), a, b, init(n);
-    for (uint32_t g = 0; g < G; g++) {
-      uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
-      int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
-                                         T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
-      if (rc != FL_OK) return set_err(rc, "outlet phase failed");
-    }
+    std::future<int> outlet_phase = std::async(std::launch::async, [&]() {
+      auto p0 = std::chrono::steady_clock::now();
+      for (uint32_t g = 0; g < G; g++) {
+        uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
+        int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
+                                           T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
+        if (rc != FL_OK) return rc;
+      }
+      T->stat_order_parts[0] =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
+      return (int)FL_OK;
+    });
+    auto p1 = std::chrono::steady_clock::now();
     for (uint32_t v = 0; v < n; v++) init[v] = v;
     for (uint32_t g = 0; g < G; g++)
-      for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++) init[first[q]] = first[T->outlet_off[g]];
+      for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++)
+        init[T->h_outlets[q]] = T->h_outlets[T->outlet_off[g]];
     std::string err;
     int rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, init, a, b, &fast, &err);
+    T->stat_order_parts[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p1).count();
+    int rc_outlets = outlet_phase.get();
     if (rc != FL_OK) return set_err(rc, "spanning forest: %s", err.c_str());
+    if (rc_outlets != FL_OK) return set_err(rc_outlets, "outlet phase failed");
     if (fast) {
+      auto p2 = std::chrono::steady_clock::now();
       rc = fl_ranks_from_msf_groups(n, G, T->outlet_off.data(), first.data(), T->site_off.data(), a.data(), b.data(),
                                     a.size(), rank.data());
+      T->stat_order_parts[2] =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
       if (rc != FL_OK) return set_err(rc, "ranks from spanning forest failed");
     }
   }
@@ -819,6 +836,9 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->host_syncs = T->stat_syncs;
   out->order_from_forest = T->stat_order_fast;
   out->visit_order_ms = T->stat_order_ms;
+  out->outlet_phase_ms = T->stat_order_parts[0];
+  out->forest_ms = T->stat_order_parts[1];
+  out->ranks_ms = T->stat_order_parts[2];
   out->solve_ms = T->stat_solve_ms;
   return FL_OK;
 }

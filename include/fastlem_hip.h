@@ -128,8 +128,11 @@ typedef struct fl_solve_stats {
   uint32_t host_syncs;      /* host-device synchronisations */
   uint32_t order_from_forest; /* 1: visit order from the spanning forest,
                                  0: exact heap simulation (ties present) */
-  double visit_order_ms;    /* host time of the lake-traversal visit order */
+  double visit_order_ms;    /* wall time of the lake-traversal visit order */
   double solve_ms;          /* wall time of the solve */
+  double outlet_phase_ms;   /* visit order parts: outlet phase (host, runs */
+  double forest_ms;         /*   concurrently with) the GPU spanning forest, */
+  double ranks_ms;          /*   then the ranks from the forest (host) */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 
