@@ -542,7 +542,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
   for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
-                       &w.jn2, &w.lw2, &w.clen, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
+                       &w.jn2, &w.lw2, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
   FLCHK(dev_alloc(P, &w.cmask, n));
   for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur})

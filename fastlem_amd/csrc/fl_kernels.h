@@ -84,7 +84,6 @@ struct fl_dev_work {
 
   uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;
   uint32_t *jc2 = nullptr, *pc2 = nullptr, *jn2 = nullptr, *lw2 = nullptr;
-  uint32_t* clen = nullptr;
   uint32_t *bcnt = nullptr, *bnodes = nullptr, *boff = nullptr, *bnoff = nullptr, *bcur = nullptr,
            *bncur = nullptr;                                  // [n+2] light-depth buckets
   uint32_t *chain_base = nullptr, *chain_len = nullptr;      // [n] chains by bucket

@@ -257,16 +257,6 @@ __global__ void k_chain_jump(uint32_t n, const uint32_t* __restrict__ jc, const 
   if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
 
-// Chain length, stored at the head by the chain's tail (the member without
-// a heavy child), for chains inside an outlet's basin.
-__global__ void k_chain_len(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ jc,
-                            const uint32_t* __restrict__ pc, const uint32_t* __restrict__ jn,
-                            const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ clen) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  if (hc[v] == FL_NONE && is_outlet[jn[v]]) clen[jc[v]] = pc[v] + 1;
-}
-
 // Buckets: 2 * light depth + (chain length >= FL_LONG), so every round has a
 // range of short chains (one lane each) followed by long ones (one
 // wavefront each).
@@ -277,10 +267,12 @@ __global__ void k_chain_len(uint32_t n, const uint32_t* __restrict__ hc, const u
 
 // Light-depth histogram of the chains (count and total length), aggregated
 // per workgroup in LDS before one global atomic per (workgroup, bucket).
+// Each chain is counted by its tail (the member without a heavy child),
+// which knows the chain's length (its position + 1) and light depth.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
-               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
-               const uint32_t* __restrict__ clen, uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes,
+k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ pc,
+               const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
+               const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes,
                uint32_t* __restrict__ maxld) {
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
@@ -293,8 +285,8 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
   uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    if (v < n && jc[v] == v && is_outlet[jn[v]]) {
-      uint32_t len = clen[v], l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
+    if (v < n && hc[v] == FL_NONE && is_outlet[jn[v]]) {
+      uint32_t len = pc[v] + 1, l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
       if (l < FL_LDS_BUCKETS) {
         atomicAdd(s_c + l, 1u);
         atomicAdd(s_n + l, len);
@@ -375,14 +367,14 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uin
   }
 }
 
-// Give every chain a slot in its light-depth bucket and a contiguous range
-// of positions: LDS atomics rank the chains of a workgroup, one global
-// atomic per (workgroup, bucket) reserves the ranges.  Order inside a bucket
-// is immaterial to the results.
+// Give every chain (through its tail) a slot in its light-depth bucket and a
+// contiguous range of positions: LDS atomics rank the chains of a workgroup,
+// one global atomic per (workgroup, bucket) reserves the ranges.  Order
+// inside a bucket is immaterial to the results.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ jn,
-               const uint32_t* __restrict__ lw, const uint8_t* __restrict__ is_outlet,
-               const uint32_t* __restrict__ clen, const uint32_t* __restrict__ boff,
+k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ jc,
+               const uint32_t* __restrict__ pc, const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
+               const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
                uint32_t* __restrict__ head_base) {
@@ -398,8 +390,8 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     lc[j] = FL_NONE;
     ln[j] = 0;
-    if (v < n && jc[v] == v && is_outlet[jn[v]]) {
-      uint32_t len = clen[v], l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
+    if (v < n && hc[v] == FL_NONE && is_outlet[jn[v]]) {
+      uint32_t len = pc[v] + 1, l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
       if (l < FL_LDS_BUCKETS) {
         lc[j] = atomicAdd(s_c + l, 1u);
         ln[j] = atomicAdd(s_n + l, len);
@@ -408,7 +400,7 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
         uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
         chain_base[c] = pos;
         chain_len[c] = len;
-        head_base[v] = pos;
+        head_base[jc[v]] = pos;
       }
     }
   }
@@ -423,12 +415,13 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __re
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     if (lc[j] == FL_NONE) continue;
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    uint32_t l = 2u * lw[v] + (clen[v] >= FL_LONG ? 1u : 0u);
+    uint32_t len = pc[v] + 1;
+    uint32_t l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
     uint32_t c = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
     chain_base[c] = pos;
-    chain_len[c] = clen[v];
-    head_base[v] = pos;
+    chain_len[c] = len;
+    head_base[jc[v]] = pos;
   }
 }
 
@@ -1026,17 +1019,16 @@ void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_w
 }
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_chain_len, t.n, s, t.n, w.hc, w.jc, w.pc, w.jn, t.is_outlet, w.clen);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.jc, w.jn, w.lw,
-                 t.is_outlet, w.clen, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
+  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.hc, w.pc, w.jn, w.lw,
+                 t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
 }
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.jc, w.jn, w.lw,
-                 t.is_outlet, w.clen, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
+  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.hc, w.jc, w.pc, w.jn,
+                 w.lw, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
   FL_LAUNCH(k_chain_perm, t.n, s, t.n, w.jc, w.pc, w.jn, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
             t.area, w.A, w.tau, w.h, w.h2);
 }
