@@ -601,8 +601,12 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     T->t_active[g] = (!empty && max_it[g] > 0) ? 1 : 0;
     active += T->t_active[g];
   }
-  if (T->t.n == 0 || active == 0) return FL_OK;
-  FLCHK(reset_state(c, T));
+  if (T->t.n == 0) return FL_OK;
+  FLCHK(reset_state(c, T));  // elevations start from the initial state even if nothing iterates
+  if (active == 0) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FL_OK;
+  }
   if (T->d_active)
     HIPCHK(hipMemcpyAsync(T->d_active, T->t_active.data(), G, hipMemcpyHostToDevice, c->stream));
   uint32_t it = 0;

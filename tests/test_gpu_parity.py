@@ -210,12 +210,13 @@ def test_max_iteration_is_honoured(plain10k):
 
     w = plain10k
     init = init_of(w)
-    for cap in (1, 3):
+    t = _dev(w)
+    for cap in (1, 3, 0):
         ref, iters, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init, w.outlets(),
                                      max_iteration=cap)
-        t = _dev(w, max_iteration=cap)
-        assert t.solve() == iters == cap
+        assert t.solve(max_iteration=cap) == iters == cap  # re-solving the same terrain
         assert _bits_equal(ref, t.elevations())
+    assert _bits_equal(t.elevations(), init)  # cap 0: nothing ran
 
 
 @pytest.fixture(scope="module")
