@@ -846,3 +846,32 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->solve_ms = T->stat_solve_ms;
   return FL_OK;
 }
+
+extern "C" int32_t fl_stage_basin_order(fl_ctx* c, fl_terrain* T, const double* elev, uint32_t outlet,
+                                        uint32_t* order_out, uint32_t* len_out) {
+  if (!c || !T || !elev || !order_out || !len_out) return set_err(FL_ERR_INVALID, "null argument");
+  if (outlet >= T->t.n) return set_err(FL_ERR_INVALID, "outlet out of range");
+  ProfScope prof_scope(c->prof);
+  HIPCHK(hipSetDevice(c->device));
+  uint32_t n = T->t.n;
+  FLCHK(upload_h(c, T, elev));
+  hipStream_t s = c->stream;
+  fl_launch_clear_iter(s, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+  fl_launch_receivers(s, T->t, T->w);
+  FLCHK(read_ctrl(c, T));
+  if (host_flag(c, FL_FLAG_LAKE)) FLCHK(join_lakes(c, T));
+  fl_launch_children(s, T->t, T->w);  // child masks of the final tree
+  // scratch: the chain-major permutation buffer and one counter
+  uint32_t* d_order = T->w.perm;
+  uint32_t* d_len = T->w.ctrl + FL_CTRL_JUMPS;
+  fl_launch_basin_order(s, T->t, T->w, T->new_of_old[outlet], d_order, d_len);
+  uint32_t len = 0;
+  HIPCHK(hipMemcpyAsync(&len, d_len, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  std::vector<uint32_t> tmp(len);
+  if (len) HIPCHK(hipMemcpyAsync(tmp.data(), d_order, (uint64_t)len * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (uint32_t q = 0; q < len; q++) order_out[q] = T->old_of_new[tmp[q]];
+  *len_out = len;
+  return FL_OK;
+}

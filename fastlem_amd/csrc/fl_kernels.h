@@ -113,3 +113,6 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);
+// Verification: the reference's BFS traversal of the basin of `outlet`.
+void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t outlet,
+                           uint32_t* order, uint32_t* len_out);

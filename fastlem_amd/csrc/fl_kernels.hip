@@ -902,6 +902,59 @@ __global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm, 
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
 
+// Verification: the reference's traversal of one basin
+// (DrainageBasin::construct, drainage_basin.rs:13-36): start at the outlet,
+// then for each site in order append its children in row order.  One
+// workgroup; children counts are prefix-summed per chunk of 1024 sites so
+// the appended order equals the sequential one.
+__global__ void __launch_bounds__(1024)
+k_basin_order(uint32_t outlet, uint32_t cap, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+              const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+              uint32_t* __restrict__ order, uint32_t* __restrict__ len_out) {
+  __shared__ uint32_t s_scan[1024];
+  __shared__ uint32_t s_tail;
+  uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    order[0] = outlet;
+    s_tail = 1;
+  }
+  __syncthreads();
+  uint32_t head = 0;
+  while (head < s_tail) {
+    uint32_t end = s_tail;
+    for (uint32_t chunk = head; chunk < end; chunk += 1024) {
+      uint32_t i = chunk + tid;
+      uint32_t u = FL_NONE, cnt = 0, b = 0, deg = 0;
+      unsigned long long m = 0;
+      if (i < end) {
+        u = order[i];
+        b = rp[u];
+        deg = rp[u + 1] - b;
+        m = cmask[u];
+        for (uint32_t s = 0; s < deg; ++s) cnt += is_child(m, deg, s, b, col, next, u) ? 1u : 0u;
+      }
+      s_scan[tid] = cnt;
+      __syncthreads();
+      for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t t = tid >= off ? s_scan[tid - off] : 0;
+        __syncthreads();
+        s_scan[tid] += t;
+        __syncthreads();
+      }
+      uint32_t base = s_tail + s_scan[tid] - cnt;
+      if (i < end && cnt) {
+        for (uint32_t s = 0; s < deg; ++s)
+          if (is_child(m, deg, s, b, col, next, u) && base < cap) order[base++] = col[b + s];
+      }
+      __syncthreads();
+      if (tid == 1023) s_tail = min(cap, s_tail + s_scan[1023]);
+      __syncthreads();
+    }
+    head = end;
+  }
+  if (tid == 0) *len_out = s_tail;
+}
+
 // Zero the control block and the first `nb` bucket entries.
 __global__ void k_clear_iter(uint32_t nb, uint32_t* __restrict__ ctrl, uint32_t* __restrict__ bcnt,
                              uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
@@ -1072,4 +1125,10 @@ void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {
 
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v) {
   FL_LAUNCH(k_fill_u32, count, s, (uint32_t)count, p, v);
+}
+
+void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t outlet,
+                           uint32_t* order, uint32_t* len_out) {
+  FL_LAUNCH_GRID(k_basin_order, dim3(1), dim3(1024), s, outlet, t.n, t.row_ptr, t.col, w.next,
+                 (const unsigned long long*)w.cmask, order, len_out);
 }

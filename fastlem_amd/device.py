@@ -100,6 +100,14 @@ class DeviceTerrain:
                                                  ctypes.byref(flag)))
         return nxt, sub, bool(flag.value)
 
+    def stage_basin_order(self, elev, outlet: int) -> np.ndarray:
+        elev = np.ascontiguousarray(elev, np.float64)
+        out = np.empty(self.n, np.uint32)
+        ln = ctypes.c_uint32()
+        check(_native.lib().fl_stage_basin_order(self._ctx, self._h, ptr(elev, ctypes.c_double), outlet,
+                                                 ptr(out, ctypes.c_uint32), ctypes.byref(ln)))
+        return out[:ln.value]
+
     def stage_iteration(self, elev):
         h = np.ascontiguousarray(elev, np.float64).copy()
         nxt = np.empty(self.n, np.uint32)

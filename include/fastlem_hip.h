@@ -160,6 +160,12 @@ int32_t fl_stage_stream_tree(fl_ctx *ctx, fl_terrain *t, const double *elev, uin
                              uint32_t *subroot_out, int32_t *flag);
 int32_t fl_stage_iteration(fl_ctx *ctx, fl_terrain *t, double *elev, uint32_t *next_out, double *area_out,
                            double *response_out, int32_t *flag);
+/* The reference's traversal of one basin (drainage_basin.rs:13-36) on the
+ * stream tree of elev: order_out[0..*len_out) (room for n entries).  The
+ * sweeps never need it (they follow heavy-path chains); it verifies the tree
+ * order. */
+int32_t fl_stage_basin_order(fl_ctx *ctx, fl_terrain *t, const double *elev, uint32_t outlet, uint32_t *order_out,
+                             uint32_t *len_out);
 
 /* ---- host setup helpers (one-time, CPU) ---- */
 

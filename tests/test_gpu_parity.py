@@ -313,3 +313,30 @@ def test_served_results_equal_solo_results(plain10k, fbm20k, strip):
         assert its == ref_it
         assert _bits_equal(ref, h)
         assert lat_ms >= q_ms >= 0
+
+
+def test_basin_traversal_order(fbm20k):
+    """Tree order: the device's basin traversal equals the reference BFS
+    (drainage_basin.rs:13-36) for the largest basins, before and after
+    lake joining."""
+    from oracle import flo
+
+    w = fbm20k
+    g = graph_of(w)
+    t = _dev(w)
+    elev = init_of(w)
+    for it in range(3):
+        nxt, _, _ = flo.stream_tree(g, elev, w.outlets())
+        roots = np.array([_root(nxt, v) for v in range(w.n)])
+        sizes = np.bincount(roots, minlength=w.n)
+        for o in np.argsort(-sizes)[:4]:
+            ref = flo.basin_order(g, nxt, int(o))
+            got = t.stage_basin_order(elev, int(o))
+            assert np.array_equal(ref, got), (it, o)
+        elev = flo.iteration(g, w.model.areas(), w.erodibility, w.uplift, w.outlets(), elev)[0]
+
+
+def _root(nxt, v):
+    while nxt[v] != v:
+        v = nxt[v]
+    return v
