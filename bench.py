@@ -35,24 +35,42 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name: str, n: int, nnz: int) -> float:
-    """Algorithmic (compulsory) bytes of one launch: every array the kernel
-    must touch, read or written once; gathers and re-reads are not counted."""
+def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
+    """Algorithmic (compulsory) bytes per work item of one kernel: every array
+    it must touch, read or written once; gathers that re-read an array are not
+    counted.  Units are sites for the per-site kernels and chain positions
+    (sites inside basins, in chain-major order) for the sweeps; the profiler
+    accumulates the units of every launch.  deg = nnz / n adjacency entries."""
+    deg = nnz / max(n, 1)
     table = {
-        # row_ptr 4 + h 8 + is_outlet 1, write next 4 + dnext 8; col 4 + len 8 per entry
-        "k_receivers": 25 * n + 12 * nnz,
-        # row_ptr 4 + prev area 8, write coff/ccnt/hc 12, clist 4; col 4 + next 4 per entry
-        "k_children": 28 * n + 8 * nnz,
-        # jc pc jn lw read + written
-        "k_chain_jump": 32 * n,
-        "k_chain_init": 28 * n,
-        # perm 4 + coff 4 + ccnt 4 + hc 4 + area 8 + A write 8 + clist 4 + children A 8 (per site)
-        "k_da_round": 44 * n,
-        # perm 4 + next 4 + jn 4 + x 8 + U 8 + tan 8 + dnext 8 + h 8 + writes tau 8 + h 8
-        "k_tau_round": 68 * n,
-        "k_celerity": 45 * n,
+        # row_ptr 4 + h 8 + is_outlet 1, writes next 4 + dnext 8; col 4 + len 8 per entry
+        "k_receivers": 25 + 12 * deg,
+        # row_ptr 4 + next 4 + previous area 8, writes cmask 8 + hc 4; col 4 per entry
+        "k_children": 28 + 4 * deg,
+        # next 4 + hc 4, writes jc pc jn lw 16
+        "k_chain_init": 24,
+        # jc pc jn lw read 16 + written 16
+        "k_chain_jump": 32,
+        # hc 4 per site (the tails' extra reads are not counted)
+        "k_bucket_count": 4,
+        "k_bucket_place": 4,
+        # jn 4 + is_outlet 1 + jc 4 + pc 4 + head_base 4, writes perm 4
+        "k_chain_perm": 21,
+        # perm 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4 + a child's A 8, writes pre/post0/post1/npost 28
+        "k_da_pre": 68,
+        # pre/post0/post1/npost 28 + perm 4, writes A 8
+        "k_da_chain": 40,
+        # perm 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4 + a child's A 8, writes A 8
+        "k_da_long": 48,
+        # perm 4 + K 8 + A 8 + dnext 8, writes x 8 (+ U, tan, h read and four gathered copies written)
+        "k_tau_prep": 92 if clamp else 36,
+        # x 8, writes tau 8 (+ U, tan, d read and h written)
+        "k_tau_round": 48 if clamp else 16,
+        "k_tau_long": 48 if clamp else 16,
+        # perm 4 + tau 8 + h 8 (+ jn 4 + U 8, or the chain-major h 8), writes tau 8 + h 8
+        "k_tau_scatter": 44 if clamp else 48,
     }
-    return float(table.get(name, 0))
+    return float(table.get(name, 0.0))
 
 
 def main() -> int:
@@ -64,6 +82,8 @@ def main() -> int:
     ap.add_argument("--width", type=float, default=1000.0)
     ap.add_argument("--cpu-iters", type=int, default=16, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-kib", type=float, default=None,
+                    help="measured HBM KiB per launch of the dominant kernel (rocprofv3 --pmc), for the roofline")
     ap.add_argument("--mode", choices=["single", "batched", "served"], default="single",
                     help="single: the metric line; batched / served: the multi-terrain forms")
     ap.add_argument("--terrains", type=int, default=128, help="batched/served: number of terrains")
@@ -134,29 +154,39 @@ def main() -> int:
     L.fl_ctx_set_profiling(ctx, 1)
     L.fl_ctx_reset_profile(ctx)
     t.solve()
-    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_chain_len", "k_bucket_count",
-             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_chain", "k_da_long", "k_tau_prep",
-             "k_tau_round", "k_tau_long", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys",
-             "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
+    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count", "k_bucket_scan",
+             "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_chain", "k_da_long", "k_tau_prep", "k_tau_round",
+             "k_tau_long", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32",
+             "k_fill_u64", "k_copy_u32"]
     prof = {}
     for nm in names:
         cnt = ctypes.c_uint64()
         ms = ctypes.c_double()
-        L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms))
+        units = ctypes.c_uint64()
+        L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(units))
         if cnt.value:
-            prof[nm] = (cnt.value, ms.value)
+            prof[nm] = (cnt.value, ms.value, units.value)
     L.fl_ctx_set_profiling(ctx, 0)
+    clamp = bool(w.max_slope is not None and np.any(~np.isnan(w.max_slope)))
     total_kernel_ms = sum(v[1] for v in prof.values())
-    for nm, (cnt, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-        log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us")
+    for nm, (cnt, ms, units) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        bpu = kernel_bytes_per_unit(nm, n, nnz, clamp)
+        gbs = bpu * units / (ms / 1e3) / 1e9 if bpu and ms else 0.0
+        log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us "
+            f"units/launch={units / cnt:11.1f} {gbs:7.1f} GB/s")
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
-    dcnt, dms = prof[dom]
+    dcnt, dms, dunits = prof[dom]
     avg_s = dms / dcnt / 1e3
-    b = kernel_bytes(dom, n, nnz)
-    achieved = b / avg_s / 1e9 if b else None
+    bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp) * dunits / dcnt
+    achieved = bytes_per_launch / avg_s / 1e9 if bytes_per_launch else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "avg_launch_us": avg_s * 1e6, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None}
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+                "launches_per_solve": dcnt, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None}
+    if args.traffic_kib is not None:
+        # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
+        # scaled by the factor the same counters read for a copy kernel of known size
+        roofline["traffic"] = args.traffic_kib * 1024.0
 
     cpu = None
     if not args.no_cpu:

@@ -89,7 +89,8 @@ def _declare(lib):
         "fl_server_destroy": (i32, [vp]),
         "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
-        "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64)]),
+        "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64),
+                                     ctypes.POINTER(u64)]),
         "fl_ctx_reset_profile": (i32, [vp]),
         "fl_stage_receivers": (i32, [vp, vp, c_f64p, c_u32p]),
         "fl_stage_stream_tree": (i32, [vp, vp, c_f64p, c_u32p, c_u32p, ctypes.POINTER(i32)]),

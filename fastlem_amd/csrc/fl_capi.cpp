@@ -339,9 +339,11 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
   for (uint32_t l = rounds; l-- > 0;)
-    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1]);
+    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2]);
   fl_launch_tau_prep(s, T->t, w, npos, clamp);
-  for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], clamp);
+  for (uint32_t l = 0; l < rounds; l++)
+    fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2],
+                        clamp);
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
   fl_launch_tau_scatter(s, T->t, w, npos, clamp, T->d_tid, T->d_active, T->d_tchanged);
   if (T->d_tchanged)
@@ -813,16 +815,19 @@ extern "C" int32_t fl_ctx_set_profiling(fl_ctx* c, int32_t enable) {
   return FL_OK;
 }
 
-extern "C" int32_t fl_ctx_kernel_time(fl_ctx* c, const char* name, uint64_t* launches, double* total_ms) {
+extern "C" int32_t fl_ctx_kernel_time(fl_ctx* c, const char* name, uint64_t* launches, double* total_ms,
+                                      uint64_t* units) {
   if (!c || !name || !launches || !total_ms) return set_err(FL_ERR_INVALID, "null argument");
   *launches = 0;
   *total_ms = 0.0;
+  if (units) *units = 0;
   if (!c->prof) return FL_OK;
   c->prof->flush();
   auto it = c->prof->totals.find(name);
   if (it != c->prof->totals.end()) {
-    *launches = it->second.first;
-    *total_ms = it->second.second;
+    *launches = it->second.launches;
+    *total_ms = it->second.ms;
+    if (units) *units = it->second.units;
   }
   return FL_OK;
 }

@@ -12,12 +12,18 @@ struct fl_profiler {
   struct Rec {
     const char* name;
     hipEvent_t a, b;
+    uint64_t units;  // work items of the launch (sites or chain positions)
+  };
+  struct Total {
+    uint64_t launches = 0;
+    double ms = 0.0;
+    uint64_t units = 0;
   };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<uint64_t, double>> totals;
+  std::map<std::string, Total> totals;
   hipEvent_t get();
-  void record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b);
+  void record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b, uint64_t units);
   void flush();
   void reset();
   ~fl_profiler();
@@ -103,13 +109,13 @@ void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_w
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 // Round l of a sweep: short chains [c0, c1) at positions [p0, p1), long
-// chains [c1, c2).
+// chains [c1, c2) at positions [p1, p2).
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1);
+                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2);
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                         uint32_t c2, bool clamp);
+                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp);
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);

@@ -988,8 +988,8 @@ hipEvent_t fl_profiler::get() {
   return e;
 }
 
-void fl_profiler::record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b) {
-  pending.push_back({name, a, b});
+void fl_profiler::record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b, uint64_t units) {
+  pending.push_back({name, a, b, units});
   if (pending.size() >= 8192) flush();
   (void)s;
 }
@@ -1000,8 +1000,9 @@ void fl_profiler::flush() {
     (void)hipEventSynchronize(r.b);
     (void)hipEventElapsedTime(&ms, r.a, r.b);
     auto& t = totals[r.name];
-    t.first += 1;
-    t.second += ms;
+    t.launches += 1;
+    t.ms += ms;
+    t.units += r.units;
     pool.push_back(r.a);
     pool.push_back(r.b);
   }
@@ -1018,7 +1019,8 @@ fl_profiler::~fl_profiler() {
   for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
-#define FL_LAUNCH_GRID(kernel, grid, block, stream, ...)                           \
+// `units`: the launch's work items for the profile (sites or positions).
+#define FL_LAUNCH_GRID(kernel, grid, block, units, stream, ...)                    \
   do {                                                                            \
     fl_profiler* prof_ = fl_current_profiler;                                     \
     if (prof_) {                                                                  \
@@ -1026,16 +1028,18 @@ fl_profiler::~fl_profiler() {
       (void)hipEventRecord(a_, (stream));                                         \
       hipLaunchKernelGGL(kernel, (grid), (block), 0, (stream), __VA_ARGS__);     \
       (void)hipEventRecord(b_, (stream));                                         \
-      prof_->record((stream), #kernel, a_, b_);                                   \
+      prof_->record((stream), #kernel, a_, b_, (units));                          \
     } else {                                                                      \
       hipLaunchKernelGGL(kernel, (grid), (block), 0, (stream), __VA_ARGS__);     \
     }                                                                             \
   } while (0)
 
-#define FL_LAUNCH(kernel, count, stream, ...)                                              \
-  do {                                                                                     \
-    if ((count) > 0) FL_LAUNCH_GRID(kernel, grid_for((count), 256), dim3(256), (stream), __VA_ARGS__); \
+#define FL_LAUNCH_U(kernel, count, units, stream, ...)                                    \
+  do {                                                                                      \
+    if ((count) > 0)                                                                        \
+      FL_LAUNCH_GRID(kernel, grid_for((count), 256), dim3(256), (units), (stream), __VA_ARGS__); \
   } while (0)
+#define FL_LAUNCH(kernel, count, stream, ...) FL_LAUNCH_U(kernel, count, count, stream, __VA_ARGS__)
 
 void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   FL_LAUNCH(k_receivers, t.n, s, t.n, t.row_ptr, t.col, t.len, w.h, t.is_outlet, w.next, w.dnext, w.flags);
@@ -1073,27 +1077,27 @@ void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_w
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.hc, w.pc, w.jn, w.lw,
+  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc, w.pc, w.jn, w.lw,
                  t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
 }
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
+  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), 0, s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), s, t.n, w.hc, w.jc, w.pc, w.jn,
+  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc, w.jc, w.pc, w.jn,
                  w.lw, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
   FL_LAUNCH(k_chain_perm, t.n, s, t.n, w.jc, w.pc, w.jn, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
             t.area, w.A, w.tau, w.h, w.h2);
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1) {
+                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2) {
   FL_LAUNCH(k_da_pre, p1 - p0, s, p0, p1, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
             w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
   if (c2 > c1)
-    FL_LAUNCH_GRID(k_da_long, dim3(c2 - c1), dim3(64), s, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col,
+    FL_LAUNCH_GRID(k_da_long, dim3(c2 - c1), dim3(64), p2 - p1, s, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col,
                    w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
-  FL_LAUNCH(k_da_chain, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
+  FL_LAUNCH_U(k_da_chain, c1 - c0, p1 - p0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
             t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, w.A);
 }
 
@@ -1103,12 +1107,12 @@ void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                         uint32_t c2, bool clamp) {
+                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp) {
   (void)t;
-  FL_LAUNCH(k_tau_round, c1 - c0, s, c0, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp,
+  FL_LAUNCH_U(k_tau_round, c1 - c0, p1 - p0, s, c0, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp,
             w.dp, w.hop, w.hnp, clamp ? 1 : 0);
   if (c2 > c1)
-    FL_LAUNCH_GRID(k_tau_long, dim3(c2 - c1), dim3(64), s, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp,
+    FL_LAUNCH_GRID(k_tau_long, dim3(c2 - c1), dim3(64), p2 - p1, s, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp,
                    w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp, clamp ? 1 : 0);
 }
 
@@ -1129,6 +1133,6 @@ void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v) 
 
 void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t outlet,
                            uint32_t* order, uint32_t* len_out) {
-  FL_LAUNCH_GRID(k_basin_order, dim3(1), dim3(1024), s, outlet, t.n, t.row_ptr, t.col, w.next,
+  FL_LAUNCH_GRID(k_basin_order, dim3(1), dim3(1024), t.n, s, outlet, t.n, t.row_ptr, t.col, w.next,
                  (const unsigned long long*)w.cmask, order, len_out);
 }

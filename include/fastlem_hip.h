@@ -143,9 +143,11 @@ int32_t fl_terrain_invalidate_cache(fl_terrain *t);
 /* Kernel timing: when enabled, every launch on the context's stream is
  * bracketed by HIP events and its duration accumulated per kernel name. */
 int32_t fl_ctx_set_profiling(fl_ctx *ctx, int32_t enable);
-/* Accumulated launches and milliseconds of kernel `name` since the last
- * reset; fl_ctx_reset_profile clears all counters. */
-int32_t fl_ctx_kernel_time(fl_ctx *ctx, const char *name, uint64_t *launches, double *total_ms);
+/* Accumulated launches, milliseconds and work items (units: sites for the
+ * per-site kernels, chain positions for the sweep rounds; may be NULL) of
+ * kernel `name` since the last reset; fl_ctx_reset_profile clears all
+ * counters. */
+int32_t fl_ctx_kernel_time(fl_ctx *ctx, const char *name, uint64_t *launches, double *total_ms, uint64_t *units);
 int32_t fl_ctx_reset_profile(fl_ctx *ctx);
 
 /* Teacher-forced stages (verification entry points): run one stage on the
