@@ -120,6 +120,32 @@ class DeviceTerrain:
         return h, nxt, da, rt, bool(flag.value)
 
 
+def batch_descs(terrains, keep: list):
+    """``fl_terrain_desc[]`` for a list of terrain dicts (keys row_ptr, col,
+    length, area, erodibility, uplift, init_elevation, outlets and optionally
+    max_slope, max_iteration); ``keep`` holds the arrays alive."""
+    descs = (TerrainDesc * len(terrains))()
+    for i, tr in enumerate(terrains):
+        class _M:
+            pass
+
+        m = _M()
+        g = (np.ascontiguousarray(tr["row_ptr"], np.uint32), np.ascontiguousarray(tr["col"], np.uint32),
+             np.ascontiguousarray(tr["length"], np.float64))
+        a = np.ascontiguousarray(tr["area"], np.float64)
+        m.graph = (lambda g=g: g)
+        m.num = (lambda g=g: len(g[0]) - 1)
+        m.areas = (lambda a=a: a)
+        n = m.num()
+        params = ParameterArrays(np.zeros(n), tr["erodibility"], tr["uplift"], np.zeros(n, bool),
+                                 tr.get("max_slope"))
+        init = np.ascontiguousarray(tr["init_elevation"], np.float64)
+        outlets = np.ascontiguousarray(tr["outlets"], np.uint32)
+        keep.extend([g, a, init, outlets, params])
+        descs[i] = make_desc(m, params, init, outlets, tr.get("max_iteration", UNBOUNDED), keep)
+    return descs
+
+
 class DeviceBatch:
     """Several terrains iterated together (C ABI ``fl_batch_*``): one launch
     sequence per iteration for all of them; each terrain stops at its own
@@ -130,27 +156,8 @@ class DeviceBatch:
         erodibility, uplift, init_elevation, outlets and optionally max_slope,
         max_iteration."""
         self._keep: list = []
-        descs = (TerrainDesc * len(terrains))()
-        self.sizes = []
-        for i, tr in enumerate(terrains):
-            class _M:
-                pass
-
-            m = _M()
-            g = (np.ascontiguousarray(tr["row_ptr"], np.uint32), np.ascontiguousarray(tr["col"], np.uint32),
-                 np.ascontiguousarray(tr["length"], np.float64))
-            a = np.ascontiguousarray(tr["area"], np.float64)
-            m.graph = (lambda g=g: g)
-            m.num = (lambda g=g: len(g[0]) - 1)
-            m.areas = (lambda a=a: a)
-            n = m.num()
-            params = ParameterArrays(np.zeros(n), tr["erodibility"], tr["uplift"], np.zeros(n, bool),
-                                     tr.get("max_slope"))
-            init = np.ascontiguousarray(tr["init_elevation"], np.float64)
-            outlets = np.ascontiguousarray(tr["outlets"], np.uint32)
-            self._keep.extend([g, a, init, outlets, params])
-            descs[i] = make_desc(m, params, init, outlets, tr.get("max_iteration", UNBOUNDED), self._keep)
-            self.sizes.append(n)
+        descs = batch_descs(terrains, self._keep)
+        self.sizes = [int(d.n) for d in descs]
         self._descs = descs
         self._ctx = context()
         h = ctypes.c_void_p()

@@ -1,12 +1,14 @@
 // Exercises the C++ host API (include/fastlem.hpp) the way the reference's
 // own tests use TerrainGenerator (tests/landscape_evolution.rs:8-54).
 //   generate_example errors          -> the three GenerationError cases
-//   generate_example run IN OUT      -> IN: n, nnz, row_ptr, col, length,
+//   generate_example run IN OUT [K]  -> IN: n, nnz, row_ptr, col, length,
 //                                       area, is_outlet(u8), erodibility,
 //                                       max_slope (NaN = None) as raw arrays;
-//                                       OUT: iterations (u32) + elevations
+//                                       OUT: iterations (u32) + elevations;
+//                                       K: set_max_iteration(K)
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -65,6 +67,7 @@ int main(int argc, char** argv) {
                            .set_is_outlet(is_outlet[i] != 0)
                            .set_max_slope(std::isnan(ms[i]) ? std::optional<double>() : std::optional<double>(ms[i])));
     auto gen = TerrainGenerator<>().set_model(model).set_parameters(params);
+    if (argc >= 5) gen = gen.set_max_iteration((uint32_t)std::strtoul(argv[4], nullptr, 10));
     auto terrain = gen.generate().unwrap();
     std::ofstream o(argv[3], std::ios::binary);
     uint32_t it = gen.last_iterations();

@@ -27,28 +27,56 @@ def test_cpp_generation_errors(example_bin):
     assert "errors 4/4" in r.stdout
 
 
-@pytest.mark.gpu
-def test_cpp_generate_matches_oracle(example_bin, plain10k, tmp_path):
-    from oracle import flo
-
-    w = plain10k
+def _write_input(path, w, ms):
     rp, col, ln = w.model.graph()
-    n = w.n
-    ms = np.full(n, np.nan)
-    ms[: n // 2] = 0.3  # half the sites have a max slope
-    inp = tmp_path / "in.bin"
-    with open(inp, "wb") as f:
-        f.write(np.array([n, len(col)], np.uint64).tobytes())
+    with open(path, "wb") as f:
+        f.write(np.array([w.n, len(col)], np.uint64).tobytes())
         for a, t in ((rp, np.uint32), (col, np.uint32), (ln, np.float64), (w.model.areas(), np.float64),
                      (w.is_outlet, np.uint8), (w.erodibility, np.float64), (ms, np.float64)):
             f.write(np.ascontiguousarray(a, t).tobytes())
-    out = tmp_path / "out.bin"
-    r = subprocess.run([example_bin, "run", str(inp), str(out)], capture_output=True, text=True, timeout=120)
+
+
+def _run_example(example_bin, tmp_path, w, ms, cap=None):
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_input(inp, w, ms)
+    args = [example_bin, "run", str(inp), str(out)] + ([str(cap)] if cap is not None else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = open(out, "rb").read()
-    iters = int(np.frombuffer(raw[:4], np.uint32)[0])
-    got = np.frombuffer(raw[4:], np.float64)
+    return np.frombuffer(raw[4:], np.float64), int(np.frombuffer(raw[:4], np.uint32)[0])
+
+
+@pytest.mark.gpu
+def test_cpp_generate_matches_oracle(example_bin, tmp_path):
+    """A 2000-site terrain, half the sites with a 0.3 rad maximum slope: the
+    reference converges (34 iterations)."""
+    from fastlem_amd.workloads import config_plain
+    from oracle import flo
+
+    w = config_plain(2000)
+    ms = np.full(w.n, np.nan)
+    ms[: w.n // 2] = 0.3
+    got, iters = _run_example(example_bin, tmp_path, w, ms)
     ref, ref_iters, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets(),
                                      max_slope=ms)
     assert iters == ref_iters
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_cpp_generate_capped_oscillation(example_bin, plain10k, tmp_path):
+    """On the 10000-site terrain with the same slopes the reference never
+    converges: from iteration 82 the elevations alternate with period 2, so
+    generate() without max_iteration would not return.  With
+    set_max_iteration(100) both stop after 100 iterations, bit-identical."""
+    from oracle import flo
+
+    w = plain10k
+    n = w.n
+    ms = np.full(n, np.nan)
+    ms[: n // 2] = 0.3  # half the sites have a max slope
+    got, iters = _run_example(example_bin, tmp_path, w, ms, cap=100)
+    ref, ref_iters, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets(),
+                                     max_slope=ms, max_iteration=100)
+    assert iters == ref_iters == 100
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))

@@ -1,0 +1,94 @@
+"""The kernel and host sources compiled for the CPU (tests/cpu_emu: g++
+against an emulation of the HIP subset they use) and run through the C ABI
+against the oracle.  This checks the kernels' logic where no GPU is present;
+it says nothing about speed, and races between concurrently running threads
+are not exercised (workgroups run one after another).  The product never
+loads this build."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import graph_of, init_of
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
+    from cpu_emu import emu
+
+    return emu.lib()
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def _oracle(w, ms=None, cap=0xFFFFFFFF):
+    from oracle import flo
+
+    return flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets(),
+                        max_slope=ms, max_iteration=cap)
+
+
+@pytest.mark.parametrize("kind", ["plain", "plain-max-slope", "fbm", "strip-max-slope"])
+def test_emulated_generate_matches_oracle(emu_lib, kind):
+    from cpu_emu import emu
+    from fastlem_amd.workloads import config_fbm, config_plain, config_strip
+
+    w = {"plain": lambda: config_plain(2000), "plain-max-slope": lambda: config_plain(2000),
+         "fbm": lambda: config_fbm(3000, 55.0, seed=5), "strip-max-slope": lambda: config_strip(48, 24)}[kind]()
+    ms = None
+    if kind.endswith("max-slope"):
+        ms = np.full(w.n, np.nan)
+        ms[: w.n // 2] = 0.3
+    got, it = emu.generate(emu_lib, w, init_of(w), ms)
+    ref, rit, _ = _oracle(w, ms)
+    assert it == rit
+    assert _bits_equal(got, ref)
+
+
+def test_emulated_capped_oscillation(emu_lib):
+    """A terrain on which the reference alternates between two states
+    forever (from iteration 82): only max_iteration ends the loop."""
+    from cpu_emu import emu
+    from fastlem_amd.workloads import config_plain
+
+    w = config_plain(10000)
+    ms = np.full(w.n, np.nan)
+    ms[: w.n // 2] = 0.3
+    got, it = emu.generate(emu_lib, w, init_of(w), ms, max_iteration=90)
+    ref, rit, _ = _oracle(w, ms, cap=90)
+    assert it == rit == 90
+    assert _bits_equal(got, ref)
+
+
+def test_emulated_batch_matches_solo(emu_lib):
+    """The batched union (per-terrain convergence, caps and freeze masks)
+    against each terrain's oracle run."""
+    from cpu_emu import emu
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceBatch, batch_descs
+    from fastlem_amd.workloads import config_fbm, config_plain
+
+    ws = [config_plain(1500), config_fbm(2500, 50.0, seed=9), config_plain(800, 60.0)]
+    caps = [0xFFFFFFFF, 0xFFFFFFFF, 3]
+    terrs = [DeviceBatch.terrain_of(w, max_iteration=c) for w, c in zip(ws, caps)]
+    keep = []
+    descs = batch_descs(terrs, keep)
+    L = emu_lib
+    ctx = ctypes.c_void_p()
+    emu.check(L, L.fl_ctx_create(0, ctypes.byref(ctx)))
+    b = ctypes.c_void_p()
+    try:
+        emu.check(L, L.fl_batch_create(ctx, descs, len(terrs), ctypes.byref(b)))
+        its = np.zeros(len(terrs), np.uint32)
+        emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
+        for i, (w, cap) in enumerate(zip(ws, caps)):
+            out = np.empty(w.n)
+            emu.check(L, L.fl_batch_download(ctx, b, i, _native.ptr(out, ctypes.c_double)))
+            ref, rit, _ = _oracle(w, cap=cap)
+            assert its[i] == rit
+            assert _bits_equal(out, ref)
+        L.fl_terrain_destroy(b)
+    finally:
+        L.fl_ctx_destroy(ctx)
