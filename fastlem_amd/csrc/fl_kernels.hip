@@ -13,11 +13,15 @@
 //   tree decomposition (heavy paths, rebuilt every iteration):
 //     k_children        children of each site in row (slot) order
 //     k_chain_init/jump chain head, position in chain, light depth, root
-//     k_chain_*         chains bucketed by light depth, laid out contiguously
+//     k_bucket_*        chains bucketed by light depth, laid out contiguously
+//     k_chain_perm      sites scattered to chain-major positions
 //   sweeps, one round per light depth (deepest first for drainage area)
-//     k_da_short/long   generator.rs:154-159  drainage area, exact fold order
+//     k_da_pre          generator.rs:154-159  light-children part of each
+//                                             site's fold, in parallel
+//     k_da_fold                               drainage area along the chains,
+//                                             exact fold order
 //     k_tau_prep        generator.rs:172      1/(K*A^0.5) * distance
-//     k_tau_round/long  generator.rs:162-203  response time (+ clamped
+//     k_tau_fold        generator.rs:162-203  response time (+ clamped
 //                                             elevation), head to tail
 //     k_tau_scatter     generator.rs:177-202  elevation, changed flag
 //
@@ -501,8 +505,9 @@ __device__ __forceinline__ void da_partial(uint32_t k, const uint32_t* __restric
     if (!after) {
       acc = acc + v;
     } else {
-      if (np == 0) pa = v;
-      else if (np == 1) pb = v;
+      // selects, not an indexed store: keeps pa and pb in registers
+      pb = (np == 1) ? v : pb;
+      pa = (np == 0) ? v : pa;
       ++np;
     }
   }
@@ -510,7 +515,7 @@ __device__ __forceinline__ void da_partial(uint32_t k, const uint32_t* __restric
 }
 
 // The light children after the heavy one beyond the first two (rare).
-__device__ __noinline__ double da_post_rest(double r, uint32_t k, const uint32_t* __restrict__ rp,
+__device__ __forceinline__ double da_post_rest(double r, uint32_t k, const uint32_t* __restrict__ rp,
                                             const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
                                             const unsigned long long* __restrict__ cmask,
                                             const uint32_t* __restrict__ hc, const double* __restrict__ A) {
@@ -552,110 +557,170 @@ __global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ 
   npost[P] = np;
 }
 
-// Drainage area, short chains, part 2: one lane per chain, tail to head:
-// A = pre (tail), A = (pre + A_heavy) + post0 + post1 + ... (others), with
-// the chain's inputs (contiguous) loaded before the dependent adds.
-__global__ void k_da_chain(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
-                           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
-                           const double* __restrict__ pre, const double* __restrict__ post0,
-                           const double* __restrict__ post1, const uint32_t* __restrict__ npost,
-                           const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                           const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-                           const uint32_t* __restrict__ hc, double* __restrict__ A) {
-  uint32_t c = c0 + gid();
-  if (c >= c1) return;
-  uint32_t base = chain_base[c], len = chain_len[c];
-  double vp[FL_LONG], va[FL_LONG], vb[FL_LONG];
-  uint32_t vn[FL_LONG], vk[FL_LONG];
-#pragma unroll
-  for (uint32_t q = 0; q < FL_LONG; ++q) {
-    if (q < len) {
-      uint32_t P = base + q;
-      vp[q] = pre[P];
-      va[q] = post0[P];
-      vb[q] = post1[P];
-      vn[q] = npost[P];
-      vk[q] = perm[P];
-    }
+// Drainage-area fold of chain positions [cs, end) (tail first) from the
+// partial folds in memory, for blocks holding the rare position with more
+// than two light children after the heavy one.
+__device__ __forceinline__ double da_fold_mem(double r, uint32_t base, uint32_t len, uint32_t cs, uint32_t end,
+                                              const uint32_t* __restrict__ perm, const double* __restrict__ pre,
+                                              const double* __restrict__ post0, const double* __restrict__ post1,
+                                              const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
+                                              const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                              const unsigned long long* __restrict__ cmask,
+                                              const uint32_t* __restrict__ hc, double* __restrict__ A) {
+  for (uint32_t q = end; q-- > cs;) {
+    uint32_t P = base + q;
+    double p = pre[P];
+    uint32_t np = npost[P], k = perm[P];
+    r = (q == len - 1) ? p : (p + r);
+    if (np >= 1) r = r + post0[P];
+    if (np >= 2) r = r + post1[P];
+    if (np >= 3) r = da_post_rest(r, k, rp, col, next, cmask, hc, A);
+    A[k] = r;
   }
-  double r = 0.0;
-#pragma unroll
-  for (uint32_t qq = 0; qq < FL_LONG; ++qq) {
-    uint32_t q = FL_LONG - 1 - qq;
-    if (q < len) {
-      r = (q == len - 1) ? vp[q] : (vp[q] + r);  // the tail has no heavy child
-      if (vn[q] >= 1) r = r + va[q];
-      if (vn[q] >= 2) r = r + vb[q];
-      if (vn[q] >= 3) r = da_post_rest(r, vk[q], rp, col, next, cmask, hc, A);
-      A[vk[q]] = r;
-    }
-  }
+  return r;
 }
 
-namespace {
-__device__ __forceinline__ double bcast_f64(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, int lane) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
-}
-}  // namespace
+#define FL_DA_BLOCK 8
 
-// Drainage area of one long chain per 64-lane wavefront.  Per block of 64
-// positions (from the tail) every lane folds its own site's light children
-// (da_partial); the sequential fold along the chain then runs with lane
-// values broadcast by readlane, each lane keeping its own site's result.
-// The next block's partial folds are computed before the current block's
-// sequential fold, so their memory latency overlaps it.
+// Drainage area, part 2, for one light depth: the sequential fold along each
+// chain, tail to head: A = pre at the tail, A = ((pre + A_heavy) + post0) +
+// post1 + ... elsewhere.  One launch covers the round's long chains (blocks
+// [0, c2 - c1), one 64-lane wavefront each) and its short chains (the
+// remaining blocks, one lane each).
+//   short chain: blocks of FL_DA_BLOCK positions are loaded into registers
+//     before their dependent adds;
+//   long chain: tiles of 64 positions are staged in LDS by the whole
+//     wavefront (the next tile's loads in flight meanwhile) and folded by
+//     lane 0 eight positions at a time; the wavefront stores the results.
 __global__ void __launch_bounds__(64)
-k_da_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A) {
-  uint32_t c = c0 + blockIdx.x;
-  uint32_t lane = threadIdx.x;
-  uint32_t base = chain_base[c], len = chain_len[c];
-  // current block [cs, end)
-  uint32_t end = len, cs = end > 64 ? end - 64 : 0;
-  uint32_t k = FL_NONE, np = 0;
-  double pre = 0.0, pa = 0.0, pb = 0.0;
-  if (lane < end - cs) {
-    k = perm[base + cs + lane];
-    da_partial(k, rp, col, next, cmask, hc, area, A, pre, pa, pb, np);
+k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ chain_base,
+          const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm, const double* __restrict__ pre,
+          const double* __restrict__ post0, const double* __restrict__ post1, const uint32_t* __restrict__ npost,
+          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A) {
+  __shared__ double s_p[64], s_a[64], s_b[64], s_r[64];
+  __shared__ uint32_t s_n[64], s_k[64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nlong = c2 - c1;
+  if (blockIdx.x >= nlong) {
+    uint32_t c = c0 + (blockIdx.x - nlong) * 64u + lane;
+    if (c >= c1) return;
+    uint32_t base = chain_base[c], len = chain_len[c];
+    double r = 0.0;
+    uint32_t end = len;
+    while (end > 0) {
+      uint32_t cs = end > FL_DA_BLOCK ? end - FL_DA_BLOCK : 0, cnt = end - cs;
+      double vp[FL_DA_BLOCK], va[FL_DA_BLOCK], vb[FL_DA_BLOCK];
+      uint32_t vn[FL_DA_BLOCK], vk[FL_DA_BLOCK];
+      bool rare = false;
+#pragma unroll
+      for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
+        if (u < cnt) {
+          uint32_t P = base + end - 1 - u;
+          vp[u] = pre[P];
+          va[u] = post0[P];
+          vb[u] = post1[P];
+          vn[u] = npost[P];
+          vk[u] = perm[P];
+          rare |= vn[u] >= 3;
+        }
+      }
+      if (!rare) {
+#pragma unroll
+        for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
+          if (u < cnt) {
+            r = (end - 1 - u == len - 1) ? vp[u] : (vp[u] + r);
+            if (vn[u] >= 1) r = r + va[u];
+            if (vn[u] >= 2) r = r + vb[u];
+            A[vk[u]] = r;
+          }
+        }
+      } else {
+        r = da_fold_mem(r, base, len, cs, end, perm, pre, post0, post1, npost, rp, col, next, cmask, hc, A);
+      }
+      end = cs;
+    }
+    return;
   }
-  double r = 0.0;
-  bool first = true;
+  const uint32_t c = c1 + blockIdx.x;
+  const uint32_t base = chain_base[c], len = chain_len[c];
+  uint32_t end = len, cs = end > 64 ? end - 64 : 0;
+  double p = 0.0, a = 0.0, b = 0.0;
+  uint32_t np = 0, k = FL_NONE;
+  if (cs + lane < end) {
+    uint32_t P = base + cs + lane;
+    p = pre[P];
+    a = post0[P];
+    b = post1[P];
+    np = npost[P];
+    k = perm[P];
+  }
+  double r = 0.0;  // lane 0's running value
   while (end > 0) {
-    uint32_t cnt = end - cs;
-    // prefetch: partial folds of the next block down the chain
-    uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
-    uint32_t nk = FL_NONE, nnp = 0;
-    double npre = 0.0, npa = 0.0, npb = 0.0;
-    if (nend > 0 && lane < nend - ncs) {
-      nk = perm[base + ncs + lane];
-      da_partial(nk, rp, col, next, cmask, hc, area, A, npre, npa, npb, nnp);
+    const uint32_t cnt = end - cs;
+    s_p[lane] = p;
+    s_a[lane] = a;
+    s_b[lane] = b;
+    s_n[lane] = np;
+    s_k[lane] = k;
+    __syncthreads();
+    const uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
+    p = a = b = 0.0;
+    np = 0;
+    k = FL_NONE;
+    if (ncs + lane < nend) {
+      uint32_t P = base + ncs + lane;
+      p = pre[P];
+      a = post0[P];
+      b = post1[P];
+      np = npost[P];
+      k = perm[P];
     }
-    double mine = 0.0;
-    for (int j = (int)cnt - 1; j >= 0; --j) {
-      double pj = bcast_f64(pre, j);
-      r = first ? pj : (pj + r);
-      first = false;
-      uint32_t npj = bcast_u32(np, j);
-      if (npj >= 1) r = r + bcast_f64(pa, j);
-      if (npj >= 2) r = r + bcast_f64(pb, j);
-      if (npj >= 3) r = da_post_rest(r, bcast_u32(k, j), rp, col, next, cmask, hc, A);
-      if ((int)lane == j) mine = r;
+    if (lane == 0) {
+      int j = (int)cnt - 1;
+      while (j >= 0) {
+        const int m = j + 1 < 8 ? j + 1 : 8;  // tile entries j, j-1, ..., j-m+1
+        double tp[8], ta[8], tb[8];
+        uint32_t tn[8];
+        bool rare = false;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (u < m) {
+            tp[u] = s_p[j - u];
+            ta[u] = s_a[j - u];
+            tb[u] = s_b[j - u];
+            tn[u] = s_n[j - u];
+            rare |= tn[u] >= 3;
+          }
+        }
+        if (!rare) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (u < m) {
+              r = (cs + (uint32_t)(j - u) == len - 1) ? tp[u] : (tp[u] + r);
+              if (tn[u] >= 1) r = r + ta[u];
+              if (tn[u] >= 2) r = r + tb[u];
+              s_r[j - u] = r;
+            }
+          }
+        } else {
+          for (int u = 0; u < m; ++u) {
+            int e = j - u;
+            r = (cs + (uint32_t)e == len - 1) ? s_p[e] : (s_p[e] + r);
+            uint32_t ne = s_n[e];
+            if (ne >= 1) r = r + s_a[e];
+            if (ne >= 2) r = r + s_b[e];
+            if (ne >= 3) r = da_post_rest(r, s_k[e], rp, col, next, cmask, hc, A);
+            s_r[e] = r;
+          }
+        }
+        j -= m;
+      }
     }
-    if (lane < cnt) A[k] = mine;
+    __syncthreads();
+    if (lane < cnt) A[s_k[lane]] = s_r[lane];
     end = nend;
     cs = ncs;
-    k = nk;
-    np = nnp;
-    pre = npre;
-    pa = npa;
-    pb = npb;
   }
 }
 
@@ -686,105 +751,104 @@ __global__ void k_tau_prep(uint32_t npos, const uint32_t* __restrict__ perm, con
 // the basin outlet first (its receiver is itself).  A head's receiver lies
 // on a chain finished in an earlier round.  With clamp, also the elevation
 // recurrence of generator.rs:177-202, whose max-slope limit depends on the
-// receiver's new elevation.
-__global__ void k_tau_round(uint32_t c0, uint32_t c1, const uint32_t* __restrict__ chain_base,
-                            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
-                            const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
-                            double* __restrict__ taup, const double* __restrict__ up,
-                            const double* __restrict__ tanp, const double* __restrict__ dp,
-                            const double* __restrict__ hop, double* __restrict__ hnp, int clamp) {
-  uint32_t c = c0 + gid();
-  if (c >= c1) return;
-  uint32_t base = chain_base[c], len = chain_len[c];
-  uint32_t pp = ppos[base];
-  double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
-  uint32_t t = 0;
-  if (pp == FL_NONE) {
-    tau = 0.0 + (0.0 + xp[base]);
-    taup[base] = tau;
-    if (clamp) {
-      double ho = hop[base];
-      double nh = ho + up[base] * rust_max(tau - tau, 0.0);
-      double tm = tanp[base], d = dp[base];
-      double slope = (nh - ho) / d;  // the receiver is the outlet itself, not yet updated
-      if (slope > tm) nh = ho + tm * d;
-      hnp[base] = nh;
-      h_p = nh;
-      h_o = nh;
-      tau_o = tau;
-    }
-    t = 1;
-  } else {
-    tau = taup[pp];
-    if (clamp) {
-      uint32_t ro = rpos[base];
-      tau_o = taup[ro];
-      h_o = hnp[ro];
-      h_p = hnp[pp];
-    }
-  }
-  if (!clamp) {
-    while (t < len) {
-      uint32_t cnt = (len - t) < 16 ? (len - t) : 16;
-      double vx[16];
-#pragma unroll
-      for (uint32_t q = 0; q < 16; ++q)
-        if (q < cnt) vx[q] = xp[base + t + q];
-#pragma unroll
-      for (uint32_t q = 0; q < 16; ++q) {
-        if (q < cnt) {
-          tau = 0.0 + (tau + vx[q]);
-          taup[base + t + q] = tau;
-        }
-      }
-      t += cnt;
-    }
-  } else {
-    while (t < len) {
-      uint32_t cnt = (len - t) < FL_BATCH ? (len - t) : FL_BATCH;
-      double vx[FL_BATCH], vu[FL_BATCH], vt[FL_BATCH], vd[FL_BATCH];
-#pragma unroll
-      for (uint32_t q = 0; q < FL_BATCH; ++q) {
-        if (q < cnt) {
-          uint32_t P = base + t + q;
-          vx[q] = xp[P];
-          vu[q] = up[P];
-          vt[q] = tanp[P];
-          vd[q] = dp[P];
-        }
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < FL_BATCH; ++q) {
-        if (q < cnt) {
-          tau = 0.0 + (tau + vx[q]);
-          double nh = h_o + vu[q] * rust_max(tau - tau_o, 0.0);
-          double slope = (nh - h_p) / vd[q];
-          if (slope > vt[q]) nh = h_p + vt[q] * vd[q];
-          taup[base + t + q] = tau;
-          hnp[base + t + q] = nh;
-          h_p = nh;
-        }
-      }
-      t += cnt;
-    }
-  }
-}
-
-// Response time (and, with clamp, elevation) of one long chain per 64-lane
-// wavefront, head to tail: lanes load 64 contiguous positions, the
-// sequential recurrence of k_tau_round runs on readlane broadcasts, each
-// lane keeps its own position's result; the next block is loaded first.
+// receiver's new elevation.  Long chains [c1, c2) take one wavefront each
+// (blocks [0, c2 - c1)): tiles of 64 positions are staged in LDS, lane 0
+// runs the recurrence, the wavefront stores; short chains [c0, c1) one lane
+// each (the remaining blocks).
+template <bool clamp>
 __global__ void __launch_bounds__(64)
-k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-           const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
-           double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
-           const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp, int clamp) {
-  uint32_t c = c0 + blockIdx.x;
-  uint32_t lane = threadIdx.x;
-  uint32_t base = chain_base[c], len = chain_len[c];
-  uint32_t pp = ppos[base];
+k_tau_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ chain_base,
+           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
+           const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
+           const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
+           const double* __restrict__ hop, double* __restrict__ hnp) {
+  __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nlong = c2 - c1;
+  if (blockIdx.x >= nlong) {
+    uint32_t c = c0 + (blockIdx.x - nlong) * 64u + lane;
+    if (c >= c1) return;
+    uint32_t base = chain_base[c], len = chain_len[c];
+    uint32_t pp = ppos[base];
+    double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
+    uint32_t t = 0;
+    if (pp == FL_NONE) {
+      tau = 0.0 + (0.0 + xp[base]);
+      taup[base] = tau;
+      if (clamp) {
+        double ho = hop[base];
+        double nh = ho + up[base] * rust_max(tau - tau, 0.0);
+        double tm = tanp[base], d = dp[base];
+        double slope = (nh - ho) / d;  // the receiver is the outlet itself, not yet updated
+        if (slope > tm) nh = ho + tm * d;
+        hnp[base] = nh;
+        h_p = nh;
+        h_o = nh;
+        tau_o = tau;
+      }
+      t = 1;
+    } else {
+      tau = taup[pp];
+      if (clamp) {
+        uint32_t ro = rpos[base];
+        tau_o = taup[ro];
+        h_o = hnp[ro];
+        h_p = hnp[pp];
+      }
+    }
+    if (!clamp) {
+      while (t < len) {
+        uint32_t cnt = (len - t) < FL_BATCH ? (len - t) : FL_BATCH;
+        double vx[FL_BATCH];
+#pragma unroll
+        for (uint32_t q = 0; q < FL_BATCH; ++q)
+          if (q < cnt) vx[q] = xp[base + t + q];
+#pragma unroll
+        for (uint32_t q = 0; q < FL_BATCH; ++q) {
+          if (q < cnt) {
+            tau = 0.0 + (tau + vx[q]);
+            taup[base + t + q] = tau;
+          }
+        }
+        t += cnt;
+      }
+    } else {
+      while (t < len) {
+        uint32_t cnt = (len - t) < FL_BATCH ? (len - t) : FL_BATCH;
+        double vx[FL_BATCH], vu[FL_BATCH], vt[FL_BATCH], vd[FL_BATCH];
+#pragma unroll
+        for (uint32_t q = 0; q < FL_BATCH; ++q) {
+          if (q < cnt) {
+            uint32_t P = base + t + q;
+            vx[q] = xp[P];
+            vu[q] = up[P];
+            vt[q] = tanp[P];
+            vd[q] = dp[P];
+          }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < FL_BATCH; ++q) {
+          if (q < cnt) {
+            tau = 0.0 + (tau + vx[q]);
+            double nh = h_o + vu[q] * rust_max(tau - tau_o, 0.0);
+            double slope = (nh - h_p) / vd[q];
+            if (slope > vt[q]) nh = h_p + vt[q] * vd[q];
+            taup[base + t + q] = tau;
+            hnp[base + t + q] = nh;
+            h_p = nh;
+          }
+        }
+        t += cnt;
+      }
+    }
+    return;
+  }
+  const uint32_t c = c1 + blockIdx.x;
+  const uint32_t base = chain_base[c], len = chain_len[c];
+  // lane 0's recurrence state
   double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
-  if (pp != FL_NONE) {
+  const uint32_t pp = ppos[base];
+  if (lane == 0 && pp != FL_NONE) {
     tau = taup[pp];
     if (clamp) {
       uint32_t ro = rpos[base];
@@ -803,64 +867,92 @@ k_tau_long(uint32_t c0, const uint32_t* __restrict__ chain_base, const uint32_t*
     }
   }
   for (uint32_t t = 0; t < len; t += 64) {
-    uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
-    double nx = 0.0, nu = 0.0, ntn = 0.0, nd = 0.0;
+    const uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
+    s_x[lane] = x;
+    if (clamp) {
+      s_u[lane] = u;
+      s_t[lane] = tn;
+      s_d[lane] = d;
+    }
+    __syncthreads();
+    x = u = tn = d = 0.0;
     if (t + 64 + lane < len) {
       uint32_t P = base + t + 64 + lane;
-      nx = xp[P];
+      x = xp[P];
       if (clamp) {
-        nu = up[P];
-        ntn = tanp[P];
-        nd = dp[P];
+        u = up[P];
+        tn = tanp[P];
+        d = dp[P];
       }
     }
-    double my_tau = 0.0, my_h = 0.0;
-    int j = 0;
-    if (t == 0 && pp == FL_NONE) {
-      // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
-      double x0 = bcast_f64(x, 0);
-      tau = 0.0 + (0.0 + x0);
-      if (clamp) {
-        double ho = hop[base];
-        double u0 = bcast_f64(u, 0), d0 = bcast_f64(d, 0), t0 = bcast_f64(tn, 0);
-        double nh = ho + u0 * rust_max(tau - tau, 0.0);
-        double slope = (nh - ho) / d0;
-        if (slope > t0) nh = ho + t0 * d0;
-        h_p = nh;
-        h_o = nh;
-        tau_o = tau;
-        if (lane == 0) my_h = nh;
+    if (lane == 0) {
+      uint32_t j = 0;
+      if (t == 0 && pp == FL_NONE) {
+        // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
+        tau = 0.0 + (0.0 + s_x[0]);
+        s_tau[0] = tau;
+        if (clamp) {
+          double ho = hop[base];
+          double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
+          double slope = (nh - ho) / s_d[0];
+          if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
+          s_h[0] = nh;
+          h_p = nh;
+          h_o = nh;
+          tau_o = tau;
+        }
+        j = 1;
       }
-      if (lane == 0) my_tau = tau;
-      j = 1;
-    }
-    if (!clamp) {
-      for (; j < (int)cnt; ++j) {
-        tau = 0.0 + (tau + bcast_f64(x, j));
-        if ((int)lane == j) my_tau = tau;
-      }
-    } else {
-      for (; j < (int)cnt; ++j) {
-        tau = 0.0 + (tau + bcast_f64(x, j));
-        double uj = bcast_f64(u, j), dj = bcast_f64(d, j), tj = bcast_f64(tn, j);
-        double nh = h_o + uj * rust_max(tau - tau_o, 0.0);
-        double slope = (nh - h_p) / dj;
-        if (slope > tj) nh = h_p + tj * dj;
-        h_p = nh;
-        if ((int)lane == j) {
-          my_tau = tau;
-          my_h = nh;
+      if (!clamp) {
+        while (j < cnt) {
+          const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
+          double tx[8];
+#pragma unroll
+          for (uint32_t q = 0; q < 8; ++q)
+            if (q < m) tx[q] = s_x[j + q];
+#pragma unroll
+          for (uint32_t q = 0; q < 8; ++q) {
+            if (q < m) {
+              tau = 0.0 + (tau + tx[q]);
+              s_tau[j + q] = tau;
+            }
+          }
+          j += m;
+        }
+      } else {
+        while (j < cnt) {
+          const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
+          double tx[8], tu[8], tt[8], td[8];
+#pragma unroll
+          for (uint32_t q = 0; q < 8; ++q) {
+            if (q < m) {
+              tx[q] = s_x[j + q];
+              tu[q] = s_u[j + q];
+              tt[q] = s_t[j + q];
+              td[q] = s_d[j + q];
+            }
+          }
+#pragma unroll
+          for (uint32_t q = 0; q < 8; ++q) {
+            if (q < m) {
+              tau = 0.0 + (tau + tx[q]);
+              double nh = h_o + tu[q] * rust_max(tau - tau_o, 0.0);
+              double slope = (nh - h_p) / td[q];
+              if (slope > tt[q]) nh = h_p + tt[q] * td[q];
+              s_tau[j + q] = tau;
+              s_h[j + q] = nh;
+              h_p = nh;
+            }
+          }
+          j += m;
         }
       }
     }
+    __syncthreads();
     if (lane < cnt) {
-      taup[base + t + lane] = my_tau;
-      if (clamp) hnp[base + t + lane] = my_h;
+      taup[base + t + lane] = s_tau[lane];
+      if (clamp) hnp[base + t + lane] = s_h[lane];
     }
-    x = nx;
-    u = nu;
-    tn = ntn;
-    d = nd;
   }
 }
 
@@ -1092,13 +1184,14 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2) {
-  FL_LAUNCH(k_da_pre, p1 - p0, s, p0, p1, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
+  (void)p1;
+  FL_LAUNCH(k_da_pre, p2 - p0, s, p0, p2, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
             w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
-  if (c2 > c1)
-    FL_LAUNCH_GRID(k_da_long, dim3(c2 - c1), dim3(64), p2 - p1, s, c1, w.chain_base, w.chain_len, w.perm, t.row_ptr, t.col,
-                   w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A);
-  FL_LAUNCH_U(k_da_chain, c1 - c0, p1 - p0, s, c0, c1, w.chain_base, w.chain_len, w.perm, w.pre, w.post0, w.post1, w.npost,
-            t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, w.A);
+  uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
+  if (blocks)
+    FL_LAUNCH_GRID(k_da_fold, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len, w.perm,
+                   w.pre, w.post0, w.post1, w.npost, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
+                   w.hc, w.A);
 }
 
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
@@ -1109,11 +1202,14 @@ void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
                          uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp) {
   (void)t;
-  FL_LAUNCH_U(k_tau_round, c1 - c0, p1 - p0, s, c0, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp,
-            w.dp, w.hop, w.hnp, clamp ? 1 : 0);
-  if (c2 > c1)
-    FL_LAUNCH_GRID(k_tau_long, dim3(c2 - c1), dim3(64), p2 - p1, s, c1, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp,
-                   w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp, clamp ? 1 : 0);
+  (void)p1;
+  uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
+  if (blocks && clamp)
+    FL_LAUNCH_GRID(k_tau_fold<true>, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len,
+                   w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
+  else if (blocks)
+    FL_LAUNCH_GRID(k_tau_fold<false>, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len,
+                   w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
 }
 
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
