@@ -539,7 +539,8 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   bool clamp = false;
   for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1})
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
+                     &w.post2, &w.post3})
     FLCHK(dev_alloc(P, p, n));
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));

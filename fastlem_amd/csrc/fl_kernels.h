@@ -86,6 +86,8 @@ struct fl_dev_work {
   double* pre = nullptr;      // [n] drainage-area partial folds, chain-major
   double* post0 = nullptr;
   double* post1 = nullptr;
+  double* post2 = nullptr;  // written where a site has three or more posts
+  double* post3 = nullptr;
   uint32_t* npost = nullptr;
 
   uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;

@@ -478,20 +478,28 @@ __device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uin
 // order (generator.rs:154-159 accumulates children in reverse BFS order, so
 // A_k = ((a_k + A_{c_m}) + A_{c_{m-1}}) + ... + A_{c_1} over the children in
 // row order).  The heavy child's value is not known here: the fold is split
-// into `pre` (the part before it) and the light children after it
-// (post0, post1, npost; more than two are re-read by da_post_rest).  Light
+// into `pre` (the part before it) and the light children after it, posts
+// p0..p3 in fold order (more than four are re-read by da_post_rest).  Light
 // children head chains of a deeper round, already finished.
+//
+// Absent posts are -0.0, and the chain fold starts from r = -0.0 at the
+// tail: x + (-0.0) == x exactly for every x (signed zeros included), so
+//     A = ((pre + r) + p0) + p1
+// is the reference's value at every position without a branch; positions
+// with three or more posts (about 1% of sites) add p2, p3, ... after it.
+#define FL_NEG0 (-0.0)
+#define FL_POSTS 4
+
 __device__ __forceinline__ void da_partial(uint32_t k, const uint32_t* __restrict__ rp,
                                            const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
                                            const unsigned long long* __restrict__ cmask,
                                            const uint32_t* __restrict__ hc, const double* __restrict__ area,
-                                           const double* __restrict__ A, double& pre, double& pa, double& pb,
-                                           uint32_t& np) {
+                                           const double* __restrict__ A, double& pre, double& p0, double& p1,
+                                           double& p2, double& p3, uint32_t& np) {
   uint32_t b = rp[k], deg = rp[k + 1] - b, hk = hc[k];
   unsigned long long m = cmask[k];
   double acc = area[k];
-  pa = 0.0;
-  pb = 0.0;
+  p0 = p1 = p2 = p3 = FL_NEG0;
   np = 0;
   bool after = false;
   for (uint32_t s = deg; s-- > 0;) {
@@ -505,20 +513,22 @@ __device__ __forceinline__ void da_partial(uint32_t k, const uint32_t* __restric
     if (!after) {
       acc = acc + v;
     } else {
-      // selects, not an indexed store: keeps pa and pb in registers
-      pb = (np == 1) ? v : pb;
-      pa = (np == 0) ? v : pa;
+      // selects, not an indexed store: keeps the posts in registers
+      p3 = (np == 3) ? v : p3;
+      p2 = (np == 2) ? v : p2;
+      p1 = (np == 1) ? v : p1;
+      p0 = (np == 0) ? v : p0;
       ++np;
     }
   }
   pre = acc;
 }
 
-// The light children after the heavy one beyond the first two (rare).
+// The light children after the heavy one beyond the first FL_POSTS.
 __device__ __forceinline__ double da_post_rest(double r, uint32_t k, const uint32_t* __restrict__ rp,
-                                            const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-                                            const unsigned long long* __restrict__ cmask,
-                                            const uint32_t* __restrict__ hc, const double* __restrict__ A) {
+                                               const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                               const unsigned long long* __restrict__ cmask,
+                                               const uint32_t* __restrict__ hc, const double* __restrict__ A) {
   uint32_t b = rp[k], deg = rp[k + 1] - b, hk = hc[k];
   unsigned long long m = cmask[k];
   bool after = false;
@@ -531,74 +541,71 @@ __device__ __forceinline__ double da_post_rest(double r, uint32_t k, const uint3
       continue;
     }
     if (after) {
-      if (seen >= 2) r = r + A[c];
+      if (seen >= FL_POSTS) r = r + A[c];
       ++seen;
     }
   }
   return r;
 }
 
-// Drainage area, short chains, part 1: the partial fold of every site of
-// the round's short chains (positions [p0, p1)), one thread per site.
+// Drainage area, part 1: the partial fold of every site of the round
+// (positions [p0, p1)), one thread per site.  post2/post3 are written only
+// where they are used (three or more posts).
 __global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ perm,
                          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
                          const uint32_t* __restrict__ hc, const double* __restrict__ area,
                          const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
-                         double* __restrict__ post1, uint32_t* __restrict__ npost) {
+                         double* __restrict__ post1, double* __restrict__ post2, double* __restrict__ post3,
+                         uint32_t* __restrict__ npost) {
   uint32_t P = p0 + gid();
   if (P >= p1) return;
-  double pr, pa, pb;
+  double pr, a, b, c, d;
   uint32_t np;
-  da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, pa, pb, np);
+  da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, a, b, c, d, np);
   pre[P] = pr;
-  post0[P] = pa;
-  post1[P] = pb;
+  post0[P] = a;
+  post1[P] = b;
   npost[P] = np;
+  if (np >= 3) {
+    post2[P] = c;
+    post3[P] = d;
+  }
 }
 
-// Drainage-area fold of chain positions [cs, end) (tail first) from the
-// partial folds in memory, for blocks holding the rare position with more
-// than two light children after the heavy one.
-__device__ __forceinline__ double da_fold_mem(double r, uint32_t base, uint32_t len, uint32_t cs, uint32_t end,
-                                              const uint32_t* __restrict__ perm, const double* __restrict__ pre,
-                                              const double* __restrict__ post0, const double* __restrict__ post1,
-                                              const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
-                                              const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-                                              const unsigned long long* __restrict__ cmask,
-                                              const uint32_t* __restrict__ hc, double* __restrict__ A) {
-  for (uint32_t q = end; q-- > cs;) {
-    uint32_t P = base + q;
-    double p = pre[P];
-    uint32_t np = npost[P], k = perm[P];
-    r = (q == len - 1) ? p : (p + r);
-    if (np >= 1) r = r + post0[P];
-    if (np >= 2) r = r + post1[P];
-    if (np >= 3) r = da_post_rest(r, k, rp, col, next, cmask, hc, A);
-    A[k] = r;
-  }
+// The terms after ((pre + r) + p0) + p1 of a position with np >= 3 posts.
+__device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c, double d, uint32_t k,
+                                                const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                                                const uint32_t* __restrict__ next,
+                                                const unsigned long long* __restrict__ cmask,
+                                                const uint32_t* __restrict__ hc, const double* __restrict__ A) {
+  r = r + c;
+  if (np >= 4) r = r + d;
+  if (np > FL_POSTS) r = da_post_rest(r, k, rp, col, next, cmask, hc, A);
   return r;
 }
 
 #define FL_DA_BLOCK 8
 
 // Drainage area, part 2, for one light depth: the sequential fold along each
-// chain, tail to head: A = pre at the tail, A = ((pre + A_heavy) + post0) +
-// post1 + ... elsewhere.  One launch covers the round's long chains (blocks
+// chain, tail to head.  One launch covers the round's long chains (blocks
 // [0, c2 - c1), one 64-lane wavefront each) and its short chains (the
 // remaining blocks, one lane each).
 //   short chain: blocks of FL_DA_BLOCK positions are loaded into registers
 //     before their dependent adds;
 //   long chain: tiles of 64 positions are staged in LDS by the whole
-//     wavefront (the next tile's loads in flight meanwhile) and folded by
-//     lane 0 eight positions at a time; the wavefront stores the results.
+//     wavefront (the next tile's loads in flight meanwhile); lane 0 folds
+//     them eight at a time (reads first, then the dependent adds), the
+//     positions with three or more posts marked by a ballot; the wavefront
+//     stores the results.
 __global__ void __launch_bounds__(64)
 k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ chain_base,
           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm, const double* __restrict__ pre,
-          const double* __restrict__ post0, const double* __restrict__ post1, const uint32_t* __restrict__ npost,
-          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+          const double* __restrict__ post0, const double* __restrict__ post1, const double* __restrict__ post2,
+          const double* __restrict__ post3, const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
+          const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
           const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A) {
-  __shared__ double s_p[64], s_a[64], s_b[64], s_r[64];
+  __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
   __shared__ uint32_t s_n[64], s_k[64];
   const uint32_t lane = threadIdx.x;
   const uint32_t nlong = c2 - c1;
@@ -606,13 +613,12 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
     uint32_t c = c0 + (blockIdx.x - nlong) * 64u + lane;
     if (c >= c1) return;
     uint32_t base = chain_base[c], len = chain_len[c];
-    double r = 0.0;
+    double r = FL_NEG0;
     uint32_t end = len;
     while (end > 0) {
       uint32_t cs = end > FL_DA_BLOCK ? end - FL_DA_BLOCK : 0, cnt = end - cs;
       double vp[FL_DA_BLOCK], va[FL_DA_BLOCK], vb[FL_DA_BLOCK];
       uint32_t vn[FL_DA_BLOCK], vk[FL_DA_BLOCK];
-      bool rare = false;
 #pragma unroll
       for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
         if (u < cnt) {
@@ -622,21 +628,18 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
           vb[u] = post1[P];
           vn[u] = npost[P];
           vk[u] = perm[P];
-          rare |= vn[u] >= 3;
         }
       }
-      if (!rare) {
 #pragma unroll
-        for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
-          if (u < cnt) {
-            r = (end - 1 - u == len - 1) ? vp[u] : (vp[u] + r);
-            if (vn[u] >= 1) r = r + va[u];
-            if (vn[u] >= 2) r = r + vb[u];
-            A[vk[u]] = r;
+      for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
+        if (u < cnt) {
+          r = ((vp[u] + r) + va[u]) + vb[u];
+          if (vn[u] >= 3) {
+            uint32_t P = base + end - 1 - u;
+            r = da_tail_terms(r, vn[u], post2[P], post3[P], vk[u], rp, col, next, cmask, hc, A);
           }
+          A[vk[u]] = r;
         }
-      } else {
-        r = da_fold_mem(r, base, len, cs, end, perm, pre, post0, post1, npost, rp, col, next, cmask, hc, A);
       }
       end = cs;
     }
@@ -645,27 +648,32 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
   const uint32_t c = c1 + blockIdx.x;
   const uint32_t base = chain_base[c], len = chain_len[c];
   uint32_t end = len, cs = end > 64 ? end - 64 : 0;
-  double p = 0.0, a = 0.0, b = 0.0;
+  double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
   uint32_t np = 0, k = FL_NONE;
   if (cs + lane < end) {
     uint32_t P = base + cs + lane;
     p = pre[P];
     a = post0[P];
     b = post1[P];
+    x = post2[P];  // meaningful where np >= 3 only
+    y = post3[P];
     np = npost[P];
     k = perm[P];
   }
-  double r = 0.0;  // lane 0's running value
+  double r = FL_NEG0;  // lane 0's running value
   while (end > 0) {
     const uint32_t cnt = end - cs;
     s_p[lane] = p;
     s_a[lane] = a;
     s_b[lane] = b;
+    s_c[lane] = x;
+    s_d[lane] = y;
     s_n[lane] = np;
     s_k[lane] = k;
+    const unsigned long long rare = __ballot(np >= 3);
     __syncthreads();
     const uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
-    p = a = b = 0.0;
+    p = a = b = x = y = 0.0;
     np = 0;
     k = FL_NONE;
     if (ncs + lane < nend) {
@@ -673,48 +681,33 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
       p = pre[P];
       a = post0[P];
       b = post1[P];
+      x = post2[P];
+      y = post3[P];
       np = npost[P];
       k = perm[P];
     }
     if (lane == 0) {
-      int j = (int)cnt - 1;
-      while (j >= 0) {
-        const int m = j + 1 < 8 ? j + 1 : 8;  // tile entries j, j-1, ..., j-m+1
+      int e = (int)cnt - 1;
+      for (; e >= 7; e -= 8) {
         double tp[8], ta[8], tb[8];
-        uint32_t tn[8];
-        bool rare = false;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          if (u < m) {
-            tp[u] = s_p[j - u];
-            ta[u] = s_a[j - u];
-            tb[u] = s_b[j - u];
-            tn[u] = s_n[j - u];
-            rare |= tn[u] >= 3;
-          }
+          tp[u] = s_p[e - u];
+          ta[u] = s_a[e - u];
+          tb[u] = s_b[e - u];
         }
-        if (!rare) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            if (u < m) {
-              r = (cs + (uint32_t)(j - u) == len - 1) ? tp[u] : (tp[u] + r);
-              if (tn[u] >= 1) r = r + ta[u];
-              if (tn[u] >= 2) r = r + tb[u];
-              s_r[j - u] = r;
-            }
-          }
-        } else {
-          for (int u = 0; u < m; ++u) {
-            int e = j - u;
-            r = (cs + (uint32_t)e == len - 1) ? s_p[e] : (s_p[e] + r);
-            uint32_t ne = s_n[e];
-            if (ne >= 1) r = r + s_a[e];
-            if (ne >= 2) r = r + s_b[e];
-            if (ne >= 3) r = da_post_rest(r, s_k[e], rp, col, next, cmask, hc, A);
-            s_r[e] = r;
-          }
+        for (int u = 0; u < 8; ++u) {
+          r = ((tp[u] + r) + ta[u]) + tb[u];
+          if ((rare >> (e - u)) & 1ull)
+            r = da_tail_terms(r, s_n[e - u], s_c[e - u], s_d[e - u], s_k[e - u], rp, col, next, cmask, hc, A);
+          s_r[e - u] = r;
         }
-        j -= m;
+      }
+      for (; e >= 0; --e) {
+        r = ((s_p[e] + r) + s_a[e]) + s_b[e];
+        if ((rare >> e) & 1ull) r = da_tail_terms(r, s_n[e], s_c[e], s_d[e], s_k[e], rp, col, next, cmask, hc, A);
+        s_r[e] = r;
       }
     }
     __syncthreads();
@@ -1186,12 +1179,12 @@ void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2) {
   (void)p1;
   FL_LAUNCH(k_da_pre, p2 - p0, s, p0, p2, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
-            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.npost);
+            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.post2, w.post3, w.npost);
   uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
   if (blocks)
     FL_LAUNCH_GRID(k_da_fold, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len, w.perm,
-                   w.pre, w.post0, w.post1, w.npost, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
-                   w.hc, w.A);
+                   w.pre, w.post0, w.post1, w.post2, w.post3, w.npost, t.row_ptr, t.col, w.next,
+                   (const unsigned long long*)w.cmask, w.hc, w.A);
 }
 
 void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {

@@ -28,7 +28,7 @@ dim3* g_tid = nullptr;
 namespace {
 
 enum State { RUNNABLE, AT_BARRIER, AT_WAVE, DONE };
-enum WaveOp { OP_ANY, OP_READLANE };
+enum WaveOp { OP_ANY, OP_BALLOT, OP_READLANE };
 
 struct Fiber {
   jmp_buf env;
@@ -38,7 +38,7 @@ struct Fiber {
   WaveOp op = OP_ANY;
   int value = 0;   // deposited operand
   int arg = 0;     // readlane source lane
-  int result = 0;  // result of the wavefront operation
+  unsigned long long result = 0;  // result of the wavefront operation
 };
 
 constexpr size_t kStack = 256 * 1024;
@@ -158,15 +158,18 @@ void run_block(uint32_t nthreads) {
           die("divergent wavefront operation");
         }
       }
-      int result = 0;
+      unsigned long long result = 0;
       if (op == OP_ANY) {
         for (uint32_t t = l0; t < l1; t++)
           if (g_fibers[t].state == AT_WAVE && g_fibers[t].value) result = 1;
+      } else if (op == OP_BALLOT) {
+        for (uint32_t t = l0; t < l1; t++)
+          if (g_fibers[t].state == AT_WAVE && g_fibers[t].value) result |= 1ull << (t - l0);
       } else {
         if (arg < 0 || (uint32_t)arg >= kWave) die("readlane lane out of range");
         uint32_t src = l0 + (uint32_t)arg;
         if (src >= l1 || g_fibers[src].state != AT_WAVE) die("readlane of an inactive lane");
-        result = g_fibers[src].value;
+        result = (unsigned long long)(unsigned)g_fibers[src].value;
       }
       for (uint32_t t = l0; t < l1; t++) {
         Fiber& f = g_fibers[t];
@@ -228,6 +231,14 @@ int wave_any(int pred) {
   g_cur->op = OP_ANY;
   g_cur->value = pred;
   yield_to_scheduler();
+  return (int)g_cur->result;
+}
+
+unsigned long long wave_ballot(int pred) {
+  g_cur->state = AT_WAVE;
+  g_cur->op = OP_BALLOT;
+  g_cur->value = pred;
+  yield_to_scheduler();
   return g_cur->result;
 }
 
@@ -237,7 +248,7 @@ int readlane(int v, int lane) {
   g_cur->value = v;
   g_cur->arg = lane;
   yield_to_scheduler();
-  return g_cur->result;
+  return (int)(unsigned)g_cur->result;
 }
 
 }  // namespace flemu

@@ -7,7 +7,7 @@
 // Model: device memory is host memory; every call is synchronous; a kernel
 // launch runs its workgroups one after another, each workgroup's threads as
 // cooperative fibers.  __syncthreads() waits for every live thread of the
-// workgroup; __any() and readlane wait for every live lane of the 64-lane
+// workgroup; __any(), __ballot() and readlane wait for every live lane of the 64-lane
 // wavefront (and must be reached by all of them, as in the kernels'
 // uniform control flow).  __shared__ arrays become function statics, shared
 // by the fibers of the running workgroup.  Atomics are plain read-modify-
@@ -63,6 +63,7 @@ extern dim3* g_tid;
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
 void block_barrier();
 int wave_any(int pred);
+unsigned long long wave_ballot(int pred);
 int readlane(int v, int lane);
 }  // namespace flemu
 
@@ -72,6 +73,7 @@ int readlane(int v, int lane);
 #define gridDim (::flemu::g_gdim)
 #define __syncthreads() ::flemu::block_barrier()
 #define __any(p) ::flemu::wave_any((p) ? 1 : 0)
+#define __ballot(p) ::flemu::wave_ballot((p) ? 1 : 0)
 #define __builtin_amdgcn_readlane(v, l) ::flemu::readlane((int)(v), (int)(l))
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
   ::flemu::launch(dim3(grid), dim3(block), [=]() { kernel(__VA_ARGS__); })

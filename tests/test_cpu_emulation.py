@@ -62,33 +62,57 @@ def test_emulated_capped_oscillation(emu_lib):
     assert _bits_equal(got, ref)
 
 
+def _emu_solve(L, terrs):
+    """fl_batch_create/solve/download through the emulated library."""
+    from cpu_emu import emu
+    from fastlem_amd import _native
+    from fastlem_amd.device import batch_descs
+
+    keep = []
+    descs = batch_descs(terrs, keep)
+    ctx = ctypes.c_void_p()
+    emu.check(L, L.fl_ctx_create(0, ctypes.byref(ctx)))
+    b = ctypes.c_void_p()
+    outs = []
+    try:
+        emu.check(L, L.fl_batch_create(ctx, descs, len(terrs), ctypes.byref(b)))
+        its = np.zeros(len(terrs), np.uint32)
+        emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
+        for i, t in enumerate(terrs):
+            out = np.empty(len(t["row_ptr"]) - 1)
+            emu.check(L, L.fl_batch_download(ctx, b, i, _native.ptr(out, ctypes.c_double)))
+            outs.append(out)
+        L.fl_terrain_destroy(b)
+    finally:
+        L.fl_ctx_destroy(ctx)
+    return outs, its
+
+
+def test_emulated_comb_many_light_children(emu_lib):
+    """Every site of a 24-site heavy chain (the long-chain path) folds six
+    light children after its heavy child, beyond the four kept per site."""
+    from conftest import comb_terrain
+    from oracle import flo
+
+    t = comb_terrain()
+    (got,), (it,) = _emu_solve(emu_lib, [t])
+    g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+    ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
+    assert it == rit
+    assert _bits_equal(got, ref)
+
+
 def test_emulated_batch_matches_solo(emu_lib):
     """The batched union (per-terrain convergence, caps and freeze masks)
     against each terrain's oracle run."""
-    from cpu_emu import emu
-    from fastlem_amd import _native
-    from fastlem_amd.device import DeviceBatch, batch_descs
+    from fastlem_amd.device import DeviceBatch
     from fastlem_amd.workloads import config_fbm, config_plain
 
     ws = [config_plain(1500), config_fbm(2500, 50.0, seed=9), config_plain(800, 60.0)]
     caps = [0xFFFFFFFF, 0xFFFFFFFF, 3]
     terrs = [DeviceBatch.terrain_of(w, max_iteration=c) for w, c in zip(ws, caps)]
-    keep = []
-    descs = batch_descs(terrs, keep)
-    L = emu_lib
-    ctx = ctypes.c_void_p()
-    emu.check(L, L.fl_ctx_create(0, ctypes.byref(ctx)))
-    b = ctypes.c_void_p()
-    try:
-        emu.check(L, L.fl_batch_create(ctx, descs, len(terrs), ctypes.byref(b)))
-        its = np.zeros(len(terrs), np.uint32)
-        emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
-        for i, (w, cap) in enumerate(zip(ws, caps)):
-            out = np.empty(w.n)
-            emu.check(L, L.fl_batch_download(ctx, b, i, _native.ptr(out, ctypes.c_double)))
-            ref, rit, _ = _oracle(w, cap=cap)
-            assert its[i] == rit
-            assert _bits_equal(out, ref)
-        L.fl_terrain_destroy(b)
-    finally:
-        L.fl_ctx_destroy(ctx)
+    outs, its = _emu_solve(emu_lib, terrs)
+    for i, (w, cap) in enumerate(zip(ws, caps)):
+        ref, rit, _ = _oracle(w, cap=cap)
+        assert its[i] == rit
+        assert _bits_equal(outs[i], ref)

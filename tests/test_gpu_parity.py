@@ -340,3 +340,20 @@ def _root(nxt, v):
     while nxt[v] != v:
         v = nxt[v]
     return v
+
+
+def test_comb_many_light_children():
+    """Every site of a 24-site heavy chain (the long-chain path) folds six
+    light children after its heavy child, beyond the four kept per site."""
+    from conftest import comb_terrain
+    from fastlem_amd.device import DeviceTerrain
+    from oracle import flo
+
+    t = comb_terrain()
+    d = DeviceTerrain(t["row_ptr"], t["col"], t["length"], t["area"], t["erodibility"], t["uplift"],
+                      t["init_elevation"], t["outlets"])
+    it = d.solve()
+    g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+    ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
+    assert it == rit
+    assert _bits_equal(d.elevations(), ref)
