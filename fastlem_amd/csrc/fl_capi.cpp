@@ -201,17 +201,24 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
     std::string err;
     int rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, init, a, b, &fast, &err);
     T->stat_order_parts[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p1).count();
+    int rc_ranks = FL_OK;
+    if (rc == FL_OK && fast) {
+      // the forest's ranks need the outlet set, not the pop order: they are
+      // computed while the outlet phase still runs
+      auto p2 = std::chrono::steady_clock::now();
+      rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(), a.data(),
+                              b.data(), a.size(), rank.data(), fl_host_threads());
+      T->stat_order_parts[2] =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
+    }
     int rc_outlets = outlet_phase.get();
     if (rc != FL_OK) return set_err(rc, "spanning forest: %s", err.c_str());
     if (rc_outlets != FL_OK) return set_err(rc_outlets, "outlet phase failed");
-    if (fast) {
-      auto p2 = std::chrono::steady_clock::now();
-      rc = fl_ranks_from_msf_groups(n, G, T->outlet_off.data(), first.data(), T->site_off.data(), a.data(), b.data(),
-                                    a.size(), rank.data());
-      T->stat_order_parts[2] =
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
-      if (rc != FL_OK) return set_err(rc, "ranks from spanning forest failed");
-    }
+    if (rc_ranks != FL_OK) return set_err(rc_ranks, "ranks from spanning forest failed");
+    if (fast)
+      for (uint32_t g = 0; g < G; g++)
+        for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++)
+          rank[first[q]] = T->site_off[g] + (q - T->outlet_off[g]);
   }
   if (!fast) {
     // equal or non-positive lengths: simulate each terrain's heap exactly

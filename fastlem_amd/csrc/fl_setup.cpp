@@ -17,9 +17,13 @@
 //
 // Built with -ffp-contract=off: every floating-point expression is evaluated
 // as written, as rustc does.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "fl_internal.h"
@@ -469,97 +473,192 @@ extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint
 // sorted by increasing length.  Kruskal's merges build the reconstruction
 // tree; Prim from S_g completes the side of each merge edge that contains
 // the entry point before crossing to the other side, which it enters at the
-// edge's endpoint there.  Group g's outlets get ranks rank_off[g] + 0..k_g-1
-// in pop order, its other reached sites rank_off[g] + k_g.. in Prim order;
-// unreached sites UINT32_MAX.
-extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,
-                                        const uint32_t* outlet_order, const uint32_t* rank_off,
-                                        const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
-                                        uint32_t* rank_out) {
+// edge's endpoint there (msf_a lies on the first-merged side, msf_b on the
+// other).  Group g's non-outlet sites reached from S_g get ranks
+// rank_off[g] + k_g + (Prim order); outlets and unreached sites are left
+// UINT32_MAX.  `outlets` may list each group's outlets in any order.
+//
+// The two top-down passes over the reconstruction tree run on `threads`
+// threads: the subtrees holding more than `cut` sites (recorded while the
+// tree is built) are visited by the calling thread in creation order
+// reversed, the smaller subtrees below them by depth-first tasks.
+namespace {
+
+template <class F>
+void parallel_range(size_t n, unsigned threads, F f) {
+  if (threads <= 1 || n < 65536) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < threads; t++)
+    th.emplace_back([&, t] { f(n * t / threads, n * (t + 1) / threads); });
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                 const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                 uint32_t* rank_out, unsigned threads) {
   const uint32_t NONE = 0xffffffffu;
-  for (size_t i = 0; i < n; i++) rank_out[i] = NONE;
-  std::vector<uint32_t> sleaf(n, NONE);  // outlet -> its group's S leaf
-  std::vector<uint8_t> is_out(n, 0);
-  for (size_t g = 0; g < groups; g++) {
-    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) {
-      uint32_t o = outlet_order[q];
-      if (o >= n) return FL_ERR_INVALID;
-      is_out[o] = 1;
-      sleaf[o] = outlet_order[outlet_off[g]];
-      rank_out[o] = rank_off[g] + (q - outlet_off[g]);
-    }
-  }
-  auto leaf_of = [&](uint32_t v) { return is_out[v] ? sleaf[v] : v; };
-  std::vector<uint32_t> uf(n), top(n);
-  for (size_t v = 0; v < n; v++) {
-    uf[v] = is_out[v] ? sleaf[v] : (uint32_t)v;
-    top[v] = (uint32_t)v;
-  }
-  auto find = [&](uint32_t v) {
-    uint32_t r = v;
-    while (uf[r] != r) r = uf[r];
-    while (uf[v] != r) {
-      uint32_t nx = uf[v];
-      uf[v] = r;
-      v = nx;
-    }
-    return r;
+  const size_t total = n + m;
+  if (threads == 0) threads = 1;
+  struct Node {
+    uint32_t ch0, ch1;
+    uint32_t cnt;   // leaves below
+    uint32_t size;  // non-outlet leaves below
   };
-  size_t total = n + m;
-  std::vector<uint32_t> ch0(m), ch1(m), ea(m), eb(m);
-  std::vector<uint32_t> cnt(total), size(total), lo(total, NONE), entry(total, NONE), off(total, 0);
-  for (size_t v = 0; v < n; v++) {
-    cnt[v] = 1;
-    size[v] = is_out[v] ? 0 : 1;
-  }
+  struct Down {
+    uint32_t lo;     // first leaf position of the subtree
+    uint32_t entry;  // site where Prim enters the subtree
+    uint32_t off;    // rank of the first site Prim reaches in it
+  };
+  std::vector<uint32_t> leaf(n), uf(n), top(n);
+  std::unique_ptr<Node[]> node(new Node[total]);
+  std::unique_ptr<Down[]> down(new Down[total]);
+  parallel_range(n, threads, [&](size_t a, size_t b) {
+    for (size_t v = a; v < b; v++) {
+      leaf[v] = (uint32_t)v;
+      rank_out[v] = NONE;
+    }
+  });
+  for (size_t g = 0; g < groups; g++)
+    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) {
+      if (outlets[q] >= n) return FL_ERR_INVALID;
+      leaf[outlets[q]] = outlets[outlet_off[g]];
+    }
+  parallel_range(n, threads, [&](size_t a, size_t b) {
+    for (size_t v = a; v < b; v++) {
+      uf[v] = leaf[v];
+      top[v] = (uint32_t)v;
+      node[v] = {NONE, NONE, 1u, 1u};
+      down[v] = {NONE, NONE, 0};
+    }
+  });
+  for (size_t g = 0; g < groups; g++)
+    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) node[outlets[q]].size = 0;
+  parallel_range(m, threads, [&](size_t a, size_t b) {
+    for (size_t i = a; i < b; i++) down[n + i] = {NONE, NONE, 0};
+  });
+  auto find = [&](uint32_t v) {
+    while (uf[v] != v) {
+      uf[v] = uf[uf[v]];
+      v = uf[v];
+    }
+    return v;
+  };
+  const uint32_t cut = (uint32_t)std::max<size_t>(4096, total / (64 * (size_t)threads));
+  std::vector<uint32_t> big;  // reconstruction nodes with more than `cut` leaves, ascending
   for (size_t i = 0; i < m; i++) {
     uint32_t a = msf_a[i], b = msf_b[i];
     if (a >= n || b >= n) return FL_ERR_INVALID;
     uint32_t ra = find(a), rb = find(b);
     if (ra == rb) return FL_ERR_INVALID;  // not a forest
-    uint32_t x = (uint32_t)(n + i);
-    ch0[i] = top[ra];
-    ch1[i] = top[rb];
-    ea[i] = a;
-    eb[i] = b;
-    cnt[x] = cnt[ch0[i]] + cnt[ch1[i]];
-    size[x] = size[ch0[i]] + size[ch1[i]];
+    Node& x = node[n + i];
+    x.ch0 = top[ra];
+    x.ch1 = top[rb];
+    const Node& A = node[x.ch0];
+    const Node& B = node[x.ch1];
+    x.cnt = A.cnt + B.cnt;
+    x.size = A.size + B.size;
     uf[rb] = ra;
-    top[ra] = x;
+    top[ra] = (uint32_t)(n + i);
+    if (x.cnt > cut) big.push_back((uint32_t)(n + i));
   }
+  std::vector<uint32_t> roots;
   for (size_t g = 0; g < groups; g++) {
     if (outlet_off[g + 1] == outlet_off[g]) continue;
-    uint32_t s_leaf = outlet_order[outlet_off[g]];
+    uint32_t s_leaf = outlets[outlet_off[g]];
     uint32_t root = top[find(s_leaf)];
-    lo[root] = 0;
-    entry[root] = s_leaf;
-    off[root] = rank_off[g] + (outlet_off[g + 1] - outlet_off[g]);
+    down[root] = {0, s_leaf, rank_off[g] + (outlet_off[g + 1] - outlet_off[g])};
+    roots.push_back(root);
   }
-  // leaf intervals (top-down; parents have larger indices than children)
-  for (size_t x = total; x-- > n;) {
-    if (lo[x] == NONE) continue;
-    size_t i = x - n;
-    lo[ch0[i]] = lo[x];
-    lo[ch1[i]] = lo[x] + cnt[ch0[i]];
-  }
-  auto inside = [&](uint32_t node, uint32_t v) {
-    uint32_t l = lo[leaf_of(v)];
-    return l >= lo[node] && l < lo[node] + cnt[node];
+  // one top-down pass: `visit(x)` fills the children of reached node x
+  auto pass = [&](auto&& visit, bool by_lo) {
+    auto reached = [&](uint32_t x) { return (by_lo ? down[x].lo : down[x].entry) != NONE; };
+    std::vector<uint32_t> tasks;
+    for (size_t q = big.size(); q-- > 0;) {
+      uint32_t x = big[q];
+      if (!reached(x)) continue;
+      visit(x);
+      for (uint32_t c : {node[x].ch0, node[x].ch1})
+        if (c >= n && node[c].cnt <= cut) tasks.push_back(c);
+    }
+    for (uint32_t r : roots)
+      if (r >= n && node[r].cnt <= cut) tasks.push_back(r);
+    std::atomic<size_t> next{0};
+    auto worker = [&] {
+      std::vector<uint32_t> st;
+      for (;;) {
+        size_t q = next.fetch_add(1);
+        if (q >= tasks.size()) break;
+        st.push_back(tasks[q]);
+        while (!st.empty()) {
+          uint32_t x = st.back();
+          st.pop_back();
+          visit(x);
+          if (node[x].ch0 >= n) st.push_back(node[x].ch0);
+          if (node[x].ch1 >= n) st.push_back(node[x].ch1);
+        }
+      }
+    };
+    if (threads <= 1 || tasks.size() < 2) {
+      worker();
+      return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < threads; t++) th.emplace_back(worker);
+    for (auto& x : th) x.join();
   };
-  for (size_t x = total; x-- > n;) {
-    if (entry[x] == NONE) continue;
-    size_t i = x - n;
-    uint32_t p = entry[x];
-    uint32_t first = inside(ch0[i], p) ? ch0[i] : ch1[i];
-    uint32_t second = first == ch0[i] ? ch1[i] : ch0[i];
-    uint32_t enter = inside(second, ea[i]) ? ea[i] : eb[i];
-    entry[first] = p;
-    off[first] = off[x];
-    entry[second] = leaf_of(enter);
-    off[second] = off[x] + size[first];
-  }
-  for (size_t v = 0; v < n; v++)
-    if (!is_out[v] && entry[v] != NONE) rank_out[v] = off[v];
+  // leaf intervals (ch0's leaves first)
+  pass(
+      [&](uint32_t x) {
+        const Node& X = node[x];
+        down[X.ch0].lo = down[x].lo;
+        down[X.ch1].lo = down[x].lo + node[X.ch0].cnt;
+      },
+      true);
+  // entry points and first ranks
+  pass(
+      [&](uint32_t x) {
+        const Down D = down[x];
+        const Node& X = node[x];
+        size_t i = x - n;
+        uint32_t l = down[leaf[D.entry]].lo;
+        const Down& C0 = down[X.ch0];
+        bool in0 = l >= C0.lo && l - C0.lo < node[X.ch0].cnt;
+        uint32_t first = in0 ? X.ch0 : X.ch1, second = in0 ? X.ch1 : X.ch0;
+        uint32_t enter = in0 ? msf_b[i] : msf_a[i];
+        down[first].entry = D.entry;
+        down[first].off = D.off;
+        down[second].entry = leaf[enter];
+        down[second].off = D.off + node[first].size;
+      },
+      false);
+  parallel_range(n, threads, [&](size_t a, size_t b) {
+    for (size_t v = a; v < b; v++)
+      if (node[v].size && down[v].entry != NONE) rank_out[v] = down[v].off;
+  });
+  return FL_OK;
+}
+
+unsigned fl_host_threads() {
+  unsigned h = std::thread::hardware_concurrency();
+  return h == 0 ? 1 : (h > 16 ? 16 : h);
+}
+
+// As fl_krt_ranks, with the outlets ranked by their pop order (outlet_order,
+// the order of fl_outlet_pop_order).
+extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,
+                                        const uint32_t* outlet_order, const uint32_t* rank_off,
+                                        const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                                        uint32_t* rank_out) {
+  int rc = fl_krt_ranks(n, groups, outlet_off, outlet_order, rank_off, msf_a, msf_b, m, rank_out, fl_host_threads());
+  if (rc != FL_OK) return rc;
+  for (size_t g = 0; g < groups; g++)
+    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++)
+      rank_out[outlet_order[q]] = rank_off[g] + (q - outlet_off[g]);
   return FL_OK;
 }
 
