@@ -276,14 +276,10 @@ int32_t decompose(fl_ctx* c, fl_terrain* T) {
   uint32_t n = T->t.n;
   fl_launch_children(s, T->t, w);
   uint32_t round = 0;
-  bool flip = false;
   uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
   uint32_t quiet = FL_NONE_ROUND;  // first round in which no pointer moved
   for (;;) {
-    for (; round < target; round++) {
-      fl_launch_chain_jump(s, T->t, w, flip, round);
-      flip = !flip;
-    }
+    for (; round < target; round++) fl_launch_chain_jump(s, T->t, w, round);
     FLCHK(read_ctrl(c, T));
     for (uint32_t r = 0; r < round; r++)
       if (c->host_ctrl[FL_CTRL_JUMPS + r] == 0) {
@@ -296,12 +292,6 @@ int32_t decompose(fl_ctx* c, fl_terrain* T) {
   }
   // quiet + 1 rounds suffice (the quiet round confirms convergence)
   T->jump_rounds = quiet + 1;
-  if (flip) {
-    std::swap(w.jc, w.jc2);
-    std::swap(w.pc, w.pc2);
-    std::swap(w.jn, w.jn2);
-    std::swap(w.lw, w.lw2);
-  }
   fl_launch_chain_count(s, T->t, w);
   fl_launch_chain_layout(s, T->t, w);
   FLCHK(read_ctrl(c, T));
@@ -346,8 +336,8 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
   for (uint32_t l = rounds; l-- > 0;)
-    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2]);
-  fl_launch_tau_prep(s, T->t, w, npos, clamp);
+    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2],
+                       clamp);
   for (uint32_t l = 0; l < rounds; l++)
     fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2],
                         clamp);
@@ -551,9 +541,11 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
     FLCHK(dev_alloc(P, p, n));
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.jc, &w.pc, &w.jn, &w.lw, &w.jc2, &w.pc2,
-                       &w.jn2, &w.lw2, &w.chain_base, &w.chain_len, &w.head_base, &w.perm, &w.ppos, &w.rpos})
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
+                       &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
+  FLCHK(dev_alloc(P, &w.cp, n));
+  FLCHK(dev_alloc(P, &w.rt, n));
   FLCHK(dev_alloc(P, &w.cmask, n));
   for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur})
     FLCHK(dev_alloc(P, p, 2 * (uint64_t)n + 4));

@@ -73,7 +73,6 @@ struct fl_dev_work {
   uint32_t* rpos = nullptr; // [n] at a chain's base position: position of the basin outlet
   double* A = nullptr;      // [n] drainage area (kept: heavy-child hint)
   double* tau = nullptr;    // [n] response time
-  double* x = nullptr;      // [n] 1/celerity * distance
   double* dnext = nullptr;  // [n] distance to the receiver
   uint32_t* next = nullptr; // [n]
   // lakes
@@ -90,8 +89,11 @@ struct fl_dev_work {
   double* post3 = nullptr;
   uint32_t* npost = nullptr;
 
-  uint32_t *jc = nullptr, *pc = nullptr, *jn = nullptr, *lw = nullptr;
-  uint32_t *jc2 = nullptr, *pc2 = nullptr, *jn2 = nullptr, *lw2 = nullptr;
+  // pointer jumping, two (pointer, count) pairs per site, pointer in the low
+  // word: cp = (chain ancestor, distance to it), rt = (tree ancestor, light
+  // edges to it); converged: (chain head, position), (root, light depth)
+  uint64_t* cp = nullptr;
+  uint64_t* rt = nullptr;
   uint32_t *bcnt = nullptr, *bnodes = nullptr, *boff = nullptr, *bnoff = nullptr, *bcur = nullptr,
            *bncur = nullptr;                                  // [n+2] light-depth buckets
   uint32_t *chain_base = nullptr, *chain_len = nullptr;      // [n] chains by bucket
@@ -107,15 +109,16 @@ void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
 void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip, uint32_t round);
+void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 // Round l of a sweep: short chains [c0, c1) at positions [p0, p1), long
 // chains [c1, c2) at positions [p1, p2).
+// The drainage-area round also writes the response-time inputs of its
+// positions (and, with clamp, the max-slope recurrence's inputs).
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2);
+                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp);
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
-void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
                          uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp);
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,

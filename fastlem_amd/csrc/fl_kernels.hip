@@ -19,8 +19,8 @@
 //     k_da_pre          generator.rs:154-159  light-children part of each
 //                                             site's fold, in parallel
 //     k_da_fold                               drainage area along the chains,
-//                                             exact fold order
-//     k_tau_prep        generator.rs:172      1/(K*A^0.5) * distance
+//                       generator.rs:172      exact fold order; then each
+//                                             site's 1/(K*A^0.5) * distance
 //     k_tau_fold        generator.rs:162-203  response time (+ clamped
 //                                             elevation), head to tail
 //     k_tau_scatter     generator.rs:177-202  elevation, changed flag
@@ -44,6 +44,13 @@
 namespace {
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// (pointer, count) pairs of the pointer jumping: pointer in the low word.
+__device__ __forceinline__ unsigned long long pair(uint32_t ptr, uint32_t cnt) {
+  return ((unsigned long long)cnt << 32) | ptr;
+}
+__device__ __forceinline__ uint32_t pair_ptr(unsigned long long p) { return (uint32_t)p; }
+__device__ __forceinline__ uint32_t pair_cnt(unsigned long long p) { return (uint32_t)(p >> 32); }
 
 // f64::powf(x, 0.5) as lowered in optimised Rust builds.
 __device__ __forceinline__ double rust_pow_half(double x) {
@@ -82,30 +89,23 @@ __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const u
     return;
   }
   const double hi = h[i];
-  double best = 0.0;
+  double best = 0.0, d = 1.0;
   uint32_t bj = i;
   const uint32_t b = rp[i], e = rp[i + 1];
   for (uint32_t k = b; k < e; ++k) {
     uint32_t j = col[k];
     double hj = h[j];
     if (hi > hj) {
-      double s = (hi - hj) / len[k];
+      double lk = len[k];
+      double s = (hi - hj) / lk;
       if (s > best) {
         best = s;
         bj = j;
+        d = lk;  // rows hold no duplicate edges: this slot is has_edge(i, bj)'s
       }
     }
   }
-  double d = 1.0;
-  if (bj != i) {
-    for (uint32_t k = b; k < e; ++k)
-      if (col[k] == bj) {
-        d = len[k];
-        break;
-      }
-  } else {
-    flags[FL_FLAG_LAKE] = 1;
-  }
+  if (bj == i) flags[FL_FLAG_LAKE] = 1;
   next[i] = bj;
   dnext[i] = d;
 }
@@ -222,41 +222,48 @@ __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const ui
 }
 
 // Chain and root pointers for pointer jumping.
-//   jc: chain parent (self at a chain head), pc: 1 (0 at a head)
-//   jn: receiver, lw: 1 at a head that is not a root (a light edge)
+//   cp: (chain parent, 1), (v, 0) at a chain head
+//   rt: (receiver, 1 at a head that is not a root: a light edge), (v, 0) at a root
 __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ hc,
-                             uint32_t* __restrict__ jc, uint32_t* __restrict__ pc, uint32_t* __restrict__ jn,
-                             uint32_t* __restrict__ lw) {
+                             unsigned long long* __restrict__ cp, unsigned long long* __restrict__ rt) {
   uint32_t v = gid();
   if (v >= n) return;
   uint32_t p = next[v];
   bool root = (p == v);
   bool head = root || hc[p] != v;
-  jc[v] = head ? v : p;
-  pc[v] = head ? 0u : 1u;
-  jn[v] = p;
-  lw[v] = (head && !root) ? 1u : 0u;
+  cp[v] = head ? pair(v, 0u) : pair(p, 1u);
+  rt[v] = pair(p, (head && !root) ? 1u : 0u);
 }
 
-// One doubling round on both pointer systems.  After convergence
-// jc[v] = chain head, pc[v] = distance to it, jn[v] = root of v's tree,
-// lw[v] = light edges between v and its root (= light depth of v's chain).
-// Movement is flagged in the round's own control word (zeroed at the start
-// of the iteration), so rounds need no separate clearing launch.
-__global__ void k_chain_jump(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ pc,
-                             const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
-                             uint32_t* __restrict__ jc2, uint32_t* __restrict__ pc2, uint32_t* __restrict__ jn2,
-                             uint32_t* __restrict__ lw2, uint32_t* __restrict__ flag_set) {
+// One doubling round on both pointer systems, in place:
+// (a, d) <- (a', d + d') with (a', d') the pair of a.  A pair read while its
+// owner rewrites it is either its old or its new value (aligned 64-bit
+// accesses are single-copy atomic), and both are (ancestor, count) pairs of
+// the same path, so rounds may mix them.  A site whose pointers reach a
+// fixed point (a head, a root) stops writing.  After convergence
+// cp[v] = (chain head, position in chain), rt[v] = (root of v's tree, light
+// edges between v and the root = light depth of v's chain).  Movement is
+// flagged in the round's own control word (zeroed at the start of the
+// iteration), so rounds need no separate clearing launch; a round without
+// movement proves convergence (a pointer at a fixed point never moves).
+__global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long long* rt,
+                             uint32_t* __restrict__ flag_set) {
   uint32_t v = gid();
   bool moved = false;
   if (v < n) {
-    uint32_t a = jc[v], b = jn[v];
-    uint32_t aa = jc[a], bb = jn[b];
-    jc2[v] = aa;
-    pc2[v] = pc[v] + pc[a];
-    jn2[v] = bb;
-    lw2[v] = lw[v] + lw[b];
-    moved = (aa != a || bb != b);
+    unsigned long long c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
+    unsigned long long r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
+    uint32_t a = pair_ptr(c), b = pair_ptr(r);
+    unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
+    unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
+    if (pair_ptr(ca) != a) {
+      __atomic_store_n(cp + v, pair(pair_ptr(ca), pair_cnt(c) + pair_cnt(ca)), __ATOMIC_RELAXED);
+      moved = true;
+    }
+    if (pair_ptr(rb) != b) {
+      __atomic_store_n(rt + v, pair(pair_ptr(rb), pair_cnt(r) + pair_cnt(rb)), __ATOMIC_RELAXED);
+      moved = true;
+    }
   }
   if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
@@ -274,10 +281,9 @@ __global__ void k_chain_jump(uint32_t n, const uint32_t* __restrict__ jc, const 
 // Each chain is counted by its tail (the member without a heavy child),
 // which knows the chain's length (its position + 1) and light depth.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ pc,
-               const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
-               const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes,
-               uint32_t* __restrict__ maxld) {
+k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
+               const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
+               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld) {
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
@@ -289,16 +295,19 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __re
   uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    if (v < n && hc[v] == FL_NONE && is_outlet[jn[v]]) {
-      uint32_t len = pc[v] + 1, l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
-      if (l < FL_LDS_BUCKETS) {
-        atomicAdd(s_c + l, 1u);
-        atomicAdd(s_n + l, len);
-      } else {
-        atomicAdd(bcnt + l, 1u);
-        atomicAdd(bnodes + l, len);
+    if (v < n && hc[v] == FL_NONE) {
+      unsigned long long r = rt[v];
+      if (is_outlet[pair_ptr(r)]) {
+        uint32_t lw = pair_cnt(r), len = pair_cnt(cp[v]) + 1, l = 2u * lw + (len >= FL_LONG ? 1u : 0u);
+        if (l < FL_LDS_BUCKETS) {
+          atomicAdd(s_c + l, 1u);
+          atomicAdd(s_n + l, len);
+        } else {
+          atomicAdd(bcnt + l, 1u);
+          atomicAdd(bnodes + l, len);
+        }
+        atomicMax(&s_max, lw);
       }
-      atomicMax(&s_max, lw[v]);
     }
   }
   __syncthreads();
@@ -376,9 +385,9 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uin
 // one global atomic per (workgroup, bucket) reserves the ranges.  Order
 // inside a bucket is immaterial to the results.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __restrict__ jc,
-               const uint32_t* __restrict__ pc, const uint32_t* __restrict__ jn, const uint32_t* __restrict__ lw,
-               const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ boff,
+k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
+               const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
+               const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
                uint32_t* __restrict__ head_base) {
@@ -394,17 +403,21 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __re
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     lc[j] = FL_NONE;
     ln[j] = 0;
-    if (v < n && hc[v] == FL_NONE && is_outlet[jn[v]]) {
-      uint32_t len = pc[v] + 1, l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
-      if (l < FL_LDS_BUCKETS) {
-        lc[j] = atomicAdd(s_c + l, 1u);
-        ln[j] = atomicAdd(s_n + l, len);
-      } else {
-        uint32_t c = boff[l] + atomicAdd(bcur + l, 1u);
-        uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
-        chain_base[c] = pos;
-        chain_len[c] = len;
-        head_base[jc[v]] = pos;
+    if (v < n && hc[v] == FL_NONE) {
+      unsigned long long r = rt[v];
+      if (is_outlet[pair_ptr(r)]) {
+        unsigned long long c = cp[v];
+        uint32_t len = pair_cnt(c) + 1, l = 2u * pair_cnt(r) + (len >= FL_LONG ? 1u : 0u);
+        if (l < FL_LDS_BUCKETS) {
+          lc[j] = atomicAdd(s_c + l, 1u);
+          ln[j] = atomicAdd(s_n + l, len);
+        } else {
+          uint32_t q = boff[l] + atomicAdd(bcur + l, 1u);
+          uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
+          chain_base[q] = pos;
+          chain_len[q] = len;
+          head_base[pair_ptr(c)] = pos;
+        }
       }
     }
   }
@@ -419,13 +432,14 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __re
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     if (lc[j] == FL_NONE) continue;
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    uint32_t len = pc[v] + 1;
-    uint32_t l = 2u * lw[v] + (len >= FL_LONG ? 1u : 0u);
-    uint32_t c = boff[l] + s_bc[l] + lc[j];
+    unsigned long long c = cp[v];
+    uint32_t len = pair_cnt(c) + 1;
+    uint32_t l = 2u * pair_cnt(rt[v]) + (len >= FL_LONG ? 1u : 0u);
+    uint32_t q = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
-    chain_base[c] = pos;
-    chain_len[c] = len;
-    head_base[jc[v]] = pos;
+    chain_base[q] = pos;
+    chain_len[q] = len;
+    head_base[pair_ptr(c)] = pos;
   }
 }
 
@@ -434,27 +448,29 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const uint32_t* __re
 // outlet) and of the basin's outlet.  Sites outside every basin keep their
 // elevation, drainage area = own area and response time 0 (generator.rs:
 // 144-145), as the reference leaves them.
-__global__ void k_chain_perm(uint32_t n, const uint32_t* __restrict__ jc, const uint32_t* __restrict__ pc,
-                             const uint32_t* __restrict__ jn, const uint32_t* __restrict__ next,
+__global__ void k_chain_perm(uint32_t n, const unsigned long long* __restrict__ cp,
+                             const unsigned long long* __restrict__ rt, const uint32_t* __restrict__ next,
                              const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
                              uint32_t* __restrict__ perm, uint32_t* __restrict__ ppos, uint32_t* __restrict__ rpos,
                              const double* __restrict__ area, double* __restrict__ A, double* __restrict__ tau,
                              const double* __restrict__ h, double* __restrict__ h2) {
   uint32_t v = gid();
   if (v >= n) return;
-  uint32_t r = jn[v];
+  uint32_t r = pair_ptr(rt[v]);
   if (!is_outlet[r]) {
     A[v] = area[v];
     tau[v] = 0.0;
     h2[v] = h[v];
     return;
   }
-  uint32_t hd = jc[v];
+  unsigned long long c = cp[v];
+  uint32_t hd = pair_ptr(c);
   uint32_t base = head_base[hd];
-  perm[base + pc[v]] = v;
+  perm[base + pair_cnt(c)] = v;
   if (hd == v) {
     uint32_t p = next[v];
-    ppos[base] = (p == v) ? FL_NONE : head_base[jc[p]] + pc[p];
+    unsigned long long cpp = cp[p];
+    ppos[base] = (p == v) ? FL_NONE : head_base[pair_ptr(cpp)] + pair_cnt(cpp);
     rpos[base] = head_base[r];
   }
 }
@@ -587,6 +603,36 @@ __device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c,
 
 #define FL_DA_BLOCK 8
 
+// Inputs of the response-time sweep for chain position P of site k, written
+// as soon as k's drainage area a is final: x = 1.0 / (K * a^0.5) * d
+// (generator.rs:172-173, the reference's association); with max slopes the
+// recurrence's inputs gathered into chain-major order.
+struct fl_tau_in {
+  const double* K;
+  const double* dnext;
+  const double* U;
+  const double* tan_ms;
+  const double* h;
+  double* xp;
+  double* up;
+  double* tanp;
+  double* dp;
+  double* hop;
+  int clamp;
+};
+
+__device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint32_t k, double a) {
+  double d = ti.dnext[k];
+  double celerity = ti.K[k] * rust_pow_half(a);
+  ti.xp[P] = 1.0 / celerity * d;
+  if (ti.clamp) {
+    ti.up[P] = ti.U[k];
+    ti.tanp[P] = ti.tan_ms[k];
+    ti.dp[P] = d;
+    ti.hop[P] = ti.h[k];
+  }
+}
+
 // Drainage area, part 2, for one light depth: the sequential fold along each
 // chain, tail to head.  One launch covers the round's long chains (blocks
 // [0, c2 - c1), one 64-lane wavefront each) and its short chains (the
@@ -604,7 +650,8 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
           const double* __restrict__ post0, const double* __restrict__ post1, const double* __restrict__ post2,
           const double* __restrict__ post3, const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
           const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A) {
+          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A,
+          fl_tau_in ti) {
   __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
   __shared__ uint32_t s_n[64], s_k[64];
   const uint32_t lane = threadIdx.x;
@@ -634,11 +681,10 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
       for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
         if (u < cnt) {
           r = ((vp[u] + r) + va[u]) + vb[u];
-          if (vn[u] >= 3) {
-            uint32_t P = base + end - 1 - u;
-            r = da_tail_terms(r, vn[u], post2[P], post3[P], vk[u], rp, col, next, cmask, hc, A);
-          }
+          uint32_t P = base + end - 1 - u;
+          if (vn[u] >= 3) r = da_tail_terms(r, vn[u], post2[P], post3[P], vk[u], rp, col, next, cmask, hc, A);
           A[vk[u]] = r;
+          tau_inputs(ti, P, vk[u], r);
         }
       }
       end = cs;
@@ -711,31 +757,12 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
       }
     }
     __syncthreads();
-    if (lane < cnt) A[s_k[lane]] = s_r[lane];
+    if (lane < cnt) {
+      A[s_k[lane]] = s_r[lane];
+      tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane]);
+    }
     end = nend;
     cs = ncs;
-  }
-}
-
-// Per chain-major position: x = 1.0 / (K * A^0.5) * d (generator.rs:172-173,
-// the reference's association); with max slopes also gather the inputs of
-// the elevation recurrence so the chain lanes read contiguous memory.
-__global__ void k_tau_prep(uint32_t npos, const uint32_t* __restrict__ perm, const double* __restrict__ K,
-                           const double* __restrict__ A, const double* __restrict__ dnext,
-                           const double* __restrict__ U, const double* __restrict__ tan_ms,
-                           const double* __restrict__ h, double* __restrict__ xp, double* __restrict__ up,
-                           double* __restrict__ tanp, double* __restrict__ dp, double* __restrict__ hop, int clamp) {
-  uint32_t P = gid();
-  if (P >= npos) return;
-  uint32_t k = perm[P];
-  double d = dnext[k];
-  double celerity = K[k] * rust_pow_half(A[k]);
-  xp[P] = 1.0 / celerity * d;
-  if (clamp) {
-    up[P] = U[k];
-    tanp[P] = tan_ms[k];
-    dp[P] = d;
-    hop[P] = h[k];
   }
 }
 
@@ -954,7 +981,8 @@ k_tau_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ c
 // h_o' = h_o + U_o * max(0, 0), generator.rs:178-180) and the changed flag
 // (generator.rs:201).  In a batch (tid != null) a terrain that stopped
 // iterating keeps its elevations and each terrain has its own flag.
-__global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ jn,
+__global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm,
+                              const unsigned long long* __restrict__ rt,
                               const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
@@ -973,7 +1001,7 @@ __global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm, 
       if (clamp) {
         nh = hnp[P];
       } else {
-        uint32_t o = jn[k];
+        uint32_t o = pair_ptr(rt[k]);
         double to = taup[head_base[o]];
         double ho = h[o] + U[o] * rust_max(to - to, 0.0);
         nh = (k == o) ? ho : ho + U[k] * rust_max(tk - to, 0.0);
@@ -1149,47 +1177,40 @@ void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc);
-  FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, w.jc, w.pc, w.jn, w.lw);
+  FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, (unsigned long long*)w.cp, (unsigned long long*)w.rt);
 }
 
-void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip, uint32_t round) {
+void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round) {
   uint32_t* fset = w.ctrl + FL_CTRL_JUMPS + (round < FL_MAX_JUMP_ROUNDS ? round : FL_MAX_JUMP_ROUNDS - 1);
-  if (!flip)
-    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc, w.pc, w.jn, w.lw, w.jc2, w.pc2, w.jn2, w.lw2, fset);
-  else
-    FL_LAUNCH(k_chain_jump, t.n, s, t.n, w.jc2, w.pc2, w.jn2, w.lw2, w.jc, w.pc, w.jn, w.lw, fset);
+  FL_LAUNCH(k_chain_jump, t.n, s, t.n, (unsigned long long*)w.cp, (unsigned long long*)w.rt, fset);
 }
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc, w.pc, w.jn, w.lw,
-                 t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
+  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc,
+                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
 }
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), 0, s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc, w.jc, w.pc, w.jn,
-                 w.lw, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
-  FL_LAUNCH(k_chain_perm, t.n, s, t.n, w.jc, w.pc, w.jn, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
+  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc,
+                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
+  FL_LAUNCH(k_chain_perm, t.n, s, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
             t.area, w.A, w.tau, w.h, w.h2);
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2) {
+                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp) {
   (void)p1;
+  fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
   FL_LAUNCH(k_da_pre, p2 - p0, s, p0, p2, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
             w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.post2, w.post3, w.npost);
   uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
   if (blocks)
     FL_LAUNCH_GRID(k_da_fold, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len, w.perm,
                    w.pre, w.post0, w.post1, w.post2, w.post3, w.npost, t.row_ptr, t.col, w.next,
-                   (const unsigned long long*)w.cmask, w.hc, w.A);
-}
-
-void fl_launch_tau_prep(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp) {
-  FL_LAUNCH(k_tau_prep, npos, s, npos, w.perm, t.erod, w.A, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp,
-            w.dp, w.hop, clamp ? 1 : 0);
+                   (const unsigned long long*)w.cmask, w.hc, w.A, ti);
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
@@ -1207,7 +1228,7 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
-  FL_LAUNCH(k_tau_scatter, npos, s, npos, w.perm, w.jn, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau,
+  FL_LAUNCH(k_tau_scatter, npos, s, npos, w.perm, (const unsigned long long*)w.rt, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau,
             w.flags, clamp ? 1 : 0, tid, active, tchanged);
 }
 
