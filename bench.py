@@ -231,7 +231,14 @@ def multi_terrain(args) -> int:
 
     kind = "boundary" if args.mode == "batched" else "ocean"
     t0 = time.time()
-    terrains = terrain_stream(args.terrains, args.min_sites, args.max_sites, 4242, kind, args.workers)
+    last = [time.time()]
+
+    def progress(done, count):
+        if done == count or time.time() - last[0] > 20:
+            last[0] = time.time()
+            log(f"built {done}/{count} terrains")
+
+    terrains = terrain_stream(args.terrains, args.min_sites, args.max_sites, 4242, kind, args.workers, progress)
     sites = sum(len(t["row_ptr"]) - 1 for t in terrains)
     log(f"{len(terrains)} terrains, {sites} sites built in {time.time() - t0:.1f}s")
     from fastlem_amd.device import DeviceBatch, DeviceTerrain
@@ -268,6 +275,7 @@ def multi_terrain(args) -> int:
         mean_service = plain_s / len(terrains)
         results = []
         for load in (0.5, 1.0, 2.0, 4.0, float("inf")):
+            log(f"served: offered load {load} x the plain service rate")
             srv = Server(slots=args.slots, hbm_budget_bytes=32 << 30)
             gaps = np.zeros(len(terrains)) if load == float("inf") else \
                 rng.exponential(mean_service / load, len(terrains))

@@ -224,20 +224,31 @@ def _terrain_dict(args):
                 outlets=outlets, init_elevation=jitter_elevations(_np.zeros(cnt), 0))
 
 
-def terrain_stream(count: int, lo: int, hi: int, seed: int, kind: str, workers: int = 8) -> list:
+def terrain_stream(count: int, lo: int, hi: int, seed: int, kind: str, workers: int = 8, progress=None) -> list:
     """`count` independent terrains with site counts log-uniform in [lo, hi]
     (seeded): uniform-random sites in [0, sqrt(N)]^2, Delaunay graph,
     erodibility = clip(10^fBm, 0.1, 10), outlets on the boundary band
     (kind="boundary", the batched workload) or where a second fBm < -0.2
     (kind="ocean", the served workload).  Built in a spawn process pool; call
-    before the GPU is initialised."""
+    before the GPU is initialised.  progress(done, count) is called as they
+    arrive."""
     import concurrent.futures as cf
     import multiprocessing as mp
 
     rng = np.random.default_rng(seed)
     sizes = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
     jobs = [(kind, int(n), seed * 100003 + i) for i, n in enumerate(sizes)]
+    out = []
     if workers <= 1:
-        return [_terrain_dict(j) for j in jobs]
+        it = map(_terrain_dict, jobs)
+        for t in it:
+            out.append(t)
+            if progress:
+                progress(len(out), count)
+        return out
     with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
-        return list(ex.map(_terrain_dict, jobs, chunksize=4))
+        for t in ex.map(_terrain_dict, jobs, chunksize=1):
+            out.append(t)
+            if progress:
+                progress(len(out), count)
+    return out
