@@ -206,8 +206,8 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
       // the forest's ranks need the outlet set, not the pop order: they are
       // computed while the outlet phase still runs
       auto p2 = std::chrono::steady_clock::now();
-      rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(), a.data(),
-                              b.data(), a.size(), rank.data(), fl_host_threads());
+      rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(),
+                              T->site_off.data(), a.data(), b.data(), a.size(), rank.data(), fl_host_threads());
       T->stat_order_parts[2] =
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
     }

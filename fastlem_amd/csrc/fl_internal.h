@@ -26,9 +26,12 @@ int fl_lake_visit_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
                               const double* len, const uint32_t* outlets, size_t k, uint32_t rank_base,
                               uint32_t* rank_out);
 // Non-outlet visit ranks from the spanning forest (outlets in any order).
+// With site_off (groups + 1 entries: group g owns sites [site_off[g],
+// site_off[g+1]) and no edge joins two groups) the groups are processed
+// independently on the threads; without it, as one forest.
 int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
-                 const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
-                 uint32_t* rank_out, unsigned threads);
+                 const uint32_t* rank_off, const uint32_t* site_off, const uint32_t* msf_a, const uint32_t* msf_b,
+                 size_t m, uint32_t* rank_out, unsigned threads);
 // Host threads for the one-time setup work (at most 16).
 unsigned fl_host_threads();
 extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,

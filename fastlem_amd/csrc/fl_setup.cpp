@@ -498,7 +498,9 @@ void parallel_range(size_t n, unsigned threads, F f) {
 
 }  // namespace
 
-int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+namespace {
+
+int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
                  const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
                  uint32_t* rank_out, unsigned threads) {
   const uint32_t NONE = 0xffffffffu;
@@ -643,6 +645,69 @@ int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint
   return FL_OK;
 }
 
+}  // namespace
+
+int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                 const uint32_t* rank_off, const uint32_t* site_off, const uint32_t* msf_a, const uint32_t* msf_b,
+                 size_t m, uint32_t* rank_out, unsigned threads) {
+  if (groups <= 1 || !site_off)
+    return krt_ranks_one(n, groups, outlet_off, outlets, rank_off, msf_a, msf_b, m, rank_out, threads);
+  // independent groups (terrains of a batch): each group's forest edges, in
+  // their sorted order, on a thread of its own, the largest groups first
+  std::vector<uint32_t> eoff(groups + 1, 0);
+  std::vector<uint32_t> group_of_edge(m);
+  for (size_t i = 0; i < m; i++) {
+    uint32_t a = msf_a[i];
+    if (a >= n) return FL_ERR_INVALID;
+    size_t g = (size_t)(std::upper_bound(site_off, site_off + groups + 1, a) - site_off) - 1;
+    group_of_edge[i] = (uint32_t)g;
+    eoff[g + 1]++;
+  }
+  for (size_t g = 0; g < groups; g++) eoff[g + 1] += eoff[g];
+  std::vector<uint32_t> ea(m), eb(m), cur(eoff.begin(), eoff.end() - 1);
+  for (size_t i = 0; i < m; i++) {
+    uint32_t q = cur[group_of_edge[i]]++;
+    ea[q] = msf_a[i];
+    eb[q] = msf_b[i];
+  }
+  std::vector<uint32_t> order(groups);
+  for (size_t g = 0; g < groups; g++) order[g] = (uint32_t)g;
+  std::sort(order.begin(), order.end(),
+            [&](uint32_t x, uint32_t y) { return site_off[x + 1] - site_off[x] > site_off[y + 1] - site_off[y]; });
+  std::atomic<size_t> next{0};
+  std::atomic<int> status{FL_OK};
+  auto worker = [&] {
+    for (;;) {
+      size_t q = next.fetch_add(1);
+      if (q >= groups) break;
+      uint32_t g = order[q];
+      uint32_t s0 = site_off[g], ng = site_off[g + 1] - s0;
+      // the group's sites relabelled to [0, ng)
+      std::vector<uint32_t> la(eoff[g + 1] - eoff[g]), lb(la.size()), lo_out(outlet_off[g + 1] - outlet_off[g]);
+      for (size_t i = 0; i < la.size(); i++) {
+        la[i] = ea[eoff[g] + i] - s0;
+        lb[i] = eb[eoff[g] + i] - s0;
+        if (la[i] >= ng || lb[i] >= ng) {
+          status = FL_ERR_INVALID;
+          return;
+        }
+      }
+      for (size_t i = 0; i < lo_out.size(); i++) lo_out[i] = outlets[outlet_off[g] + i] - s0;
+      uint32_t ooff[2] = {0, (uint32_t)lo_out.size()}, roff[1] = {rank_off[g]};
+      int rc = krt_ranks_one(ng, 1, ooff, lo_out.data(), roff, la.data(), lb.data(), la.size(), rank_out + s0, 1);
+      if (rc != FL_OK) status = rc;
+    }
+  };
+  if (threads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < threads; t++) th.emplace_back(worker);
+    for (auto& x : th) x.join();
+  }
+  return status.load();
+}
+
 unsigned fl_host_threads() {
   unsigned h = std::thread::hardware_concurrency();
   return h == 0 ? 1 : (h > 16 ? 16 : h);
@@ -654,7 +719,8 @@ extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t*
                                         const uint32_t* outlet_order, const uint32_t* rank_off,
                                         const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
                                         uint32_t* rank_out) {
-  int rc = fl_krt_ranks(n, groups, outlet_off, outlet_order, rank_off, msf_a, msf_b, m, rank_out, fl_host_threads());
+  int rc = fl_krt_ranks(n, groups, outlet_off, outlet_order, rank_off, nullptr, msf_a, msf_b, m, rank_out,
+                        fl_host_threads());
   if (rc != FL_OK) return rc;
   for (size_t g = 0; g < groups; g++)
     for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++)
