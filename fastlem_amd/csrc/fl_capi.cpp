@@ -59,6 +59,7 @@ struct ProfScope {
 
 struct fl_ctx {
   int device = 0;
+  uint32_t compact_min_idle = 1u << 16;  // see solve_union
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -74,7 +75,8 @@ struct fl_terrain {
   uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
   bool have_order = false;
   // statistics of the last solve
-  uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0;
+  uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0,
+           stat_compactions = 0;
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   uint64_t bytes = 0;
@@ -87,7 +89,7 @@ struct fl_terrain {
   // terrains packed into this object (one for fl_terrain_create); terrain g
   // owns sites [site_off[g], site_off[g+1]) and outlets [outlet_off[g], ...)
   uint32_t nterr = 1;
-  std::vector<uint32_t> site_off, outlet_off, t_max_it, t_iters;
+  std::vector<uint32_t> site_off, outlet_off, entry_off, t_max_it, t_iters;
   std::vector<uint8_t> t_active;
   uint32_t* d_tid = nullptr;      // [n] terrain of each site (batches)
   uint8_t* d_active = nullptr;    // [nterr] terrains still iterating
@@ -440,6 +442,57 @@ void gather_perm(const V* src, const std::vector<uint32_t>& old_of_new, std::vec
 
 namespace {
 
+// Device arrays of a union of `count` terrains with n sites and nnz entries;
+// the bucket histograms start at zero.
+int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, bool clamp) {
+  fl_dev_terrain& t = P->t;
+  fl_dev_work& w = P->w;
+  t.n = n;
+  t.nnz = nnz;
+  FLCHK(dev_alloc(P, &t.row_ptr, (uint64_t)n + 1));
+  FLCHK(dev_alloc(P, &t.col, nnz));
+  FLCHK(dev_alloc(P, &t.len, nnz));
+  FLCHK(dev_alloc(P, &t.area, n));
+  FLCHK(dev_alloc(P, &t.erod, n));
+  FLCHK(dev_alloc(P, &t.uplift, n));
+  FLCHK(dev_alloc(P, &t.tan_ms, n));
+  FLCHK(dev_alloc(P, &t.init_h, n));
+  FLCHK(dev_alloc(P, &t.is_outlet, n));
+  FLCHK(dev_alloc(P, &t.rank, n));
+  FLCHK(dev_alloc(P, &t.order, n));
+  t.any_max_slope = clamp;
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
+                     &w.post2, &w.post3})
+    FLCHK(dev_alloc(P, p, n));
+  if (clamp)
+    for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
+                       &w.perm, &w.ppos, &w.rpos})
+    FLCHK(dev_alloc(P, p, n));
+  FLCHK(dev_alloc(P, &w.cp, n));
+  FLCHK(dev_alloc(P, &w.rt, n));
+  FLCHK(dev_alloc(P, &w.cmask, n));
+  for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur})
+    FLCHK(dev_alloc(P, p, 2 * (uint64_t)n + 4));
+  FLCHK(dev_alloc(P, &w.lake_key, n));
+  FLCHK(dev_alloc(P, &w.ctrl, FL_CTRL_WORDS));
+  w.flags = (int*)w.ctrl;
+  w.counters = w.ctrl + FL_NUM_FLAGS;
+  if (count > 1) {
+    FLCHK(dev_alloc(P, &P->d_tid, n));
+    FLCHK(dev_alloc(P, &P->d_active, count));
+    FLCHK(dev_alloc(P, &P->d_tchanged, count));
+    HIPCHK(hipHostMalloc((void**)&P->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
+  }
+  hipStream_t s = c->stream;
+  fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bnodes, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bcur, 2 * (uint64_t)n + 4, 0);
+  fl_launch_fill_u32(s, w.bncur, 2 * (uint64_t)n + 4, 0);
+  P->prev_nb = 0;
+  return FL_OK;
+}
+
 // Upload `count` terrains as one union graph: terrain g's sites occupy
 // [site_off[g], site_off[g+1]), each relabelled for locality inside its range;
 // no edge joins two terrains.
@@ -520,47 +573,14 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   }
 
   fl_dev_terrain& t = T->t;
-  fl_dev_work& w = T->w;
   fl_terrain* P = T.get();
-  FLCHK(dev_alloc(P, &t.row_ptr, (uint64_t)n + 1));
-  FLCHK(dev_alloc(P, &t.col, nnz));
-  FLCHK(dev_alloc(P, &t.len, nnz));
-  FLCHK(dev_alloc(P, &t.area, n));
-  FLCHK(dev_alloc(P, &t.erod, n));
-  FLCHK(dev_alloc(P, &t.uplift, n));
-  FLCHK(dev_alloc(P, &t.tan_ms, n));
-  FLCHK(dev_alloc(P, &t.init_h, n));
-  FLCHK(dev_alloc(P, &t.is_outlet, n));
-  FLCHK(dev_alloc(P, &t.rank, n));
-  FLCHK(dev_alloc(P, &t.order, n));
   bool clamp = false;
   for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
-  t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
-                     &w.post2, &w.post3})
-    FLCHK(dev_alloc(P, p, n));
-  if (clamp)
-    for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
-                       &w.perm, &w.ppos, &w.rpos})
-    FLCHK(dev_alloc(P, p, n));
-  FLCHK(dev_alloc(P, &w.cp, n));
-  FLCHK(dev_alloc(P, &w.rt, n));
-  FLCHK(dev_alloc(P, &w.cmask, n));
-  for (uint32_t** p : {&w.bcnt, &w.bnodes, &w.boff, &w.bnoff, &w.bcur, &w.bncur})
-    FLCHK(dev_alloc(P, p, 2 * (uint64_t)n + 4));
-  FLCHK(dev_alloc(P, &w.lake_key, n));
-  FLCHK(dev_alloc(P, &w.ctrl, FL_CTRL_WORDS));
-  w.flags = (int*)w.ctrl;
-  w.counters = w.ctrl + FL_NUM_FLAGS;
+  FLCHK(alloc_device(c, P, n, nnz, count, clamp));
   hipStream_t s = c->stream;
-  if (count > 1) {
-    FLCHK(dev_alloc(P, &T->d_tid, n));
-    FLCHK(dev_alloc(P, &T->d_active, count));
-    FLCHK(dev_alloc(P, &T->d_tchanged, count));
-    HIPCHK(hipHostMalloc((void**)&T->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
-    HIPCHK(hipMemcpyAsync(T->d_tid, tid.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
-  }
+  T->entry_off.resize(count + 1);
+  for (uint32_t g = 0; g <= count; g++) T->entry_off[g] = T->h_row_ptr[T->site_off[g]];
+  if (count > 1) HIPCHK(hipMemcpyAsync(T->d_tid, tid.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.row_ptr, T->h_row_ptr.data(), ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
   if (nnz) {
     HIPCHK(hipMemcpyAsync(t.col, T->h_col.data(), (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
@@ -572,15 +592,74 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   HIPCHK(hipMemcpyAsync(t.tan_ms, tan_ms.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.init_h, init.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(t.is_outlet, is_out.data(), n, hipMemcpyHostToDevice, s));
-  fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
-  fl_launch_fill_u32(s, w.bnodes, 2 * (uint64_t)n + 4, 0);
-  fl_launch_fill_u32(s, w.bcur, 2 * (uint64_t)n + 4, 0);
-  fl_launch_fill_u32(s, w.bncur, 2 * (uint64_t)n + 4, 0);
-  T->prev_nb = 0;
   FLCHK(reset_state(c, P));
   // the pageable staging vectors above must outlive the copies
   HIPCHK(hipStreamSynchronize(s));
   *out = T.release();
+  return FL_OK;
+}
+
+// A batch keeps sweeping the sites of terrains that stopped iterating; once
+// the running terrains hold less than half of the union (and at least
+// fl_ctx::compact_min_idle sites are idle), they are gathered into a smaller
+// union.
+
+// The running terrains of `cur` (a union derived from T, or T itself) gathered
+// on the device into a new union; orig maps cur's terrains to T's.  The
+// stopped terrains' elevations are copied to their ranges of T->w.h first.
+int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t* caps,
+                      const std::vector<uint32_t>& orig, fl_terrain** out, std::vector<uint32_t>& new_orig) {
+  hipStream_t s = c->stream;
+  std::vector<uint32_t> keep;
+  for (uint32_t g = 0; g < cur->nterr; g++) {
+    if (cur->t_active[g]) {
+      keep.push_back(g);
+    } else if (cur != T) {
+      uint32_t s0 = cur->site_off[g], ng = cur->site_off[g + 1] - s0;
+      if (ng)
+        HIPCHK(hipMemcpyAsync(T->w.h + T->site_off[orig[g]], cur->w.h + s0, (uint64_t)ng * 8,
+                              hipMemcpyDeviceToDevice, s));
+    }
+  }
+  uint32_t G = (uint32_t)keep.size();
+  std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> N(new fl_terrain(), fl_terrain_destroy);
+  N->ctx = c;
+  N->nterr = G;
+  N->site_off.assign(G + 1, 0);
+  N->entry_off.assign(G + 1, 0);
+  N->outlet_off.assign(G + 1, 0);
+  N->t_max_it.resize(G);
+  N->t_iters.resize(G);
+  N->t_active.assign(G, 1);
+  new_orig.resize(G);
+  std::vector<uint32_t> map(4 * (size_t)G + 1);  // new_off[G+1], old_off[G], new_eoff[G], old_eoff[G]
+  for (uint32_t q = 0; q < G; q++) {
+    uint32_t g = keep[q];
+    N->site_off[q + 1] = N->site_off[q] + (cur->site_off[g + 1] - cur->site_off[g]);
+    N->entry_off[q + 1] = N->entry_off[q] + (cur->entry_off[g + 1] - cur->entry_off[g]);
+    N->outlet_off[q + 1] = N->outlet_off[q] + (cur->outlet_off[g + 1] - cur->outlet_off[g]);
+    N->t_max_it[q] = caps[g];
+    N->t_iters[q] = cur->t_iters[g];
+    new_orig[q] = orig[g];
+  }
+  for (uint32_t q = 0; q <= G; q++) map[q] = N->site_off[q];
+  for (uint32_t q = 0; q < G; q++) {
+    map[G + 1 + q] = cur->site_off[keep[q]];
+    map[2 * G + 1 + q] = N->entry_off[q];
+    map[3 * G + 1 + q] = cur->entry_off[keep[q]];
+  }
+  uint32_t n = N->site_off[G], nnz = N->entry_off[G];
+  FLCHK(alloc_device(c, N.get(), n, nnz, G > 1 ? G : 2, cur->t.any_max_slope));
+  uint32_t* d_map = nullptr;
+  FLCHK(dev_alloc(N.get(), &d_map, map.size()));
+  HIPCHK(hipMemcpyAsync(d_map, map.data(), map.size() * 4, hipMemcpyHostToDevice, s));
+  fl_compact_map m{G, d_map, d_map + G + 1, d_map + 2 * G + 1, d_map + 3 * G + 1};
+  fl_launch_compact(s, m, cur->t, cur->w, N->t, N->w, N->d_tid);
+  HIPCHK(hipMemcpyAsync(N->d_active, N->t_active.data(), G, hipMemcpyHostToDevice, s));
+  N->have_order = true;
+  N->jump_rounds = cur->jump_rounds;
+  HIPCHK(hipStreamSynchronize(s));  // `map` is pageable
+  *out = N.release();
   return FL_OK;
 }
 
@@ -594,7 +673,9 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   T->stat_syncs = 0;
   T->stat_order_ms = 0.0;
   T->stat_lake_iterations = 0;
+  T->stat_compactions = 0;
   uint32_t active = 0;
+  uint64_t active_sites = 0;
   for (uint32_t g = 0; g < G; g++) {
     T->t_iters[g] = 0;
     bool empty = T->site_off[g + 1] == T->site_off[g];
@@ -602,6 +683,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     if (empty && max_it[g] > 0) T->t_iters[g] = 1;
     T->t_active[g] = (!empty && max_it[g] > 0) ? 1 : 0;
     active += T->t_active[g];
+    if (T->t_active[g]) active_sites += T->site_off[g + 1] - T->site_off[g];
   }
   if (T->t.n == 0) return FL_OK;
   FLCHK(reset_state(c, T));  // elevations start from the initial state even if nothing iterates
@@ -611,25 +693,55 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   }
   if (T->d_active)
     HIPCHK(hipMemcpyAsync(T->d_active, T->t_active.data(), G, hipMemcpyHostToDevice, c->stream));
+  fl_terrain* cur = T;
+  const uint32_t* caps = max_it;  // iteration caps of cur's terrains
+  std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> owned(nullptr, fl_terrain_destroy);
+  std::vector<uint32_t> orig(G);
+  for (uint32_t g = 0; g < G; g++) orig[g] = g;
   uint32_t it = 0;
   while (active) {
     bool changed = false, lake = false;
-    FLCHK(one_iteration(c, T, &changed, &lake));
+    if (cur != T) cur->stat_syncs = 0;
+    FLCHK(one_iteration(c, cur, &changed, &lake));
+    if (cur != T) T->stat_syncs += cur->stat_syncs;
     it++;
     T->stat_lake_iterations += lake ? 1 : 0;
     bool mask_changed = false;
-    for (uint32_t g = 0; g < G; g++) {
-      if (!T->t_active[g]) continue;
-      T->t_iters[g]++;
-      bool ch = T->d_tchanged ? T->h_tchanged[g] != 0 : changed;
-      if (!ch || T->t_iters[g] >= max_it[g]) {
-        T->t_active[g] = 0;
+    for (uint32_t g = 0; g < cur->nterr; g++) {
+      if (!cur->t_active[g]) continue;
+      cur->t_iters[g]++;
+      T->t_iters[orig[g]] = cur->t_iters[g];
+      bool ch = cur->d_tchanged ? cur->h_tchanged[g] != 0 : changed;
+      if (!ch || cur->t_iters[g] >= caps[g]) {
+        cur->t_active[g] = 0;
         active--;
+        active_sites -= cur->site_off[g + 1] - cur->site_off[g];
         mask_changed = true;
       }
     }
-    if (mask_changed && active && T->d_active)
-      HIPCHK(hipMemcpyAsync(T->d_active, T->t_active.data(), G, hipMemcpyHostToDevice, c->stream));
+    if (!active) break;
+    if (cur->nterr > 1 && active_sites * 2 < cur->t.n && cur->t.n - active_sites >= c->compact_min_idle) {
+      fl_terrain* next_union = nullptr;
+      std::vector<uint32_t> next_orig;
+      FLCHK(compact_union(c, T, cur, caps, orig, &next_union, next_orig));
+      owned.reset(next_union);  // destroys the previous derived union, if any
+      cur = next_union;
+      caps = cur->t_max_it.data();
+      orig.swap(next_orig);
+      T->stat_compactions++;
+      continue;
+    }
+    if (mask_changed && cur->d_active)
+      HIPCHK(hipMemcpyAsync(cur->d_active, cur->t_active.data(), cur->nterr, hipMemcpyHostToDevice, c->stream));
+  }
+  if (cur != T) {
+    for (uint32_t g = 0; g < cur->nterr; g++) {
+      uint32_t s0 = cur->site_off[g], ng = cur->site_off[g + 1] - s0;
+      if (ng)
+        HIPCHK(hipMemcpyAsync(T->w.h + T->site_off[orig[g]], cur->w.h + s0, (uint64_t)ng * 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   T->stat_iterations = it;
   T->stat_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -799,6 +911,12 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
   return FL_OK;
 }
 
+extern "C" int32_t fl_ctx_set_batch_compaction(fl_ctx* c, uint32_t min_idle_sites) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  c->compact_min_idle = min_idle_sites;
+  return FL_OK;
+}
+
 extern "C" int32_t fl_terrain_invalidate_cache(fl_terrain* T) {
   if (!T) return set_err(FL_ERR_INVALID, "null terrain");
   T->have_order = false;
@@ -849,6 +967,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->forest_ms = T->stat_order_parts[1];
   out->ranks_ms = T->stat_order_parts[2];
   out->solve_ms = T->stat_solve_ms;
+  out->compactions = T->stat_compactions;
   return FL_OK;
 }
 

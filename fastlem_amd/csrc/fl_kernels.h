@@ -124,6 +124,22 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);
+
+// Gather of whole terrains of a union into a smaller union (in the same
+// order, each keeping its internal layout): new terrain q takes the sites
+// [new_off[q], new_off[q+1]) from old sites starting at old_off[q], and its
+// entries from old entry old_eoff[q] to new entry new_eoff[q].
+struct fl_compact_map {
+  uint32_t count;
+  const uint32_t* new_off;   // [count + 1]
+  const uint32_t* old_off;   // [count]
+  const uint32_t* new_eoff;  // [count]
+  const uint32_t* old_eoff;  // [count]
+};
+// Copies the graph, parameters, visit ranks and order, current elevations
+// and drainage areas (shifting site and entry indices) and writes new_tid.
+void fl_launch_compact(hipStream_t s, const fl_compact_map& m, const fl_dev_terrain& ot, const fl_dev_work& ow,
+                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid);
 // Verification: the reference's BFS traversal of the basin of `outlet`.
 void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t outlet,
                            uint32_t* order, uint32_t* len_out);

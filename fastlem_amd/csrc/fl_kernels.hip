@@ -1015,6 +1015,58 @@ __global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm,
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
 
+// ----------------------------------------------------------- compaction ---
+
+namespace {
+// Terrain of new site v: the last q with new_off[q] <= v.
+__device__ __forceinline__ uint32_t terrain_of(const fl_compact_map& m, uint32_t v) {
+  uint32_t lo = 0, hi = m.count;  // new_off[lo] <= v < new_off[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (m.new_off[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t shift_site(uint32_t x, const fl_compact_map& m, uint32_t q) {
+  return x == FL_NONE ? FL_NONE : x - m.old_off[q] + m.new_off[q];
+}
+}  // namespace
+
+__global__ void k_compact_sites(uint32_t n_new, uint32_t nnz_new, fl_compact_map m, fl_dev_terrain ot,
+                                fl_dev_work ow, fl_dev_terrain nt, fl_dev_work nw, uint32_t* __restrict__ new_tid) {
+  uint32_t v = gid();
+  if (v >= n_new) return;
+  uint32_t q = terrain_of(m, v);
+  uint32_t src = v - m.new_off[q] + m.old_off[q];
+  nt.row_ptr[v] = ot.row_ptr[src] - m.old_eoff[q] + m.new_eoff[q];
+  if (v == n_new - 1) nt.row_ptr[n_new] = nnz_new;
+  nt.area[v] = ot.area[src];
+  nt.erod[v] = ot.erod[src];
+  nt.uplift[v] = ot.uplift[src];
+  nt.tan_ms[v] = ot.tan_ms[src];
+  nt.init_h[v] = ot.init_h[src];
+  nt.is_outlet[v] = ot.is_outlet[src];
+  nt.rank[v] = shift_site(ot.rank[src], m, q);
+  nt.order[v] = shift_site(ot.order[src], m, q);  // rank positions share the terrain's site range
+  nw.h[v] = ow.h[src];
+  nw.A[v] = ow.A[src];
+  new_tid[v] = q;
+}
+
+__global__ void k_compact_entries(uint32_t n_new, fl_compact_map m, fl_dev_terrain ot, fl_dev_terrain nt) {
+  uint32_t v = gid();
+  if (v >= n_new) return;
+  uint32_t q = terrain_of(m, v);
+  uint32_t src = v - m.new_off[q] + m.old_off[q];
+  uint32_t b = ot.row_ptr[src], e = ot.row_ptr[src + 1];
+  uint32_t d = m.new_eoff[q] - m.old_eoff[q];
+  for (uint32_t k = b; k < e; ++k) {
+    nt.col[k + d] = ot.col[k] - m.old_off[q] + m.new_off[q];
+    nt.len[k + d] = ot.len[k];
+  }
+}
+
 // Verification: the reference's traversal of one basin
 // (DrainageBasin::construct, drainage_basin.rs:13-36): start at the outlet,
 // then for each site in order append its children in row order.  One
@@ -1245,4 +1297,10 @@ void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_
                            uint32_t* order, uint32_t* len_out) {
   FL_LAUNCH_GRID(k_basin_order, dim3(1), dim3(1024), t.n, s, outlet, t.n, t.row_ptr, t.col, w.next,
                  (const unsigned long long*)w.cmask, order, len_out);
+}
+
+void fl_launch_compact(hipStream_t s, const fl_compact_map& m, const fl_dev_terrain& ot, const fl_dev_work& ow,
+                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid) {
+  FL_LAUNCH(k_compact_sites, nt.n, s, nt.n, nt.nnz, m, ot, ow, nt, nw, new_tid);
+  FL_LAUNCH(k_compact_entries, nt.n, s, nt.n, m, ot, nt);
 }

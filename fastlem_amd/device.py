@@ -184,9 +184,19 @@ class DeviceBatch:
         except Exception:
             pass
 
-    def solve(self) -> np.ndarray:
+    def solve(self, compact_min_idle: int = None) -> np.ndarray:
+        """Iteration counts per terrain.  compact_min_idle: idle sites before
+        the running terrains are gathered into a smaller union (the library's
+        default when None)."""
         its = np.zeros(len(self.sizes), np.uint32)
-        check(_native.lib().fl_batch_solve(self._ctx, self._h, ptr(its, ctypes.c_uint32)))
+        L = _native.lib()
+        if compact_min_idle is not None:
+            check(L.fl_ctx_set_batch_compaction(self._ctx, int(compact_min_idle)))
+        try:
+            check(L.fl_batch_solve(self._ctx, self._h, ptr(its, ctypes.c_uint32)))
+        finally:
+            if compact_min_idle is not None:
+                check(L.fl_ctx_set_batch_compaction(self._ctx, 1 << 16))
         return its
 
     def elevations(self, index: int) -> np.ndarray:

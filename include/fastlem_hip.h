@@ -103,6 +103,11 @@ int32_t fl_batch_create(fl_ctx *ctx, const fl_terrain_desc *descs, uint32_t coun
 int32_t fl_batch_solve(fl_ctx *ctx, fl_terrain *batch, uint32_t *iterations_out /* [count] or NULL */);
 int32_t fl_batch_download(fl_ctx *ctx, fl_terrain *batch, uint32_t index, double *elevations_out);
 int32_t fl_batch_count(const fl_terrain *batch, uint32_t *count);
+/* A batch solve gathers its still-running terrains into a smaller union when
+ * they hold less than half of the current one and at least min_idle_sites
+ * sites belong to terrains that have stopped (default 65536; UINT32_MAX never
+ * compacts).  Results do not depend on it. */
+int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
 
 /* Serving: a request loop on one GPU with continuous batching.  `slots`
  * worker threads each own a context (stream); a request is admitted to a free
@@ -133,6 +138,8 @@ typedef struct fl_solve_stats {
   double outlet_phase_ms;   /* visit order parts: outlet phase (host, runs */
   double forest_ms;         /*   concurrently with) the GPU spanning forest, */
   double ranks_ms;          /*   then the ranks from the forest (host) */
+  uint32_t compactions;     /* batch: times the running terrains were gathered */
+  uint32_t reserved;
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

@@ -62,7 +62,7 @@ def test_emulated_capped_oscillation(emu_lib):
     assert _bits_equal(got, ref)
 
 
-def _emu_solve(L, terrs):
+def _emu_solve(L, terrs, compact=None):
     """fl_batch_create/solve/download through the emulated library."""
     from cpu_emu import emu
     from fastlem_amd import _native
@@ -76,13 +76,18 @@ def _emu_solve(L, terrs):
     outs = []
     try:
         emu.check(L, L.fl_batch_create(ctx, descs, len(terrs), ctypes.byref(b)))
+        if compact is not None:
+            emu.check(L, L.fl_ctx_set_batch_compaction(ctx, compact))
         its = np.zeros(len(terrs), np.uint32)
         emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
         for i, t in enumerate(terrs):
             out = np.empty(len(t["row_ptr"]) - 1)
             emu.check(L, L.fl_batch_download(ctx, b, i, _native.ptr(out, ctypes.c_double)))
             outs.append(out)
+        st = _native.SolveStats()
+        emu.check(L, L.fl_terrain_last_stats(b, ctypes.byref(st)))
         L.fl_terrain_destroy(b)
+        _emu_solve.last_compactions = st.compactions
     finally:
         L.fl_ctx_destroy(ctx)
     return outs, its
@@ -116,3 +121,22 @@ def test_emulated_batch_matches_solo(emu_lib):
         ref, rit, _ = _oracle(w, cap=cap)
         assert its[i] == rit
         assert _bits_equal(outs[i], ref)
+
+
+def test_emulated_batch_compaction(emu_lib):
+    """Terrains that converge at different iterations, with a cap: the union
+    is compacted as they stop (forced at this size); every result and count
+    still equals its solo run."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm, config_plain
+
+    ws = [config_plain(400, 20.0), config_fbm(2500, 50.0, seed=9), config_plain(900, 30.0),
+          config_fbm(1200, 35.0, seed=4), config_plain(3000, 55.0)]
+    caps = [0xFFFFFFFF, 0xFFFFFFFF, 5, 0xFFFFFFFF, 0xFFFFFFFF]
+    terrs = [DeviceBatch.terrain_of(w, max_iteration=c) for w, c in zip(ws, caps)]
+    outs, its = _emu_solve(emu_lib, terrs, compact=1)
+    assert _emu_solve.last_compactions >= 1
+    for i, (w, cap) in enumerate(zip(ws, caps)):
+        ref, rit, _ = _oracle(w, cap=cap)
+        assert its[i] == rit, i
+        assert _bits_equal(outs[i], ref), i

@@ -357,3 +357,24 @@ def test_comb_many_light_children():
     ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
     assert it == rit
     assert _bits_equal(d.elevations(), ref)
+
+
+def test_batch_compaction_matches_solo_runs():
+    """The union is gathered into smaller ones as terrains stop (forced at
+    this size); results and counts equal the solo runs."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm, config_plain
+    from oracle import flo
+
+    ws = [config_plain(400, 20.0), config_fbm(2500, 50.0, seed=9), config_plain(900, 30.0),
+          config_fbm(1200, 35.0, seed=4), config_plain(3000, 55.0)]
+    caps = [0xFFFFFFFF, 0xFFFFFFFF, 5, 0xFFFFFFFF, 0xFFFFFFFF]
+    b = DeviceBatch([DeviceBatch.terrain_of(w, max_iteration=c) for w, c in zip(ws, caps)])
+    for rep in range(2):  # a second solve starts again from the full union
+        its = b.solve(compact_min_idle=1)
+        assert b.last_stats()["compactions"] >= 1
+        for i, (w, cap) in enumerate(zip(ws, caps)):
+            ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w),
+                                       w.outlets(), max_iteration=cap)
+            assert its[i] == rit, (rep, i)
+            assert _bits_equal(b.elevations(i), ref), (rep, i)
