@@ -516,7 +516,13 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
     uint32_t entry;  // site where Prim enters the subtree
     uint32_t off;    // rank of the first site Prim reaches in it
   };
-  std::vector<uint32_t> leaf(n), uf(n), top(n);
+  // union-find root record: parent, the component's current reconstruction
+  // node and its counts (kept at the root, so a merge reads no child node)
+  struct Root {
+    uint32_t parent, top, cnt, size;
+  };
+  std::vector<uint32_t> leaf(n);
+  std::unique_ptr<Root[]> uf(new Root[n]);
   std::unique_ptr<Node[]> node(new Node[total]);
   std::unique_ptr<Down[]> down(new Down[total]);
   parallel_range(n, threads, [&](size_t a, size_t b) {
@@ -532,21 +538,23 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
     }
   parallel_range(n, threads, [&](size_t a, size_t b) {
     for (size_t v = a; v < b; v++) {
-      uf[v] = leaf[v];
-      top[v] = (uint32_t)v;
+      uf[v] = {leaf[v], (uint32_t)v, 1u, 1u};
       node[v] = {NONE, NONE, 1u, 1u};
       down[v] = {NONE, NONE, 0};
     }
   });
   for (size_t g = 0; g < groups; g++)
-    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) node[outlets[q]].size = 0;
+    for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) {
+      node[outlets[q]].size = 0;
+      uf[outlets[q]].size = 0;
+    }
   parallel_range(m, threads, [&](size_t a, size_t b) {
     for (size_t i = a; i < b; i++) down[n + i] = {NONE, NONE, 0};
   });
   auto find = [&](uint32_t v) {
-    while (uf[v] != v) {
-      uf[v] = uf[uf[v]];
-      v = uf[v];
+    while (uf[v].parent != v) {
+      uf[v].parent = uf[uf[v].parent].parent;
+      v = uf[v].parent;
     }
     return v;
   };
@@ -557,22 +565,25 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
     if (a >= n || b >= n) return FL_ERR_INVALID;
     uint32_t ra = find(a), rb = find(b);
     if (ra == rb) return FL_ERR_INVALID;  // not a forest
+    Root& A = uf[ra];
+    Root& B = uf[rb];
     Node& x = node[n + i];
-    x.ch0 = top[ra];
-    x.ch1 = top[rb];
-    const Node& A = node[x.ch0];
-    const Node& B = node[x.ch1];
+    x.ch0 = A.top;  // the side of msf_a first
+    x.ch1 = B.top;
     x.cnt = A.cnt + B.cnt;
     x.size = A.size + B.size;
-    uf[rb] = ra;
-    top[ra] = (uint32_t)(n + i);
+    Root& keep = A.cnt >= B.cnt ? A : B;  // union by size
+    (A.cnt >= B.cnt ? B : A).parent = A.cnt >= B.cnt ? ra : rb;
+    keep.top = (uint32_t)(n + i);
+    keep.cnt = x.cnt;
+    keep.size = x.size;
     if (x.cnt > cut) big.push_back((uint32_t)(n + i));
   }
   std::vector<uint32_t> roots;
   for (size_t g = 0; g < groups; g++) {
     if (outlet_off[g + 1] == outlet_off[g]) continue;
     uint32_t s_leaf = outlets[outlet_off[g]];
-    uint32_t root = top[find(s_leaf)];
+    uint32_t root = uf[find(s_leaf)].top;
     down[root] = {0, s_leaf, rank_off[g] + (outlet_off[g + 1] - outlet_off[g])};
     roots.push_back(root);
   }
