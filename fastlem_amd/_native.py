@@ -49,7 +49,7 @@ class SolveStats(ctypes.Structure):
                 ("host_syncs", ctypes.c_uint32), ("order_from_forest", ctypes.c_uint32),
                 ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("outlet_phase_ms", ctypes.c_double), ("forest_ms", ctypes.c_double),
-                ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32)]
 
 
 class NativeError(RuntimeError):
@@ -90,6 +90,7 @@ def _declare(lib):
         "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
         "fl_ctx_set_batch_compaction": (i32, [vp, u32]),
+        "fl_ctx_set_pipelining": (i32, [vp, i32]),
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64),
                                      ctypes.POINTER(u64)]),
         "fl_ctx_reset_profile": (i32, [vp]),

@@ -60,6 +60,8 @@ struct ProfScope {
 struct fl_ctx {
   int device = 0;
   uint32_t compact_min_idle = 1u << 16;  // see solve_union
+  int pipelined = 1;                     // one_iteration_fast after the first iteration (2: minimal
+                                         // speculation, most iterations redone)
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -71,12 +73,14 @@ struct fl_terrain {
   fl_dev_terrain t;
   fl_dev_work w;
   uint32_t max_iteration = FL_UNBOUNDED_ITERATIONS;
-  uint32_t prev_nb = 0;
+  uint32_t prev_nb = 0;      // bucket entries the previous decomposition used (cleared next)
   uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
+  uint32_t last_rounds = 0;  // sweep rounds of the previous iteration
+  std::vector<uint32_t> last_boff;  // its bucket offsets (boff[nb+1], bnoff[nb+1]): launch-size estimates
   bool have_order = false;
   // statistics of the last solve
   uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0,
-           stat_compactions = 0;
+           stat_compactions = 0, stat_redone = 0;
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   uint64_t bytes = 0;
@@ -294,8 +298,9 @@ int32_t decompose(fl_ctx* c, fl_terrain* T) {
   }
   // quiet + 1 rounds suffice (the quiet round confirms convergence)
   T->jump_rounds = quiet + 1;
+  w.guard = w.ctrl + FL_CTRL_JUMPS + (round - 1);  // a round at or after the quiet one: zero
   fl_launch_chain_count(s, T->t, w);
-  fl_launch_chain_layout(s, T->t, w);
+  fl_launch_chain_layout(s, T->t, w, 0);
   FLCHK(read_ctrl(c, T));
   uint32_t nb = 2 * (host_counter(c, FL_CNT_MAXLD) + 1);  // short and long bucket per light depth
   if (nb > 2 * n + 2) return set_err(FL_ERR_HIP, "light depth out of range");
@@ -312,12 +317,28 @@ int32_t decompose(fl_ctx* c, fl_terrain* T) {
     HIPCHK(hipStreamSynchronize(s));
     T->stat_syncs++;
   }
+  T->last_boff = c->host_boff;
+  T->last_rounds = nb / 2;
   return FL_OK;
 }
 
-// One outer iteration (generator.rs:141-209).  The decomposition is built
-// speculatively before the lake test: lakes occur in the first iteration
-// only in practice, so the common path costs two host synchronisations.
+// Launch shape of round l from bucket offsets (boff[nb+1] then bnoff[nb+1]).
+fl_round_grid exact_grid(const std::vector<uint32_t>& off, uint32_t l) {
+  uint32_t nb = (uint32_t)off.size() / 2 - 1;
+  const uint32_t* bo = off.data();
+  const uint32_t* bn = bo + nb + 1;
+  fl_round_grid g{};
+  if (2 * l + 2 > nb) return g;
+  g.positions = bn[2 * l + 2] - bn[2 * l];
+  g.pre_blocks = (uint32_t)((g.positions + 255) / 256);
+  g.fold_blocks = (bo[2 * l + 2] - bo[2 * l + 1]) + (bo[2 * l + 1] - bo[2 * l] + 63) / 64;
+  return g;
+}
+
+// One outer iteration (generator.rs:141-209), synchronous form: the host
+// confirms pointer-jumping convergence and reads the bucket offsets before
+// the sweeps.  The decomposition is built speculatively before the lake test
+// (lakes occur in the first iteration only in practice).
 int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
@@ -332,22 +353,111 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
     FLCHK(decompose(c, T));
   }
   uint32_t nb = T->prev_nb;
-  const uint32_t* bo = c->host_boff.data();
-  const uint32_t* bn = bo + nb + 1;
-  uint32_t npos = bn[nb];
+  const std::vector<uint32_t>& off = c->host_boff;
+  uint32_t npos = off[2 * nb + 1];
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
-  for (uint32_t l = rounds; l-- > 0;)
-    fl_launch_da_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2],
-                       clamp);
-  for (uint32_t l = 0; l < rounds; l++)
-    fl_launch_tau_round(s, T->t, w, bo[2 * l], bo[2 * l + 1], bo[2 * l + 2], bn[2 * l], bn[2 * l + 1], bn[2 * l + 2],
-                        clamp);
+  for (uint32_t l = rounds; l-- > 0;) fl_launch_da_round(s, T->t, w, l, exact_grid(off, l), clamp);
+  for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, l, exact_grid(off, l), clamp);
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
   fl_launch_tau_scatter(s, T->t, w, npos, clamp, T->d_tid, T->d_active, T->d_tchanged);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   FLCHK(read_ctrl(c, T));
+  *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
+  std::swap(w.h, w.h2);
+  return FL_OK;
+}
+
+// One outer iteration enqueued whole, with one host synchronisation at its
+// end.  Three things are taken from the previous iteration instead of being
+// read back in between: the number of pointer-jumping rounds (one more is
+// launched; a round after a quiet one returns at once), the number of sweep
+// rounds (one more; rounds past the deepest light depth are empty, their
+// bounds read on the device) and the absence of lakes.  *valid is false when
+// any of them did not hold: the decomposition had not converged (then the
+// kernels after it did nothing), a lake was found, or the tree was deeper.
+// The input elevations are untouched then, and the caller redoes the
+// iteration with one_iteration.
+int32_t one_iteration_fast(fl_ctx* c, fl_terrain* T, bool* valid, bool* changed) {
+  hipStream_t s = c->stream;
+  fl_dev_work& w = T->w;
+  uint32_t n = T->t.n;
+  bool clamp = T->t.any_max_slope;
+  uint32_t J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
+  uint32_t R = std::max<uint32_t>(T->last_rounds, 1) + 1;
+  if (c->pipelined == 2) J = R = 1;
+  fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+  fl_launch_receivers(s, T->t, w);
+  fl_launch_children(s, T->t, w);
+  for (uint32_t r = 0; r < J; r++) fl_launch_chain_jump(s, T->t, w, r);
+  w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
+  fl_launch_chain_count(s, T->t, w);
+  fl_launch_chain_layout(s, T->t, w, R);
+  // launch sizes from the previous iteration's rounds (grid-stride kernels:
+  // any size is correct), profiler units filled in once the bounds are known
+  uint32_t pre_default = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 2048));
+  uint32_t fold_default = std::max<uint32_t>(1, std::min<uint32_t>((n + 127) / 128, 8192));
+  auto guess = [&](uint32_t l) {
+    fl_round_grid g = T->last_boff.empty() ? fl_round_grid{} : exact_grid(T->last_boff, l);
+    g.pre_blocks = g.pre_blocks ? std::min<uint32_t>(g.pre_blocks + g.pre_blocks / 4 + 8, pre_default) : 8;
+    g.fold_blocks = g.fold_blocks ? std::min<uint32_t>(g.fold_blocks + g.fold_blocks / 4 + 8, fold_default) : 8;
+    if (T->last_boff.empty()) {
+      g.pre_blocks = pre_default;
+      g.fold_blocks = fold_default;
+    }
+    g.positions = 0;
+    return g;
+  };
+  std::vector<size_t> rec_da(R, SIZE_MAX), rec_tau(R, SIZE_MAX);
+  for (uint32_t l = R; l-- > 0;) {
+    if (c->prof) rec_da[l] = c->prof->pending.size();
+    fl_launch_da_round(s, T->t, w, l, guess(l), clamp);
+  }
+  for (uint32_t l = 0; l < R; l++) {
+    if (c->prof) rec_tau[l] = c->prof->pending.size();
+    fl_launch_tau_round(s, T->t, w, l, guess(l), clamp);
+  }
+  if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
+  size_t rec_scatter = c->prof ? c->prof->pending.size() : SIZE_MAX;
+  fl_launch_tau_scatter(s, T->t, w, 0, clamp, T->d_tid, T->d_active, T->d_tchanged);
+  if (T->d_tchanged)
+    HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  FLCHK(read_ctrl(c, T));
+  bool lake = host_flag(c, FL_FLAG_LAKE) != 0;
+  bool converged = c->host_ctrl[FL_CTRL_JUMPS + J - 1] == 0;
+  uint32_t nb = 2 * (host_counter(c, FL_CNT_MAXLD) + 1);
+  if (!converged) {
+    // nothing after the decomposition ran; the histograms are untouched
+    T->jump_rounds = std::max(T->jump_rounds, J + 1);
+    *valid = false;
+    return FL_OK;
+  }
+  if (nb > 2 * n + 2) return set_err(FL_ERR_HIP, "light depth out of range");
+  T->prev_nb = nb;  // histogram entries to clear next time
+  for (uint32_t r = 0; r < J; r++)
+    if (c->host_ctrl[FL_CTRL_JUMPS + r] == 0) {
+      T->jump_rounds = r + 1;
+      break;
+    }
+  *valid = !lake && nb / 2 <= R;
+  if (!*valid) return FL_OK;
+  T->last_rounds = nb / 2;
+  if (nb <= FL_CTRL_HEAD) {
+    const uint32_t* hc = c->host_ctrl + FL_NUM_FLAGS + FL_NUM_COUNTERS;
+    T->last_boff.assign(hc, hc + nb + 1);
+    T->last_boff.insert(T->last_boff.end(), hc + FL_CTRL_HEAD + 1, hc + FL_CTRL_HEAD + 1 + nb + 1);
+    if (c->prof) {
+      for (uint32_t l = 0; l < R; l++) {
+        uint64_t units = exact_grid(T->last_boff, l).positions;
+        for (size_t q : {rec_da[l], rec_da[l] + 1, rec_tau[l]})
+          if (q < c->prof->pending.size()) c->prof->pending[q].units = units;
+      }
+      if (rec_scatter < c->prof->pending.size()) c->prof->pending[rec_scatter].units = T->last_boff[2 * nb + 1];
+    }
+  } else {
+    T->last_boff.clear();
+  }
   *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
   std::swap(w.h, w.h2);
   return FL_OK;
@@ -658,6 +768,7 @@ int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t*
   HIPCHK(hipMemcpyAsync(N->d_active, N->t_active.data(), G, hipMemcpyHostToDevice, s));
   N->have_order = true;
   N->jump_rounds = cur->jump_rounds;
+  N->last_rounds = cur->last_rounds;
   HIPCHK(hipStreamSynchronize(s));  // `map` is pageable
   *out = N.release();
   return FL_OK;
@@ -674,6 +785,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   T->stat_order_ms = 0.0;
   T->stat_lake_iterations = 0;
   T->stat_compactions = 0;
+  T->stat_redone = 0;
   uint32_t active = 0;
   uint64_t active_sites = 0;
   for (uint32_t g = 0; g < G; g++) {
@@ -700,9 +812,13 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   for (uint32_t g = 0; g < G; g++) orig[g] = g;
   uint32_t it = 0;
   while (active) {
-    bool changed = false, lake = false;
+    bool changed = false, lake = false, valid = false;
     if (cur != T) cur->stat_syncs = 0;
-    FLCHK(one_iteration(c, cur, &changed, &lake));
+    if (it > 0 && c->pipelined) {
+      FLCHK(one_iteration_fast(c, cur, &valid, &changed));
+      T->stat_redone += valid ? 0 : 1;
+    }
+    if (!valid) FLCHK(one_iteration(c, cur, &changed, &lake));
     if (cur != T) T->stat_syncs += cur->stat_syncs;
     it++;
     T->stat_lake_iterations += lake ? 1 : 0;
@@ -917,6 +1033,13 @@ extern "C" int32_t fl_ctx_set_batch_compaction(fl_ctx* c, uint32_t min_idle_site
   return FL_OK;
 }
 
+extern "C" int32_t fl_ctx_set_pipelining(fl_ctx* c, int32_t enable) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  if (enable < 0 || enable > 2) return set_err(FL_ERR_INVALID, "pipelining mode must be 0, 1 or 2");
+  c->pipelined = enable;
+  return FL_OK;
+}
+
 extern "C" int32_t fl_terrain_invalidate_cache(fl_terrain* T) {
   if (!T) return set_err(FL_ERR_INVALID, "null terrain");
   T->have_order = false;
@@ -968,6 +1091,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->ranks_ms = T->stat_order_parts[2];
   out->solve_ms = T->stat_solve_ms;
   out->compactions = T->stat_compactions;
+  out->redone = T->stat_redone;
   return FL_OK;
 }
 

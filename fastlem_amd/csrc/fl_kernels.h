@@ -100,6 +100,9 @@ struct fl_dev_work {
   uint32_t* head_base = nullptr;                             // [n]
   uint32_t* perm = nullptr;                                  // [n] chain-major site order
   uint32_t* ctrl = nullptr;                                  // [FL_CTRL_WORDS]
+  // movement flag of the last pointer-jumping round launched: the kernels
+  // after the decomposition do nothing unless it is zero (converged)
+  const uint32_t* guard = nullptr;
   int* flags = nullptr;                                      // = ctrl
   uint32_t* counters = nullptr;                              // = ctrl + FL_NUM_FLAGS
 };
@@ -111,17 +114,28 @@ void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-// Round l of a sweep: short chains [c0, c1) at positions [p0, p1), long
-// chains [c1, c2) at positions [p1, p2).
-// The drainage-area round also writes the response-time inputs of its
-// positions (and, with clamp, the max-slope recurrence's inputs).
-void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp);
+// Bucket offsets (scan), chain placement and the chain-major permutation;
+// offsets are written up to round `rounds` (rounds past the deepest light
+// depth are empty).
+void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t rounds);
+// Launch shape of one sweep round; the round's chain and position bounds are
+// read by the kernels from the bucket offsets (boff/bnoff at 2l..2l+2), so a
+// round can be enqueued before its bounds are known on the host.  Kernels
+// loop over their range, so any non-zero grid is correct.
+struct fl_round_grid {
+  uint32_t pre_blocks;   // k_da_pre, 256 threads each
+  uint32_t fold_blocks;  // k_da_fold / k_tau_fold, 64 threads each
+  uint64_t positions;    // profiler units (0 when not known at launch)
+};
+// Round l of the drainage-area sweep; also writes the response-time inputs
+// of its positions (and, with clamp, the max-slope recurrence's inputs).
+void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
+                        const fl_round_grid& g, bool clamp);
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
-void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp);
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
+void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
+                         const fl_round_grid& g, bool clamp);
+// Back to site order; the number of positions is read from the control block.
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint64_t units, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);
 

@@ -245,9 +245,12 @@ __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, cons
 // edges between v and the root = light depth of v's chain).  Movement is
 // flagged in the round's own control word (zeroed at the start of the
 // iteration), so rounds need no separate clearing launch; a round without
-// movement proves convergence (a pointer at a fixed point never moves).
+// movement proves convergence (a pointer at a fixed point never moves), and
+// the rounds launched after it return at once.
 __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long long* rt,
-                             uint32_t* __restrict__ flag_set) {
+                             uint32_t* __restrict__ flag_set, const uint32_t* __restrict__ prev_flag) {
+  // a round after a round without movement has nothing to do
+  if (prev_flag && *prev_flag == 0) return;
   uint32_t v = gid();
   bool moved = false;
   if (v < n) {
@@ -281,9 +284,10 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
 // Each chain is counted by its tail (the member without a heavy child),
 // which knows the chain's length (its position + 1) and light depth.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
+k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
@@ -322,10 +326,11 @@ k_bucket_count(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long 
 
 // Exclusive scans of the bucket histograms over the light depths actually
 // present (2 * (counters[FL_CNT_MAXLD] + 1) buckets), one workgroup.  The first
-// FL_CTRL_HEAD + 1 offsets are mirrored into the control block.
-__global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uint32_t* __restrict__ bcnt,
-                              const uint32_t* __restrict__ bnodes, uint32_t* __restrict__ boff,
-                              uint32_t* __restrict__ bnoff) {
+// FL_CTRL_HEAD + 1 offsets are mirrored into the control block; offsets up to
+// `fill` are written.
+__global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ ctrl,
+                              const uint32_t* __restrict__ bcnt, const uint32_t* __restrict__ bnodes,
+                              uint32_t* __restrict__ boff, uint32_t* __restrict__ bnoff) {
   __shared__ uint32_t s_a[1024], s_b[1024];
   __shared__ uint32_t carry_a, carry_b;
   uint32_t* counters = ctrl + FL_NUM_FLAGS;
@@ -378,6 +383,12 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uin
     counters[FL_CNT_CHAINS] = carry_a;
     counters[FL_CNT_POSITIONS] = carry_b;
   }
+  // offsets nb+1..fill repeat the totals: rounds launched beyond the deepest
+  // light depth find empty ranges
+  for (uint32_t i = nb + 1 + threadIdx.x; i <= fill; i += blockDim.x) {
+    boff[i] = carry_a;
+    bnoff[i] = carry_b;
+  }
 }
 
 // Give every chain (through its tail) a slot in its light-depth bucket and a
@@ -385,12 +396,13 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t* __restrict__ ctrl, const uin
 // one global atomic per (workgroup, bucket) reserves the ranges.  Order
 // inside a bucket is immaterial to the results.
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
-k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
+k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
                uint32_t* __restrict__ head_base) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
     s_c[t] = 0;
@@ -448,12 +460,13 @@ k_bucket_place(uint32_t n, const uint32_t* __restrict__ hc, const unsigned long 
 // outlet) and of the basin's outlet.  Sites outside every basin keep their
 // elevation, drainage area = own area and response time 0 (generator.rs:
 // 144-145), as the reference leaves them.
-__global__ void k_chain_perm(uint32_t n, const unsigned long long* __restrict__ cp,
+__global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
                              const unsigned long long* __restrict__ rt, const uint32_t* __restrict__ next,
                              const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
                              uint32_t* __restrict__ perm, uint32_t* __restrict__ ppos, uint32_t* __restrict__ rpos,
                              const double* __restrict__ area, double* __restrict__ A, double* __restrict__ tau,
                              const double* __restrict__ h, double* __restrict__ h2) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   uint32_t v = gid();
   if (v >= n) return;
   uint32_t r = pair_ptr(rt[v]);
@@ -564,28 +577,30 @@ __device__ __forceinline__ double da_post_rest(double r, uint32_t k, const uint3
   return r;
 }
 
-// Drainage area, part 1: the partial fold of every site of the round
-// (positions [p0, p1)), one thread per site.  post2/post3 are written only
-// where they are used (three or more posts).
-__global__ void k_da_pre(uint32_t p0, uint32_t p1, const uint32_t* __restrict__ perm,
+// Drainage area, part 1: the partial fold of every site of round l
+// (positions [bnoff[2l], bnoff[2l+2])), one thread per site.  post2/post3
+// are written only where they are used (three or more posts).
+__global__ void k_da_pre(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ bnoff, const uint32_t* __restrict__ perm,
                          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
                          const uint32_t* __restrict__ hc, const double* __restrict__ area,
                          const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
                          double* __restrict__ post1, double* __restrict__ post2, double* __restrict__ post3,
                          uint32_t* __restrict__ npost) {
-  uint32_t P = p0 + gid();
-  if (P >= p1) return;
-  double pr, a, b, c, d;
-  uint32_t np;
-  da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, a, b, c, d, np);
-  pre[P] = pr;
-  post0[P] = a;
-  post1[P] = b;
-  npost[P] = np;
-  if (np >= 3) {
-    post2[P] = c;
-    post3[P] = d;
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
+  const uint32_t p0 = bnoff[2 * l], p1 = bnoff[2 * l + 2];
+  for (uint32_t P = p0 + gid(); P < p1; P += gridDim.x * blockDim.x) {
+    double pr, a, b, c, d;
+    uint32_t np;
+    da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, a, b, c, d, np);
+    pre[P] = pr;
+    post0[P] = a;
+    post1[P] = b;
+    npost[P] = np;
+    if (np >= 3) {
+      post2[P] = c;
+      post3[P] = d;
+    }
   }
 }
 
@@ -645,20 +660,96 @@ __device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint
 //     positions with three or more posts marked by a ballot; the wavefront
 //     stores the results.
 __global__ void __launch_bounds__(64)
-k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ chain_base,
+k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm, const double* __restrict__ pre,
           const double* __restrict__ post0, const double* __restrict__ post1, const double* __restrict__ post2,
           const double* __restrict__ post3, const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
           const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
           const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A,
           fl_tau_in ti) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
   __shared__ uint32_t s_n[64], s_k[64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t nlong = c2 - c1;
-  if (blockIdx.x >= nlong) {
-    uint32_t c = c0 + (blockIdx.x - nlong) * 64u + lane;
-    if (c >= c1) return;
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  // long chains: one per block (uniform per block)
+  for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
+    const uint32_t base = chain_base[c], len = chain_len[c];
+    uint32_t end = len, cs = end > 64 ? end - 64 : 0;
+    double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
+    uint32_t np = 0, k = FL_NONE;
+    if (cs + lane < end) {
+      uint32_t P = base + cs + lane;
+      p = pre[P];
+      a = post0[P];
+      b = post1[P];
+      x = post2[P];  // meaningful where np >= 3 only
+      y = post3[P];
+      np = npost[P];
+      k = perm[P];
+    }
+    double r = FL_NEG0;  // lane 0's running value
+    while (end > 0) {
+      const uint32_t cnt = end - cs;
+      s_p[lane] = p;
+      s_a[lane] = a;
+      s_b[lane] = b;
+      s_c[lane] = x;
+      s_d[lane] = y;
+      s_n[lane] = np;
+      s_k[lane] = k;
+      const unsigned long long rare = __ballot(np >= 3);
+      __syncthreads();
+      const uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
+      p = a = b = x = y = 0.0;
+      np = 0;
+      k = FL_NONE;
+      if (ncs + lane < nend) {
+        uint32_t P = base + ncs + lane;
+        p = pre[P];
+        a = post0[P];
+        b = post1[P];
+        x = post2[P];
+        y = post3[P];
+        np = npost[P];
+        k = perm[P];
+      }
+      if (lane == 0) {
+        int e = (int)cnt - 1;
+        for (; e >= 7; e -= 8) {
+          double tp[8], ta[8], tb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            tp[u] = s_p[e - u];
+            ta[u] = s_a[e - u];
+            tb[u] = s_b[e - u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            r = ((tp[u] + r) + ta[u]) + tb[u];
+            if ((rare >> (e - u)) & 1ull)
+              r = da_tail_terms(r, s_n[e - u], s_c[e - u], s_d[e - u], s_k[e - u], rp, col, next, cmask, hc, A);
+            s_r[e - u] = r;
+          }
+        }
+        for (; e >= 0; --e) {
+          r = ((s_p[e] + r) + s_a[e]) + s_b[e];
+          if ((rare >> e) & 1ull) r = da_tail_terms(r, s_n[e], s_c[e], s_d[e], s_k[e], rp, col, next, cmask, hc, A);
+          s_r[e] = r;
+        }
+      }
+      __syncthreads();
+      if (lane < cnt) {
+        A[s_k[lane]] = s_r[lane];
+        tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane]);
+      }
+      end = nend;
+      cs = ncs;
+    }
+    __syncthreads();  // the tile arrays are reused by the next chain
+  }
+  // short chains: one lane each, the last blocks first (they hold no long chain)
+  for (uint32_t c = c0 + (gridDim.x - 1 - blockIdx.x) * 64u + lane; c < c1; c += gridDim.x * 64u) {
     uint32_t base = chain_base[c], len = chain_len[c];
     double r = FL_NEG0;
     uint32_t end = len;
@@ -689,80 +780,6 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
       }
       end = cs;
     }
-    return;
-  }
-  const uint32_t c = c1 + blockIdx.x;
-  const uint32_t base = chain_base[c], len = chain_len[c];
-  uint32_t end = len, cs = end > 64 ? end - 64 : 0;
-  double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
-  uint32_t np = 0, k = FL_NONE;
-  if (cs + lane < end) {
-    uint32_t P = base + cs + lane;
-    p = pre[P];
-    a = post0[P];
-    b = post1[P];
-    x = post2[P];  // meaningful where np >= 3 only
-    y = post3[P];
-    np = npost[P];
-    k = perm[P];
-  }
-  double r = FL_NEG0;  // lane 0's running value
-  while (end > 0) {
-    const uint32_t cnt = end - cs;
-    s_p[lane] = p;
-    s_a[lane] = a;
-    s_b[lane] = b;
-    s_c[lane] = x;
-    s_d[lane] = y;
-    s_n[lane] = np;
-    s_k[lane] = k;
-    const unsigned long long rare = __ballot(np >= 3);
-    __syncthreads();
-    const uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
-    p = a = b = x = y = 0.0;
-    np = 0;
-    k = FL_NONE;
-    if (ncs + lane < nend) {
-      uint32_t P = base + ncs + lane;
-      p = pre[P];
-      a = post0[P];
-      b = post1[P];
-      x = post2[P];
-      y = post3[P];
-      np = npost[P];
-      k = perm[P];
-    }
-    if (lane == 0) {
-      int e = (int)cnt - 1;
-      for (; e >= 7; e -= 8) {
-        double tp[8], ta[8], tb[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          tp[u] = s_p[e - u];
-          ta[u] = s_a[e - u];
-          tb[u] = s_b[e - u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          r = ((tp[u] + r) + ta[u]) + tb[u];
-          if ((rare >> (e - u)) & 1ull)
-            r = da_tail_terms(r, s_n[e - u], s_c[e - u], s_d[e - u], s_k[e - u], rp, col, next, cmask, hc, A);
-          s_r[e - u] = r;
-        }
-      }
-      for (; e >= 0; --e) {
-        r = ((s_p[e] + r) + s_a[e]) + s_b[e];
-        if ((rare >> e) & 1ull) r = da_tail_terms(r, s_n[e], s_c[e], s_d[e], s_k[e], rp, col, next, cmask, hc, A);
-        s_r[e] = r;
-      }
-    }
-    __syncthreads();
-    if (lane < cnt) {
-      A[s_k[lane]] = s_r[lane];
-      tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane]);
-    }
-    end = nend;
-    cs = ncs;
   }
 }
 
@@ -777,17 +794,131 @@ k_da_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ ch
 // each (the remaining blocks).
 template <bool clamp>
 __global__ void __launch_bounds__(64)
-k_tau_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ chain_base,
+k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
            const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
            const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
            const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
            const double* __restrict__ hop, double* __restrict__ hnp) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t nlong = c2 - c1;
-  if (blockIdx.x >= nlong) {
-    uint32_t c = c0 + (blockIdx.x - nlong) * 64u + lane;
-    if (c >= c1) return;
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  // long chains: one per block (uniform per block)
+  for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
+    const uint32_t base = chain_base[c], len = chain_len[c];
+    // lane 0's recurrence state
+    double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
+    const uint32_t pp = ppos[base];
+    if (lane == 0 && pp != FL_NONE) {
+      tau = taup[pp];
+      if (clamp) {
+        uint32_t ro = rpos[base];
+        tau_o = taup[ro];
+        h_o = hnp[ro];
+        h_p = hnp[pp];
+      }
+    }
+    double x = 0.0, u = 0.0, tn = 0.0, d = 0.0;
+    if (lane < len) {
+      x = xp[base + lane];
+      if (clamp) {
+        u = up[base + lane];
+        tn = tanp[base + lane];
+        d = dp[base + lane];
+      }
+    }
+    for (uint32_t t = 0; t < len; t += 64) {
+      const uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
+      s_x[lane] = x;
+      if (clamp) {
+        s_u[lane] = u;
+        s_t[lane] = tn;
+        s_d[lane] = d;
+      }
+      __syncthreads();
+      x = u = tn = d = 0.0;
+      if (t + 64 + lane < len) {
+        uint32_t P = base + t + 64 + lane;
+        x = xp[P];
+        if (clamp) {
+          u = up[P];
+          tn = tanp[P];
+          d = dp[P];
+        }
+      }
+      if (lane == 0) {
+        uint32_t j = 0;
+        if (t == 0 && pp == FL_NONE) {
+          // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
+          tau = 0.0 + (0.0 + s_x[0]);
+          s_tau[0] = tau;
+          if (clamp) {
+            double ho = hop[base];
+            double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
+            double slope = (nh - ho) / s_d[0];
+            if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
+            s_h[0] = nh;
+            h_p = nh;
+            h_o = nh;
+            tau_o = tau;
+          }
+          j = 1;
+        }
+        if (!clamp) {
+          while (j < cnt) {
+            const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
+            double tx[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q)
+              if (q < m) tx[q] = s_x[j + q];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+              if (q < m) {
+                tau = 0.0 + (tau + tx[q]);
+                s_tau[j + q] = tau;
+              }
+            }
+            j += m;
+          }
+        } else {
+          while (j < cnt) {
+            const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
+            double tx[8], tu[8], tt[8], td[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+              if (q < m) {
+                tx[q] = s_x[j + q];
+                tu[q] = s_u[j + q];
+                tt[q] = s_t[j + q];
+                td[q] = s_d[j + q];
+              }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+              if (q < m) {
+                tau = 0.0 + (tau + tx[q]);
+                double nh = h_o + tu[q] * rust_max(tau - tau_o, 0.0);
+                double slope = (nh - h_p) / td[q];
+                if (slope > tt[q]) nh = h_p + tt[q] * td[q];
+                s_tau[j + q] = tau;
+                s_h[j + q] = nh;
+                h_p = nh;
+              }
+            }
+            j += m;
+          }
+        }
+      }
+      __syncthreads();
+      if (lane < cnt) {
+        taup[base + t + lane] = s_tau[lane];
+        if (clamp) hnp[base + t + lane] = s_h[lane];
+      }
+    }
+    __syncthreads();  // the tile arrays are reused by the next chain
+  }
+  // short chains: one lane each, the last blocks first (they hold no long chain)
+  for (uint32_t c = c0 + (gridDim.x - 1 - blockIdx.x) * 64u + lane; c < c1; c += gridDim.x * 64u) {
     uint32_t base = chain_base[c], len = chain_len[c];
     uint32_t pp = ppos[base];
     double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
@@ -861,118 +992,6 @@ k_tau_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ c
         t += cnt;
       }
     }
-    return;
-  }
-  const uint32_t c = c1 + blockIdx.x;
-  const uint32_t base = chain_base[c], len = chain_len[c];
-  // lane 0's recurrence state
-  double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
-  const uint32_t pp = ppos[base];
-  if (lane == 0 && pp != FL_NONE) {
-    tau = taup[pp];
-    if (clamp) {
-      uint32_t ro = rpos[base];
-      tau_o = taup[ro];
-      h_o = hnp[ro];
-      h_p = hnp[pp];
-    }
-  }
-  double x = 0.0, u = 0.0, tn = 0.0, d = 0.0;
-  if (lane < len) {
-    x = xp[base + lane];
-    if (clamp) {
-      u = up[base + lane];
-      tn = tanp[base + lane];
-      d = dp[base + lane];
-    }
-  }
-  for (uint32_t t = 0; t < len; t += 64) {
-    const uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
-    s_x[lane] = x;
-    if (clamp) {
-      s_u[lane] = u;
-      s_t[lane] = tn;
-      s_d[lane] = d;
-    }
-    __syncthreads();
-    x = u = tn = d = 0.0;
-    if (t + 64 + lane < len) {
-      uint32_t P = base + t + 64 + lane;
-      x = xp[P];
-      if (clamp) {
-        u = up[P];
-        tn = tanp[P];
-        d = dp[P];
-      }
-    }
-    if (lane == 0) {
-      uint32_t j = 0;
-      if (t == 0 && pp == FL_NONE) {
-        // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
-        tau = 0.0 + (0.0 + s_x[0]);
-        s_tau[0] = tau;
-        if (clamp) {
-          double ho = hop[base];
-          double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
-          double slope = (nh - ho) / s_d[0];
-          if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
-          s_h[0] = nh;
-          h_p = nh;
-          h_o = nh;
-          tau_o = tau;
-        }
-        j = 1;
-      }
-      if (!clamp) {
-        while (j < cnt) {
-          const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
-          double tx[8];
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q)
-            if (q < m) tx[q] = s_x[j + q];
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q) {
-            if (q < m) {
-              tau = 0.0 + (tau + tx[q]);
-              s_tau[j + q] = tau;
-            }
-          }
-          j += m;
-        }
-      } else {
-        while (j < cnt) {
-          const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
-          double tx[8], tu[8], tt[8], td[8];
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q) {
-            if (q < m) {
-              tx[q] = s_x[j + q];
-              tu[q] = s_u[j + q];
-              tt[q] = s_t[j + q];
-              td[q] = s_d[j + q];
-            }
-          }
-#pragma unroll
-          for (uint32_t q = 0; q < 8; ++q) {
-            if (q < m) {
-              tau = 0.0 + (tau + tx[q]);
-              double nh = h_o + tu[q] * rust_max(tau - tau_o, 0.0);
-              double slope = (nh - h_p) / td[q];
-              if (slope > tt[q]) nh = h_p + tt[q] * td[q];
-              s_tau[j + q] = tau;
-              s_h[j + q] = nh;
-              h_p = nh;
-            }
-          }
-          j += m;
-        }
-      }
-    }
-    __syncthreads();
-    if (lane < cnt) {
-      taup[base + t + lane] = s_tau[lane];
-      if (clamp) hnp[base + t + lane] = s_h[lane];
-    }
   }
 }
 
@@ -981,13 +1000,15 @@ k_tau_fold(uint32_t c0, uint32_t c1, uint32_t c2, const uint32_t* __restrict__ c
 // h_o' = h_o + U_o * max(0, 0), generator.rs:178-180) and the changed flag
 // (generator.rs:201).  In a batch (tid != null) a terrain that stopped
 // iterating keeps its elevations and each terrain has its own flag.
-__global__ void k_tau_scatter(uint32_t npos, const uint32_t* __restrict__ perm,
+__global__ void k_tau_scatter(const uint32_t* __restrict__ guard, const uint32_t* __restrict__ npos_ptr, const uint32_t* __restrict__ perm,
                               const unsigned long long* __restrict__ rt,
                               const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ active, uint32_t* __restrict__ tchanged) {
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
+  const uint32_t npos = *npos_ptr;  // positions of the iteration's chains
   uint32_t P = gid();
   bool changed = false;
   if (P < npos) {
@@ -1154,8 +1175,7 @@ hipEvent_t fl_profiler::get() {
 }
 
 void fl_profiler::record(hipStream_t s, const char* name, hipEvent_t a, hipEvent_t b, uint64_t units) {
-  pending.push_back({name, a, b, units});
-  if (pending.size() >= 8192) flush();
+  pending.push_back({name, a, b, units});  // flushed on query: the host may fill in units later
   (void)s;
 }
 
@@ -1233,55 +1253,58 @@ void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 }
 
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round) {
-  uint32_t* fset = w.ctrl + FL_CTRL_JUMPS + (round < FL_MAX_JUMP_ROUNDS ? round : FL_MAX_JUMP_ROUNDS - 1);
-  FL_LAUNCH(k_chain_jump, t.n, s, t.n, (unsigned long long*)w.cp, (unsigned long long*)w.rt, fset);
+  uint32_t r = round < FL_MAX_JUMP_ROUNDS ? round : FL_MAX_JUMP_ROUNDS - 1;
+  uint32_t* fset = w.ctrl + FL_CTRL_JUMPS + r;
+  const uint32_t* prev = r > 0 ? w.ctrl + FL_CTRL_JUMPS + r - 1 : nullptr;
+  FL_LAUNCH(k_chain_jump, t.n, s, t.n, (unsigned long long*)w.cp, (unsigned long long*)w.rt, fset, prev);
 }
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc,
+  FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
 }
 
-void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), 0, s, t.n, w.ctrl, w.bcnt, w.bnodes, w.boff, w.bnoff);
+void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t rounds) {
+  uint64_t fill = std::min<uint64_t>(2ull * rounds, 2ull * t.n + 2);
+  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), 0, s, t.n, (uint32_t)fill, w.ctrl, w.bcnt, w.bnodes, w.boff,
+                 w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
-  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, t.n, w.hc,
+  FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
-  FL_LAUNCH(k_chain_perm, t.n, s, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
+  FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
             t.area, w.A, w.tau, w.h, w.h2);
 }
 
-void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                        uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp) {
-  (void)p1;
+void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
+                        const fl_round_grid& g, bool clamp) {
   fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
-  FL_LAUNCH(k_da_pre, p2 - p0, s, p0, p2, w.perm, t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask,
-            w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.post2, w.post3, w.npost);
-  uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
-  if (blocks)
-    FL_LAUNCH_GRID(k_da_fold, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len, w.perm,
-                   w.pre, w.post0, w.post1, w.post2, w.post3, w.npost, t.row_ptr, t.col, w.next,
+  if (g.pre_blocks)
+    FL_LAUNCH_GRID(k_da_pre, dim3(g.pre_blocks), dim3(256), g.positions, s, w.guard, l, w.bnoff, w.perm, t.row_ptr, t.col,
+                   w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.post2,
+                   w.post3, w.npost);
+  if (g.fold_blocks)
+    FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base, w.chain_len,
+                   w.perm, w.pre, w.post0, w.post1, w.post2, w.post3, w.npost, t.row_ptr, t.col, w.next,
                    (const unsigned long long*)w.cmask, w.hc, w.A, ti);
 }
 
-void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t c0, uint32_t c1,
-                         uint32_t c2, uint32_t p0, uint32_t p1, uint32_t p2, bool clamp) {
+void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
+                         const fl_round_grid& g, bool clamp) {
   (void)t;
-  (void)p1;
-  uint32_t blocks = (c2 - c1) + (c1 - c0 + 63) / 64;
-  if (blocks && clamp)
-    FL_LAUNCH_GRID(k_tau_fold<true>, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len,
-                   w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
-  else if (blocks)
-    FL_LAUNCH_GRID(k_tau_fold<false>, dim3(blocks), dim3(64), p2 - p0, s, c0, c1, c2, w.chain_base, w.chain_len,
-                   w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
+  if (!g.fold_blocks) return;
+  if (clamp)
+    FL_LAUNCH_GRID(k_tau_fold<true>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
+                   w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
+  else
+    FL_LAUNCH_GRID(k_tau_fold<false>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
+                   w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
 }
 
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t npos, bool clamp,
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint64_t units, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
-  FL_LAUNCH(k_tau_scatter, npos, s, npos, w.perm, (const unsigned long long*)w.rt, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau,
-            w.flags, clamp ? 1 : 0, tid, active, tchanged);
+  FL_LAUNCH_U(k_tau_scatter, t.n, units, s, w.guard, w.counters + FL_CNT_POSITIONS, w.perm, (const unsigned long long*)w.rt,
+              w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, active, tchanged);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {

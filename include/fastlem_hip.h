@@ -108,6 +108,13 @@ int32_t fl_batch_count(const fl_terrain *batch, uint32_t *count);
  * sites belong to terrains that have stopped (default 65536; UINT32_MAX never
  * compacts).  Results do not depend on it. */
 int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
+/* Iterations after the first are enqueued whole with one host
+ * synchronisation each, their pointer-jumping and sweep round counts taken
+ * from the previous iteration (and redone synchronously when that did not
+ * hold).  enable = 0 synchronises inside every iteration instead; 2
+ * speculates a single round of each kind, so that most iterations take the
+ * redo path (for testing it).  Results do not depend on it. */
+int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
 
 /* Serving: a request loop on one GPU with continuous batching.  `slots`
  * worker threads each own a context (stream); a request is admitted to a free
@@ -139,7 +146,7 @@ typedef struct fl_solve_stats {
   double forest_ms;         /*   concurrently with) the GPU spanning forest, */
   double ranks_ms;          /*   then the ranks from the forest (host) */
   uint32_t compactions;     /* batch: times the running terrains were gathered */
-  uint32_t reserved;
+  uint32_t redone;          /* pipelined iterations redone synchronously */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

@@ -140,3 +140,44 @@ def test_emulated_batch_compaction(emu_lib):
         ref, rit, _ = _oracle(w, cap=cap)
         assert its[i] == rit, i
         assert _bits_equal(outs[i], ref), i
+
+
+@pytest.mark.parametrize("kind", ["fbm", "strip"])
+def test_emulated_pipelining_modes_agree(emu_lib, kind):
+    """The pipelined iteration (one host synchronisation, round counts taken
+    from the previous iteration), its redo path (mode 2 speculates one round
+    of each kind, so most iterations are redone) and the synchronous one give
+    the same bits, counts and oracle results."""
+    from cpu_emu import emu
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceBatch, batch_descs
+    from fastlem_amd.workloads import config_fbm, config_strip
+
+    w = config_fbm(3000, 55.0, seed=5) if kind == "fbm" else config_strip(48, 24)
+    L = emu_lib
+    runs = {}
+    for mode in (0, 1, 2):
+        keep = []
+        d = batch_descs([DeviceBatch.terrain_of(w)], keep)
+        ctx = ctypes.c_void_p()
+        emu.check(L, L.fl_ctx_create(0, ctypes.byref(ctx)))
+        try:
+            emu.check(L, L.fl_ctx_set_pipelining(ctx, mode))
+            t = ctypes.c_void_p()
+            emu.check(L, L.fl_terrain_create(ctx, ctypes.byref(d[0]), ctypes.byref(t)))
+            it = ctypes.c_uint32()
+            emu.check(L, L.fl_terrain_solve(ctx, t, 0xFFFFFFFF, ctypes.byref(it)))
+            out = np.empty(w.n)
+            emu.check(L, L.fl_terrain_download(ctx, t, _native.ptr(out, ctypes.c_double)))
+            st = _native.SolveStats()
+            emu.check(L, L.fl_terrain_last_stats(t, ctypes.byref(st)))
+            L.fl_terrain_destroy(t)
+        finally:
+            L.fl_ctx_destroy(ctx)
+        runs[mode] = (it.value, out, st.host_syncs, st.redone)
+    ref, rit, _ = _oracle(w)
+    for mode in (0, 1, 2):
+        assert runs[mode][0] == rit
+        assert _bits_equal(runs[mode][1], ref)
+    assert runs[1][2] < runs[0][2]  # fewer host synchronisations
+    assert runs[2][3] > 0  # the redo path ran

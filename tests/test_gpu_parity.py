@@ -378,3 +378,30 @@ def test_batch_compaction_matches_solo_runs():
                                        w.outlets(), max_iteration=cap)
             assert its[i] == rit, (rep, i)
             assert _bits_equal(b.elevations(i), ref), (rep, i)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_pipelining_modes_agree(fbm20k, mode):
+    """Synchronous iterations (mode 0) and minimal speculation with most
+    iterations redone (mode 2) against the oracle; mode 1 (the default) is
+    what every other test runs."""
+    import ctypes
+
+    from fastlem_amd import _native
+    from fastlem_amd.generator import context
+    from oracle import flo
+
+    w = fbm20k
+    L = _native.lib()
+    _native.check(L.fl_ctx_set_pipelining(context(), mode))
+    try:
+        t = _dev(w)
+        it = t.solve()
+        st = t.last_stats()
+    finally:
+        _native.check(L.fl_ctx_set_pipelining(context(), 1))
+    ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+    assert it == rit
+    assert _bits_equal(t.elevations(), ref)
+    if mode == 2:
+        assert st["redone"] > 0
