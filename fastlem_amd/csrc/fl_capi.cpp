@@ -62,6 +62,7 @@ struct fl_ctx {
   uint32_t compact_min_idle = 1u << 16;  // see solve_union
   int pipelined = 1;                     // one_iteration_fast after the first iteration (2: minimal
                                          // speculation, most iterations redone)
+  bool graphs = true;                    // pipelined iterations replayed from captured graphs
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -77,6 +78,12 @@ struct fl_terrain {
   uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
   uint32_t last_rounds = 0;  // sweep rounds of the previous iteration
   std::vector<uint32_t> last_boff;  // its bucket offsets (boff[nb+1], bnoff[nb+1]): launch-size estimates
+  // captured pipelined iterations, one per elevation-buffer parity, for the
+  // (jump rounds, sweep rounds) they were captured with
+  double* hbuf0 = nullptr;  // the buffer w.h held at creation
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  uint32_t gJ[2] = {0, 0}, gR[2] = {0, 0};
+  bool graphs_failed = false;
   bool have_order = false;
   // statistics of the last solve
   uint32_t stat_iterations = 0, stat_lake_iterations = 0, stat_syncs = 0, stat_order_fast = 0,
@@ -379,15 +386,17 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
 // kernels after it did nothing), a lake was found, or the tree was deeper.
 // The input elevations are untouched then, and the caller redoes the
 // iteration with one_iteration.
-int32_t one_iteration_fast(fl_ctx* c, fl_terrain* T, bool* valid, bool* changed) {
+// The kernels of one pipelined iteration (J jump rounds, R sweep rounds),
+// enqueued on the context's stream or recorded by a capture.  rec_* receive
+// the profiler records of the rounds (when profiling).
+void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::vector<size_t>* rec_da,
+                       std::vector<size_t>* rec_tau, size_t* rec_scatter) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
   bool clamp = T->t.any_max_slope;
-  uint32_t J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
-  uint32_t R = std::max<uint32_t>(T->last_rounds, 1) + 1;
-  if (c->pipelined == 2) J = R = 1;
-  fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+  // the previous iteration used 2 * (R - 1) bucket entries at most
+  fl_launch_clear_iter(s, w, std::min<uint32_t>(2 * R + 1, 2 * n + 2));
   fl_launch_receivers(s, T->t, w);
   fl_launch_children(s, T->t, w);
   for (uint32_t r = 0; r < J; r++) fl_launch_chain_jump(s, T->t, w, r);
@@ -409,18 +418,74 @@ int32_t one_iteration_fast(fl_ctx* c, fl_terrain* T, bool* valid, bool* changed)
     g.positions = 0;
     return g;
   };
-  std::vector<size_t> rec_da(R, SIZE_MAX), rec_tau(R, SIZE_MAX);
   for (uint32_t l = R; l-- > 0;) {
-    if (c->prof) rec_da[l] = c->prof->pending.size();
+    if (c->prof && rec_da) (*rec_da)[l] = c->prof->pending.size();
     fl_launch_da_round(s, T->t, w, l, guess(l), clamp);
   }
   for (uint32_t l = 0; l < R; l++) {
-    if (c->prof) rec_tau[l] = c->prof->pending.size();
+    if (c->prof && rec_tau) (*rec_tau)[l] = c->prof->pending.size();
     fl_launch_tau_round(s, T->t, w, l, guess(l), clamp);
   }
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
-  size_t rec_scatter = c->prof ? c->prof->pending.size() : SIZE_MAX;
+  if (c->prof && rec_scatter) *rec_scatter = c->prof->pending.size();
   fl_launch_tau_scatter(s, T->t, w, 0, clamp, T->d_tid, T->d_active, T->d_tchanged);
+}
+
+// The pipelined iteration for (J, R) and the current buffer parity as a
+// graph, captured on first use.  False when capture is not possible (the
+// caller launches directly).
+bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
+  if (T->graphs_failed) return false;
+  int p = T->w.h == T->hbuf0 ? 0 : 1;
+  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R) {
+    if (T->gexec[p]) {
+      hipGraphExecDestroy(T->gexec[p]);
+      T->gexec[p] = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      T->graphs_failed = true;
+      return false;
+    }
+    enqueue_fast_body(c, T, J, R, nullptr, nullptr, nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&T->gexec[p], graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      T->gexec[p] = nullptr;
+      T->graphs_failed = true;
+      (void)hipGetLastError();
+      return false;
+    }
+    T->gJ[p] = J;
+    T->gR[p] = R;
+  }
+  return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
+}
+
+// One outer iteration enqueued whole, with one host synchronisation at its
+// end.  Three things are taken from the previous iteration instead of being
+// read back in between: the number of pointer-jumping rounds (one more is
+// launched; a round after a quiet one returns at once), the number of sweep
+// rounds (one more; rounds past the deepest light depth are empty, their
+// bounds read on the device) and the absence of lakes.  *valid is false when
+// any of them did not hold: the decomposition had not converged (then the
+// kernels after it did nothing), a lake was found, or the tree was deeper.
+// The input elevations are untouched then, and the caller redoes the
+// iteration with one_iteration.  Without profiling the kernels are replayed
+// from a captured graph.
+int32_t one_iteration_fast(fl_ctx* c, fl_terrain* T, bool* valid, bool* changed) {
+  hipStream_t s = c->stream;
+  fl_dev_work& w = T->w;
+  uint32_t n = T->t.n;
+  uint32_t J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
+  uint32_t R = std::max<uint32_t>(T->last_rounds, 1) + 1;
+  if (c->pipelined == 2) J = R = 1;
+  std::vector<size_t> rec_da(R, SIZE_MAX), rec_tau(R, SIZE_MAX);
+  size_t rec_scatter = SIZE_MAX;
+  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, J, R);
+  if (!graphed) enqueue_fast_body(c, T, J, R, &rec_da, &rec_tau, &rec_scatter);
+  w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   FLCHK(read_ctrl(c, T));
@@ -508,6 +573,8 @@ extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
 extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   if (!T) return FL_OK;
   if (T->ctx) hipSetDevice(T->ctx->device);
+  for (hipGraphExec_t& e : T->gexec)
+    if (e) hipGraphExecDestroy(e);
   for (void* p : T->allocations) hipFree(p);
   if (T->h_tchanged) hipHostFree(T->h_tchanged);
   delete T;
@@ -574,6 +641,7 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
   for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
                      &w.post2, &w.post3})
     FLCHK(dev_alloc(P, p, n));
+  P->hbuf0 = w.h;
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
   for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
@@ -1035,8 +1103,9 @@ extern "C" int32_t fl_ctx_set_batch_compaction(fl_ctx* c, uint32_t min_idle_site
 
 extern "C" int32_t fl_ctx_set_pipelining(fl_ctx* c, int32_t enable) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
-  if (enable < 0 || enable > 2) return set_err(FL_ERR_INVALID, "pipelining mode must be 0, 1 or 2");
-  c->pipelined = enable;
+  if (enable < 0 || enable > 3) return set_err(FL_ERR_INVALID, "pipelining mode must be 0, 1, 2 or 3");
+  c->pipelined = enable == 3 ? 1 : enable;
+  c->graphs = enable != 3;
   return FL_OK;
 }
 

@@ -465,9 +465,21 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
                              const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
                              uint32_t* __restrict__ perm, uint32_t* __restrict__ ppos, uint32_t* __restrict__ rpos,
                              const double* __restrict__ area, double* __restrict__ A, double* __restrict__ tau,
-                             const double* __restrict__ h, double* __restrict__ h2) {
+                             const double* __restrict__ h, double* __restrict__ h2,
+                             const uint32_t* __restrict__ maxld, uint32_t* __restrict__ bcnt,
+                             uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
+                             uint32_t* __restrict__ bncur) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   uint32_t v = gid();
+  // the bucket histograms are used up (count, scan, place): clear the
+  // entries this decomposition touched for the next one
+  const uint32_t nb = 2u * (*maxld + 1u);
+  for (uint32_t i = v; i <= nb && i < 2u * n + 4u; i += gridDim.x * blockDim.x) {
+    bcnt[i] = 0;
+    bnodes[i] = 0;
+    bcur[i] = 0;
+    bncur[i] = 0;
+  }
   if (v >= n) return;
   uint32_t r = pair_ptr(rt[v]);
   if (!is_outlet[r]) {
@@ -1273,7 +1285,7 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
   FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
-            t.area, w.A, w.tau, w.h, w.h2);
+            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur);
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,

@@ -111,9 +111,10 @@ int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
 /* Iterations after the first are enqueued whole with one host
  * synchronisation each, their pointer-jumping and sweep round counts taken
  * from the previous iteration (and redone synchronously when that did not
- * hold).  enable = 0 synchronises inside every iteration instead; 2
- * speculates a single round of each kind, so that most iterations take the
- * redo path (for testing it).  Results do not depend on it. */
+ * hold), and replayed from a captured HIP graph (mode 1, the default).
+ * Mode 0 synchronises inside every iteration instead; 2 speculates a single
+ * round of each kind, so that most iterations take the redo path (for
+ * testing it); 3 is mode 1 without graphs.  Results do not depend on it. */
 int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
 
 /* Serving: a request loop on one GPU with continuous batching.  `slots`

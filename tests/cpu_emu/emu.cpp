@@ -202,7 +202,21 @@ void run_block(uint32_t nthreads) {
 
 }  // namespace
 
+// Work recorded by a stream capture on this host thread (null: none).
+thread_local std::vector<std::function<void()>>* g_capture = nullptr;
+
+void run_launch(dim3 grid, dim3 block, const std::function<void()>& body);
+
 void launch(dim3 grid, dim3 block, const std::function<void()>& body) {
+  if (g_capture) {
+    std::function<void()> b = body;
+    g_capture->push_back([grid, block, b]() { run_launch(grid, block, b); });
+    return;
+  }
+  run_launch(grid, block, body);
+}
+
+void run_launch(dim3 grid, dim3 block, const std::function<void()>& body) {
   std::lock_guard<std::mutex> lk(g_launch_mu);
   if (block.y != 1 || block.z != 1 || grid.y != 1 || grid.z != 1) die("only 1-D launches are emulated");
   if (block.x == 0 || block.x > kMaxThreads) die("block size out of range");
@@ -285,14 +299,57 @@ hipError_t hipFree(void* p) {
 }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) { return hipMalloc(p, bytes); }
 hipError_t hipHostFree(void* p) { return hipFree(p); }
+struct ihipGraph_t {
+  std::vector<std::function<void()>> work;
+};
+struct ihipGraphExec_t {
+  std::vector<std::function<void()>> work;
+};
+
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t) {
+  if (flemu::g_capture) {
+    flemu::g_capture->push_back([=]() {
+      if (bytes) std::memmove(dst, src, bytes);
+    });
+    return hipSuccess;
+  }
   if (bytes) std::memmove(dst, src, bytes);
+  return hipSuccess;
+}
+
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  if (flemu::g_capture) return hipErrorInvalidValue;
+  flemu::g_capture = new std::vector<std::function<void()>>();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* graph) {
+  if (!flemu::g_capture) return hipErrorInvalidValue;
+  *graph = new ihipGraph_t{std::move(*flemu::g_capture)};
+  delete flemu::g_capture;
+  flemu::g_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* exec, hipGraph_t graph, hipGraphNode_t*, char*, size_t) {
+  *exec = new ihipGraphExec_t{graph->work};
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
+  for (auto& f : exec->work) f();
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t graph) {
+  delete graph;
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t exec) {
+  delete exec;
   return hipSuccess;
 }
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   return hipMemcpyAsync(dst, src, bytes, kind, nullptr);
 }
 hipError_t hipMemsetAsync(void* dst, int value, size_t bytes, hipStream_t) {
+  if (flemu::g_capture) return hipErrorInvalidValue;  // no memset nodes in captured work
   if (bytes) std::memset(dst, value, bytes);
   return hipSuccess;
 }

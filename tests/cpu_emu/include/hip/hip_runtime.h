@@ -4,7 +4,8 @@
 // and with host sanitizers where no GPU is present.  Nothing in the product
 // loads it.
 //
-// Model: device memory is host memory; every call is synchronous; a kernel
+// Model: device memory is host memory; every call is synchronous (work
+// enqueued during a stream capture is recorded and run by hipGraphLaunch); a kernel
 // launch runs its workgroups one after another, each workgroup's threads as
 // cooperative fibers.  __syncthreads() waits for every live thread of the
 // workgroup; __any(), __ballot() and readlane wait for every live lane of the 64-lane
@@ -40,6 +41,17 @@ struct ihipStream_t;
 typedef ihipStream_t* hipStream_t;
 struct ihipEvent_t;
 typedef ihipEvent_t* hipEvent_t;
+struct ihipGraph_t;
+typedef ihipGraph_t* hipGraph_t;
+struct ihipGraphExec_t;
+typedef ihipGraphExec_t* hipGraphExec_t;
+struct hipGraphNode;
+typedef hipGraphNode* hipGraphNode_t;
+enum hipStreamCaptureMode {
+  hipStreamCaptureModeGlobal = 0,
+  hipStreamCaptureModeThreadLocal = 1,
+  hipStreamCaptureModeRelaxed = 2
+};
 
 #define hipStreamNonBlocking 0x01
 #define hipHostMallocDefault 0x0
@@ -153,4 +165,13 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipGetLastError();
+// Stream capture: work enqueued between begin and end is recorded (not run)
+// and replayed in order by hipGraphLaunch.
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* graph);
+hipError_t hipGraphInstantiate(hipGraphExec_t* exec, hipGraph_t graph, hipGraphNode_t* error_node, char* log,
+                               size_t log_size);
+hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t s);
+hipError_t hipGraphDestroy(hipGraph_t graph);
+hipError_t hipGraphExecDestroy(hipGraphExec_t exec);
 const char* hipGetErrorString(hipError_t e);

@@ -145,9 +145,10 @@ def test_emulated_batch_compaction(emu_lib):
 @pytest.mark.parametrize("kind", ["fbm", "strip"])
 def test_emulated_pipelining_modes_agree(emu_lib, kind):
     """The pipelined iteration (one host synchronisation, round counts taken
-    from the previous iteration), its redo path (mode 2 speculates one round
-    of each kind, so most iterations are redone) and the synchronous one give
-    the same bits, counts and oracle results."""
+    from the previous iteration; replayed from a captured graph in mode 1,
+    launched directly in mode 3), its redo path (mode 2 speculates one round
+    of each kind, so most iterations are redone) and the synchronous one
+    (mode 0) give the same bits, counts and oracle results."""
     from cpu_emu import emu
     from fastlem_amd import _native
     from fastlem_amd.device import DeviceBatch, batch_descs
@@ -156,7 +157,7 @@ def test_emulated_pipelining_modes_agree(emu_lib, kind):
     w = config_fbm(3000, 55.0, seed=5) if kind == "fbm" else config_strip(48, 24)
     L = emu_lib
     runs = {}
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         keep = []
         d = batch_descs([DeviceBatch.terrain_of(w)], keep)
         ctx = ctypes.c_void_p()
@@ -176,7 +177,7 @@ def test_emulated_pipelining_modes_agree(emu_lib, kind):
             L.fl_ctx_destroy(ctx)
         runs[mode] = (it.value, out, st.host_syncs, st.redone)
     ref, rit, _ = _oracle(w)
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         assert runs[mode][0] == rit
         assert _bits_equal(runs[mode][1], ref)
     assert runs[1][2] < runs[0][2]  # fewer host synchronisations
