@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <future>
 #include <memory>
 #include <mutex>
@@ -62,10 +63,13 @@ struct fl_ctx {
   uint32_t compact_min_idle = 1u << 16;  // see solve_union
   int pipelined = 1;                     // one_iteration_fast after the first iteration (2: minimal
                                          // speculation, most iterations redone)
-  bool graphs = true;                    // pipelined iterations replayed from captured graphs
+  bool graphs = false;                   // pipelined iterations replayed from captured graphs
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
+  // pinned readbacks and completion events of pipelined iterations in flight
+  uint32_t* host_slot[2] = {nullptr, nullptr};
+  hipEvent_t slot_ev[2] = {nullptr, nullptr};
   std::vector<uint32_t> host_boff;
 };
 
@@ -80,9 +84,13 @@ struct fl_terrain {
   std::vector<uint32_t> last_boff;  // its bucket offsets (boff[nb+1], bnoff[nb+1]): launch-size estimates
   // captured pipelined iterations, one per elevation-buffer parity, for the
   // (jump rounds, sweep rounds) they were captured with
-  double* hbuf0 = nullptr;  // the buffer w.h held at creation
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  uint32_t gJ[2] = {0, 0}, gR[2] = {0, 0};
+  // elevation buffers: an iteration reads hb[i] and writes hb[(i+1) % 3], so
+  // an iteration in flight never overwrites the input of the one before it
+  double* hb[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
+  uint32_t* h_tslot[2] = {nullptr, nullptr};  // pinned per-terrain changed flags of iterations in flight
+  uint8_t* h_mask = nullptr;                  // pinned staging of the active mask
   bool graphs_failed = false;
   bool have_order = false;
   // statistics of the last solve
@@ -376,16 +384,6 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   return FL_OK;
 }
 
-// One outer iteration enqueued whole, with one host synchronisation at its
-// end.  Three things are taken from the previous iteration instead of being
-// read back in between: the number of pointer-jumping rounds (one more is
-// launched; a round after a quiet one returns at once), the number of sweep
-// rounds (one more; rounds past the deepest light depth are empty, their
-// bounds read on the device) and the absence of lakes.  *valid is false when
-// any of them did not hold: the decomposition had not converged (then the
-// kernels after it did nothing), a lake was found, or the tree was deeper.
-// The input elevations are untouched then, and the caller redoes the
-// iteration with one_iteration.
 // The kernels of one pipelined iteration (J jump rounds, R sweep rounds),
 // enqueued on the context's stream or recorded by a capture.  rec_* receive
 // the profiler records of the rounds (when profiling).
@@ -436,7 +434,7 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::ve
 // caller launches directly).
 bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
   if (T->graphs_failed) return false;
-  int p = T->w.h == T->hbuf0 ? 0 : 1;
+  int p = T->w.h == T->hb[0] ? 0 : (T->w.h == T->hb[1] ? 1 : 2);
   if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R) {
     if (T->gexec[p]) {
       hipGraphExecDestroy(T->gexec[p]);
@@ -463,68 +461,88 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
   return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
 }
 
-// One outer iteration enqueued whole, with one host synchronisation at its
-// end.  Three things are taken from the previous iteration instead of being
+// A pipelined iteration in flight: it reads hb[in] and writes hb[(in+1) % 3].
+struct Flight {
+  int in = 0;
+  int slot = 0;
+  uint32_t J = 0, R = 0;
+  std::vector<uint8_t> mask;  // terrains iterating in it
+  std::vector<size_t> rec_da, rec_tau;
+  size_t rec_scatter = SIZE_MAX;
+};
+
+// One outer iteration enqueued whole, without waiting for the previous one.
+// Three things are taken from the last confirmed iteration instead of being
 // read back in between: the number of pointer-jumping rounds (one more is
 // launched; a round after a quiet one returns at once), the number of sweep
 // rounds (one more; rounds past the deepest light depth are empty, their
-// bounds read on the device) and the absence of lakes.  *valid is false when
-// any of them did not hold: the decomposition had not converged (then the
-// kernels after it did nothing), a lake was found, or the tree was deeper.
-// The input elevations are untouched then, and the caller redoes the
-// iteration with one_iteration.  Without profiling the kernels are replayed
-// from a captured graph.
-int32_t one_iteration_fast(fl_ctx* c, fl_terrain* T, bool* valid, bool* changed) {
+// bounds read on the device) and the absence of lakes.  Its control block
+// and changed flags are copied to the host slot f.slot and an event marks
+// its end.
+int32_t enqueue_fast(fl_ctx* c, fl_terrain* T, Flight& f) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
-  uint32_t n = T->t.n;
-  uint32_t J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
-  uint32_t R = std::max<uint32_t>(T->last_rounds, 1) + 1;
-  if (c->pipelined == 2) J = R = 1;
-  std::vector<size_t> rec_da(R, SIZE_MAX), rec_tau(R, SIZE_MAX);
-  size_t rec_scatter = SIZE_MAX;
-  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, J, R);
-  if (!graphed) enqueue_fast_body(c, T, J, R, &rec_da, &rec_tau, &rec_scatter);
-  w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
+  f.J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
+  f.R = std::max<uint32_t>(T->last_rounds, 1) + 1;
+  if (c->pipelined == 2) f.J = f.R = 1;
+  w.h = T->hb[f.in];
+  w.h2 = T->hb[(f.in + 1) % 3];
+  f.rec_da.assign(f.R, SIZE_MAX);
+  f.rec_tau.assign(f.R, SIZE_MAX);
+  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R);
+  if (!graphed) enqueue_fast_body(c, T, f.J, f.R, &f.rec_da, &f.rec_tau, &f.rec_scatter);
   if (T->d_tchanged)
-    HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  FLCHK(read_ctrl(c, T));
-  bool lake = host_flag(c, FL_FLAG_LAKE) != 0;
-  bool converged = c->host_ctrl[FL_CTRL_JUMPS + J - 1] == 0;
-  uint32_t nb = 2 * (host_counter(c, FL_CNT_MAXLD) + 1);
+    HIPCHK(hipMemcpyAsync(T->h_tslot[f.slot], T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->host_slot[f.slot], w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(c->slot_ev[f.slot], s));
+  return FL_OK;
+}
+
+// Wait for flight f and check what it assumed.  *valid is false when any
+// assumption failed: the decomposition had not converged (the kernels after
+// it then did nothing), a lake was found, or the tree was deeper.  Its input
+// hb[f.in] is untouched either way.  *changed: the changed flag; per-terrain
+// flags are in T->h_tslot[f.slot].
+int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool* changed) {
+  HIPCHK(hipEventSynchronize(c->slot_ev[f.slot]));
+  T->stat_syncs++;
+  const uint32_t* hc = c->host_slot[f.slot];
+  uint32_t n = T->t.n;
+  bool lake = hc[FL_FLAG_LAKE] != 0;
+  bool converged = hc[FL_CTRL_JUMPS + f.J - 1] == 0;
+  uint32_t nb = 2 * (hc[FL_NUM_FLAGS + FL_CNT_MAXLD] + 1);
+  *valid = false;
+  *changed = false;
   if (!converged) {
-    // nothing after the decomposition ran; the histograms are untouched
-    T->jump_rounds = std::max(T->jump_rounds, J + 1);
-    *valid = false;
+    T->jump_rounds = std::max(T->jump_rounds, f.J + 1);
     return FL_OK;
   }
   if (nb > 2 * n + 2) return set_err(FL_ERR_HIP, "light depth out of range");
-  T->prev_nb = nb;  // histogram entries to clear next time
-  for (uint32_t r = 0; r < J; r++)
-    if (c->host_ctrl[FL_CTRL_JUMPS + r] == 0) {
+  for (uint32_t r = 0; r < f.J; r++)
+    if (hc[FL_CTRL_JUMPS + r] == 0) {
       T->jump_rounds = r + 1;
       break;
     }
-  *valid = !lake && nb / 2 <= R;
-  if (!*valid) return FL_OK;
+  if (lake || nb / 2 > f.R) return FL_OK;
+  *valid = true;
+  T->prev_nb = nb;
   T->last_rounds = nb / 2;
   if (nb <= FL_CTRL_HEAD) {
-    const uint32_t* hc = c->host_ctrl + FL_NUM_FLAGS + FL_NUM_COUNTERS;
-    T->last_boff.assign(hc, hc + nb + 1);
-    T->last_boff.insert(T->last_boff.end(), hc + FL_CTRL_HEAD + 1, hc + FL_CTRL_HEAD + 1 + nb + 1);
+    const uint32_t* head = hc + FL_NUM_FLAGS + FL_NUM_COUNTERS;
+    T->last_boff.assign(head, head + nb + 1);
+    T->last_boff.insert(T->last_boff.end(), head + FL_CTRL_HEAD + 1, head + FL_CTRL_HEAD + 1 + nb + 1);
     if (c->prof) {
-      for (uint32_t l = 0; l < R; l++) {
+      for (uint32_t l = 0; l < f.R; l++) {
         uint64_t units = exact_grid(T->last_boff, l).positions;
-        for (size_t q : {rec_da[l], rec_da[l] + 1, rec_tau[l]})
+        for (size_t q : {f.rec_da[l], f.rec_da[l] + 1, f.rec_tau[l]})
           if (q < c->prof->pending.size()) c->prof->pending[q].units = units;
       }
-      if (rec_scatter < c->prof->pending.size()) c->prof->pending[rec_scatter].units = T->last_boff[2 * nb + 1];
+      if (f.rec_scatter < c->prof->pending.size()) c->prof->pending[f.rec_scatter].units = T->last_boff[2 * nb + 1];
     }
   } else {
     T->last_boff.clear();
   }
-  *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
-  std::swap(w.h, w.h2);
+  *changed = hc[FL_FLAG_CHANGED] != 0;
   return FL_OK;
 }
 
@@ -556,6 +574,10 @@ extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipHostMalloc((void**)&c->host_ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+  for (int q = 0; q < 2; q++) {
+    HIPCHK(hipHostMalloc((void**)&c->host_slot[q], FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&c->slot_ev[q], hipEventDisableTiming));
+  }
   *out = c.release();
   return FL_OK;
 }
@@ -566,6 +588,10 @@ extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
   delete c->prof;
   if (c->stream) hipStreamDestroy(c->stream);
   if (c->host_ctrl) hipHostFree(c->host_ctrl);
+  for (int q = 0; q < 2; q++) {
+    if (c->host_slot[q]) hipHostFree(c->host_slot[q]);
+    if (c->slot_ev[q]) hipEventDestroy(c->slot_ev[q]);
+  }
   delete c;
   return FL_OK;
 }
@@ -577,6 +603,9 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
     if (e) hipGraphExecDestroy(e);
   for (void* p : T->allocations) hipFree(p);
   if (T->h_tchanged) hipHostFree(T->h_tchanged);
+  for (uint32_t* q : T->h_tslot)
+    if (q) hipHostFree(q);
+  if (T->h_mask) hipHostFree(T->h_mask);
   delete T;
   return FL_OK;
 }
@@ -641,7 +670,9 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
   for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
                      &w.post2, &w.post3})
     FLCHK(dev_alloc(P, p, n));
-  P->hbuf0 = w.h;
+  FLCHK(dev_alloc(P, &P->hb[2], n));
+  P->hb[0] = w.h;
+  P->hb[1] = w.h2;
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
   for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
@@ -661,6 +692,8 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
     FLCHK(dev_alloc(P, &P->d_active, count));
     FLCHK(dev_alloc(P, &P->d_tchanged, count));
     HIPCHK(hipHostMalloc((void**)&P->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
+    for (uint32_t*& q : P->h_tslot) HIPCHK(hipHostMalloc((void**)&q, count * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&P->h_mask, count, hipHostMallocDefault));
   }
   hipStream_t s = c->stream;
   fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
@@ -866,6 +899,8 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     if (T->t_active[g]) active_sites += T->site_off[g + 1] - T->site_off[g];
   }
   if (T->t.n == 0) return FL_OK;
+  T->w.h = T->hb[0];
+  T->w.h2 = T->hb[1];
   FLCHK(reset_state(c, T));  // elevations start from the initial state even if nothing iterates
   if (active == 0) {
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -878,33 +913,104 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> owned(nullptr, fl_terrain_destroy);
   std::vector<uint32_t> orig(G);
   for (uint32_t g = 0; g < G; g++) orig[g] = g;
+  hipStream_t s = c->stream;
   uint32_t it = 0;
-  while (active) {
-    bool changed = false, lake = false, valid = false;
-    if (cur != T) cur->stat_syncs = 0;
-    if (it > 0 && c->pipelined) {
-      FLCHK(one_iteration_fast(c, cur, &valid, &changed));
-      T->stat_redone += valid ? 0 : 1;
-    }
-    if (!valid) FLCHK(one_iteration(c, cur, &changed, &lake));
-    if (cur != T) T->stat_syncs += cur->stat_syncs;
-    it++;
-    T->stat_lake_iterations += lake ? 1 : 0;
-    bool mask_changed = false;
+  int hin = 0;  // cur->hb[hin] holds the last confirmed elevations (reset_state wrote hb[0])
+  std::deque<Flight> q;
+  std::vector<uint32_t> enq(cur->t_iters);  // iterations enqueued per terrain (confirmed + in flight)
+  std::vector<uint8_t> dev_mask(cur->t_active);  // the mask the device holds
+  int next_slot = 0;
+  bool drain = false;  // stop enqueueing: a compaction waits for the flights to land
+  auto upload_mask = [&](const std::vector<uint8_t>& m) -> int32_t {
+    if (!cur->d_active || m == dev_mask) return FL_OK;
+    std::copy(m.begin(), m.end(), cur->h_mask);
+    HIPCHK(hipMemcpyAsync(cur->d_active, cur->h_mask, cur->nterr, hipMemcpyHostToDevice, s));
+    dev_mask = m;
+    return FL_OK;
+  };
+  // account one confirmed iteration of the terrains in `mask`
+  auto confirm = [&](const std::vector<uint8_t>& mask, const uint32_t* tflags, bool changed) {
     for (uint32_t g = 0; g < cur->nterr; g++) {
-      if (!cur->t_active[g]) continue;
+      if (!mask[g] || !cur->t_active[g]) continue;
       cur->t_iters[g]++;
       T->t_iters[orig[g]] = cur->t_iters[g];
-      bool ch = cur->d_tchanged ? cur->h_tchanged[g] != 0 : changed;
+      bool ch = tflags ? tflags[g] != 0 : changed;
       if (!ch || cur->t_iters[g] >= caps[g]) {
         cur->t_active[g] = 0;
         active--;
         active_sites -= cur->site_off[g + 1] - cur->site_off[g];
-        mask_changed = true;
       }
     }
+  };
+  while (active) {
+    if (cur != T) cur->stat_syncs = 0;
+    if (it == 0 || !c->pipelined) {
+      // the first iteration (lakes in practice) and the unpipelined form
+      bool changed = false, lake = false;
+      std::vector<uint8_t> mask(cur->t_active);
+      FLCHK(upload_mask(mask));
+      cur->w.h = cur->hb[hin];
+      cur->w.h2 = cur->hb[(hin + 1) % 3];
+      FLCHK(one_iteration(c, cur, &changed, &lake));
+      T->stat_lake_iterations += lake ? 1 : 0;
+      for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
+      confirm(mask, cur->d_tchanged ? cur->h_tchanged : nullptr, changed);
+      hin = (hin + 1) % 3;
+      it++;
+    } else {
+      // keep two iterations in flight: the host enqueues the next one while
+      // the device runs the current one
+      while (!drain && q.size() < 2) {
+        std::vector<uint8_t> mask(cur->nterr);
+        bool any = false;
+        for (uint32_t g = 0; g < cur->nterr; g++) {
+          mask[g] = cur->t_active[g] && enq[g] < caps[g];
+          any |= mask[g] != 0;
+        }
+        if (!any) break;
+        FLCHK(upload_mask(mask));
+        Flight f;
+        f.in = q.empty() ? hin : (q.back().in + 1) % 3;
+        f.slot = next_slot;
+        next_slot ^= 1;
+        f.mask = mask;
+        FLCHK(enqueue_fast(c, cur, f));
+        for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
+        q.push_back(std::move(f));
+      }
+      if (q.empty()) break;  // cannot happen while a terrain is active
+      Flight f = std::move(q.front());
+      q.pop_front();
+      bool valid = false, changed = false;
+      FLCHK(finish_fast(c, cur, f, &valid, &changed));
+      const uint32_t* tflags = cur->d_tchanged ? cur->h_tslot[f.slot] : nullptr;
+      if (!valid) {
+        // the flights after it started from a wrong input: let them land,
+        // then redo this iteration synchronously on its own input and mask
+        HIPCHK(hipStreamSynchronize(s));
+        for (const Flight& x : q)
+          for (uint32_t g = 0; g < cur->nterr; g++) enq[g] -= x.mask[g];
+        q.clear();
+        T->stat_redone++;
+        bool lake = false;
+        FLCHK(upload_mask(f.mask));
+        cur->w.h = cur->hb[f.in];
+        cur->w.h2 = cur->hb[(f.in + 1) % 3];
+        FLCHK(one_iteration(c, cur, &changed, &lake));
+        T->stat_lake_iterations += lake ? 1 : 0;
+        tflags = cur->d_tchanged ? cur->h_tchanged : nullptr;
+      }
+      confirm(f.mask, tflags, changed);
+      hin = (f.in + 1) % 3;
+      it++;
+    }
+    if (cur != T) T->stat_syncs += cur->stat_syncs;
     if (!active) break;
-    if (cur->nterr > 1 && active_sites * 2 < cur->t.n && cur->t.n - active_sites >= c->compact_min_idle) {
+    bool want = cur->nterr > 1 && active_sites * 2 < cur->t.n && cur->t.n - active_sites >= c->compact_min_idle;
+    if (want) drain = true;
+    if (drain && q.empty()) {
+      cur->w.h = cur->hb[hin];
+      cur->w.h2 = cur->hb[(hin + 1) % 3];
       fl_terrain* next_union = nullptr;
       std::vector<uint32_t> next_orig;
       FLCHK(compact_union(c, T, cur, caps, orig, &next_union, next_orig));
@@ -912,20 +1018,26 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       cur = next_union;
       caps = cur->t_max_it.data();
       orig.swap(next_orig);
+      hin = 0;
+      enq = cur->t_iters;
+      dev_mask = cur->t_active;
+      drain = false;
       T->stat_compactions++;
-      continue;
     }
-    if (mask_changed && cur->d_active)
-      HIPCHK(hipMemcpyAsync(cur->d_active, cur->t_active.data(), cur->nterr, hipMemcpyHostToDevice, c->stream));
   }
+  // iterations still in flight after the last one that counted recompute a
+  // fixed point (or masked terrains): wait for them, their results are unused
+  HIPCHK(hipStreamSynchronize(s));
+  cur->w.h = cur->hb[hin];
+  cur->w.h2 = cur->hb[(hin + 1) % 3];
   if (cur != T) {
     for (uint32_t g = 0; g < cur->nterr; g++) {
       uint32_t s0 = cur->site_off[g], ng = cur->site_off[g + 1] - s0;
       if (ng)
         HIPCHK(hipMemcpyAsync(T->w.h + T->site_off[orig[g]], cur->w.h + s0, (uint64_t)ng * 8,
-                              hipMemcpyDeviceToDevice, c->stream));
+                              hipMemcpyDeviceToDevice, s));
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(s));
   }
   T->stat_iterations = it;
   T->stat_solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1105,7 +1217,7 @@ extern "C" int32_t fl_ctx_set_pipelining(fl_ctx* c, int32_t enable) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
   if (enable < 0 || enable > 3) return set_err(FL_ERR_INVALID, "pipelining mode must be 0, 1, 2 or 3");
   c->pipelined = enable == 3 ? 1 : enable;
-  c->graphs = enable != 3;
+  c->graphs = enable == 3;
   return FL_OK;
 }
 

@@ -108,13 +108,15 @@ int32_t fl_batch_count(const fl_terrain *batch, uint32_t *count);
  * sites belong to terrains that have stopped (default 65536; UINT32_MAX never
  * compacts).  Results do not depend on it. */
 int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
-/* Iterations after the first are enqueued whole with one host
- * synchronisation each, their pointer-jumping and sweep round counts taken
- * from the previous iteration (and redone synchronously when that did not
- * hold), and replayed from a captured HIP graph (mode 1, the default).
+/* Mode 1 (the default): iterations after the first are enqueued whole, the
+ * next one while the device still runs the current one, their
+ * pointer-jumping and sweep round counts taken from the last confirmed
+ * iteration (an iteration whose assumptions did not hold is redone
+ * synchronously from its input, which three elevation buffers keep intact).
  * Mode 0 synchronises inside every iteration instead; 2 speculates a single
  * round of each kind, so that most iterations take the redo path (for
- * testing it); 3 is mode 1 without graphs.  Results do not depend on it. */
+ * testing it); 3 is mode 1 with each iteration replayed from a captured HIP
+ * graph.  Results do not depend on the mode. */
 int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
 
 /* Serving: a request loop on one GPU with continuous batching.  `slots`

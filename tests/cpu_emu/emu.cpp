@@ -367,6 +367,7 @@ hipError_t hipEventCreate(hipEvent_t* e) {
   *e = new ihipEvent_t();
   return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) {
   delete e;
   return hipSuccess;

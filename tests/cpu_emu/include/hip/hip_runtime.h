@@ -54,6 +54,7 @@ enum hipStreamCaptureMode {
 };
 
 #define hipStreamNonBlocking 0x01
+#define hipEventDisableTiming 0x2
 #define hipHostMallocDefault 0x0
 
 struct dim3 {
@@ -160,6 +161,7 @@ hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipDeviceSynchronize();
 hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);

@@ -144,11 +144,12 @@ def test_emulated_batch_compaction(emu_lib):
 
 @pytest.mark.parametrize("kind", ["fbm", "strip"])
 def test_emulated_pipelining_modes_agree(emu_lib, kind):
-    """The pipelined iteration (one host synchronisation, round counts taken
-    from the previous iteration; replayed from a captured graph in mode 1,
-    launched directly in mode 3), its redo path (mode 2 speculates one round
-    of each kind, so most iterations are redone) and the synchronous one
-    (mode 0) give the same bits, counts and oracle results."""
+    """The pipelined iteration (the next one enqueued while the current one
+    runs, round counts taken from the last confirmed iteration; launched
+    directly in mode 1, replayed from a captured graph in mode 3), its redo
+    path (mode 2 speculates one round of each kind, so most iterations are
+    redone) and the synchronous one (mode 0) give the same bits, counts and
+    oracle results."""
     from cpu_emu import emu
     from fastlem_amd import _native
     from fastlem_amd.device import DeviceBatch, batch_descs

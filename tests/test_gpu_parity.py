@@ -383,9 +383,9 @@ def test_batch_compaction_matches_solo_runs():
 @pytest.mark.parametrize("mode", [0, 2, 3])
 def test_pipelining_modes_agree(fbm20k, mode):
     """Synchronous iterations (mode 0), minimal speculation with most
-    iterations redone (mode 2) and pipelined iterations launched without a
-    graph (mode 3) against the oracle; mode 1 (graphs, the default) is what
-    every other test runs."""
+    iterations redone (mode 2) and pipelined iterations replayed from
+    captured graphs (mode 3) against the oracle; mode 1 (the default) is
+    what every other test runs."""
     import ctypes
 
     from fastlem_amd import _native
