@@ -54,13 +54,12 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
         # hc 4 per site (the tails' extra reads are not counted)
         "k_bucket_count": 4,
         "k_bucket_place": 4,
-        # jn 4 + is_outlet 1 + jc 4 + pc 4 + head_base 4, writes perm 4
-        "k_chain_perm": 21,
-        # perm 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4 + a child's A 8, writes pre/post0/post1/npost 28
-        "k_da_pre": 68,
-        # pre/post0/post1/npost 28 + perm 4, writes A 8; then K 8 + dnext 8, writes x 8
-        # (+ U, tan, h read and four chain-major copies written with max slopes)
-        "k_da_fold": 120 if clamp else 64,
+        # rt 8 + is_outlet 1 + cp 8 + head_base 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4,
+        # writes perm 4 + list 16 + count 4 + area 8
+        "k_chain_perm": 81,
+        # perm 4 + count 4 + list 16 + area 8 + a child's A 8, writes A 8; then K 8 + dnext 8,
+        # writes x 8 (+ U, tan, h read and four chain-major copies written with max slopes)
+        "k_da_fold": 128 if clamp else 72,
         # x 8, writes tau 8 (+ U, tan, d read and h written)
         "k_tau_fold<false>": 16,
         "k_tau_fold<true>": 48,
@@ -152,7 +151,7 @@ def main() -> int:
     L.fl_ctx_reset_profile(ctx)
     t.solve()
     names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count", "k_bucket_scan",
-             "k_bucket_place", "k_chain_perm", "k_da_pre", "k_da_fold", "k_tau_fold<false>",
+             "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
              "k_tau_fold<true>", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32",
              "k_fill_u64", "k_copy_u32"]
     prof = {}

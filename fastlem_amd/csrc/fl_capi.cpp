@@ -667,15 +667,14 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
   FLCHK(dev_alloc(P, &t.rank, n));
   FLCHK(dev_alloc(P, &t.order, n));
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.pre, &w.post0, &w.post1,
-                     &w.post2, &w.post3})
-    FLCHK(dev_alloc(P, p, n));
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.parea}) FLCHK(dev_alloc(P, p, n));
+  FLCHK(dev_alloc(P, &w.plist, 4 * (uint64_t)n));
   FLCHK(dev_alloc(P, &P->hb[2], n));
   P->hb[0] = w.h;
   P->hb[1] = w.h2;
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.npost, &w.chain_base, &w.chain_len, &w.head_base,
+  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.pcnt, &w.chain_base, &w.chain_len, &w.head_base,
                        &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
   FLCHK(dev_alloc(P, &w.cp, n));

@@ -82,12 +82,11 @@ struct fl_dev_work {
   // decomposition
   uint64_t* cmask = nullptr;  // [n] child slots of each site
   uint32_t* hc = nullptr;     // [n] heavy child
-  double* pre = nullptr;      // [n] drainage-area partial folds, chain-major
-  double* post0 = nullptr;
-  double* post1 = nullptr;
-  double* post2 = nullptr;  // written where a site has three or more posts
-  double* post3 = nullptr;
-  uint32_t* npost = nullptr;
+  // drainage-area inputs in chain-major order (written by k_chain_perm):
+  // light children (FL_LIST per position, fold order), counts, own area
+  uint32_t* plist = nullptr;  // [4n]
+  uint32_t* pcnt = nullptr;   // [n] npre | npost << 8, or the overflow flag
+  double* parea = nullptr;    // [n]
 
   // pointer jumping, two (pointer, count) pairs per site, pointer in the low
   // word: cp = (chain ancestor, distance to it), rt = (tree ancestor, light
@@ -123,7 +122,7 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
 // round can be enqueued before its bounds are known on the host.  Kernels
 // loop over their range, so any non-zero grid is correct.
 struct fl_round_grid {
-  uint32_t pre_blocks;   // k_da_pre, 256 threads each
+  uint32_t pre_blocks;   // unused (kept for the launch-shape estimates)
   uint32_t fold_blocks;  // k_da_fold / k_tau_fold, 64 threads each
   uint64_t positions;    // profiler units (0 when not known at launch)
 };

@@ -14,11 +14,10 @@
 //     k_children        children of each site in row (slot) order
 //     k_chain_init/jump chain head, position in chain, light depth, root
 //     k_bucket_*        chains bucketed by light depth, laid out contiguously
-//     k_chain_perm      sites scattered to chain-major positions
+//     k_chain_perm      sites scattered to chain-major positions, with
+//                                             each site's light children
 //   sweeps, one round per light depth (deepest first for drainage area)
-//     k_da_pre          generator.rs:154-159  light-children part of each
-//                                             site's fold, in parallel
-//     k_da_fold                               drainage area along the chains,
+//     k_da_fold         generator.rs:154-159  drainage area along the chains,
 //                       generator.rs:172      exact fold order; then each
 //                                             site's 1/(K*A^0.5) * distance
 //     k_tau_fold        generator.rs:162-203  response time (+ clamped
@@ -40,6 +39,8 @@
 #include "fl_kernels.h"
 
 #define FL_NONE 0xffffffffu
+#define FL_LIST 4                   // light children listed per chain position
+#define FL_LIST_OVERFLOW 0x10000u   // pcnt flag: more light children than listed
 
 namespace {
 
@@ -51,6 +52,13 @@ __device__ __forceinline__ unsigned long long pair(uint32_t ptr, uint32_t cnt) {
 }
 __device__ __forceinline__ uint32_t pair_ptr(unsigned long long p) { return (uint32_t)p; }
 __device__ __forceinline__ uint32_t pair_cnt(unsigned long long p) { return (uint32_t)(p >> 32); }
+
+// Is slot s of row k (starting at b) a child of k?
+__device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uint32_t s, uint32_t b,
+                                         const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+                                         uint32_t k) {
+  return deg <= 64 ? ((m >> s) & 1ull) != 0 : next[col[b + s]] == k;
+}
 
 // f64::powf(x, 0.5) as lowered in optimised Rust builds.
 __device__ __forceinline__ double rust_pow_half(double x) {
@@ -468,7 +476,10 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
                              const double* __restrict__ h, double* __restrict__ h2,
                              const uint32_t* __restrict__ maxld, uint32_t* __restrict__ bcnt,
                              uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
-                             uint32_t* __restrict__ bncur) {
+                             uint32_t* __restrict__ bncur, const uint32_t* __restrict__ rp,
+                             const uint32_t* __restrict__ col, const unsigned long long* __restrict__ cmask,
+                             const uint32_t* __restrict__ hc, uint32_t* __restrict__ pcnt,
+                             uint32_t* __restrict__ plist, double* __restrict__ parea) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   uint32_t v = gid();
   // the bucket histograms are used up (count, scan, place): clear the
@@ -491,7 +502,33 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
   unsigned long long c = cp[v];
   uint32_t hd = pair_ptr(c);
   uint32_t base = head_base[hd];
-  perm[base + pair_cnt(c)] = v;
+  const uint32_t P = base + pair_cnt(c);
+  perm[P] = v;
+  // the drainage-area inputs of the site, in chain-major order: its area and
+  // its light children in fold order (descending slot), those before the
+  // heavy child first; more than FL_LIST of them set the overflow bit and
+  // the fold reads the graph instead
+  {
+    const uint32_t b = rp[v], deg = rp[v + 1] - b, hk = hc[v];
+    const unsigned long long m = cmask[v];
+    uint32_t npre = 0, npost = 0;
+    bool after = false;
+    for (uint32_t s = deg; s-- > 0;) {
+      if (!is_child(m, deg, s, b, col, next, v)) continue;
+      uint32_t ch = col[b + s];
+      if (ch == hk) {
+        after = true;
+        continue;
+      }
+      uint32_t idx = npre + npost;
+      if (idx < FL_LIST) plist[(size_t)FL_LIST * P + idx] = ch;
+      if (after) ++npost;
+      else ++npre;
+    }
+    bool over = npre + npost > FL_LIST;
+    pcnt[P] = over ? FL_LIST_OVERFLOW : (npre | (npost << 8));
+    parea[P] = area[v];
+  }
   if (hd == v) {
     uint32_t p = next[v];
     unsigned long long cpp = cp[p];
@@ -501,17 +538,6 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
 }
 
 // --------------------------------------------------------------- sweeps ---
-
-namespace {
-
-// Is slot s of row k (starting at b) a child of k?
-__device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uint32_t s, uint32_t b,
-                                         const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-                                         uint32_t k) {
-  return deg <= 64 ? ((m >> s) & 1ull) != 0 : next[col[b + s]] == k;
-}
-
-}  // namespace
 
 #define FL_BATCH 8
 
@@ -589,33 +615,6 @@ __device__ __forceinline__ double da_post_rest(double r, uint32_t k, const uint3
   return r;
 }
 
-// Drainage area, part 1: the partial fold of every site of round l
-// (positions [bnoff[2l], bnoff[2l+2])), one thread per site.  post2/post3
-// are written only where they are used (three or more posts).
-__global__ void k_da_pre(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ bnoff, const uint32_t* __restrict__ perm,
-                         const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-                         const uint32_t* __restrict__ hc, const double* __restrict__ area,
-                         const double* __restrict__ A, double* __restrict__ pre, double* __restrict__ post0,
-                         double* __restrict__ post1, double* __restrict__ post2, double* __restrict__ post3,
-                         uint32_t* __restrict__ npost) {
-  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
-  const uint32_t p0 = bnoff[2 * l], p1 = bnoff[2 * l + 2];
-  for (uint32_t P = p0 + gid(); P < p1; P += gridDim.x * blockDim.x) {
-    double pr, a, b, c, d;
-    uint32_t np;
-    da_partial(perm[P], rp, col, next, cmask, hc, area, A, pr, a, b, c, d, np);
-    pre[P] = pr;
-    post0[P] = a;
-    post1[P] = b;
-    npost[P] = np;
-    if (np >= 3) {
-      post2[P] = c;
-      post3[P] = d;
-    }
-  }
-}
-
 // The terms after ((pre + r) + p0) + p1 of a position with np >= 3 posts.
 __device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c, double d, uint32_t k,
                                                 const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
@@ -626,6 +625,37 @@ __device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c,
   if (np >= 4) r = r + d;
   if (np > FL_POSTS) r = da_post_rest(r, k, rp, col, next, cmask, hc, A);
   return r;
+}
+
+// The partial fold of chain position P from its list (written by
+// k_chain_perm): pre = area + the light children before the heavy one (in
+// fold order), posts p0..p3 after it, np posts; absent terms are -0.0.  One
+// level of gathers (the children's A).  A site with more light children than
+// listed folds from the graph instead.
+__device__ __forceinline__ void da_inputs(uint32_t P, uint32_t k, uint32_t cn, uint4 ids, double area_k,
+                                          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+                                          const uint32_t* __restrict__ next,
+                                          const unsigned long long* __restrict__ cmask,
+                                          const uint32_t* __restrict__ hc, const double* __restrict__ area,
+                                          const double* __restrict__ A, double& pre, double& p0, double& p1,
+                                          double& p2, double& p3, uint32_t& np) {
+  (void)P;
+  if (cn & FL_LIST_OVERFLOW) {
+    da_partial(k, rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
+    return;
+  }
+  const uint32_t npre = cn & 0xffu, nl = npre + ((cn >> 8) & 0xffu);
+  const double v0 = nl > 0 ? A[ids.x] : FL_NEG0, v1 = nl > 1 ? A[ids.y] : FL_NEG0;
+  const double v2 = nl > 2 ? A[ids.z] : FL_NEG0, v3 = nl > 3 ? A[ids.w] : FL_NEG0;
+  // pre: x + (-0.0) == x, so the terms past npre are added as -0.0
+  pre = (((area_k + (npre > 0 ? v0 : FL_NEG0)) + (npre > 1 ? v1 : FL_NEG0)) + (npre > 2 ? v2 : FL_NEG0)) +
+        (npre > 3 ? v3 : FL_NEG0);
+  // posts: list entries npre, npre+1, ... (entries past nl read as -0.0)
+  p0 = npre == 0 ? v0 : npre == 1 ? v1 : npre == 2 ? v2 : npre == 3 ? v3 : FL_NEG0;
+  p1 = npre == 0 ? v1 : npre == 1 ? v2 : npre == 2 ? v3 : FL_NEG0;
+  p2 = npre == 0 ? v2 : npre == 1 ? v3 : FL_NEG0;
+  p3 = npre == 0 ? v3 : FL_NEG0;
+  np = nl - npre;
 }
 
 #define FL_DA_BLOCK 8
@@ -660,49 +690,44 @@ __device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint
   }
 }
 
-// Drainage area, part 2, for one light depth: the sequential fold along each
-// chain, tail to head.  One launch covers the round's long chains (blocks
-// [0, c2 - c1), one 64-lane wavefront each) and its short chains (the
-// remaining blocks, one lane each).
-//   short chain: blocks of FL_DA_BLOCK positions are loaded into registers
-//     before their dependent adds;
-//   long chain: tiles of 64 positions are staged in LDS by the whole
-//     wavefront (the next tile's loads in flight meanwhile); lane 0 folds
-//     them eight at a time (reads first, then the dependent adds), the
-//     positions with three or more posts marked by a ballot; the wavefront
-//     stores the results.
+// Drainage area of one light depth: the sequential fold along each chain,
+// tail to head, A = ((pre + A_heavy) + p0) + p1 (+ p2 + p3 ...), with the
+// partial folds of each position made from its list (da_inputs).  Long
+// chains [c1, c2) take one 64-lane wavefront each: tiles of 64 positions are
+// prepared by the whole wavefront and staged in LDS, lane 0 folds them eight
+// at a time (reads first, then the dependent adds; positions with three or
+// more posts marked by a ballot), the wavefront stores the results.  Short
+// chains [c0, c1) take one lane each, FL_DA_BLOCK positions at a time.  As
+// each site's A is final the response-time inputs of its position are
+// written (tau_inputs).  Bounds from the bucket offsets of round l.
 __global__ void __launch_bounds__(64)
-k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
-          const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm, const double* __restrict__ pre,
-          const double* __restrict__ post0, const double* __restrict__ post1, const double* __restrict__ post2,
-          const double* __restrict__ post3, const uint32_t* __restrict__ npost, const uint32_t* __restrict__ rp,
-          const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc, double* __restrict__ A,
+k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff,
+          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
+          const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
           fl_tau_in ti) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
   __shared__ uint32_t s_n[64], s_k[64];
+  const uint4* __restrict__ lists = reinterpret_cast<const uint4*>(plist);
   const uint32_t lane = threadIdx.x;
   const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   // long chains: one per block (uniform per block)
   for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
     const uint32_t base = chain_base[c], len = chain_len[c];
     uint32_t end = len, cs = end > 64 ? end - 64 : 0;
-    double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
-    uint32_t np = 0, k = FL_NONE;
-    if (cs + lane < end) {
-      uint32_t P = base + cs + lane;
-      p = pre[P];
-      a = post0[P];
-      b = post1[P];
-      x = post2[P];  // meaningful where np >= 3 only
-      y = post3[P];
-      np = npost[P];
-      k = perm[P];
-    }
     double r = FL_NEG0;  // lane 0's running value
     while (end > 0) {
       const uint32_t cnt = end - cs;
+      double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
+      uint32_t np = 0, k = FL_NONE;
+      if (lane < cnt) {
+        const uint32_t P = base + cs + lane;
+        k = perm[P];
+        da_inputs(P, k, pcnt[P], lists[P], parea[P], rp, col, next, cmask, hc, area, A, p, a, b, x, y, np);
+      }
       s_p[lane] = p;
       s_a[lane] = a;
       s_b[lane] = b;
@@ -712,20 +737,6 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
       s_k[lane] = k;
       const unsigned long long rare = __ballot(np >= 3);
       __syncthreads();
-      const uint32_t nend = cs, ncs = nend > 64 ? nend - 64 : 0;
-      p = a = b = x = y = 0.0;
-      np = 0;
-      k = FL_NONE;
-      if (ncs + lane < nend) {
-        uint32_t P = base + ncs + lane;
-        p = pre[P];
-        a = post0[P];
-        b = post1[P];
-        x = post2[P];
-        y = post3[P];
-        np = npost[P];
-        k = perm[P];
-      }
       if (lane == 0) {
         int e = (int)cnt - 1;
         for (; e >= 7; e -= 8) {
@@ -755,37 +766,40 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
         A[s_k[lane]] = s_r[lane];
         tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane]);
       }
-      end = nend;
-      cs = ncs;
+      __syncthreads();  // the tile arrays are rewritten next
+      end = cs;
+      cs = end > 64 ? end - 64 : 0;
     }
-    __syncthreads();  // the tile arrays are reused by the next chain
   }
   // short chains: one lane each, the last blocks first (they hold no long chain)
   for (uint32_t c = c0 + (gridDim.x - 1 - blockIdx.x) * 64u + lane; c < c1; c += gridDim.x * 64u) {
-    uint32_t base = chain_base[c], len = chain_len[c];
+    const uint32_t base = chain_base[c], len = chain_len[c];
     double r = FL_NEG0;
     uint32_t end = len;
     while (end > 0) {
-      uint32_t cs = end > FL_DA_BLOCK ? end - FL_DA_BLOCK : 0, cnt = end - cs;
-      double vp[FL_DA_BLOCK], va[FL_DA_BLOCK], vb[FL_DA_BLOCK];
-      uint32_t vn[FL_DA_BLOCK], vk[FL_DA_BLOCK];
+      const uint32_t cs = end > FL_DA_BLOCK ? end - FL_DA_BLOCK : 0, cnt = end - cs;
+      uint32_t vk[FL_DA_BLOCK], vc[FL_DA_BLOCK];
+      uint4 vl[FL_DA_BLOCK];
+      double va[FL_DA_BLOCK];
 #pragma unroll
       for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
         if (u < cnt) {
-          uint32_t P = base + end - 1 - u;
-          vp[u] = pre[P];
-          va[u] = post0[P];
-          vb[u] = post1[P];
-          vn[u] = npost[P];
+          const uint32_t P = base + end - 1 - u;
           vk[u] = perm[P];
+          vc[u] = pcnt[P];
+          vl[u] = lists[P];
+          va[u] = parea[P];
         }
       }
 #pragma unroll
       for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
         if (u < cnt) {
-          r = ((vp[u] + r) + va[u]) + vb[u];
-          uint32_t P = base + end - 1 - u;
-          if (vn[u] >= 3) r = da_tail_terms(r, vn[u], post2[P], post3[P], vk[u], rp, col, next, cmask, hc, A);
+          const uint32_t P = base + end - 1 - u;
+          double pre, p0, p1, p2, p3;
+          uint32_t np;
+          da_inputs(P, vk[u], vc[u], vl[u], va[u], rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
+          r = ((pre + r) + p0) + p1;
+          if (np >= 3) r = da_tail_terms(r, np, p2, p3, vk[u], rp, col, next, cmask, hc, A);
           A[vk[u]] = r;
           tau_inputs(ti, P, vk[u], r);
         }
@@ -1285,20 +1299,16 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
   FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
-            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur);
+            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.pcnt, w.plist, w.parea);
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
                         const fl_round_grid& g, bool clamp) {
   fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
-  if (g.pre_blocks)
-    FL_LAUNCH_GRID(k_da_pre, dim3(g.pre_blocks), dim3(256), g.positions, s, w.guard, l, w.bnoff, w.perm, t.row_ptr, t.col,
-                   w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A, w.pre, w.post0, w.post1, w.post2,
-                   w.post3, w.npost);
   if (g.fold_blocks)
-    FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base, w.chain_len,
-                   w.perm, w.pre, w.post0, w.post1, w.post2, w.post3, w.npost, t.row_ptr, t.col, w.next,
-                   (const unsigned long long*)w.cmask, w.hc, w.A, ti);
+    FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
+                   w.chain_len, w.perm, w.pcnt, w.plist, w.parea, t.row_ptr, t.col, w.next,
+                   (const unsigned long long*)w.cmask, w.hc, t.area, w.A, ti);
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,

@@ -57,6 +57,10 @@ enum hipStreamCaptureMode {
 #define hipEventDisableTiming 0x2
 #define hipHostMallocDefault 0x0
 
+struct alignas(16) uint4 {
+  unsigned int x, y, z, w;
+};
+
 struct dim3 {
   uint32_t x, y, z;
   constexpr dim3(uint32_t x_ = 1, uint32_t y_ = 1, uint32_t z_ = 1) : x(x_), y(y_), z(z_) {}
