@@ -80,8 +80,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="measured HBM KiB per launch of the dominant kernel (rocprofv3 --pmc), for the roofline")
-    ap.add_argument("--mode", choices=["single", "batched", "served"], default="single",
-                    help="single: the metric line; batched / served: the multi-terrain forms")
+    ap.add_argument("--mode", choices=["single", "batched", "served", "copy"], default="single",
+                    help="single: the metric line; batched / served: the multi-terrain forms; copy: a 1 GiB "
+                         "device copy, the known-size kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--terrains", type=int, default=128, help="batched/served: number of terrains")
     ap.add_argument("--min-sites", type=int, default=1000)
     ap.add_argument("--max-sites", type=int, default=65536)
@@ -93,6 +94,8 @@ def main() -> int:
                                    "(terrains shard across GPUs without communication, see DESIGN.md)"}))
         return 2
 
+    if args.mode == "copy":
+        return copy_calibration()
     if args.mode != "single":
         return multi_terrain(args)
 
@@ -183,6 +186,8 @@ def main() -> int:
         # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
         # scaled by the factor the same counters read for a copy kernel of known size
         roofline["traffic"] = args.traffic_kib * 1024.0
+    else:
+        roofline["traffic"] = measured_traffic(dom, bytes_per_launch)
 
     cpu = None
     if not args.no_cpu:
@@ -220,6 +225,51 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_da_fold_traffic_fbm1M.md")
+
+
+def measured_traffic(kernel: str, algorithmic_bytes: float):
+    """HBM bytes per launch of ``kernel`` as recorded by the committed counter
+    summary (lines ``traffic_kernel:``, ``traffic_algorithmic_bytes:`` and
+    ``traffic_bytes_per_launch:``).  Used only when the summary was made for the
+    same kernel and the same work per launch (algorithmic bytes within 1%);
+    otherwise None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            fields = dict(line.split(":", 1) for line in f if line.startswith("traffic_"))
+        if fields["traffic_kernel"].strip() != kernel:
+            return None
+        ref = float(fields["traffic_algorithmic_bytes"])
+        if abs(ref - algorithmic_bytes) > 0.01 * algorithmic_bytes:
+            return None
+        return float(fields["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration() -> int:
+    """A device-to-device copy of 1 GiB (read 1 GiB, write 1 GiB), three times:
+    the kernel whose rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch calibrates
+    the counters against a known byte count."""
+    import torch
+
+    nbytes = 1 << 30
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(3):
+        ts = time.perf_counter()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - ts)
+    assert bool((y == 1).all())
+    print(json.dumps({"mode": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "best_s": min(times), "GBps": 2 * nbytes / min(times) / 1e9}), flush=True)
     return 0
 
 

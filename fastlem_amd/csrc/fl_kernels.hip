@@ -33,6 +33,7 @@
 // reverse-BFS accumulation produces; response time is a sequential
 // root-to-leaf prefix per chain.  Results are therefore bit-identical to the
 // reference for the same inputs.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1230,15 +1231,17 @@ fl_profiler::~fl_profiler() {
   for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
-// `units`: the launch's work items for the profile (sites or positions).
+// `units`: the launch's work items for the profile (sites or positions).  A
+// profiled launch passes its two events to the dispatch itself
+// (hipExtLaunchKernelGGL), which stamps them with the kernel's own start and
+// end: the kernel's execution time, without the queue's dispatch gap.
 #define FL_LAUNCH_GRID(kernel, grid, block, units, stream, ...)                    \
   do {                                                                            \
     fl_profiler* prof_ = fl_current_profiler;                                     \
     if (prof_) {                                                                  \
       hipEvent_t a_ = prof_->get(), b_ = prof_->get();                            \
-      (void)hipEventRecord(a_, (stream));                                         \
-      hipLaunchKernelGGL(kernel, (grid), (block), 0, (stream), __VA_ARGS__);     \
-      (void)hipEventRecord(b_, (stream));                                         \
+      hipExtLaunchKernelGGL(kernel, (grid), (block), 0, (stream), a_, b_, 0,      \
+                            __VA_ARGS__);                                         \
       prof_->record((stream), #kernel, a_, b_, (units));                          \
     } else {                                                                      \
       hipLaunchKernelGGL(kernel, (grid), (block), 0, (stream), __VA_ARGS__);     \

@@ -406,3 +406,43 @@ def test_pipelining_modes_agree(fbm20k, mode):
     assert _bits_equal(t.elevations(), ref)
     if mode == 2:
         assert st["redone"] > 0
+
+
+def test_profiled_solve_is_exact_and_timed(fbm20k):
+    """A profiled solve (every launch carries start / end events stamped by
+    its dispatch) gives the oracle's elevations, and the profile holds every
+    per-iteration kernel with positive time and the launch counts the
+    iteration count implies."""
+    import ctypes
+
+    from fastlem_amd import _native
+    from fastlem_amd.generator import context
+    from oracle import flo
+
+    w = fbm20k
+    L = _native.lib()
+    ctx = context()
+    _native.check(L.fl_ctx_set_profiling(ctx, 1))
+    try:
+        _native.check(L.fl_ctx_reset_profile(ctx))
+        t = _dev(w)
+        it = t.solve()
+        redone = t.last_stats()["redone"]
+        prof = {}
+        for nm in ("k_receivers", "k_children", "k_chain_perm", "k_da_fold", "k_tau_fold<false>", "k_tau_scatter"):
+            cnt, ms, units = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
+            _native.check(L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms),
+                                               ctypes.byref(units)))
+            prof[nm] = (cnt.value, ms.value, units.value)
+    finally:
+        _native.check(L.fl_ctx_set_profiling(ctx, 0))
+    ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+    assert it == rit
+    assert _bits_equal(t.elevations(), ref)
+    for nm, (cnt, ms, units) in prof.items():
+        assert cnt > 0 and ms > 0.0 and units > 0, (nm, cnt, ms, units)
+    # one receivers launch per iteration, per redone iteration and per
+    # speculative iteration cancelled at convergence
+    assert it <= prof["k_receivers"][0] <= it + redone + 3
+    assert prof["k_receivers"][2] == prof["k_receivers"][0] * w.n
+    assert prof["k_da_fold"][0] >= it
