@@ -560,9 +560,17 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
   };
   const uint32_t cut = (uint32_t)std::max<size_t>(4096, total / (64 * (size_t)threads));
   std::vector<uint32_t> big;  // reconstruction nodes with more than `cut` leaves, ascending
+  // the union-find records of the edges a few steps ahead are fetched early:
+  // the loop is bound by their cache misses
+  constexpr size_t kAhead = 16;
   for (size_t i = 0; i < m; i++) {
     uint32_t a = msf_a[i], b = msf_b[i];
     if (a >= n || b >= n) return FL_ERR_INVALID;
+    if (i + kAhead < m) {
+      const uint32_t pa = msf_a[i + kAhead], pb = msf_b[i + kAhead];
+      if (pa < n) __builtin_prefetch(&uf[pa]);
+      if (pb < n) __builtin_prefetch(&uf[pb]);
+    }
     uint32_t ra = find(a), rb = find(b);
     if (ra == rb) return FL_ERR_INVALID;  // not a forest
     Root& A = uf[ra];
@@ -593,6 +601,23 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
     std::vector<uint32_t> tasks;
     for (size_t q = big.size(); q-- > 0;) {
       uint32_t x = big[q];
+      // the sequential walk down the large nodes: records 16 and 8 nodes
+      // ahead (and their children's) are fetched early
+      if (q >= 16) {
+        const uint32_t y = big[q - 16];
+        __builtin_prefetch(&node[y]);
+        __builtin_prefetch(&down[y]);
+      }
+      if (q >= 8) {
+        const uint32_t y = big[q - 8];
+        const Node& Y = node[y];
+        __builtin_prefetch(&node[Y.ch0]);
+        __builtin_prefetch(&down[Y.ch0]);
+        __builtin_prefetch(&node[Y.ch1]);
+        __builtin_prefetch(&down[Y.ch1]);
+        __builtin_prefetch(&leaf[msf_a[y - n]]);
+        __builtin_prefetch(&leaf[msf_b[y - n]]);
+      }
       if (!reached(x)) continue;
       visit(x);
       for (uint32_t c : {node[x].ch0, node[x].ch1})

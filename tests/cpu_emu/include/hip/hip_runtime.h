@@ -60,6 +60,7 @@ enum hipStreamCaptureMode {
 struct alignas(16) uint4 {
   unsigned int x, y, z, w;
 };
+inline uint4 make_uint4(unsigned int x, unsigned int y, unsigned int z, unsigned int w) { return uint4{x, y, z, w}; }
 
 struct dim3 {
   uint32_t x, y, z;
