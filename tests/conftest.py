@@ -11,6 +11,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    config.addinivalue_line("markers", "fullsize: parity at the workloads' full sizes (minutes each)")
 
 
 @pytest.fixture(scope="session")
