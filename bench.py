@@ -148,6 +148,16 @@ def main() -> int:
     value = site_iters / elapsed
 
     stats = t.last_stats()
+    # the drop-in with host buffers (fl_generate_f64: validation, relabelling,
+    # upload over PCIe, solve, download), outside the timed region
+    out = np.empty(n, np.float64)
+    hit = ctypes.c_uint32()
+    ts = time.perf_counter()
+    _native.check(L.fl_generate_f64(ctx, ctypes.byref(t._desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    ctypes.byref(hit)))
+    host_s = time.perf_counter() - ts
+    host_buffers = {"seconds": host_s, "site_iter_per_s": float(n) * hit.value / host_s, "iterations": hit.value}
+    log(f"host buffers (fl_generate_f64): {hit.value} iterations in {host_s:.3f}s")
     # roofline of the dominant kernel: HIP events around every launch on the
     # library's stream, one extra solve
     L.fl_ctx_set_profiling(ctx, 1)
@@ -223,6 +233,7 @@ def main() -> int:
                    "host_syncs_per_solve": stats["host_syncs"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_buffers": host_buffers,
     }
     print(json.dumps(line), flush=True)
     return 0

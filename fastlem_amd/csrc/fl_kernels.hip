@@ -679,14 +679,22 @@ struct fl_tau_in {
   int clamp;
 };
 
-__device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint32_t k, double a) {
-  double d = ti.dnext[k];
-  double celerity = ti.K[k] * rust_pow_half(a);
-  ti.xp[P] = 1.0 / celerity * d;
+// The site values tau_inputs reads, loaded together with a position's
+// gathers: after the store of A the compiler cannot move loads above it.
+struct fl_tau_site {
+  double K, d;
+};
+__device__ __forceinline__ fl_tau_site tau_site(const fl_tau_in& ti, uint32_t k) {
+  return fl_tau_site{ti.K[k], ti.dnext[k]};
+}
+__device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint32_t k, double a,
+                                           fl_tau_site v) {
+  double celerity = v.K * rust_pow_half(a);
+  ti.xp[P] = 1.0 / celerity * v.d;
   if (ti.clamp) {
     ti.up[P] = ti.U[k];
     ti.tanp[P] = ti.tan_ms[k];
-    ti.dp[P] = d;
+    ti.dp[P] = v.d;
     ti.hop[P] = ti.h[k];
   }
 }
@@ -724,9 +732,11 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
       const uint32_t cnt = end - cs;
       double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
       uint32_t np = 0, k = FL_NONE;
+      fl_tau_site ts{0.0, 0.0};
       if (lane < cnt) {
         const uint32_t P = base + cs + lane;
         k = perm[P];
+        ts = tau_site(ti, k);
         da_inputs(P, k, pcnt[P], lists[P], parea[P], rp, col, next, cmask, hc, area, A, p, a, b, x, y, np);
       }
       s_p[lane] = p;
@@ -765,7 +775,7 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
       __syncthreads();
       if (lane < cnt) {
         A[s_k[lane]] = s_r[lane];
-        tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane]);
+        tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane], ts);
       }
       __syncthreads();  // the tile arrays are rewritten next
       end = cs;
@@ -798,11 +808,12 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
           const uint32_t P = base + end - 1 - u;
           double pre, p0, p1, p2, p3;
           uint32_t np;
+          const fl_tau_site ts = tau_site(ti, vk[u]);
           da_inputs(P, vk[u], vc[u], vl[u], va[u], rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
           r = ((pre + r) + p0) + p1;
           if (np >= 3) r = da_tail_terms(r, np, p2, p3, vk[u], rp, col, next, cmask, hc, A);
           A[vk[u]] = r;
-          tau_inputs(ti, P, vk[u], r);
+          tau_inputs(ti, P, vk[u], r, ts);
         }
       }
       end = cs;
