@@ -534,7 +534,7 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
     if (c->prof) {
       for (uint32_t l = 0; l < f.R; l++) {
         uint64_t units = exact_grid(T->last_boff, l).positions;
-        for (size_t q : {f.rec_da[l], f.rec_da[l] + 1, f.rec_tau[l]})
+        for (size_t q : {f.rec_da[l], f.rec_tau[l]})
           if (q < c->prof->pending.size()) c->prof->pending[q].units = units;
       }
       if (f.rec_scatter < c->prof->pending.size()) c->prof->pending[f.rec_scatter].units = T->last_boff[2 * nb + 1];
