@@ -258,13 +258,17 @@ __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, cons
 // the rounds launched after it return at once.
 __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long long* rt,
                              uint32_t* __restrict__ flag_set, const uint32_t* __restrict__ prev_flag) {
+  uint32_t v = gid();
+  // the site's own pairs are read together with the previous round's flag
+  unsigned long long c = 0, r = 0;
+  if (v < n) {
+    c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
+    r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
+  }
   // a round after a round without movement has nothing to do
   if (prev_flag && *prev_flag == 0) return;
-  uint32_t v = gid();
   bool moved = false;
   if (v < n) {
-    unsigned long long c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
-    unsigned long long r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
     uint32_t a = pair_ptr(c), b = pair_ptr(r);
     unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
     unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
@@ -481,11 +485,14 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
                              const uint32_t* __restrict__ col, const unsigned long long* __restrict__ cmask,
                              const uint32_t* __restrict__ hc, uint32_t* __restrict__ pcnt,
                              uint32_t* __restrict__ plist, double* __restrict__ parea) {
-  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   uint32_t v = gid();
+  // read together with the guard (one load latency)
+  const uint32_t mld = *maxld;
+  const unsigned long long rv = v < n ? rt[v] : 0ull, cv = v < n ? cp[v] : 0ull;
+  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   // the bucket histograms are used up (count, scan, place): clear the
   // entries this decomposition touched for the next one
-  const uint32_t nb = 2u * (*maxld + 1u);
+  const uint32_t nb = 2u * (mld + 1u);
   for (uint32_t i = v; i <= nb && i < 2u * n + 4u; i += gridDim.x * blockDim.x) {
     bcnt[i] = 0;
     bnodes[i] = 0;
@@ -493,14 +500,14 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
     bncur[i] = 0;
   }
   if (v >= n) return;
-  uint32_t r = pair_ptr(rt[v]);
+  uint32_t r = pair_ptr(rv);
   if (!is_outlet[r]) {
     A[v] = area[v];
     tau[v] = 0.0;
     h2[v] = h[v];
     return;
   }
-  unsigned long long c = cp[v];
+  unsigned long long c = cv;
   uint32_t hd = pair_ptr(c);
   uint32_t base = head_base[hd];
   const uint32_t P = base + pair_cnt(c);
@@ -717,12 +724,15 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
           const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
           const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
           fl_tau_in ti) {
-  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
+  // the round's bounds are read together with the guard (one load latency;
+  // the bounds enter the test so that their loads stay above it: offsets are
+  // ascending, c2 < c0 never holds once the guard is clear)
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
   __shared__ uint32_t s_n[64], s_k[64];
   const uint4* __restrict__ lists = reinterpret_cast<const uint4*>(plist);
   const uint32_t lane = threadIdx.x;
-  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   // long chains: one per block (uniform per block)
   for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
     const uint32_t base = chain_base[c], len = chain_len[c];
@@ -837,10 +847,13 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
            const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
            const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
            const double* __restrict__ hop, double* __restrict__ hnp) {
-  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
+  // the round's bounds are read together with the guard (one load latency;
+  // the bounds enter the test so that their loads stay above it: offsets are
+  // ascending, c2 < c0 never holds once the guard is clear)
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   // long chains: one per block (uniform per block)
   for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
     const uint32_t base = chain_base[c], len = chain_len[c];
@@ -1045,8 +1058,9 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, const uint32_t
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ active, uint32_t* __restrict__ tchanged) {
-  if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   const uint32_t npos = *npos_ptr;  // positions of the iteration's chains
+  // (npos enters the test so that its load stays above it)
+  if (*guard != 0 || npos == 0xffffffffu) return;  // pointer jumping has not converged: the iteration is redone
   uint32_t P = gid();
   bool changed = false;
   if (P < npos) {
