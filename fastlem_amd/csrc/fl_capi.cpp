@@ -345,7 +345,6 @@ fl_round_grid exact_grid(const std::vector<uint32_t>& off, uint32_t l) {
   fl_round_grid g{};
   if (2 * l + 2 > nb) return g;
   g.positions = bn[2 * l + 2] - bn[2 * l];
-  g.pre_blocks = (uint32_t)((g.positions + 255) / 256);
   g.fold_blocks = (bo[2 * l + 2] - bo[2 * l + 1]) + (bo[2 * l + 1] - bo[2 * l] + 63) / 64;
   return g;
 }
@@ -403,16 +402,11 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::ve
   fl_launch_chain_layout(s, T->t, w, R);
   // launch sizes from the previous iteration's rounds (grid-stride kernels:
   // any size is correct), profiler units filled in once the bounds are known
-  uint32_t pre_default = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 2048));
   uint32_t fold_default = std::max<uint32_t>(1, std::min<uint32_t>((n + 127) / 128, 8192));
   auto guess = [&](uint32_t l) {
     fl_round_grid g = T->last_boff.empty() ? fl_round_grid{} : exact_grid(T->last_boff, l);
-    g.pre_blocks = g.pre_blocks ? std::min<uint32_t>(g.pre_blocks + g.pre_blocks / 4 + 8, pre_default) : 8;
     g.fold_blocks = g.fold_blocks ? std::min<uint32_t>(g.fold_blocks + g.fold_blocks / 4 + 8, fold_default) : 8;
-    if (T->last_boff.empty()) {
-      g.pre_blocks = pre_default;
-      g.fold_blocks = fold_default;
-    }
+    if (T->last_boff.empty()) g.fold_blocks = fold_default;
     g.positions = 0;
     return g;
   };

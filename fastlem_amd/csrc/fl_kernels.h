@@ -122,7 +122,6 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
 // round can be enqueued before its bounds are known on the host.  Kernels
 // loop over their range, so any non-zero grid is correct.
 struct fl_round_grid {
-  uint32_t pre_blocks;   // unused (kept for the launch-shape estimates)
   uint32_t fold_blocks;  // k_da_fold / k_tau_fold, 64 threads each
   uint64_t positions;    // profiler units (0 when not known at launch)
 };
