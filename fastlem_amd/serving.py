@@ -19,6 +19,31 @@ from ._native import check, ptr
 from .generator import ParameterArrays, make_desc
 
 
+def request(terrain: dict):
+    """The generate() call of a terrain dict (as for DeviceBatch): its
+    ``fl_terrain_desc``, the arrays it points to (kept alive by the caller)
+    and the output array."""
+
+    class _M:
+        pass
+
+    m = _M()
+    g = (np.ascontiguousarray(terrain["row_ptr"], np.uint32), np.ascontiguousarray(terrain["col"], np.uint32),
+         np.ascontiguousarray(terrain["length"], np.float64))
+    a = np.ascontiguousarray(terrain["area"], np.float64)
+    m.graph = lambda: g
+    m.num = lambda: len(g[0]) - 1
+    m.areas = lambda: a
+    n = m.num()
+    params = ParameterArrays(np.zeros(n), terrain["erodibility"], terrain["uplift"], np.zeros(n, bool),
+                             terrain.get("max_slope"))
+    init = np.ascontiguousarray(terrain["init_elevation"], np.float64)
+    outlets = np.ascontiguousarray(terrain["outlets"], np.uint32)
+    keep = [g, a, params, init, outlets]
+    desc = make_desc(m, params, init, outlets, terrain.get("max_iteration", 0xFFFFFFFF), keep)
+    return desc, keep, np.empty(n, np.float64)
+
+
 class Server:
     def __init__(self, slots: int = 8, hbm_budget_bytes: int = 64 << 30, device: int = 0):
         h = ctypes.c_void_p()
@@ -28,24 +53,7 @@ class Server:
 
     def submit(self, terrain: dict) -> int:
         """terrain: dict as for fastlem_amd.device.DeviceBatch."""
-        class _M:
-            pass
-
-        m = _M()
-        g = (np.ascontiguousarray(terrain["row_ptr"], np.uint32), np.ascontiguousarray(terrain["col"], np.uint32),
-             np.ascontiguousarray(terrain["length"], np.float64))
-        a = np.ascontiguousarray(terrain["area"], np.float64)
-        m.graph = lambda: g
-        m.num = lambda: len(g[0]) - 1
-        m.areas = lambda: a
-        n = m.num()
-        params = ParameterArrays(np.zeros(n), terrain["erodibility"], terrain["uplift"], np.zeros(n, bool),
-                                 terrain.get("max_slope"))
-        init = np.ascontiguousarray(terrain["init_elevation"], np.float64)
-        outlets = np.ascontiguousarray(terrain["outlets"], np.uint32)
-        keep = [g, a, params, init, outlets]
-        desc = make_desc(m, params, init, outlets, terrain.get("max_iteration", 0xFFFFFFFF), keep)
-        out = np.empty(n, np.float64)
+        desc, keep, out = request(terrain)
         ticket = ctypes.c_uint64()
         check(_native.lib().fl_server_submit(self._h, ctypes.byref(desc), ptr(out, ctypes.c_double),
                                              ctypes.byref(ticket)))
