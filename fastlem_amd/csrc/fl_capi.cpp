@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -14,6 +15,8 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "fl_internal.h"
@@ -130,6 +133,66 @@ int32_t dev_alloc(fl_terrain* T_, T** p, uint64_t count) {
   return FL_OK;
 }
 
+// f(begin, end) over [0, count) in contiguous ranges on up to
+// fl_host_threads() threads, none smaller than min_per_thread; runs on the
+// calling thread when one range suffices.
+template <class F>
+void host_parallel(uint64_t count, uint64_t min_per_thread, F f) {
+  uint64_t th = std::min<uint64_t>(fl_host_threads(), std::max<uint64_t>(count / min_per_thread, 1));
+  if (th <= 1) {
+    f(0, count);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (uint64_t t = 0; t < th; t++) pool.emplace_back([&, t] { f(count * t / th, count * (t + 1) / th); });
+  for (auto& x : pool) x.join();
+}
+
+// f(g) for every g in [0, count), largest items first (by size(g)), on up to
+// fl_host_threads() threads.
+template <class S, class F>
+void host_each(uint32_t count, S size, F f) {
+  std::vector<uint32_t> order(count);
+  for (uint32_t g = 0; g < count; g++) order[g] = g;
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return size(x) > size(y); });
+  std::atomic<uint32_t> next{0};
+  auto worker = [&] {
+    for (uint32_t q; (q = next.fetch_add(1)) < count;) f(order[q]);
+  };
+  uint32_t th = std::min<uint32_t>(fl_host_threads(), count);
+  if (th <= 1) {
+    worker();
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < th; t++) pool.emplace_back(worker);
+  for (auto& x : pool) x.join();
+}
+
+// The first failing row of a per-row check, over host threads: the same row
+// (and entry) a sequential scan would report.
+struct RowError {
+  uint64_t row = UINT64_MAX;
+  int kind = 0;  // 0 none; 1 index out of range; 2 self-loop; 3 duplicate; 4 not symmetric
+  uint32_t k = 0, j = 0;
+};
+template <class C>
+RowError first_row_error(uint64_t n, C check_row) {
+  std::mutex mu;
+  RowError best;
+  host_parallel(n, 65536, [&](uint64_t a, uint64_t b) {
+    for (uint64_t i = a; i < b; i++) {
+      RowError e = check_row(i);
+      if (e.kind) {
+        std::lock_guard<std::mutex> g(mu);
+        if (i < best.row) best = e;
+        return;
+      }
+    }
+  });
+  return best;
+}
+
 int32_t validate(const fl_terrain_desc* d) {
   if (!d) return set_err(FL_ERR_INVALID, "null terrain description");
   if (d->n >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "n=%llu exceeds 2^32-2 sites", (unsigned long long)d->n);
@@ -141,34 +204,64 @@ int32_t validate(const fl_terrain_desc* d) {
   if (d->n_outlets && !d->outlets) return set_err(FL_ERR_INVALID, "null outlets");
   if (d->has_max_slope && !d->max_slope) return set_err(FL_ERR_INVALID, "has_max_slope without max_slope");
   uint64_t n = d->n;
-  if (d->row_ptr[0] != 0 || d->row_ptr[n] != d->nnz) return set_err(FL_ERR_INVALID, "row_ptr must span [0, nnz]");
+  const uint32_t* rp = d->row_ptr;
+  const uint32_t* col = d->col;
+  if (rp[0] != 0 || rp[n] != d->nnz) return set_err(FL_ERR_INVALID, "row_ptr must span [0, nnz]");
   for (uint64_t i = 0; i < n; i++)
-    if (d->row_ptr[i + 1] < d->row_ptr[i]) return set_err(FL_ERR_INVALID, "row_ptr decreases at %llu",
-                                                          (unsigned long long)i);
-  // neighbour indices, self-loops, duplicates (the reference's BFS would
-  // visit such a site twice), symmetry (terrain-graph is undirected)
-  std::vector<uint32_t> mark(n, 0xffffffffu);
-  for (uint64_t i = 0; i < n; i++) {
-    for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++) {
-      uint32_t j = d->col[k];
-      if (j >= n) return set_err(FL_ERR_INVALID, "col[%u]=%u out of range", k, j);
-      if (j == i) return set_err(FL_ERR_INVALID, "self-loop at site %llu", (unsigned long long)i);
-      if (mark[j] == (uint32_t)i) return set_err(FL_ERR_INVALID, "duplicate edge %llu-%u", (unsigned long long)i, j);
-      mark[j] = (uint32_t)i;
+    if (rp[i + 1] < rp[i]) return set_err(FL_ERR_INVALID, "row_ptr decreases at %llu", (unsigned long long)i);
+  // neighbour indices, self-loops and duplicates (the reference's BFS would
+  // visit such a site twice), in entry order within each row
+  RowError e = first_row_error(n, [&](uint64_t i) {
+    RowError r;
+    const uint32_t b = rp[i], deg = rp[i + 1] - b;
+    std::unordered_set<uint32_t> seen;  // rows longer than 32 entries only
+    for (uint32_t k = b; k < b + deg; k++) {
+      const uint32_t j = col[k];
+      r.row = i;
+      r.k = k;
+      r.j = j;
+      if (j >= n) {
+        r.kind = 1;
+        return r;
+      }
+      if (j == i) {
+        r.kind = 2;
+        return r;
+      }
+      bool dup = false;
+      if (deg <= 32) {
+        for (uint32_t q = b; q < k && !dup; q++) dup = col[q] == j;
+      } else {
+        dup = !seen.insert(j).second;
+      }
+      if (dup) {
+        r.kind = 3;
+        return r;
+      }
     }
-  }
-  for (uint64_t i = 0; i < n; i++) {
-    for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++) {
-      uint32_t j = d->col[k];
+    return RowError{};
+  });
+  if (e.kind == 1) return set_err(FL_ERR_INVALID, "col[%u]=%u out of range", e.k, e.j);
+  if (e.kind == 2) return set_err(FL_ERR_INVALID, "self-loop at site %llu", (unsigned long long)e.row);
+  if (e.kind == 3) return set_err(FL_ERR_INVALID, "duplicate edge %llu-%u", (unsigned long long)e.row, e.j);
+  // symmetry (terrain-graph is undirected)
+  e = first_row_error(n, [&](uint64_t i) {
+    for (uint32_t k = rp[i]; k < rp[i + 1]; k++) {
+      const uint32_t j = col[k];
       bool found = false;
-      for (uint32_t q = d->row_ptr[j]; q < d->row_ptr[j + 1]; q++)
-        if (d->col[q] == i) {
-          found = true;
-          break;
-        }
-      if (!found) return set_err(FL_ERR_INVALID, "edge %llu-%u is not symmetric", (unsigned long long)i, j);
+      for (uint32_t q = rp[j]; q < rp[j + 1] && !found; q++) found = col[q] == i;
+      if (!found) {
+        RowError r;
+        r.row = i;
+        r.kind = 4;
+        r.k = k;
+        r.j = j;
+        return r;
+      }
     }
-  }
+    return RowError{};
+  });
+  if (e.kind == 4) return set_err(FL_ERR_INVALID, "edge %llu-%u is not symmetric", (unsigned long long)e.row, e.j);
   std::vector<uint8_t> seen(n, 0);
   for (uint64_t k = 0; k < d->n_outlets; k++) {
     if (d->outlets[k] >= n) return set_err(FL_ERR_INVALID, "outlet %u out of range", d->outlets[k]);
@@ -704,12 +797,22 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
   if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
   uint64_t N = 0, NNZ = 0, K = 0;
+  if (count == 1) {
+    FLCHK(validate(&descs[0]));
+  } else {
+    // terrains on host threads; the first failing terrain's error is reported
+    std::vector<int32_t> rc(count, FL_OK);
+    std::vector<std::string> msg(count);
+    host_each(
+        count, [&](uint32_t g) { return descs[g].nnz; },
+        [&](uint32_t g) {
+          rc[g] = validate(&descs[g]);
+          if (rc[g] != FL_OK) msg[g] = g_err;
+        });
+    for (uint32_t g = 0; g < count; g++)
+      if (rc[g] != FL_OK) return set_err(rc[g], "terrain %u: %s", g, msg[g].c_str());
+  }
   for (uint32_t g = 0; g < count; g++) {
-    int32_t rc = validate(&descs[g]);
-    if (rc != FL_OK) {
-      if (count > 1) g_err = "terrain " + std::to_string(g) + ": " + g_err;
-      return rc;
-    }
     N += descs[g].n;
     NNZ += descs[g].nnz;
     K += descs[g].n_outlets;
@@ -746,35 +849,59 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   std::vector<double> area(n), erod(n), uplift(n), init(n), tan_ms(n);
   std::vector<uint8_t> is_out(n, 0);
   std::vector<uint32_t> tid(count > 1 ? n : 0);
-  std::vector<uint32_t> o2, n2;
-  uint32_t q = 0;
-  for (uint32_t g = 0; g < count; g++) {
+  // entry offset of each terrain in the union
+  std::vector<uint64_t> q0(count + 1, 0);
+  for (uint32_t g = 0; g < count; g++) q0[g + 1] = q0[g] + descs[g].nnz;
+  // terrain g relabelled into its site and entry ranges (disjoint between
+  // terrains): breadth-first order, then its rows and site values copied,
+  // over host threads when `inner`
+  auto relabel = [&](uint32_t g, bool inner) {
     const fl_terrain_desc* d = &descs[g];
-    uint32_t s0 = T->site_off[g], ng = (uint32_t)d->n;
-    if (ng == 0) continue;
+    const uint32_t s0 = T->site_off[g], ng = (uint32_t)d->n;
+    if (ng == 0) return;
+    std::vector<uint32_t> o2, n2;
     locality_order(ng, d->row_ptr, d->col, o2, n2);
+    uint32_t q = (uint32_t)q0[g];
     for (uint32_t u = 0; u < ng; u++) {
-      uint32_t i = o2[u];
-      T->old_of_new[s0 + u] = s0 + i;
-      T->new_of_old[s0 + i] = s0 + u;
-      for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++, q++) {
-        T->h_col[q] = s0 + n2[d->col[k]];
-        T->h_len[q] = d->length[k];
-      }
+      const uint32_t i = o2[u];
+      q += d->row_ptr[i + 1] - d->row_ptr[i];
       T->h_row_ptr[s0 + u + 1] = q;
-      area[s0 + u] = d->area[i];
-      erod[s0 + u] = d->erodibility[i];
-      uplift[s0 + u] = d->uplift_rate[i];
-      init[s0 + u] = d->init_elevation[i];
-      tan_ms[s0 + u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
-      if (count > 1) tid[s0 + u] = g;
     }
+    auto fill = [&](uint64_t lo, uint64_t hi) {
+      for (uint64_t u = lo; u < hi; u++) {
+        const uint32_t i = o2[u];
+        T->old_of_new[s0 + u] = s0 + i;
+        T->new_of_old[s0 + i] = s0 + (uint32_t)u;
+        // (the start of the terrain's first row is its own offset: the entry
+        // before it belongs to the previous terrain)
+        uint32_t e = u == 0 ? (uint32_t)q0[g] : T->h_row_ptr[s0 + u];
+        for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++, e++) {
+          T->h_col[e] = s0 + n2[d->col[k]];
+          T->h_len[e] = d->length[k];
+        }
+        area[s0 + u] = d->area[i];
+        erod[s0 + u] = d->erodibility[i];
+        uplift[s0 + u] = d->uplift_rate[i];
+        init[s0 + u] = d->init_elevation[i];
+        tan_ms[s0 + u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
+        if (count > 1) tid[s0 + u] = g;
+      }
+    };
+    if (inner)
+      host_parallel(ng, 65536, fill);
+    else
+      fill(0, ng);
     for (uint64_t k = 0; k < d->n_outlets; k++) {
       uint32_t o = s0 + n2[d->outlets[k]];
       T->h_outlets[T->outlet_off[g] + k] = o;
       is_out[o] = 1;
     }
-  }
+  };
+  if (count == 1)
+    relabel(0, true);
+  else
+    host_each(
+        count, [&](uint32_t g) { return descs[g].nnz; }, [&](uint32_t g) { relabel(g, false); });
 
   fl_dev_terrain& t = T->t;
   fl_terrain* P = T.get();

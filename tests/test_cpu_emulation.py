@@ -183,3 +183,78 @@ def test_emulated_pipelining_modes_agree(emu_lib, kind):
         assert _bits_equal(runs[mode][1], ref)
     assert runs[1][2] < runs[0][2]  # fewer host synchronisations
     assert runs[2][3] > 0  # the redo path ran
+
+
+def _create_error(L, terrains):
+    """(status, message) of fl_batch_create on the terrain dicts (emulated
+    library: validation and relabelling are host code)."""
+    from cpu_emu import emu
+    from fastlem_amd.device import batch_descs
+
+    keep = []
+    d = batch_descs(terrains, keep)
+    ctx = ctypes.c_void_p()
+    emu.check(L, L.fl_ctx_create(0, ctypes.byref(ctx)))
+    h = ctypes.c_void_p()
+    rc = L.fl_batch_create(ctx, d, len(terrains), ctypes.byref(h))
+    msg = L.fl_last_error().decode()
+    if rc == 0:
+        L.fl_terrain_destroy(h)
+    L.fl_ctx_destroy(ctx)
+    return rc, msg
+
+
+def _symmetric_edit(t, i, j_old, j_new):
+    """Terrain dict t with row i's entry j_old replaced by j_new (one-sided)."""
+    t = dict(t)
+    col = np.array(t["col"], np.uint32)
+    rp = t["row_ptr"]
+    k = rp[i] + int(np.nonzero(col[rp[i]:rp[i + 1]] == j_old)[0][0])
+    col[k] = j_new
+    t["col"] = col
+    return t
+
+
+def test_emulated_validation_reports_the_first_error(emu_lib):
+    """Malformed graphs are rejected at upload with FL_ERR_INVALID (-1): the
+    message names the first failing row, as a sequential scan would, although
+    rows are checked on host threads (the 300k-site terrain takes the
+    threaded path); in a batch, the first failing terrain is named."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_plain
+
+    L = emu_lib
+    big = DeviceBatch.terrain_of(config_plain(300_000, 548.0))
+    small = DeviceBatch.terrain_of(config_plain(500, 22.0, 1.0))
+    rp, col = big["row_ptr"], big["col"]
+    assert _create_error(L, [big]) == (0, "")
+    # a self-loop at row 250000 and a duplicate at row 120000: the duplicate's row comes first
+    a = _symmetric_edit(big, 250_000, int(col[rp[250_000]]), 250_000)
+    r1 = 120_000
+    a = _symmetric_edit(a, r1, int(col[rp[r1] + 1]), int(col[rp[r1]]))
+    rc, msg = _create_error(L, [a])
+    assert rc == -1 and msg == f"duplicate edge {r1}-{int(col[rp[r1]])}", msg
+    # an out-of-range index
+    b = _symmetric_edit(big, 200_000, int(col[rp[200_000]]), 300_000)
+    rc, msg = _create_error(L, [b])
+    assert rc == -1 and msg == f"col[{rp[200_000]}]=300000 out of range", msg
+    # one-sided edges (rows 180000 and 90000 lose a neighbour and gain a
+    # distant site): the first entry in row order whose reverse is missing,
+    # found by brute force, is the one reported
+    far = 5
+    c = _symmetric_edit(big, 180_000, int(col[rp[180_000]]), far)
+    c = _symmetric_edit(c, 90_000, int(col[rp[90_000]]), far)
+    n = len(rp) - 1
+    cc = np.asarray(c["col"], np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    keys = np.sort(rows * n + cc)
+    rev = cc * n + rows
+    pos = np.minimum(np.searchsorted(keys, rev), len(keys) - 1)
+    bad_k = int(np.nonzero(keys[pos] != rev)[0][0])
+    rc, msg = _create_error(L, [c])
+    assert rc == -1 and msg == f"edge {int(rows[bad_k])}-{int(cc[bad_k])} is not symmetric", msg
+    # a batch: the first failing terrain is named
+    bad = dict(small)
+    bad["outlets"] = np.concatenate([small["outlets"], small["outlets"][:1]]).astype(np.uint32)
+    rc, msg = _create_error(L, [small, small, bad, a])
+    assert rc == -1 and msg == f"terrain 2: duplicate outlet {int(small['outlets'][0])}", msg
