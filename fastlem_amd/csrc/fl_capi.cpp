@@ -307,6 +307,14 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
       return (int)FL_OK;
     });
+    // one forest: the ranks' working arrays are set up (from the outlets
+    // alone) on host threads while the GPU builds the forest
+    int prep_rc = FL_OK;
+    std::future<fl_krt_work*> prep;
+    if (G == 1)
+      prep = std::async(std::launch::async, [&]() {
+        return fl_krt_prepare(n, G, T->outlet_off.data(), T->h_outlets.data(), fl_host_threads(), &prep_rc);
+      });
     auto p1 = std::chrono::steady_clock::now();
     for (uint32_t v = 0; v < n; v++) init[v] = v;
     for (uint32_t g = 0; g < G; g++)
@@ -316,12 +324,19 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
     int rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, init, a, b, &fast, &err);
     T->stat_order_parts[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p1).count();
     int rc_ranks = FL_OK;
+    std::unique_ptr<fl_krt_work, void (*)(fl_krt_work*)> work(prep.valid() ? prep.get() : nullptr, fl_krt_release);
     if (rc == FL_OK && fast) {
       // the forest's ranks need the outlet set, not the pop order: they are
       // computed while the outlet phase still runs
       auto p2 = std::chrono::steady_clock::now();
-      rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(),
-                              T->site_off.data(), a.data(), b.data(), a.size(), rank.data(), fl_host_threads());
+      if (G == 1)
+        rc_ranks = work ? fl_krt_ranks_prepared(work.get(), G, T->outlet_off.data(), T->h_outlets.data(),
+                                                T->site_off.data(), a.data(), b.data(), a.size(), rank.data(),
+                                                fl_host_threads())
+                        : prep_rc;
+      else
+        rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(),
+                                T->site_off.data(), a.data(), b.data(), a.size(), rank.data(), fl_host_threads());
       T->stat_order_parts[2] =
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
     }

@@ -32,6 +32,17 @@ int fl_lake_visit_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
 int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
                  const uint32_t* rank_off, const uint32_t* site_off, const uint32_t* msf_a, const uint32_t* msf_b,
                  size_t m, uint32_t* rank_out, unsigned threads);
+// The one-forest form in two steps: the working arrays set up from the
+// sites and outlets alone (while the forest is still being built), then the
+// ranks from the forest.  fl_krt_prepare returns null and sets *rc on invalid
+// outlets; fl_krt_ranks_prepared consumes the arrays (release them after).
+struct fl_krt_work;
+fl_krt_work* fl_krt_prepare(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                            unsigned threads, int* rc);
+int fl_krt_ranks_prepared(fl_krt_work* work, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                          const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                          uint32_t* rank_out, unsigned threads);
+void fl_krt_release(fl_krt_work* work);
 // Host threads for the one-time setup work (at most 16).
 unsigned fl_host_threads();
 extern "C" int fl_ranks_from_msf_groups(size_t n, size_t groups, const uint32_t* outlet_off,

@@ -500,47 +500,65 @@ void parallel_range(size_t n, unsigned threads, F f) {
 
 namespace {
 
-int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
-                 const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
-                 uint32_t* rank_out, unsigned threads) {
-  const uint32_t NONE = 0xffffffffu;
-  const size_t total = n + m;
+const uint32_t kNone = 0xffffffffu;
+struct KrtNode {
+  uint32_t ch0, ch1;
+  uint32_t cnt;   // leaves below
+  uint32_t size;  // non-outlet leaves below
+};
+struct KrtDown {
+  uint32_t lo;     // first leaf position of the subtree
+  uint32_t entry;  // site where Prim enters the subtree
+  uint32_t off;    // rank of the first site Prim reaches in it
+};
+// union-find root record: parent, the component's current reconstruction
+// node and its counts (kept at the root, so a merge reads no child node)
+struct KrtRoot {
+  uint32_t parent, top, cnt, size;
+};
+
+}  // namespace
+
+// The ranks' working arrays for n sites (reconstruction-tree nodes for up to
+// n forest edges), set up from the outlets alone.
+struct fl_krt_work {
+  size_t n = 0;
+  std::vector<uint32_t> leaf;
+  std::unique_ptr<KrtRoot[]> uf;
+  std::unique_ptr<KrtNode[]> node;
+  std::unique_ptr<KrtDown[]> down;
+};
+
+fl_krt_work* fl_krt_prepare(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                            unsigned threads, int* rc) {
   if (threads == 0) threads = 1;
-  struct Node {
-    uint32_t ch0, ch1;
-    uint32_t cnt;   // leaves below
-    uint32_t size;  // non-outlet leaves below
-  };
-  struct Down {
-    uint32_t lo;     // first leaf position of the subtree
-    uint32_t entry;  // site where Prim enters the subtree
-    uint32_t off;    // rank of the first site Prim reaches in it
-  };
-  // union-find root record: parent, the component's current reconstruction
-  // node and its counts (kept at the root, so a merge reads no child node)
-  struct Root {
-    uint32_t parent, top, cnt, size;
-  };
-  std::vector<uint32_t> leaf(n);
-  std::unique_ptr<Root[]> uf(new Root[n]);
-  std::unique_ptr<Node[]> node(new Node[total]);
-  std::unique_ptr<Down[]> down(new Down[total]);
+  std::unique_ptr<fl_krt_work> w(new fl_krt_work());
+  w->n = n;
+  w->leaf.resize(n);
+  w->uf.reset(new KrtRoot[n]);
+  w->node.reset(new KrtNode[2 * n]);
+  w->down.reset(new KrtDown[2 * n]);
+  uint32_t* leaf = w->leaf.data();
+  KrtRoot* uf = w->uf.get();
+  KrtNode* node = w->node.get();
+  KrtDown* down = w->down.get();
   parallel_range(n, threads, [&](size_t a, size_t b) {
-    for (size_t v = a; v < b; v++) {
-      leaf[v] = (uint32_t)v;
-      rank_out[v] = NONE;
-    }
+    for (size_t v = a; v < b; v++) leaf[v] = (uint32_t)v;
   });
   for (size_t g = 0; g < groups; g++)
     for (uint32_t q = outlet_off[g]; q < outlet_off[g + 1]; q++) {
-      if (outlets[q] >= n) return FL_ERR_INVALID;
+      if (outlets[q] >= n) {
+        *rc = FL_ERR_INVALID;
+        return nullptr;
+      }
       leaf[outlets[q]] = outlets[outlet_off[g]];
     }
   parallel_range(n, threads, [&](size_t a, size_t b) {
     for (size_t v = a; v < b; v++) {
       uf[v] = {leaf[v], (uint32_t)v, 1u, 1u};
-      node[v] = {NONE, NONE, 1u, 1u};
-      down[v] = {NONE, NONE, 0};
+      node[v] = {kNone, kNone, 1u, 1u};
+      down[v] = {kNone, kNone, 0};
+      down[n + v] = {kNone, kNone, 0};
     }
   });
   for (size_t g = 0; g < groups; g++)
@@ -548,8 +566,33 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
       node[outlets[q]].size = 0;
       uf[outlets[q]].size = 0;
     }
-  parallel_range(m, threads, [&](size_t a, size_t b) {
-    for (size_t i = a; i < b; i++) down[n + i] = {NONE, NONE, 0};
+  *rc = FL_OK;
+  return w.release();
+}
+
+void fl_krt_release(fl_krt_work* w) { delete w; }
+
+namespace {
+
+// Ranks of one forest on the arrays of `work` (prepared for the same sites
+// and outlets; consumed).
+int krt_ranks_run(fl_krt_work* work, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                  const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                  uint32_t* rank_out, unsigned threads) {
+  const uint32_t NONE = kNone;
+  const size_t n = work->n;
+  if (m > n) return FL_ERR_INVALID;  // a forest on n sites has fewer than n edges
+  const size_t total = n + m;
+  if (threads == 0) threads = 1;
+  using Node = KrtNode;
+  using Down = KrtDown;
+  using Root = KrtRoot;
+  const uint32_t* leaf = work->leaf.data();
+  Root* uf = work->uf.get();
+  Node* node = work->node.get();
+  Down* down = work->down.get();
+  parallel_range(n, threads, [&](size_t a, size_t b) {
+    for (size_t v = a; v < b; v++) rank_out[v] = NONE;
   });
   auto find = [&](uint32_t v) {
     while (uf[v].parent != v) {
@@ -681,7 +724,22 @@ int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uin
   return FL_OK;
 }
 
+int krt_ranks_one(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                  const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                  uint32_t* rank_out, unsigned threads) {
+  int rc = FL_OK;
+  std::unique_ptr<fl_krt_work> work(fl_krt_prepare(n, groups, outlet_off, outlets, threads, &rc));
+  if (!work) return rc;
+  return krt_ranks_run(work.get(), groups, outlet_off, outlets, rank_off, msf_a, msf_b, m, rank_out, threads);
+}
+
 }  // namespace
+
+int fl_krt_ranks_prepared(fl_krt_work* work, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
+                          const uint32_t* rank_off, const uint32_t* msf_a, const uint32_t* msf_b, size_t m,
+                          uint32_t* rank_out, unsigned threads) {
+  return krt_ranks_run(work, groups, outlet_off, outlets, rank_off, msf_a, msf_b, m, rank_out, threads);
+}
 
 int fl_krt_ranks(size_t n, size_t groups, const uint32_t* outlet_off, const uint32_t* outlets,
                  const uint32_t* rank_off, const uint32_t* site_off, const uint32_t* msf_a, const uint32_t* msf_b,
