@@ -640,6 +640,27 @@ __device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c,
 // fold order), posts p0..p3 after it, np posts; absent terms are -0.0.  One
 // level of gathers (the children's A).  A site with more light children than
 // listed folds from the graph instead.
+// The partial folds of a listed position from its gathered values (v_i =
+// the i-th listed child's A, or anything when the list has fewer entries).
+__device__ __forceinline__ void da_combine(uint32_t cn, double area_k, double v0, double v1, double v2, double v3,
+                                           double& pre, double& p0, double& p1, double& p2, double& p3,
+                                           uint32_t& np) {
+  const uint32_t npre = cn & 0xffu, nl = npre + ((cn >> 8) & 0xffu);
+  v0 = nl > 0 ? v0 : FL_NEG0;
+  v1 = nl > 1 ? v1 : FL_NEG0;
+  v2 = nl > 2 ? v2 : FL_NEG0;
+  v3 = nl > 3 ? v3 : FL_NEG0;
+  // pre: x + (-0.0) == x, so the terms past npre are added as -0.0
+  pre = (((area_k + (npre > 0 ? v0 : FL_NEG0)) + (npre > 1 ? v1 : FL_NEG0)) + (npre > 2 ? v2 : FL_NEG0)) +
+        (npre > 3 ? v3 : FL_NEG0);
+  // posts: list entries npre, npre+1, ... (entries past nl read as -0.0)
+  p0 = npre == 0 ? v0 : npre == 1 ? v1 : npre == 2 ? v2 : npre == 3 ? v3 : FL_NEG0;
+  p1 = npre == 0 ? v1 : npre == 1 ? v2 : npre == 2 ? v3 : FL_NEG0;
+  p2 = npre == 0 ? v2 : npre == 1 ? v3 : FL_NEG0;
+  p3 = npre == 0 ? v3 : FL_NEG0;
+  np = nl - npre;
+}
+
 __device__ __forceinline__ void da_inputs(uint32_t P, uint32_t k, uint32_t cn, uint4 ids, double area_k,
                                           const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                                           const uint32_t* __restrict__ next,
@@ -652,18 +673,10 @@ __device__ __forceinline__ void da_inputs(uint32_t P, uint32_t k, uint32_t cn, u
     da_partial(k, rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
     return;
   }
-  const uint32_t npre = cn & 0xffu, nl = npre + ((cn >> 8) & 0xffu);
+  const uint32_t nl = (cn & 0xffu) + ((cn >> 8) & 0xffu);
   const double v0 = nl > 0 ? A[ids.x] : FL_NEG0, v1 = nl > 1 ? A[ids.y] : FL_NEG0;
   const double v2 = nl > 2 ? A[ids.z] : FL_NEG0, v3 = nl > 3 ? A[ids.w] : FL_NEG0;
-  // pre: x + (-0.0) == x, so the terms past npre are added as -0.0
-  pre = (((area_k + (npre > 0 ? v0 : FL_NEG0)) + (npre > 1 ? v1 : FL_NEG0)) + (npre > 2 ? v2 : FL_NEG0)) +
-        (npre > 3 ? v3 : FL_NEG0);
-  // posts: list entries npre, npre+1, ... (entries past nl read as -0.0)
-  p0 = npre == 0 ? v0 : npre == 1 ? v1 : npre == 2 ? v2 : npre == 3 ? v3 : FL_NEG0;
-  p1 = npre == 0 ? v1 : npre == 1 ? v2 : npre == 2 ? v3 : FL_NEG0;
-  p2 = npre == 0 ? v2 : npre == 1 ? v3 : FL_NEG0;
-  p3 = npre == 0 ? v3 : FL_NEG0;
-  np = nl - npre;
+  da_combine(cn, area_k, v0, v1, v2, v3, pre, p0, p1, p2, p3, np);
 }
 
 #define FL_DA_BLOCK 8
@@ -733,21 +746,56 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
   __shared__ uint32_t s_n[64], s_k[64];
   const uint4* __restrict__ lists = reinterpret_cast<const uint4*>(plist);
   const uint32_t lane = threadIdx.x;
-  // long chains: one per block (uniform per block)
+  // long chains: one per block (uniform per block).  Tiles of 64 positions
+  // from the tail, software-pipelined: the gathers of the next tile and the
+  // list loads of the one after are issued before lane 0 folds the current
+  // tile, with in-chain indices and no branches, so nothing waits for them
+  // before the fold; their values are combined when their tile is staged.
+  // No tile input depends on this round's results (light children head
+  // chains of deeper rounds; a listed slot past the list reads the site's
+  // own A and is dropped).
   for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
     const uint32_t base = chain_base[c], len = chain_len[c];
-    uint32_t end = len, cs = end > 64 ? end - 64 : 0;
+    struct Lists {
+      uint32_t k, cn;
+      uint4 ids;
+      double area;
+    };
+    struct Raw {
+      double g0, g1, g2, g3, area;
+      uint32_t k, cn;
+      fl_tau_site ts;
+    };
+    // position of this lane in the tile ending at `end` (lanes past the
+    // tile read its first position)
+    auto pos_of = [&](uint32_t end) {
+      const uint32_t cs = end > 64 ? end - 64 : 0;
+      return base + (lane < end - cs ? cs + lane : cs);
+    };
+    auto lists_of = [&](uint32_t end) {
+      const uint32_t P = pos_of(end);
+      return Lists{perm[P], pcnt[P], lists[P], parea[P]};
+    };
+    auto gather = [&](const Lists& q) {
+      const uint32_t nl = (q.cn & FL_LIST_OVERFLOW) ? 0u : (q.cn & 0xffu) + ((q.cn >> 8) & 0xffu);
+      return Raw{A[nl > 0 ? q.ids.x : q.k], A[nl > 1 ? q.ids.y : q.k], A[nl > 2 ? q.ids.z : q.k],
+                 A[nl > 3 ? q.ids.w : q.k], q.area, q.k, q.cn, tau_site(ti, q.k)};
+    };
+    uint32_t end = len;
+    uint32_t end1 = end > 64 ? end - 64 : 0;
+    Raw cur = gather(lists_of(end));
+    Lists nxt = lists_of(end1 > 0 ? end1 : end);
     double r = FL_NEG0;  // lane 0's running value
     while (end > 0) {
-      const uint32_t cnt = end - cs;
+      const uint32_t cs = end > 64 ? end - 64 : 0, cnt = end - cs;
+      const bool mine = lane < cnt;
       double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
-      uint32_t np = 0, k = FL_NONE;
-      fl_tau_site ts{0.0, 0.0};
-      if (lane < cnt) {
-        const uint32_t P = base + cs + lane;
-        k = perm[P];
-        ts = tau_site(ti, k);
-        da_inputs(P, k, pcnt[P], lists[P], parea[P], rp, col, next, cmask, hc, area, A, p, a, b, x, y, np);
+      uint32_t np = 0;
+      if (mine) {
+        if (cur.cn & FL_LIST_OVERFLOW)
+          da_partial(cur.k, rp, col, next, cmask, hc, area, A, p, a, b, x, y, np);
+        else
+          da_combine(cur.cn, cur.area, cur.g0, cur.g1, cur.g2, cur.g3, p, a, b, x, y, np);
       }
       s_p[lane] = p;
       s_a[lane] = a;
@@ -755,10 +803,49 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
       s_c[lane] = x;
       s_d[lane] = y;
       s_n[lane] = np;
-      s_k[lane] = k;
+      s_k[lane] = mine ? cur.k : FL_NONE;
       const unsigned long long rare = __ballot(np >= 3);
+      // a position with more posts than staged reads the graph in the fold
+      // (rare): such tiles take the fold with loads, the others one without,
+      // so the loads in flight are not waited for during the fold
+      const bool deep = __any(np > FL_POSTS);
+      const uint32_t kk = cur.k;
+      const fl_tau_site ts = cur.ts;
       __syncthreads();
-      if (lane == 0) {
+      const uint32_t end2 = end1 > 64 ? end1 - 64 : 0;
+      if (end1 > 0) cur = gather(nxt);
+      if (end2 > 0) nxt = lists_of(end2);
+      if (lane == 0 && !deep) {
+        // the terms of da_tail_terms for np <= FL_POSTS: p2 (and p3)
+        int e = (int)cnt - 1;
+        for (; e >= 7; e -= 8) {
+          double tp[8], ta[8], tb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            tp[u] = s_p[e - u];
+            ta[u] = s_a[e - u];
+            tb[u] = s_b[e - u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            r = ((tp[u] + r) + ta[u]) + tb[u];
+            if ((rare >> (e - u)) & 1ull) {
+              r = r + s_c[e - u];
+              if (s_n[e - u] >= 4) r = r + s_d[e - u];
+            }
+            s_r[e - u] = r;
+          }
+        }
+        for (; e >= 0; --e) {
+          r = ((s_p[e] + r) + s_a[e]) + s_b[e];
+          if ((rare >> e) & 1ull) {
+            r = r + s_c[e];
+            if (s_n[e] >= 4) r = r + s_d[e];
+          }
+          s_r[e] = r;
+        }
+      }
+      if (lane == 0 && deep) {
         int e = (int)cnt - 1;
         for (; e >= 7; e -= 8) {
           double tp[8], ta[8], tb[8];
@@ -783,13 +870,13 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
         }
       }
       __syncthreads();
-      if (lane < cnt) {
-        A[s_k[lane]] = s_r[lane];
-        tau_inputs(ti, base + cs + lane, s_k[lane], s_r[lane], ts);
+      if (mine) {
+        A[kk] = s_r[lane];
+        tau_inputs(ti, base + cs + lane, kk, s_r[lane], ts);
       }
       __syncthreads();  // the tile arrays are rewritten next
-      end = cs;
-      cs = end > 64 ? end - 64 : 0;
+      end = end1;
+      end1 = end2;
     }
   }
   // short chains: one lane each, the last blocks first (they hold no long chain)
