@@ -1088,10 +1088,11 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
     if (!clamp) {
       while (t < len) {
         uint32_t cnt = (len - t) < FL_BATCH ? (len - t) : FL_BATCH;
+        // every slot loads (slots past the chain read its last position): no
+        // branch per load, so the batch's loads are in flight together
         double vx[FL_BATCH];
 #pragma unroll
-        for (uint32_t q = 0; q < FL_BATCH; ++q)
-          if (q < cnt) vx[q] = xp[base + t + q];
+        for (uint32_t q = 0; q < FL_BATCH; ++q) vx[q] = xp[base + t + (q < cnt ? q : cnt - 1)];
 #pragma unroll
         for (uint32_t q = 0; q < FL_BATCH; ++q) {
           if (q < cnt) {
@@ -1107,13 +1108,11 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
         double vx[FL_BATCH], vu[FL_BATCH], vt[FL_BATCH], vd[FL_BATCH];
 #pragma unroll
         for (uint32_t q = 0; q < FL_BATCH; ++q) {
-          if (q < cnt) {
-            uint32_t P = base + t + q;
-            vx[q] = xp[P];
-            vu[q] = up[P];
-            vt[q] = tanp[P];
-            vd[q] = dp[P];
-          }
+          const uint32_t P = base + t + (q < cnt ? q : cnt - 1);  // every slot loads (see above)
+          vx[q] = xp[P];
+          vu[q] = up[P];
+          vt[q] = tanp[P];
+          vd[q] = dp[P];
         }
 #pragma unroll
         for (uint32_t q = 0; q < FL_BATCH; ++q) {
