@@ -124,7 +124,9 @@ namespace {
 template <typename T>
 int32_t dev_alloc(fl_terrain* T_, T** p, uint64_t count) {
   size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc((void**)p, bytes);
+  // stream-ordered on the terrain's context: no device-wide synchronisation
+  // when terrains come and go while others run (serving, batch compaction)
+  hipError_t e = hipMallocAsync((void**)p, bytes, T_->ctx->stream);
   if (e != hipSuccess)
     return set_err(e == hipErrorOutOfMemory ? FL_ERR_OUT_OF_MEMORY : FL_ERR_HIP, "hipMalloc(%zu): %s", bytes,
                    hipGetErrorString(e));
@@ -675,6 +677,12 @@ extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   c->device = device;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // the device pool keeps freed memory for the next terrain instead of
+  // returning it at every synchronisation
+  hipMemPool_t pool;
+  HIPCHK(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t keep = UINT64_MAX;
+  HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
   HIPCHK(hipHostMalloc((void**)&c->host_ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
   for (int q = 0; q < 2; q++) {
     HIPCHK(hipHostMalloc((void**)&c->host_slot[q], FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
@@ -703,7 +711,14 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   if (T->ctx) hipSetDevice(T->ctx->device);
   for (hipGraphExec_t& e : T->gexec)
     if (e) hipGraphExecDestroy(e);
-  for (void* p : T->allocations) hipFree(p);
+  // every call that used the terrain has returned (they end synchronised),
+  // so the frees are ordered after its last work on any stream
+  if (T->ctx) {
+    for (void* p : T->allocations) hipFreeAsync(p, T->ctx->stream);
+    hipStreamSynchronize(T->ctx->stream);
+  } else {
+    for (void* p : T->allocations) hipFree(p);
+  }
   if (T->h_tchanged) hipHostFree(T->h_tchanged);
   for (uint32_t* q : T->h_tslot)
     if (q) hipHostFree(q);

@@ -165,15 +165,19 @@ __global__ void k_iota(uint32_t m, uint32_t* __restrict__ out) {
 
 inline dim3 grid_for(uint64_t n) { return dim3((uint32_t)((n + 255) / 256)); }
 
+// Scratch of one forest build, stream-ordered on the build's stream: no
+// device-wide synchronisation, and the memory returns to the device's pool
+// (which the contexts keep) for the next build.
 struct DevBuf {
+  hipStream_t s;
   std::vector<void*> ptrs;
   hipError_t alloc(void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes < 8 ? 8 : bytes);
+    hipError_t e = hipMallocAsync(p, bytes < 8 ? 8 : bytes, s);
     if (e == hipSuccess) ptrs.push_back(*p);
     return e;
   }
   ~DevBuf() {
-    for (void* p : ptrs) (void)hipFree(p);
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
   }
 };
 
@@ -200,7 +204,7 @@ int fl_device_msf(hipStream_t s, uint32_t n, uint32_t nnz, const uint32_t* rp, c
                   const std::vector<uint32_t>& init_comp, std::vector<uint32_t>& a_sorted,
                   std::vector<uint32_t>& b_sorted, bool* applies, std::string* err) {
   *applies = false;
-  DevBuf buf;
+  DevBuf buf{s, {}};
   uint32_t* d_flag = nullptr;
   HIPTRY(buf.alloc((void**)&d_flag, 4 * sizeof(uint32_t)));
   HIPTRY(hipMemsetAsync(d_flag, 0, 4 * sizeof(uint32_t), s));

@@ -156,6 +156,16 @@ hipError_t hipGetDeviceCount(int* count);
 hipError_t hipSetDevice(int device);
 hipError_t hipMalloc(void** p, size_t bytes);
 hipError_t hipFree(void* p);
+// stream-ordered allocation: the emulation's streams run in order on the host
+inline hipError_t hipMallocAsync(void** p, size_t bytes, hipStream_t) { return hipMalloc(p, bytes); }
+inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+typedef void* hipMemPool_t;
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4 };
+inline hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int) {
+  *pool = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);
 hipError_t hipHostFree(void* p);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s = nullptr);
