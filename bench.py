@@ -63,6 +63,10 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
         # x 8, writes tau 8 (+ U, tan, d read and h written)
         "k_tau_fold<false>": 16,
         "k_tau_fold<true>": 48,
+        # the deep rounds of both sweeps, one launch each: as their round kernels
+        "k_da_tail": 128 if clamp else 72,
+        "k_tau_tail<false>": 16,
+        "k_tau_tail<true>": 48,
         # perm 4 + tau 8 + h 8 (+ jn 4 + U 8, or the chain-major h 8), writes tau 8 + h 8
         "k_tau_scatter": 44 if clamp else 48,
     }
@@ -165,7 +169,7 @@ def main() -> int:
     t.solve()
     names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count", "k_bucket_scan",
              "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
-             "k_tau_fold<true>", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32",
+             "k_tau_fold<true>", "k_da_tail", "k_tau_tail<false>", "k_tau_tail<true>", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32",
              "k_fill_u64", "k_copy_u32"]
     prof = {}
     for nm in names:

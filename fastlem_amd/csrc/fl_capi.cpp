@@ -85,13 +85,14 @@ struct fl_terrain {
   uint32_t jump_rounds = 4;  // pointer-jumping rounds of the previous iteration
   uint32_t last_rounds = 0;  // sweep rounds of the previous iteration
   std::vector<uint32_t> last_boff;  // its bucket offsets (boff[nb+1], bnoff[nb+1]): launch-size estimates
+  uint32_t tail_split = 0;          // first round of the pipelined iterations' tail launches
   // captured pipelined iterations, one per elevation-buffer parity, for the
   // (jump rounds, sweep rounds) they were captured with
   // elevation buffers: an iteration reads hb[i] and writes hb[(i+1) % 3], so
   // an iteration in flight never overwrites the input of the one before it
   double* hb[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
+  uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
   uint32_t* h_tslot[2] = {nullptr, nullptr};  // pinned per-terrain changed flags of iterations in flight
   uint8_t* h_mask = nullptr;                  // pinned staging of the active mask
   bool graphs_failed = false;
@@ -459,6 +460,21 @@ fl_round_grid exact_grid(const std::vector<uint32_t>& off, uint32_t l) {
   return g;
 }
 
+// First round of the tail launches: the rounds from it on each hold at most
+// FL_TAIL_WAVES fold units (one pass of the tail's wavefronts); `rounds`
+// (no tail) when fewer than two rounds qualify, as one round gains nothing.
+uint32_t tail_start(const std::vector<uint32_t>& off, uint32_t rounds) {
+  uint32_t l = rounds;
+  while (l > 0 && exact_grid(off, l - 1).fold_blocks <= FL_TAIL_WAVES) l--;
+  return rounds - l >= 2 ? l : rounds;
+}
+
+uint64_t tail_positions(const std::vector<uint32_t>& off, uint32_t from, uint32_t rounds) {
+  uint64_t units = 0;
+  for (uint32_t l = from; l < rounds; l++) units += exact_grid(off, l).positions;
+  return units;
+}
+
 // One outer iteration (generator.rs:141-209), synchronous form: the host
 // confirms pointer-jumping convergence and reads the bucket offsets before
 // the sweeps.  The decomposition is built speculatively before the lake test
@@ -481,8 +497,13 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   uint32_t npos = off[2 * nb + 1];
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
-  for (uint32_t l = rounds; l-- > 0;) fl_launch_da_round(s, T->t, w, l, exact_grid(off, l), clamp);
-  for (uint32_t l = 0; l < rounds; l++) fl_launch_tau_round(s, T->t, w, l, exact_grid(off, l), clamp);
+  // the deep rounds, which hold few chains, go to one launch per sweep
+  uint32_t split = tail_start(off, rounds);
+  uint64_t tail_units = tail_positions(off, split, rounds);
+  if (split < rounds) fl_launch_da_tail(s, T->t, w, split, rounds - 1, tail_units, clamp);
+  for (uint32_t l = split; l-- > 0;) fl_launch_da_round(s, T->t, w, l, exact_grid(off, l), clamp);
+  for (uint32_t l = 0; l < split; l++) fl_launch_tau_round(s, T->t, w, l, exact_grid(off, l), clamp);
+  if (split < rounds) fl_launch_tau_tail(s, T->t, w, split, rounds - 1, tail_units, clamp);
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
   fl_launch_tau_scatter(s, T->t, w, npos, clamp, T->d_tid, T->d_active, T->d_tchanged);
   if (T->d_tchanged)
@@ -493,11 +514,12 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   return FL_OK;
 }
 
-// The kernels of one pipelined iteration (J jump rounds, R sweep rounds),
-// enqueued on the context's stream or recorded by a capture.  rec_* receive
-// the profiler records of the rounds (when profiling).
-void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::vector<size_t>* rec_da,
-                       std::vector<size_t>* rec_tau, size_t* rec_scatter) {
+// The kernels of one pipelined iteration (J jump rounds, R sweep rounds,
+// rounds S..R-1 in the tail launches), enqueued on the context's stream or
+// recorded by a capture.  rec_* receive the profiler records of the launches
+// (when profiling).
+void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, std::vector<size_t>* rec_da,
+                       std::vector<size_t>* rec_tau, size_t* rec_tail, size_t* rec_scatter) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
@@ -520,13 +542,21 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::ve
     g.positions = 0;
     return g;
   };
-  for (uint32_t l = R; l-- > 0;) {
+  if (S < R) {
+    if (c->prof && rec_tail) rec_tail[0] = c->prof->pending.size();
+    fl_launch_da_tail(s, T->t, w, S, R - 1, 0, clamp);
+  }
+  for (uint32_t l = S; l-- > 0;) {
     if (c->prof && rec_da) (*rec_da)[l] = c->prof->pending.size();
     fl_launch_da_round(s, T->t, w, l, guess(l), clamp);
   }
-  for (uint32_t l = 0; l < R; l++) {
+  for (uint32_t l = 0; l < S; l++) {
     if (c->prof && rec_tau) (*rec_tau)[l] = c->prof->pending.size();
     fl_launch_tau_round(s, T->t, w, l, guess(l), clamp);
+  }
+  if (S < R) {
+    if (c->prof && rec_tail) rec_tail[1] = c->prof->pending.size();
+    fl_launch_tau_tail(s, T->t, w, S, R - 1, 0, clamp);
   }
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
   if (c->prof && rec_scatter) *rec_scatter = c->prof->pending.size();
@@ -536,10 +566,10 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, std::ve
 // The pipelined iteration for (J, R) and the current buffer parity as a
 // graph, captured on first use.  False when capture is not possible (the
 // caller launches directly).
-bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
+bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S) {
   if (T->graphs_failed) return false;
   int p = T->w.h == T->hb[0] ? 0 : (T->w.h == T->hb[1] ? 1 : 2);
-  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R) {
+  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S) {
     if (T->gexec[p]) {
       hipGraphExecDestroy(T->gexec[p]);
       T->gexec[p] = nullptr;
@@ -549,7 +579,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
       T->graphs_failed = true;
       return false;
     }
-    enqueue_fast_body(c, T, J, R, nullptr, nullptr, nullptr);
+    enqueue_fast_body(c, T, J, R, S, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&T->gexec[p], graph, nullptr, nullptr, 0);
     if (graph) hipGraphDestroy(graph);
@@ -561,6 +591,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
     }
     T->gJ[p] = J;
     T->gR[p] = R;
+    T->gS[p] = S;
   }
   return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
 }
@@ -569,9 +600,10 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R) {
 struct Flight {
   int in = 0;
   int slot = 0;
-  uint32_t J = 0, R = 0;
+  uint32_t J = 0, R = 0, S = 0;
   std::vector<uint8_t> mask;  // terrains iterating in it
   std::vector<size_t> rec_da, rec_tau;
+  size_t rec_tail[2] = {SIZE_MAX, SIZE_MAX};
   size_t rec_scatter = SIZE_MAX;
 };
 
@@ -589,12 +621,23 @@ int32_t enqueue_fast(fl_ctx* c, fl_terrain* T, Flight& f) {
   f.J = std::min<uint32_t>(T->jump_rounds + 1, FL_MAX_JUMP_ROUNDS);
   f.R = std::max<uint32_t>(T->last_rounds, 1) + 1;
   if (c->pipelined == 2) f.J = f.R = 1;
+  // tail launches for the rounds that were small in the last confirmed
+  // iteration; a split within one round of the previous one is kept (the
+  // captured graphs stay valid)
+  f.S = f.R;
+  if (!T->last_boff.empty()) {
+    uint32_t want = tail_start(T->last_boff, f.R);
+    uint32_t keep = T->tail_split;
+    f.S = (want < f.R && keep < f.R && want + 1 >= keep && want <= keep + 1) ? keep : want;
+  }
+  T->tail_split = f.S;
   w.h = T->hb[f.in];
   w.h2 = T->hb[(f.in + 1) % 3];
   f.rec_da.assign(f.R, SIZE_MAX);
   f.rec_tau.assign(f.R, SIZE_MAX);
-  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R);
-  if (!graphed) enqueue_fast_body(c, T, f.J, f.R, &f.rec_da, &f.rec_tau, &f.rec_scatter);
+  f.rec_tail[0] = f.rec_tail[1] = SIZE_MAX;
+  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R, f.S);
+  if (!graphed) enqueue_fast_body(c, T, f.J, f.R, f.S, &f.rec_da, &f.rec_tau, f.rec_tail, &f.rec_scatter);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tslot[f.slot], T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->host_slot[f.slot], w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -641,6 +684,9 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
         for (size_t q : {f.rec_da[l], f.rec_tau[l]})
           if (q < c->prof->pending.size()) c->prof->pending[q].units = units;
       }
+      uint64_t tail_units = tail_positions(T->last_boff, f.S, f.R);
+      for (size_t q : f.rec_tail)
+        if (q < c->prof->pending.size()) c->prof->pending[q].units = tail_units;
       if (f.rec_scatter < c->prof->pending.size()) c->prof->pending[f.rec_scatter].units = T->last_boff[2 * nb + 1];
     }
   } else {

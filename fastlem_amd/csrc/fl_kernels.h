@@ -35,7 +35,7 @@ extern thread_local fl_profiler* fl_current_profiler;
 // Control block read back once per phase: [flags | counters | boff head | bnoff head].
 enum { FL_FLAG_LAKE = 0, FL_FLAG_JUMP = 1 /* and 2: alternating */, FL_FLAG_CHANGED = 3, FL_NUM_FLAGS = 4 };
 enum { FL_CNT_MAXLD = 0, FL_CNT_CHAINS = 1, FL_CNT_POSITIONS = 2, FL_NUM_COUNTERS = 4 };
-enum { FL_CTRL_HEAD = 128 };  // bucket offsets mirrored into the control block
+enum { FL_CTRL_HEAD = 512 };  // bucket offsets mirrored into the control block
 enum { FL_MAX_JUMP_ROUNDS = 96 };  // one "pointers moved" word per doubling round
 enum { FL_CTRL_JUMPS = FL_NUM_FLAGS + FL_NUM_COUNTERS + 2 * (FL_CTRL_HEAD + 1) };
 enum { FL_CTRL_WORDS = FL_CTRL_JUMPS + FL_MAX_JUMP_ROUNDS };
@@ -129,6 +129,15 @@ struct fl_round_grid {
 // of its positions (and, with clamp, the max-slope recurrence's inputs).
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
                         const fl_round_grid& g, bool clamp);
+// Rounds lsplit..min(deepest light depth, cap) of the drainage-area sweep
+// (deepest first), then of the response-time sweep (shallowest first), in
+// one launch of one workgroup of FL_TAIL_WAVES wavefronts each: for the
+// deep rounds, which hold few chains.
+enum { FL_TAIL_WAVES = 16 };
+void fl_launch_da_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
+                       uint64_t units, bool clamp);
+void fl_launch_tau_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
+                        uint64_t units, bool clamp);
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
                          const fl_round_grid& g, bool clamp);

@@ -729,23 +729,32 @@ __device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint
 // chains [c0, c1) take one lane each, FL_DA_BLOCK positions at a time.  As
 // each site's A is final the response-time inputs of its position are
 // written (tau_inputs).  Bounds from the bucket offsets of round l.
-__global__ void __launch_bounds__(64)
-k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff,
-          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
-          const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
-          fl_tau_in ti) {
-  // the round's bounds are read together with the guard (one load latency;
-  // the bounds enter the test so that their loads stay above it: offsets are
-  // ascending, c2 < c0 never holds once the guard is clear)
-  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
-  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
-  __shared__ double s_p[64], s_a[64], s_b[64], s_c[64], s_d[64], s_r[64];
-  __shared__ uint32_t s_n[64], s_k[64];
+// One wavefront's LDS tile of a drainage-area round.
+struct fl_da_lds {
+  double p[64], a[64], b[64], c[64], d[64], r[64];
+  uint32_t n[64], k[64];
+};
+
+// Synchronisation of one wavefront's lanes around its own LDS tile: a
+// wavefront-scope fence (orders the LDS accesses; the wavefront's LDS
+// operations execute in order) and a wave barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront `wave` of `nwaves` in drainage-area round l (short chains
+// [c0, c1), long chains [c1, c2)).
+__device__ __forceinline__ void
+da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, uint32_t lane, fl_da_lds& L,
+         const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+         const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
+         const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+         const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
+         const fl_tau_in& ti) {
   const uint4* __restrict__ lists = reinterpret_cast<const uint4*>(plist);
-  const uint32_t lane = threadIdx.x;
   // long chains: one per block (uniform per block).  Tiles of 64 positions
   // from the tail, software-pipelined: the gathers of the next tile and the
   // list loads of the one after are issued before lane 0 folds the current
@@ -754,7 +763,7 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
   // No tile input depends on this round's results (light children head
   // chains of deeper rounds; a listed slot past the list reads the site's
   // own A and is dropped).
-  for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
+  for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
     const uint32_t base = chain_base[c], len = chain_len[c];
     struct Lists {
       uint32_t k, cn;
@@ -797,13 +806,13 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
         else
           da_combine(cur.cn, cur.area, cur.g0, cur.g1, cur.g2, cur.g3, p, a, b, x, y, np);
       }
-      s_p[lane] = p;
-      s_a[lane] = a;
-      s_b[lane] = b;
-      s_c[lane] = x;
-      s_d[lane] = y;
-      s_n[lane] = np;
-      s_k[lane] = mine ? cur.k : FL_NONE;
+      L.p[lane] = p;
+      L.a[lane] = a;
+      L.b[lane] = b;
+      L.c[lane] = x;
+      L.d[lane] = y;
+      L.n[lane] = np;
+      L.k[lane] = mine ? cur.k : FL_NONE;
       const unsigned long long rare = __ballot(np >= 3);
       // a position with more posts than staged reads the graph in the fold
       // (rare): such tiles take the fold with loads, the others one without,
@@ -811,7 +820,7 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
       const bool deep = __any(np > FL_POSTS);
       const uint32_t kk = cur.k;
       const fl_tau_site ts = cur.ts;
-      __syncthreads();
+      wave_sync();
       const uint32_t end2 = end1 > 64 ? end1 - 64 : 0;
       if (end1 > 0) cur = gather(nxt);
       if (end2 > 0) nxt = lists_of(end2);
@@ -822,27 +831,27 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
           double tp[8], ta[8], tb[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
-            tp[u] = s_p[e - u];
-            ta[u] = s_a[e - u];
-            tb[u] = s_b[e - u];
+            tp[u] = L.p[e - u];
+            ta[u] = L.a[e - u];
+            tb[u] = L.b[e - u];
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             r = ((tp[u] + r) + ta[u]) + tb[u];
             if ((rare >> (e - u)) & 1ull) {
-              r = r + s_c[e - u];
-              if (s_n[e - u] >= 4) r = r + s_d[e - u];
+              r = r + L.c[e - u];
+              if (L.n[e - u] >= 4) r = r + L.d[e - u];
             }
-            s_r[e - u] = r;
+            L.r[e - u] = r;
           }
         }
         for (; e >= 0; --e) {
-          r = ((s_p[e] + r) + s_a[e]) + s_b[e];
+          r = ((L.p[e] + r) + L.a[e]) + L.b[e];
           if ((rare >> e) & 1ull) {
-            r = r + s_c[e];
-            if (s_n[e] >= 4) r = r + s_d[e];
+            r = r + L.c[e];
+            if (L.n[e] >= 4) r = r + L.d[e];
           }
-          s_r[e] = r;
+          L.r[e] = r;
         }
       }
       if (lane == 0 && deep) {
@@ -851,36 +860,36 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
           double tp[8], ta[8], tb[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
-            tp[u] = s_p[e - u];
-            ta[u] = s_a[e - u];
-            tb[u] = s_b[e - u];
+            tp[u] = L.p[e - u];
+            ta[u] = L.a[e - u];
+            tb[u] = L.b[e - u];
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             r = ((tp[u] + r) + ta[u]) + tb[u];
             if ((rare >> (e - u)) & 1ull)
-              r = da_tail_terms(r, s_n[e - u], s_c[e - u], s_d[e - u], s_k[e - u], rp, col, next, cmask, hc, A);
-            s_r[e - u] = r;
+              r = da_tail_terms(r, L.n[e - u], L.c[e - u], L.d[e - u], L.k[e - u], rp, col, next, cmask, hc, A);
+            L.r[e - u] = r;
           }
         }
         for (; e >= 0; --e) {
-          r = ((s_p[e] + r) + s_a[e]) + s_b[e];
-          if ((rare >> e) & 1ull) r = da_tail_terms(r, s_n[e], s_c[e], s_d[e], s_k[e], rp, col, next, cmask, hc, A);
-          s_r[e] = r;
+          r = ((L.p[e] + r) + L.a[e]) + L.b[e];
+          if ((rare >> e) & 1ull) r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], rp, col, next, cmask, hc, A);
+          L.r[e] = r;
         }
       }
-      __syncthreads();
+      wave_sync();
       if (mine) {
-        A[kk] = s_r[lane];
-        tau_inputs(ti, base + cs + lane, kk, s_r[lane], ts);
+        A[kk] = L.r[lane];
+        tau_inputs(ti, base + cs + lane, kk, L.r[lane], ts);
       }
-      __syncthreads();  // the tile arrays are rewritten next
+      wave_sync();  // the tile arrays are rewritten next
       end = end1;
       end1 = end2;
     }
   }
   // short chains: one lane each, the last blocks first (they hold no long chain)
-  for (uint32_t c = c0 + (gridDim.x - 1 - blockIdx.x) * 64u + lane; c < c1; c += gridDim.x * 64u) {
+  for (uint32_t c = c0 + (nwaves - 1 - wave) * 64u + lane; c < c1; c += nwaves * 64u) {
     const uint32_t base = chain_base[c], len = chain_len[c];
     double r = FL_NEG0;
     uint32_t end = len;
@@ -918,6 +927,49 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
   }
 }
 
+__global__ void __launch_bounds__(64)
+k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff,
+          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
+          const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
+          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
+          fl_tau_in ti) {
+  // the round's bounds are read together with the guard (one load latency;
+  // the bounds enter the test so that their loads stay above it: offsets are
+  // ascending, c2 < c0 never holds once the guard is clear)
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
+  __shared__ fl_da_lds L;
+  da_round(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, chain_base, chain_len, perm, pcnt, plist, parea, rp,
+           col, next, cmask, hc, area, A, ti);
+}
+
+// Drainage-area rounds maxld, maxld - 1, ..., lsplit (the small deep ones) in
+// one workgroup of FL_TAIL_WAVES wavefronts, a workgroup barrier between rounds (a
+// round's results are read by the next one, within the workgroup): one
+// launch instead of one per round.  Rounds above `cap` are left out.
+__global__ void __launch_bounds__(64 * FL_TAIL_WAVES)
+k_da_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, const uint32_t* __restrict__ maxld,
+          const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
+          const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
+          const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist, const double* __restrict__ parea,
+          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
+          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc,
+          const double* __restrict__ area, double* __restrict__ A, fl_tau_in ti) {
+  uint32_t top = *maxld;
+  if (*guard != 0 || top > 0xfffffffeu) return;  // pointer jumping has not converged: the iteration is redone
+  top = top < cap ? top : cap;
+  __shared__ fl_da_lds L[FL_TAIL_WAVES];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
+  for (uint32_t l = top + 1; l-- > lsplit;) {
+    const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+    da_round(c0, c1, c2, wave, nwaves, lane, L[wave], chain_base, chain_len, perm, pcnt, plist, parea, rp, col,
+             next, cmask, hc, area, A, ti);
+    __syncthreads();
+  }
+}
+
 // Response time along the chains of one light depth, head to tail:
 // rt_i = 0.0 + (rt_next + x_i) (generator.rs:173 with rt_i initially 0),
 // the basin outlet first (its receiver is itself).  A head's receiver lies
@@ -927,22 +979,22 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
 // (blocks [0, c2 - c1)): tiles of 64 positions are staged in LDS, lane 0
 // runs the recurrence, the wavefront stores; short chains [c0, c1) one lane
 // each (the remaining blocks).
+// One wavefront's LDS tile of a response-time round.
+struct fl_tau_lds {
+  double x[64], u[64], t[64], d[64], tau[64], h[64];
+};
+
+// Wavefront `wave` of `nwaves` in response-time round l (short chains
+// [c0, c1), long chains [c1, c2)).
 template <bool clamp>
-__global__ void __launch_bounds__(64)
-k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
-           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
-           const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
-           const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
-           const double* __restrict__ hop, double* __restrict__ hnp) {
-  // the round's bounds are read together with the guard (one load latency;
-  // the bounds enter the test so that their loads stay above it: offsets are
-  // ascending, c2 < c0 never holds once the guard is clear)
-  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
-  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
-  __shared__ double s_x[64], s_u[64], s_t[64], s_d[64], s_tau[64], s_h[64];
-  const uint32_t lane = threadIdx.x;
+__device__ __forceinline__ void
+tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, uint32_t lane, fl_tau_lds& L,
+          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+          const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+          double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
+          const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp) {
   // long chains: one per block (uniform per block)
-  for (uint32_t c = c1 + blockIdx.x; c < c2; c += gridDim.x) {
+  for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
     const uint32_t base = chain_base[c], len = chain_len[c];
     // lane 0's recurrence state
     double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
@@ -967,13 +1019,13 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
     }
     for (uint32_t t = 0; t < len; t += 64) {
       const uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
-      s_x[lane] = x;
+      L.x[lane] = x;
       if (clamp) {
-        s_u[lane] = u;
-        s_t[lane] = tn;
-        s_d[lane] = d;
+        L.u[lane] = u;
+        L.t[lane] = tn;
+        L.d[lane] = d;
       }
-      __syncthreads();
+      wave_sync();
       x = u = tn = d = 0.0;
       if (t + 64 + lane < len) {
         uint32_t P = base + t + 64 + lane;
@@ -988,14 +1040,14 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
         uint32_t j = 0;
         if (t == 0 && pp == FL_NONE) {
           // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
-          tau = 0.0 + (0.0 + s_x[0]);
-          s_tau[0] = tau;
+          tau = 0.0 + (0.0 + L.x[0]);
+          L.tau[0] = tau;
           if (clamp) {
             double ho = hop[base];
-            double nh = ho + s_u[0] * rust_max(tau - tau, 0.0);
-            double slope = (nh - ho) / s_d[0];
-            if (slope > s_t[0]) nh = ho + s_t[0] * s_d[0];
-            s_h[0] = nh;
+            double nh = ho + L.u[0] * rust_max(tau - tau, 0.0);
+            double slope = (nh - ho) / L.d[0];
+            if (slope > L.t[0]) nh = ho + L.t[0] * L.d[0];
+            L.h[0] = nh;
             h_p = nh;
             h_o = nh;
             tau_o = tau;
@@ -1008,12 +1060,12 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
             double tx[8];
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q)
-              if (q < m) tx[q] = s_x[j + q];
+              if (q < m) tx[q] = L.x[j + q];
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q) {
               if (q < m) {
                 tau = 0.0 + (tau + tx[q]);
-                s_tau[j + q] = tau;
+                L.tau[j + q] = tau;
               }
             }
             j += m;
@@ -1025,10 +1077,10 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q) {
               if (q < m) {
-                tx[q] = s_x[j + q];
-                tu[q] = s_u[j + q];
-                tt[q] = s_t[j + q];
-                td[q] = s_d[j + q];
+                tx[q] = L.x[j + q];
+                tu[q] = L.u[j + q];
+                tt[q] = L.t[j + q];
+                td[q] = L.d[j + q];
               }
             }
 #pragma unroll
@@ -1038,8 +1090,8 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
                 double nh = h_o + tu[q] * rust_max(tau - tau_o, 0.0);
                 double slope = (nh - h_p) / td[q];
                 if (slope > tt[q]) nh = h_p + tt[q] * td[q];
-                s_tau[j + q] = tau;
-                s_h[j + q] = nh;
+                L.tau[j + q] = tau;
+                L.h[j + q] = nh;
                 h_p = nh;
               }
             }
@@ -1047,16 +1099,16 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
           }
         }
       }
-      __syncthreads();
+      wave_sync();
       if (lane < cnt) {
-        taup[base + t + lane] = s_tau[lane];
-        if (clamp) hnp[base + t + lane] = s_h[lane];
+        taup[base + t + lane] = L.tau[lane];
+        if (clamp) hnp[base + t + lane] = L.h[lane];
       }
     }
-    __syncthreads();  // the tile arrays are reused by the next chain
+    wave_sync();  // the tile arrays are reused by the next chain
   }
   // short chains: one lane each, the last blocks first (they hold no long chain)
-  for (uint32_t c = c0 + (gridDim.x - 1 - blockIdx.x) * 64u + lane; c < c1; c += gridDim.x * 64u) {
+  for (uint32_t c = c0 + (nwaves - 1 - wave) * 64u + lane; c < c1; c += nwaves * 64u) {
     uint32_t base = chain_base[c], len = chain_len[c];
     uint32_t pp = ppos[base];
     double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
@@ -1129,6 +1181,45 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
         t += cnt;
       }
     }
+  }
+}
+
+template <bool clamp>
+__global__ void __launch_bounds__(64)
+k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
+           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
+           const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
+           const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
+           const double* __restrict__ hop, double* __restrict__ hnp) {
+  // the round's bounds are read together with the guard (one load latency;
+  // the bounds enter the test so that their loads stay above it: offsets are
+  // ascending, c2 < c0 never holds once the guard is clear)
+  const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+  if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
+  __shared__ fl_tau_lds L;
+  tau_round<clamp>(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, chain_base, chain_len, ppos, rpos, xp, taup, up, tanp, dp, hop, hnp);
+}
+
+// Response-time rounds lsplit, lsplit + 1, ..., maxld (the small deep ones)
+// in one workgroup of FL_TAIL_WAVES wavefronts, a workgroup barrier between rounds (a
+// round reads the response times and elevations of the previous ones): one
+// launch instead of one per round.  Rounds above `cap` are left out.
+template <bool clamp>
+__global__ void __launch_bounds__(64 * FL_TAIL_WAVES)
+k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, const uint32_t* __restrict__ maxld,
+           const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
+          const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+          double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
+          const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp) {
+  uint32_t top = *maxld;
+  if (*guard != 0 || top > 0xfffffffeu) return;  // pointer jumping has not converged: the iteration is redone
+  top = top < cap ? top : cap;
+  __shared__ fl_tau_lds L[FL_TAIL_WAVES];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
+  for (uint32_t l = lsplit; l <= top; ++l) {
+    const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
+    tau_round<clamp>(c0, c1, c2, wave, nwaves, lane, L[wave], chain_base, chain_len, ppos, rpos, xp, taup, up, tanp, dp, hop, hnp);
+    __syncthreads();
   }
 }
 
@@ -1423,6 +1514,27 @@ void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
     FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
                    w.chain_len, w.perm, w.pcnt, w.plist, w.parea, t.row_ptr, t.col, w.next,
                    (const unsigned long long*)w.cmask, w.hc, t.area, w.A, ti);
+}
+
+void fl_launch_da_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
+                       uint64_t units, bool clamp) {
+  fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
+  FL_LAUNCH_GRID(k_da_tail, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap, w.counters + FL_CNT_MAXLD,
+                 w.boff, w.chain_base, w.chain_len, w.perm, w.pcnt, w.plist, w.parea, t.row_ptr, t.col, w.next,
+                 (const unsigned long long*)w.cmask, w.hc, t.area, w.A, ti);
+}
+
+void fl_launch_tau_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
+                        uint64_t units, bool clamp) {
+  (void)t;
+  if (clamp)
+    FL_LAUNCH_GRID(k_tau_tail<true>, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap,
+                   w.counters + FL_CNT_MAXLD, w.boff, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up,
+                   w.tanp, w.dp, w.hop, w.hnp);
+  else
+    FL_LAUNCH_GRID(k_tau_tail<false>, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap,
+                   w.counters + FL_CNT_MAXLD, w.boff, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up,
+                   w.tanp, w.dp, w.hop, w.hnp);
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,

@@ -93,6 +93,10 @@ int readlane(int v, int lane);
 #define __any(p) ::flemu::wave_any((p) ? 1 : 0)
 #define __ballot(p) ::flemu::wave_ballot((p) ? 1 : 0)
 #define __builtin_amdgcn_readlane(v, l) ::flemu::readlane((int)(v), (int)(l))
+// a wave barrier waits for every live lane of the wavefront; fences order
+// nothing further here (one fiber runs at a time)
+#define __builtin_amdgcn_wave_barrier() ((void)::flemu::wave_any(0))
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
   ::flemu::launch(dim3(grid), dim3(block), [=]() { kernel(__VA_ARGS__); })
 

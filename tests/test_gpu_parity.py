@@ -429,7 +429,8 @@ def test_profiled_solve_is_exact_and_timed(fbm20k):
         it = t.solve()
         redone = t.last_stats()["redone"]
         prof = {}
-        for nm in ("k_receivers", "k_children", "k_chain_perm", "k_da_fold", "k_tau_fold<false>", "k_tau_scatter"):
+        for nm in ("k_receivers", "k_children", "k_chain_perm", "k_da_fold", "k_tau_fold<false>", "k_da_tail",
+                   "k_tau_tail<false>", "k_tau_scatter"):
             cnt, ms, units = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
             _native.check(L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms),
                                                ctypes.byref(units)))
@@ -446,3 +447,5 @@ def test_profiled_solve_is_exact_and_timed(fbm20k):
     assert it <= prof["k_receivers"][0] <= it + redone + 3
     assert prof["k_receivers"][2] == prof["k_receivers"][0] * w.n
     assert prof["k_da_fold"][0] >= it
+    # the deep rounds of an iteration share one launch per sweep
+    assert prof["k_da_tail"][0] == prof["k_tau_tail<false>"][0] <= prof["k_receivers"][0]
