@@ -45,7 +45,18 @@
 
 namespace {
 
-__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Workgroups are dealt to the GPU's 8 XCDs in turn (workgroup b to XCD
+// b % 8), each XCD with an L2 of its own.  The per-site kernels hand each XCD
+// one contiguous eighth of the index range instead, so that sites close in
+// the breadth-first labels, whose neighbour gathers touch the same lines,
+// share an L2.  A bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t gid() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 // (pointer, count) pairs of the pointer jumping: pointer in the low word.
 __device__ __forceinline__ unsigned long long pair(uint32_t ptr, uint32_t cnt) {
@@ -309,7 +320,7 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
   }
   if (threadIdx.x == 0) s_max = 0;
   __syncthreads();
-  uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     if (v < n && hc[v] == FL_NONE) {
@@ -422,7 +433,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     s_n[t] = 0;
   }
   __syncthreads();
-  uint32_t base = blockIdx.x * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   uint32_t lc[FL_BUCKET_PER_THREAD], ln[FL_BUCKET_PER_THREAD];
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
@@ -1238,7 +1249,9 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, const uint32_t
   const uint32_t npos = *npos_ptr;  // positions of the iteration's chains
   // (npos enters the test so that its load stays above it)
   if (*guard != 0 || npos == 0xffffffffu) return;  // pointer jumping has not converged: the iteration is redone
-  uint32_t P = gid();
+  // chain positions in launch order: the scattered site writes fare better
+  // spread over the XCDs
+  uint32_t P = blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false;
   if (P < npos) {
     uint32_t k = perm[P];
