@@ -67,8 +67,9 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
         "k_da_tail": 128 if clamp else 72,
         "k_tau_tail<false>": 16,
         "k_tau_tail<true>": 48,
-        # perm 4 + tau 8 + h 8 (+ jn 4 + U 8, or the chain-major h 8), writes tau 8 + h 8
-        "k_tau_scatter": 44 if clamp else 48,
+        # per site: rt 8 + cp 8 + head_base 4 + tau 8 + h 8 + U 8 (or the chain-major h 8),
+        # writes tau 8 + h 8 (the roots' values are shared)
+        "k_tau_scatter": 60,
     }
     return float(table.get(name, 0.0))
 

@@ -494,7 +494,6 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   }
   uint32_t nb = T->prev_nb;
   const std::vector<uint32_t>& off = c->host_boff;
-  uint32_t npos = off[2 * nb + 1];
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
   // the deep rounds, which hold few chains, go to one launch per sweep
@@ -505,7 +504,7 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   for (uint32_t l = 0; l < split; l++) fl_launch_tau_round(s, T->t, w, l, exact_grid(off, l), clamp);
   if (split < rounds) fl_launch_tau_tail(s, T->t, w, split, rounds - 1, tail_units, clamp);
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
-  fl_launch_tau_scatter(s, T->t, w, npos, clamp, T->d_tid, T->d_active, T->d_tchanged);
+  fl_launch_tau_scatter(s, T->t, w, clamp, T->d_tid, T->d_active, T->d_tchanged);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tchanged, T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   FLCHK(read_ctrl(c, T));
@@ -560,7 +559,7 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
   }
   if (T->d_tchanged) fl_launch_fill_u32(s, T->d_tchanged, T->nterr, 0);
   if (c->prof && rec_scatter) *rec_scatter = c->prof->pending.size();
-  fl_launch_tau_scatter(s, T->t, w, 0, clamp, T->d_tid, T->d_active, T->d_tchanged);
+  fl_launch_tau_scatter(s, T->t, w, clamp, T->d_tid, T->d_active, T->d_tchanged);
 }
 
 // The pipelined iteration for (J, R) and the current buffer parity as a
@@ -687,7 +686,6 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
       uint64_t tail_units = tail_positions(T->last_boff, f.S, f.R);
       for (size_t q : f.rec_tail)
         if (q < c->prof->pending.size()) c->prof->pending[q].units = tail_units;
-      if (f.rec_scatter < c->prof->pending.size()) c->prof->pending[f.rec_scatter].units = T->last_boff[2 * nb + 1];
     }
   } else {
     T->last_boff.clear();

@@ -141,8 +141,8 @@ void fl_launch_tau_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb);
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
                          const fl_round_grid& g, bool clamp);
-// Back to site order; the number of positions is read from the control block.
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint64_t units, bool clamp,
+// Back to site order, one thread per site.
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged);
 void fl_launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t count, uint32_t v);
 

@@ -1237,41 +1237,43 @@ k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, co
 // Back to site order: response time, new elevation (computed here when no
 // site has a max slope: h_i' = h_o' + U_i * max(rt_i - rt_o, 0) with
 // h_o' = h_o + U_o * max(0, 0), generator.rs:178-180) and the changed flag
-// (generator.rs:201).  In a batch (tid != null) a terrain that stopped
-// iterating keeps its elevations and each terrain has its own flag.
-__global__ void k_tau_scatter(const uint32_t* __restrict__ guard, const uint32_t* __restrict__ npos_ptr, const uint32_t* __restrict__ perm,
-                              const unsigned long long* __restrict__ rt,
+// (generator.rs:201).  One thread per site, which finds its chain position
+// from its converged chain pair (head, position): the site-ordered accesses
+// are coalesced, the position-ordered ones are gathers.  Sites outside every
+// basin were settled by k_chain_perm.  In a batch (tid != null) a terrain
+// that stopped iterating keeps its elevations and each terrain has its own
+// flag.
+__global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
+                              const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                               const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ active, uint32_t* __restrict__ tchanged) {
-  const uint32_t npos = *npos_ptr;  // positions of the iteration's chains
-  // (npos enters the test so that its load stays above it)
-  if (*guard != 0 || npos == 0xffffffffu) return;  // pointer jumping has not converged: the iteration is redone
-  // chain positions in launch order: the scattered site writes fare better
-  // spread over the XCDs
-  uint32_t P = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = gid();
+  // the site's pairs are read together with the guard (one load latency)
+  const unsigned long long rv = v < n ? rt[v] : 0ull, cv = v < n ? cp[v] : 0ull;
+  if (*guard != 0) return;  // pointer jumping has not converged: the iteration is redone
   bool changed = false;
-  if (P < npos) {
-    uint32_t k = perm[P];
-    uint32_t g = tid ? tid[k] : 0;
+  const uint32_t o = pair_ptr(rv);
+  if (v < n && is_outlet[o]) {
+    const uint32_t g = tid ? tid[v] : 0;
     if (tid && !active[g]) {
-      h2[k] = h[k];
+      h2[v] = h[v];
     } else {
-      double tk = taup[P];
+      const uint32_t P = head_base[pair_ptr(cv)] + pair_cnt(cv);
+      const double tk = taup[P];
       double nh;
       if (clamp) {
         nh = hnp[P];
       } else {
-        uint32_t o = pair_ptr(rt[k]);
-        double to = taup[head_base[o]];
-        double ho = h[o] + U[o] * rust_max(to - to, 0.0);
-        nh = (k == o) ? ho : ho + U[k] * rust_max(tk - to, 0.0);
+        const double to = taup[head_base[o]];
+        const double ho = h[o] + U[o] * rust_max(to - to, 0.0);
+        nh = (v == o) ? ho : ho + U[v] * rust_max(tk - to, 0.0);
       }
-      tau[k] = tk;
-      h2[k] = nh;
-      changed = nh != h[k];
+      tau[v] = tk;
+      h2[v] = nh;
+      changed = nh != h[v];
       if (changed && tchanged) tchanged[g] = 1;
     }
   }
@@ -1562,10 +1564,11 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
                    w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
 }
 
-void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint64_t units, bool clamp,
+void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
-  FL_LAUNCH_U(k_tau_scatter, t.n, units, s, w.guard, w.counters + FL_CNT_POSITIONS, w.perm, (const unsigned long long*)w.rt,
-              w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, active, tchanged);
+  FL_LAUNCH(k_tau_scatter, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
+            t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, active,
+            tchanged);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {

@@ -22,7 +22,13 @@
 
 namespace {
 
-__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Workgroup b runs on XCD b % 8; each XCD takes one contiguous eighth of the
+// index range (a bijection of [0, gridDim.x)), so neighbouring sites share an L2.
+__device__ __forceinline__ uint32_t gid() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+  return (x * q + (x < r ? x : r) + (b >> 3)) * blockDim.x + threadIdx.x;
+}
 
 __device__ __forceinline__ unsigned long long len_bits(double w) {
   return (unsigned long long)__double_as_longlong(w);
