@@ -899,8 +899,17 @@ da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, 
       end1 = end2;
     }
   }
-  // short chains: one lane each, the last blocks first (they hold no long chain)
-  for (uint32_t c = c0 + (nwaves - 1 - wave) * 64u + lane; c < c1; c += nwaves * 64u) {
+  // short chains: one lane each, the last blocks first (they hold no long
+  // chain).  When one pass covers them, the 64-chain groups of the wavefronts
+  // that share an XCD (workgroup index mod 8) are contiguous: chains placed
+  // together hold nearby sites, whose site-ordered reads then share an L2.
+  const uint32_t groups = (c1 - c0 + 63u) >> 6;
+  uint32_t sw = nwaves - 1 - wave;
+  if (groups <= nwaves && sw < groups) {
+    const uint32_t q = groups >> 3, rq = groups & 7u, x = sw & 7u;
+    sw = x * q + (x < rq ? x : rq) + (sw >> 3);
+  }
+  for (uint32_t c = c0 + sw * 64u + lane; c < c1; c += nwaves * 64u) {
     const uint32_t base = chain_base[c], len = chain_len[c];
     double r = FL_NEG0;
     uint32_t end = len;
@@ -1118,8 +1127,16 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
     }
     wave_sync();  // the tile arrays are reused by the next chain
   }
-  // short chains: one lane each, the last blocks first (they hold no long chain)
-  for (uint32_t c = c0 + (nwaves - 1 - wave) * 64u + lane; c < c1; c += nwaves * 64u) {
+  // short chains: one lane each, the last blocks first (they hold no long
+  // chain); the groups of the wavefronts that share an XCD are contiguous
+  // when one pass covers them
+  const uint32_t groups = (c1 - c0 + 63u) >> 6;
+  uint32_t sw = nwaves - 1 - wave;
+  if (groups <= nwaves && sw < groups) {
+    const uint32_t q = groups >> 3, rq = groups & 7u, x = sw & 7u;
+    sw = x * q + (x < rq ? x : rq) + (sw >> 3);
+  }
+  for (uint32_t c = c0 + sw * 64u + lane; c < c1; c += nwaves * 64u) {
     uint32_t base = chain_base[c], len = chain_len[c];
     uint32_t pp = ppos[base];
     double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
