@@ -301,7 +301,7 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
 #define FL_LONG 16
 #define FL_LDS_BUCKETS 128
 #define FL_BUCKET_BLOCK 1024
-#define FL_BUCKET_PER_THREAD 4
+#define FL_BUCKET_PER_THREAD 2
 
 // Light-depth histogram of the chains (count and total length), aggregated
 // per workgroup in LDS before one global atomic per (workgroup, bucket).
