@@ -428,6 +428,14 @@ int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
   for (size_t step = 0; step < k; step++) {
     uint64_t L = len;
     uint32_t result = zid[0];
+    // the next pop is one of the root's children and the one after it one
+    // of their children (each sift-down moves one child up): fetch their
+    // adjacency two pops ahead (row offsets) and one pop ahead (entries)
+    __builtin_prefetch(col + row_ptr[result]);
+    for (uint64_t q = 1; q <= 2 && q < k; q++)
+      if (zid[q] != NONE) __builtin_prefetch(col + row_ptr[zid[q]]);
+    for (uint64_t q = 3; q <= 6 && q < k; q++)
+      if (zid[q] != NONE) __builtin_prefetch(row_ptr + zid[q]);
     uint64_t last = L - 1;
     uint32_t last_z = last < k ? zid[last] : NONE;
     if (last < k) zid[last] = NONE;
