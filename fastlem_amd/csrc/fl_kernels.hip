@@ -22,7 +22,12 @@
 //                                             site's 1/(K*A^0.5) * distance
 //     k_tau_fold        generator.rs:162-203  response time (+ clamped
 //                                             elevation), head to tail
+//     k_da_tail, k_tau_tail                   the deep rounds (few chains
+//                                             each) in one launch per sweep
 //     k_tau_scatter     generator.rs:177-202  elevation, changed flag
+//
+// Per-site kernels take their site ranges XCD by XCD (xcd_block): each of
+// the 8 XCDs works on one contiguous eighth of the sites.
 //
 // Exactness.  Every floating-point expression is evaluated in the order the
 // reference writes it; the library is built with -ffp-contract=off so no
