@@ -611,9 +611,10 @@ int krt_ranks_run(fl_krt_work* work, size_t groups, const uint32_t* outlet_off, 
   };
   const uint32_t cut = (uint32_t)std::max<size_t>(4096, total / (64 * (size_t)threads));
   std::vector<uint32_t> big;  // reconstruction nodes with more than `cut` leaves, ascending
-  // the union-find records of the edges a few steps ahead are fetched early:
-  // the loop is bound by their cache misses
-  constexpr size_t kAhead = 16;
+  // the union-find records of the edges a few steps ahead are fetched early,
+  // and half as far ahead their parents' records: the loop is bound by their
+  // cache misses
+  constexpr size_t kAhead = 32;
   for (size_t i = 0; i < m; i++) {
     uint32_t a = msf_a[i], b = msf_b[i];
     if (a >= n || b >= n) return FL_ERR_INVALID;
@@ -621,6 +622,11 @@ int krt_ranks_run(fl_krt_work* work, size_t groups, const uint32_t* outlet_off, 
       const uint32_t pa = msf_a[i + kAhead], pb = msf_b[i + kAhead];
       if (pa < n) __builtin_prefetch(&uf[pa]);
       if (pb < n) __builtin_prefetch(&uf[pb]);
+    }
+    if (i + kAhead / 2 < m) {
+      const uint32_t qa = msf_a[i + kAhead / 2], qb = msf_b[i + kAhead / 2];
+      if (qa < n) __builtin_prefetch(&uf[uf[qa].parent]);
+      if (qb < n) __builtin_prefetch(&uf[uf[qb].parent]);
     }
     uint32_t ra = find(a), rb = find(b);
     if (ra == rb) return FL_ERR_INVALID;  // not a forest
