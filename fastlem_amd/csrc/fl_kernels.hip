@@ -303,7 +303,7 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
 // Buckets: 2 * light depth + (chain length >= FL_LONG), so every round has a
 // range of short chains (one lane each) followed by long ones (one
 // wavefront each).
-#define FL_LONG 16
+#define FL_LONG 6
 #define FL_LDS_BUCKETS 128
 #define FL_BUCKET_BLOCK 1024
 #define FL_BUCKET_PER_THREAD 2
