@@ -103,6 +103,9 @@ struct fl_terrain {
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   uint64_t bytes = 0;
+  // where its device arrays come from (the device pool unless a caller
+  // supplies a region); region buffers are never freed one by one
+  fl_arena arena;
   // host copies (relabelled) needed for the one-time visit order
   std::vector<uint32_t> h_row_ptr, h_col, h_outlets;
   std::vector<double> h_len;
@@ -114,6 +117,7 @@ struct fl_terrain {
   uint32_t nterr = 1;
   std::vector<uint32_t> site_off, outlet_off, entry_off, t_max_it, t_iters;
   std::vector<uint8_t> t_active;
+  std::vector<uint8_t> t_asym;    // [nterr] some edge's two lengths differ (exact visit order)
   uint32_t* d_tid = nullptr;      // [n] terrain of each site (batches)
   uint8_t* d_active = nullptr;    // [nterr] terrains still iterating
   uint32_t* d_tchanged = nullptr; // [nterr] per-terrain changed flags
@@ -124,14 +128,19 @@ namespace {
 
 template <typename T>
 int32_t dev_alloc(fl_terrain* T_, T** p, uint64_t count) {
-  size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
-  // stream-ordered on the terrain's context: no device-wide synchronisation
-  // when terrains come and go while others run (serving, batch compaction)
-  hipError_t e = hipMallocAsync((void**)p, bytes, T_->ctx->stream);
-  if (e != hipSuccess)
-    return set_err(e == hipErrorOutOfMemory ? FL_ERR_OUT_OF_MEMORY : FL_ERR_HIP, "hipMalloc(%zu): %s", bytes,
-                   hipGetErrorString(e));
-  T_->allocations.push_back((void*)*p);
+  uint64_t bytes = fl_round256(std::max<uint64_t>(count, 1) * sizeof(T));
+  // pool allocations are stream-ordered on the terrain's context: no
+  // device-wide synchronisation when terrains come and go while others run
+  // (serving, batch compaction)
+  hipError_t e = fl_arena_alloc(&T_->arena, T_->ctx ? T_->ctx->stream : nullptr, (void**)p, bytes);
+  if (e != hipSuccess) {
+    if (T_->arena.kind == fl_arena::REGION)
+      return set_err(FL_ERR_TOO_LARGE, "device region of %llu bytes exhausted (%llu more needed)",
+                     (unsigned long long)T_->arena.size, (unsigned long long)bytes);
+    return set_err(e == hipErrorOutOfMemory ? FL_ERR_OUT_OF_MEMORY : FL_ERR_HIP, "hipMalloc(%llu): %s",
+                   (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  if (T_->arena.kind == fl_arena::POOL) T_->allocations.push_back((void*)*p);
   T_->bytes += bytes;
   return FL_OK;
 }
@@ -196,7 +205,12 @@ RowError first_row_error(uint64_t n, C check_row) {
   return best;
 }
 
-int32_t validate(const fl_terrain_desc* d) {
+// Checks a description; *asym is set when some edge's two entries carry
+// different length bits (legal input: the reference keys each heap push by
+// the popped site's own entry, stream_tree.rs:231-235, which the exact
+// traversal reproduces; the spanning-forest form needs one length per edge).
+int32_t validate(const fl_terrain_desc* d, bool* asym) {
+  *asym = false;
   if (!d) return set_err(FL_ERR_INVALID, "null terrain description");
   if (d->n >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "n=%llu exceeds 2^32-2 sites", (unsigned long long)d->n);
   if (d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "nnz exceeds 2^32-2 entries");
@@ -247,12 +261,22 @@ int32_t validate(const fl_terrain_desc* d) {
   if (e.kind == 1) return set_err(FL_ERR_INVALID, "col[%u]=%u out of range", e.k, e.j);
   if (e.kind == 2) return set_err(FL_ERR_INVALID, "self-loop at site %llu", (unsigned long long)e.row);
   if (e.kind == 3) return set_err(FL_ERR_INVALID, "duplicate edge %llu-%u", (unsigned long long)e.row, e.j);
-  // symmetry (terrain-graph is undirected)
+  // symmetry (terrain-graph is undirected), and whether both entries of
+  // every edge carry the same length bits
+  std::atomic<bool> len_differs{false};
   e = first_row_error(n, [&](uint64_t i) {
+    bool differs = false;
     for (uint32_t k = rp[i]; k < rp[i + 1]; k++) {
       const uint32_t j = col[k];
       bool found = false;
-      for (uint32_t q = rp[j]; q < rp[j + 1] && !found; q++) found = col[q] == i;
+      for (uint32_t q = rp[j]; q < rp[j + 1] && !found; q++) {
+        if (col[q] != i) continue;
+        found = true;
+        uint64_t a, b;
+        std::memcpy(&a, d->length + k, 8);
+        std::memcpy(&b, d->length + q, 8);
+        differs |= a != b;
+      }
       if (!found) {
         RowError r;
         r.row = i;
@@ -262,8 +286,10 @@ int32_t validate(const fl_terrain_desc* d) {
         return r;
       }
     }
+    if (differs) len_differs.store(true, std::memory_order_relaxed);
     return RowError{};
   });
+  *asym = len_differs.load();
   if (e.kind == 4) return set_err(FL_ERR_INVALID, "edge %llu-%u is not symmetric", (unsigned long long)e.row, e.j);
   std::vector<uint8_t> seen(n, 0);
   for (uint64_t k = 0; k < d->n_outlets; k++) {
@@ -288,88 +314,117 @@ inline uint32_t host_counter(const fl_ctx* c, int k) { return c->host_ctrl[FL_NU
 int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
   if (T->have_order) return FL_OK;
   auto t0 = std::chrono::steady_clock::now();
-  uint32_t n = T->t.n, G = T->nterr;
-  size_t k = T->h_outlets.size();
-  std::vector<uint32_t> rank(n), order(n, 0xffffffffu);
-  bool fast = false;
-  if (k > 0) {
-    // per terrain: outlet phase (exact heap tie handling, host thread) while
-    // the GPU builds one spanning forest of all terrains with each terrain's
-    // outlets merged into its own component (labelled by its first listed
-    // outlet; the ranks only need the component, not the label)
-    std::vector<uint32_t> first(k), a, b, init(n);
-    std::future<int> outlet_phase = std::async(std::launch::async, [&]() {
-      auto p0 = std::chrono::steady_clock::now();
-      for (uint32_t g = 0; g < G; g++) {
-        uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
-        int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
-                                           T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
-        if (rc != FL_OK) return rc;
-      }
-      T->stat_order_parts[0] =
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
-      return (int)FL_OK;
-    });
-    // one forest: the ranks' working arrays are set up (from the outlets
-    // alone) on host threads while the GPU builds the forest
-    int prep_rc = FL_OK;
-    std::future<fl_krt_work*> prep;
-    if (G == 1)
-      prep = std::async(std::launch::async, [&]() {
-        return fl_krt_prepare(n, G, T->outlet_off.data(), T->h_outlets.data(), fl_host_threads(), &prep_rc);
-      });
-    auto p1 = std::chrono::steady_clock::now();
-    for (uint32_t v = 0; v < n; v++) init[v] = v;
-    for (uint32_t g = 0; g < G; g++)
-      for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++)
-        init[T->h_outlets[q]] = T->h_outlets[T->outlet_off[g]];
-    std::string err;
-    int rc = fl_device_msf(c->stream, n, T->t.nnz, T->t.row_ptr, T->t.col, T->t.len, init, a, b, &fast, &err);
-    T->stat_order_parts[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p1).count();
-    int rc_ranks = FL_OK;
-    std::unique_ptr<fl_krt_work, void (*)(fl_krt_work*)> work(prep.valid() ? prep.get() : nullptr, fl_krt_release);
-    if (rc == FL_OK && fast) {
-      // the forest's ranks need the outlet set, not the pop order: they are
-      // computed while the outlet phase still runs
-      auto p2 = std::chrono::steady_clock::now();
-      if (G == 1)
-        rc_ranks = work ? fl_krt_ranks_prepared(work.get(), G, T->outlet_off.data(), T->h_outlets.data(),
-                                                T->site_off.data(), a.data(), b.data(), a.size(), rank.data(),
-                                                fl_host_threads())
-                        : prep_rc;
-      else
-        rc_ranks = fl_krt_ranks(n, G, T->outlet_off.data(), T->h_outlets.data(), T->site_off.data(),
-                                T->site_off.data(), a.data(), b.data(), a.size(), rank.data(), fl_host_threads());
-      T->stat_order_parts[2] =
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p2).count();
-    }
-    int rc_outlets = outlet_phase.get();
-    if (rc != FL_OK) return set_err(rc, "spanning forest: %s", err.c_str());
-    if (rc_outlets != FL_OK) return set_err(rc_outlets, "outlet phase failed");
-    if (rc_ranks != FL_OK) return set_err(rc_ranks, "ranks from spanning forest failed");
-    if (fast)
-      for (uint32_t g = 0; g < G; g++)
-        for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++)
-          rank[first[q]] = T->site_off[g] + (q - T->outlet_off[g]);
-  }
-  if (!fast) {
-    // equal or non-positive lengths: simulate each terrain's heap exactly
+  const uint32_t n = T->t.n, G = T->nterr;
+  const size_t K = T->h_outlets.size();
+  hipStream_t s = c->stream;
+  fl_launch_fill_u32(s, T->t.rank, n, 0xffffffffu);
+  // per terrain: 0 fast form (spanning forest), 1 exact heap simulation
+  // (repeated, non-positive or asymmetric lengths), 2 nothing to rank (no
+  // outlets: no site is ever reached)
+  std::vector<uint8_t> how(G, 0);
+  uint32_t n_fast = 0, n_exact = 0;
+  std::string err;
+  if (K > 0) {
+    std::vector<uint8_t> bad;
+    int rc = fl_device_len_check(s, n, T->t.row_ptr, T->t.col, T->t.len, T->d_tid, G, T->entry_off.data(), bad,
+                                 &err, &T->arena);
+    if (rc != FL_OK) return set_err(rc, "length check: %s", err.c_str());
     for (uint32_t g = 0; g < G; g++) {
-      uint32_t s0 = T->site_off[g], o0 = T->outlet_off[g];
-      int rc = fl_lake_visit_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
-                                         T->h_len.data(), T->h_outlets.data() + o0, T->outlet_off[g + 1] - o0, s0,
-                                         rank.data());
-      if (rc == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
-      if (rc != FL_OK) return set_err(rc, "visit order failed");
+      how[g] = T->outlet_off[g + 1] == T->outlet_off[g] ? 2 : (bad[g] || T->t_asym[g]) ? 1 : 0;
+      n_fast += how[g] == 0;
+      n_exact += how[g] == 1;
     }
   }
-  for (uint32_t i = 0; i < n; i++)
-    if (rank[i] != 0xffffffffu) order[rank[i]] = i;
-  HIPCHK(hipMemcpyAsync(T->t.rank, rank.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(T->t.order, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // host threads: the outlet phase of the fast terrains (exact std heap tie
+  // handling), and the whole traversal of the exact ones
+  std::vector<uint32_t> first(K), exact_rank(n_exact ? n : 0);
+  std::future<int> outlet_phase = std::async(std::launch::async, [&]() {
+    auto p0 = std::chrono::steady_clock::now();
+    for (uint32_t g = 0; g < G; g++) {
+      if (how[g] != 0) continue;
+      uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
+      int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
+                                         T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
+      if (rc != FL_OK) return rc;
+    }
+    T->stat_order_parts[0] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
+    return (int)FL_OK;
+  });
+  std::future<int> exact_phase = std::async(std::launch::async, [&]() {
+    if (!n_exact) return (int)FL_OK;
+    std::vector<int> rcs(G, FL_OK);
+    host_each(
+        G, [&](uint32_t g) { return how[g] == 1 ? T->site_off[g + 1] - T->site_off[g] : 0u; },
+        [&](uint32_t g) {
+          if (how[g] != 1) return;
+          uint32_t s0 = T->site_off[g], o0 = T->outlet_off[g];
+          rcs[g] = fl_lake_visit_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
+                                             T->h_len.data(), T->h_outlets.data() + o0, T->outlet_off[g + 1] - o0,
+                                             s0, exact_rank.data());
+        });
+    for (int rc : rcs)
+      if (rc != FL_OK) return rc;
+    return (int)FL_OK;
+  });
+  // the device: spanning forest and Prim ranks of the fast terrains, with
+  // each terrain's outlets merged into one component (labelled by its first
+  // listed outlet) and the other terrains merged whole (no forest edges)
+  int rc_dev = FL_OK;
+  if (n_fast) {
+    auto p1 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> init(n), base(G);
+    host_parallel(n, 1 << 18, [&](uint64_t a, uint64_t b) {
+      for (uint64_t v = a; v < b; v++) init[v] = (uint32_t)v;
+    });
+    for (uint32_t g = 0; g < G; g++) {
+      const uint32_t s0 = T->site_off[g], s1 = T->site_off[g + 1];
+      base[g] = s0 + (T->outlet_off[g + 1] - T->outlet_off[g]);
+      if (how[g] == 0) {
+        for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++)
+          init[T->h_outlets[q]] = T->h_outlets[T->outlet_off[g]];
+      } else {
+        for (uint32_t v = s0; v < s1; v++) init[v] = s0;
+      }
+    }
+    fl_visit_stats vs;
+    rc_dev = fl_device_ranks(s, n, T->t.row_ptr, T->t.col, T->t.len, T->t.is_outlet, T->d_tid, init, base,
+                             T->t.rank, &vs, &err, &T->arena);
+    // the forest part includes the set-up of its labels on the host
+    T->stat_order_parts[1] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p1).count() - vs.ranks_ms;
+    T->stat_order_parts[2] = vs.ranks_ms;
+  }
+  int rc_outlets = outlet_phase.get();
+  int rc_exact = exact_phase.get();
+  if (rc_dev != FL_OK) return set_err(rc_dev, "visit order on the device: %s", err.c_str());
+  if (rc_outlets != FL_OK) return set_err(rc_outlets, "outlet phase failed");
+  if (rc_exact == FL_ERR_NAN_LENGTH) return set_err(FL_ERR_NAN_LENGTH, "NaN edge length reached by the lake traversal");
+  if (rc_exact != FL_OK) return set_err(rc_exact, "visit order failed");
+  // the outlets of the fast terrains in pop order, the exact terrains whole
+  std::vector<uint32_t> osite, orank;
+  osite.reserve(K);
+  orank.reserve(K);
+  for (uint32_t g = 0; g < G; g++) {
+    if (how[g] != 0) continue;
+    for (uint32_t q = T->outlet_off[g]; q < T->outlet_off[g + 1]; q++) {
+      osite.push_back(first[q]);
+      orank.push_back(T->site_off[g] + (q - T->outlet_off[g]));
+    }
+  }
+  if (!osite.empty()) {
+    int rc = fl_device_scatter_u32(s, (uint32_t)osite.size(), osite.data(), orank.data(), T->t.rank, &err, &T->arena);
+    if (rc != FL_OK) return set_err(rc, "outlet ranks: %s", err.c_str());
+  }
+  for (uint32_t g = 0; g < G; g++) {
+    if (how[g] != 1) continue;
+    const uint32_t s0 = T->site_off[g], ng = T->site_off[g + 1] - s0;
+    HIPCHK(hipMemcpyAsync(T->t.rank + s0, exact_rank.data() + s0, (uint64_t)ng * 4, hipMemcpyHostToDevice, s));
+  }
+  fl_device_order(s, n, T->t.rank, T->t.order);
+  HIPCHK(hipStreamSynchronize(s));  // the host arrays above must outlive the copies
   T->have_order = true;
-  T->stat_order_fast = fast ? 1 : 0;
+  T->stat_order_fast = (n_fast > 0 && n_exact == 0) ? 1 : 0;
   T->stat_order_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return FL_OK;
 }
@@ -709,6 +764,19 @@ extern "C" const char* fl_last_error(void) { return g_err.c_str(); }
 
 void fl_internal_set_error(const char* msg) { g_err = msg ? msg : ""; }
 
+hipError_t fl_arena_alloc(fl_arena* a, hipStream_t s, void** p, uint64_t bytes) {
+  bytes = fl_round256(bytes);
+  if (!a || a->kind == fl_arena::POOL) return hipMallocAsync(p, bytes, s);
+  if (a->kind == fl_arena::REGION && a->used + bytes > a->size) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  *p = a->kind == fl_arena::REGION ? (void*)(a->base + a->used) : nullptr;
+  a->used += bytes;
+  a->peak = std::max(a->peak, a->used);
+  return hipSuccess;
+}
+
 extern "C" const char* fl_version(void) { return "fastlem_hip 0.1.0 (gfx950, f64)"; }
 
 extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
@@ -809,9 +877,9 @@ void gather_perm(const V* src, const std::vector<uint32_t>& old_of_new, std::vec
 
 namespace {
 
-// Device arrays of a union of `count` terrains with n sites and nnz entries;
-// the bucket histograms start at zero.
-int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, bool clamp) {
+// The device arrays of a union of `count` terrains with n sites and nnz
+// entries, from P->arena (with a COUNT arena: their sizes only).
+int32_t layout_device(fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, bool clamp) {
   fl_dev_terrain& t = P->t;
   fl_dev_work& w = P->w;
   t.n = n;
@@ -851,6 +919,26 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
     FLCHK(dev_alloc(P, &P->d_tid, n));
     FLCHK(dev_alloc(P, &P->d_active, count));
     FLCHK(dev_alloc(P, &P->d_tchanged, count));
+  }
+  return FL_OK;
+}
+
+// HBM of a union of `count` terrains (n sites, nnz entries) as
+// layout_device takes it.
+uint64_t union_device_bytes(uint64_t n, uint64_t nnz, uint32_t count, bool clamp) {
+  fl_terrain P;
+  P.arena.kind = fl_arena::COUNT;
+  if (layout_device(&P, (uint32_t)n, (uint32_t)nnz, count, clamp) != FL_OK) return UINT64_MAX;
+  return P.bytes;
+}
+
+// Device arrays of a union of `count` terrains with n sites and nnz entries
+// (layout_device), pinned host slots for a batch; the bucket histograms
+// start at zero.
+int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, bool clamp) {
+  FLCHK(layout_device(P, n, nnz, count, clamp));
+  fl_dev_work& w = P->w;
+  if (count > 1) {
     HIPCHK(hipHostMalloc((void**)&P->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
     for (uint32_t*& q : P->h_tslot) HIPCHK(hipHostMalloc((void**)&q, count * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&P->h_mask, count, hipHostMallocDefault));
@@ -871,8 +959,11 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
   if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
   uint64_t N = 0, NNZ = 0, K = 0;
+  std::vector<uint8_t> asym(count, 0);
   if (count == 1) {
-    FLCHK(validate(&descs[0]));
+    bool a = false;
+    FLCHK(validate(&descs[0], &a));
+    asym[0] = a;
   } else {
     // terrains on host threads; the first failing terrain's error is reported
     std::vector<int32_t> rc(count, FL_OK);
@@ -880,7 +971,9 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
     host_each(
         count, [&](uint32_t g) { return descs[g].nnz; },
         [&](uint32_t g) {
-          rc[g] = validate(&descs[g]);
+          bool a = false;
+          rc[g] = validate(&descs[g], &a);
+          asym[g] = a;
           if (rc[g] != FL_OK) msg[g] = g_err;
         });
     for (uint32_t g = 0; g < count; g++)
@@ -905,6 +998,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   T->t_max_it.resize(count);
   T->t_iters.assign(count, 0);
   T->t_active.assign(count, 0);
+  T->t_asym = asym;
   for (uint32_t g = 0; g < count; g++) {
     T->site_off[g + 1] = T->site_off[g] + (uint32_t)descs[g].n;
     T->outlet_off[g + 1] = T->outlet_off[g] + (uint32_t)descs[g].n_outlets;
@@ -1036,10 +1130,12 @@ int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t*
   N->t_max_it.resize(G);
   N->t_iters.resize(G);
   N->t_active.assign(G, 1);
+  N->t_asym.assign(G, 0);
   new_orig.resize(G);
   std::vector<uint32_t> map(4 * (size_t)G + 1);  // new_off[G+1], old_off[G], new_eoff[G], old_eoff[G]
   for (uint32_t q = 0; q < G; q++) {
     uint32_t g = keep[q];
+    N->t_asym[q] = cur->t_asym[g];
     N->site_off[q + 1] = N->site_off[q] + (cur->site_off[g + 1] - cur->site_off[g]);
     N->entry_off[q + 1] = N->entry_off[q] + (cur->entry_off[g + 1] - cur->entry_off[g]);
     N->outlet_off[q + 1] = N->outlet_off[q] + (cur->outlet_off[g + 1] - cur->outlet_off[g]);
@@ -1467,6 +1563,35 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->solve_ms = T->stat_solve_ms;
   out->compactions = T->stat_compactions;
   out->redone = T->stat_redone;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_stage_visit_order(fl_ctx* c, fl_terrain* T, uint32_t* rank_out) {
+  if (!c || !T || (!rank_out && T->t.n)) return set_err(FL_ERR_INVALID, "null argument");
+  ProfScope prof_scope(c->prof);
+  const uint32_t n = T->t.n;
+  if (n == 0) return FL_OK;
+  HIPCHK(hipSetDevice(c->device));
+  FLCHK(ensure_visit_order(c, T));
+  std::vector<uint32_t> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), T->t.rank, (uint64_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // ranks of terrain g start at site_off[g]: report positions within it
+  for (uint32_t g = 0; g < T->nterr; g++)
+    for (uint32_t i = T->site_off[g]; i < T->site_off[g + 1]; i++) {
+      const uint32_t r = tmp[T->new_of_old[i]];
+      rank_out[i] = r == 0xffffffffu ? r : r - T->site_off[g];
+    }
+  return FL_OK;
+}
+
+extern "C" int32_t fl_request_device_bytes(const fl_terrain_desc* d, uint64_t* bytes) {
+  if (!d || !bytes) return set_err(FL_ERR_INVALID, "null argument");
+  if (d->n >= 0xffffffffull || d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "terrain too large");
+  bool clamp = false;
+  if (d->has_max_slope)
+    for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;
+  *bytes = union_device_bytes(d->n, d->nnz, 1, clamp) + fl_msf_scratch_bytes((uint32_t)d->n, (uint32_t)d->nnz);
   return FL_OK;
 }
 

@@ -13,10 +13,59 @@ void fl_internal_set_error(const char* msg);
 #include <string>
 #include <vector>
 
-// fl_order.hip: minimum spanning forest with the outlets merged, sorted edges.
-int fl_device_msf(hipStream_t s, uint32_t n, uint32_t nnz, const uint32_t* rp, const uint32_t* col, const double* len,
-                  const std::vector<uint32_t>& init_comp, std::vector<uint32_t>& a_sorted,
-                  std::vector<uint32_t>& b_sorted, bool* applies, std::string* err);
+// Where device buffers come from.
+//   POOL:   the device's memory pool, stream-ordered (hipMallocAsync /
+//           hipFreeAsync);
+//   REGION: a fixed device region handed out front to back, no allocation
+//           call; buffers are released by rewinding `used`;
+//   COUNT:  nowhere: the bytes are counted and the pointers stay null (sizing).
+// Every buffer takes a multiple of 256 bytes, so the three agree on sizes.
+struct fl_arena {
+  enum Kind { POOL = 0, REGION = 1, COUNT = 2 };
+  int kind = POOL;
+  uint8_t* base = nullptr;
+  uint64_t size = 0;  // REGION: bytes available
+  uint64_t used = 0;  // REGION / COUNT: bytes handed out
+  uint64_t peak = 0;  // REGION / COUNT: largest `used` seen
+};
+inline uint64_t fl_round256(uint64_t bytes) { return ((bytes < 8 ? 8 : bytes) + 255) & ~255ull; }
+// A buffer of `bytes` from `a` (null = POOL) on stream s.  A full REGION
+// returns hipErrorOutOfMemory and sets *p to null.
+hipError_t fl_arena_alloc(fl_arena* a, hipStream_t s, void** p, uint64_t bytes);
+
+// fl_order.hip, the device part of the lake-traversal visit order.  Scratch
+// comes from `arena` (null: the device pool) and is released before return.
+//
+// Per terrain g of a union (terrain of each site: tid, device, null for one
+// terrain; entry_off: G + 1 union entry offsets, host): bad[g] = 1 when its
+// edge lengths are not all positive and pairwise distinct.
+int fl_device_len_check(hipStream_t s, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* len,
+                        const uint32_t* tid, uint32_t G, const uint32_t* entry_off, std::vector<uint8_t>& bad,
+                        std::string* err, fl_arena* arena);
+struct fl_visit_stats {
+  uint32_t forest_edges = 0;
+  double forest_ms = 0.0;  // length order of the forest included
+  double ranks_ms = 0.0;
+};
+// Prim ranks of the traversal after its outlet phase, from the minimum
+// spanning forest of the graph with the sites of each init_comp component
+// (host, n labels: each terrain's outlets share one outlet's label) merged:
+// every site reached from an outlet through a forest edge gets
+// rank[v] = d_base[tid[v]] + its position in Prim order.  Other entries of
+// `rank` are not written.  Lengths must be positive and pairwise distinct
+// within every component that holds an outlet.
+int fl_device_ranks(hipStream_t s, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* len,
+                    const uint8_t* is_outlet, const uint32_t* tid, const std::vector<uint32_t>& init_comp,
+                    const std::vector<uint32_t>& base, uint32_t* rank, fl_visit_stats* st, std::string* err,
+                    fl_arena* arena);
+// dst[idx[q]] = val[q] for the `count` host pairs.
+int fl_device_scatter_u32(hipStream_t s, uint32_t count, const uint32_t* idx, const uint32_t* val, uint32_t* dst,
+                          std::string* err, fl_arena* arena);
+// order[rank[v]] = v for every site v with a rank (rank != UINT32_MAX).
+void fl_device_order(hipStream_t s, uint32_t n, const uint32_t* rank, uint32_t* order);
+// Peak scratch bytes of fl_device_len_check, fl_device_ranks and
+// fl_device_scatter_u32 (up to n pairs) for n sites and nnz entries.
+uint64_t fl_msf_scratch_bytes(uint32_t n, uint32_t nnz);
 
 // fl_setup.cpp: per-terrain forms on a union graph whose terrain occupies
 // sites [s0, s0 + nl) (its rows reference only those sites).

@@ -53,9 +53,12 @@ struct fl_server {
 
 namespace {
 
-// HBM a request holds while it runs: the terrain's arrays (about 230 B per
-// site and 12 B per adjacency entry) plus the one-time visit-order scratch.
-uint64_t estimate_bytes(const fl_terrain_desc& d) { return d.n * 400ull + d.nnz * 24ull + (1ull << 20); }
+// HBM a request holds at its peak: the terrain's device arrays plus the
+// visit order's scratch, from the sizes the allocations themselves use.
+uint64_t estimate_bytes(const fl_terrain_desc& d) {
+  uint64_t b = 0;
+  return fl_request_device_bytes(&d, &b) == FL_OK ? b : UINT64_MAX;
+}
 
 void worker_main(fl_server* S) {
   fl_ctx* ctx = nullptr;

@@ -44,6 +44,15 @@ def request(terrain: dict):
     return desc, keep, np.empty(n, np.float64)
 
 
+def request_bytes(terrain: dict) -> int:
+    """HBM the generate() call of a terrain dict holds at its peak (the
+    server's admission estimate, C ABI ``fl_request_device_bytes``)."""
+    desc, keep, _ = request(terrain)
+    b = ctypes.c_uint64()
+    check(_native.lib().fl_request_device_bytes(ctypes.byref(desc), ctypes.byref(b)))
+    return b.value
+
+
 class Server:
     def __init__(self, slots: int = 8, hbm_budget_bytes: int = 64 << 30, device: int = 0):
         h = ctypes.c_void_p()

@@ -186,6 +186,18 @@ int32_t fl_stage_iteration(fl_ctx *ctx, fl_terrain *t, double *elev, uint32_t *n
 int32_t fl_stage_basin_order(fl_ctx *ctx, fl_terrain *t, const double *elev, uint32_t outlet, uint32_t *order_out,
                              uint32_t *len_out);
 
+/* The lake traversal's visit order (stream_tree.rs:184-240) as the solve
+ * uses it: rank_out[i] = position of caller site i in the order in which
+ * its terrain's traversal first pops it (outlets first), UINT32_MAX when it
+ * is never reached.  For a batch, i runs over the terrains' sites in order
+ * and positions are within each terrain. */
+int32_t fl_stage_visit_order(fl_ctx *ctx, fl_terrain *t, uint32_t *rank_out);
+
+/* HBM a generate() call of this terrain holds at its peak: its device
+ * arrays plus the visit order's scratch (the serving loop's admission
+ * estimate).  No device is needed. */
+int32_t fl_request_device_bytes(const fl_terrain_desc *desc, uint64_t *bytes);
+
 /* ---- host setup helpers (one-time, CPU) ---- */
 
 /* StdRng (rand 0.8.5 ChaCha12) restatement used for the jitter of

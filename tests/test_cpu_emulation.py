@@ -258,3 +258,17 @@ def test_emulated_validation_reports_the_first_error(emu_lib):
     bad["outlets"] = np.concatenate([small["outlets"], small["outlets"][:1]]).astype(np.uint32)
     rc, msg = _create_error(L, [small, small, bad, a])
     assert rc == -1 and msg == f"terrain 2: duplicate outlet {int(small['outlets'][0])}", msg
+
+
+def test_emulated_visit_order_matches_heap_traversal(emu_lib):
+    """The device visit order (spanning forest, Euler tours, heavier-ancestor
+    jumping, preorder ranking; exact simulation for ties and asymmetric
+    lengths) equals the oracle's heap traversal, alone and in batches."""
+    import visit_cases
+
+    ctx = ctypes.c_void_p()
+    assert emu_lib.fl_ctx_create(0, ctypes.byref(ctx)) == 0
+    try:
+        visit_cases.check_all(emu_lib, ctx)
+    finally:
+        emu_lib.fl_ctx_destroy(ctx)

@@ -449,3 +449,14 @@ def test_profiled_solve_is_exact_and_timed(fbm20k):
     assert prof["k_da_fold"][0] >= it
     # the deep rounds of an iteration share one launch per sweep
     assert prof["k_da_tail"][0] == prof["k_tau_tail<false>"][0] <= prof["k_receivers"][0]
+
+
+def test_visit_order_matches_heap_traversal():
+    """The device visit order equals the oracle's heap traversal on random
+    Delaunay graphs, an equal-length grid, asymmetric lengths and an
+    unreachable component, alone and in batches."""
+    import visit_cases
+    from fastlem_amd import _native
+    from fastlem_amd.generator import context
+
+    visit_cases.check_all(_native.lib(), context())

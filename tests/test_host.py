@@ -318,11 +318,13 @@ def test_server_budget_and_errors_without_gpu():
 
     from fastlem_amd import _native
     from fastlem_amd.device import DeviceBatch
-    from fastlem_amd.serving import Server
+    from fastlem_amd.serving import Server, request_bytes
     from fastlem_amd.workloads import config_plain
 
     w = config_plain(500, 22.0, 1.0)
-    small = Server(slots=2, hbm_budget_bytes=1 << 20)  # 1 MiB: too small for any terrain
+    need = request_bytes(DeviceBatch.terrain_of(w))
+    assert need > 500 * 300  # device arrays (~300 B per site) and the visit order's scratch
+    small = Server(slots=2, hbm_budget_bytes=need - 1)  # just too small for this terrain
     with pytest.raises(_native.NativeError) as e:
         small.submit(DeviceBatch.terrain_of(w))
     assert e.value.code == -7  # FL_ERR_BUDGET
