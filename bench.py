@@ -12,8 +12,13 @@ HBM, then outer iterations until no elevation changes (src/lem/generator.rs:
 call computes.  value = sites * iterations / seconds, over the timed steps.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
-(algorithmic bytes per launch / HIP-event-timed launch duration) and the CPU
-baseline (the oracle, one core, on a capped number of iterations).
+(algorithmic bytes per launch / HIP-event-timed launch duration) and of the
+whole solve, the CPU baseline (the oracle, one thread, to convergence, with
+the host's CPU model and core count), and two bounded legs on their own
+terrain streams: batched (64 terrains of 1,000..262,144 sites in one union
+against one solve per terrain) and served (32 generate() requests of
+4,096..1,048,576 sites through the request loop, Poisson arrivals, p50/p99,
+against the plain paths).
 """
 from __future__ import annotations
 
@@ -74,6 +79,181 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
     return float(table.get(name, 0.0))
 
 
+def host_cpu() -> dict:
+    """The host's CPU model and core counts (for the CPU baseline record)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "host_cores_total": os.cpu_count(), "host_cores_usable": usable}
+
+
+def iteration_bytes(n: int, nnz: int, clamp: bool = False) -> float:
+    """Compulsory HBM bytes of one outer iteration: 141 per site and 20 per
+    adjacency entry (receivers, lake labels, tree structure, drainage area,
+    response time and elevation, each array read or written once), 8 more per
+    site with max slopes."""
+    return 141.0 * n + 20.0 * nnz + (8.0 * n if clamp else 0.0)
+
+
+def build_streams(args) -> dict:
+    """The terrains of the batched and served legs, built in a spawn process
+    pool before the GPU is first used."""
+    from fastlem_amd.workloads import terrain_stream
+
+    out = {}
+    t0 = time.time()
+    if args.batched_terrains:
+        out["batched"] = terrain_stream(args.batched_terrains, args.batched_min, args.batched_max, 4242, "boundary",
+                                        args.workers)
+    if args.served_requests:
+        out["served"] = terrain_stream(args.served_requests, args.served_min, args.served_max, 5151, "ocean",
+                                       args.workers)
+    if out:
+        log(f"streams built in {time.time() - t0:.1f}s: "
+            + ", ".join(f"{k} {len(v)} terrains / {sum(len(t['row_ptr']) - 1 for t in v)} sites"
+                        for k, v in out.items()))
+    return out
+
+
+def plain_resident(terrains):
+    """The plain path with inputs resident in HBM: one terrain per solve, one
+    after another (visit order recomputed as a fresh generate() would).
+    Returns (seconds, iterations per terrain, site-iterations)."""
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceTerrain
+
+    devs = [DeviceTerrain(t["row_ptr"], t["col"], t["length"], t["area"], t["erodibility"], t["uplift"],
+                          t["init_elevation"], t["outlets"]) for t in terrains]
+    devs[0].solve()  # warm-up
+    L = _native.lib()
+    for d in devs:
+        L.fl_terrain_invalidate_cache(d._h)
+    ts = time.perf_counter()
+    iters = [d.solve() for d in devs]
+    secs = time.perf_counter() - ts
+    for d in devs:
+        d.close()
+    work = float(sum((len(t["row_ptr"]) - 1) * i for t, i in zip(terrains, iters)))
+    return secs, iters, work
+
+
+def batched_leg(terrains) -> dict:
+    """Many terrains in one union solve against the plain path on the same
+    terrains (resident inputs both)."""
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceBatch
+
+    plain_s, iters, work = plain_resident(terrains)
+    b = DeviceBatch(terrains)
+    b.solve()  # warm-up
+    _native.lib().fl_terrain_invalidate_cache(b._h)  # the timed solve recomputes the visit orders, as plain does
+    ts = time.perf_counter()
+    its = b.solve()
+    batch_s = time.perf_counter() - ts
+    assert list(its) == iters, "batched iteration counts differ from the plain runs"
+    st = b.last_stats()
+    b.close()
+    sizes = [len(t["row_ptr"]) - 1 for t in terrains]
+    r = {"terrains": len(terrains), "sites": int(sum(sizes)), "site_range": [min(sizes), max(sizes)],
+         "site_iterations": work, "plain_site_iter_per_s": work / plain_s, "plain_seconds": plain_s,
+         "batched_site_iter_per_s": work / batch_s, "batched_seconds": batch_s,
+         "speedup_vs_plain": plain_s / batch_s, "compactions": st["compactions"],
+         "visit_order_ms": st["visit_order_ms"]}
+    log(f"batched: {batch_s:.3f}s vs plain {plain_s:.3f}s: x{plain_s / batch_s:.2f}")
+    return r
+
+
+def served_leg(terrains, slots: int, loads, budget: int) -> dict:
+    """A Poisson stream of generate() requests (host buffers in and out)
+    through the server, against the plain paths: resident solves one after
+    another, and generate() calls one after another."""
+    import ctypes
+
+    from fastlem_amd import _native
+    from fastlem_amd.generator import context
+    from fastlem_amd.serving import Server, request
+
+    plain_s, iters, work = plain_resident(terrains)
+    L = _native.lib()
+    ts = time.perf_counter()
+    gen_lat = []
+    for t, it_ref in zip(terrains, iters):
+        desc, keep, out = request(t)
+        it = ctypes.c_uint32()
+        t1 = time.perf_counter()
+        _native.check(L.fl_generate_f64(context(), ctypes.byref(desc),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(it)))
+        assert it.value == it_ref, "generate() iteration count differs from the resident solve"
+        gen_lat.append((time.perf_counter() - t1) * 1e3)
+    gen_s = time.perf_counter() - ts
+    sizes = [len(t["row_ptr"]) - 1 for t in terrains]
+    r = {"requests": len(terrains), "sites": int(sum(sizes)), "site_range": [min(sizes), max(sizes)],
+         "site_iterations": work, "slots": slots, "hbm_budget_bytes": budget,
+         "plain_resident_site_iter_per_s": work / plain_s, "plain_resident_seconds": plain_s,
+         "plain_generate_site_iter_per_s": work / gen_s, "plain_generate_seconds": gen_s,
+         "plain_generate_p50_ms": float(np.percentile(gen_lat, 50)),
+         "plain_generate_p99_ms": float(np.percentile(gen_lat, 99)), "loads": []}
+    rng = np.random.default_rng(7)
+    mean_service = gen_s / len(terrains)
+    for load in loads:
+        srv = Server(slots=slots, hbm_budget_bytes=budget)
+        sat = load == float("inf")
+        gaps = np.zeros(len(terrains)) if sat else rng.exponential(mean_service / load, len(terrains))
+        ts = time.perf_counter()
+        tickets = []
+        for t, gap in zip(terrains, gaps):
+            if gap > 0:
+                time.sleep(gap)
+            tickets.append(srv.submit(t))
+        lat = []
+        for k, it_ref in zip(tickets, iters):
+            _, its, _, ms = srv.wait(k)
+            assert its == it_ref, "served iteration count differs from the resident solve"
+            lat.append(ms)
+        span = time.perf_counter() - ts
+        st = srv.stats()
+        srv.close()
+        x = {"offered_load": "saturation" if sat else load, "site_iter_per_s": work / span,
+             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span,
+             "peak_running": st["peak_running"], "peak_bytes": st["peak_bytes"],
+             "vs_plain_resident": (work / span) / (work / plain_s), "vs_plain_generate": (work / span) / (work / gen_s)}
+        r["loads"].append(x)
+        log(f"served load={x['offered_load']}: {x['site_iter_per_s']:.3e} site-it/s p50={x['p50_ms']:.1f}ms "
+            f"p99={x['p99_ms']:.1f}ms x{x['vs_plain_resident']:.2f} resident x{x['vs_plain_generate']:.2f} generate()")
+    return r
+
+
+def cpu_baseline(w, init, cap=None) -> dict:
+    """The oracle (the reference's algorithm restated in C, one thread) on
+    the same terrain, to convergence unless capped."""
+    from oracle import flo
+
+    rp, col, ln = w.model.graph()
+    g = flo.Graph(rp, col, ln)
+    ts = time.perf_counter()
+    kw = {} if cap is None else {"max_iteration": cap}
+    _, cit, _ = flo.generate(g, w.model.areas(), w.erodibility, w.uplift, init, w.outlets(), **kw)
+    cs = time.perf_counter() - ts
+    log(f"cpu baseline: {cit} iterations in {cs:.2f}s")
+    r = {"value": float(w.n) * cit / cs, "unit": "site-iterations/s", "cores": 1, "kind": "port",
+         "sample": (f"oracle generate() on the same {w.n}-site terrain, "
+                    + (f"to convergence ({cit} iterations)" if cap is None else f"capped at {cit} iterations")
+                    + f", {cs:.1f} s, one thread"),
+         "iterations": cit, "seconds": cs}
+    r.update(host_cpu())
+    return r
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,17 +261,23 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sites", type=int, default=1_000_000)
     ap.add_argument("--width", type=float, default=1000.0)
-    ap.add_argument("--cpu-iters", type=int, default=16, help="iterations of the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=None,
+                    help="cap the CPU baseline's iterations (default: run it to convergence)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="measured HBM KiB per launch of the dominant kernel (rocprofv3 --pmc), for the roofline")
     ap.add_argument("--mode", choices=["single", "batched", "served", "copy"], default="single",
-                    help="single: the metric line; batched / served: the multi-terrain forms; copy: a 1 GiB "
-                         "device copy, the known-size kernel that calibrates rocprofv3's byte counters")
-    ap.add_argument("--terrains", type=int, default=128, help="batched/served: number of terrains")
-    ap.add_argument("--min-sites", type=int, default=1000)
-    ap.add_argument("--max-sites", type=int, default=65536)
-    ap.add_argument("--slots", type=int, default=8, help="served: worker slots (streams)")
+                    help="single: the metric line (with bounded batched and served legs); batched / served: one "
+                         "multi-terrain form alone; copy: a 1 GiB device copy, the known-size kernel that "
+                         "calibrates rocprofv3's byte counters")
+    ap.add_argument("--batched-terrains", type=int, default=64, help="batched leg: terrains (0: no leg)")
+    ap.add_argument("--batched-min", type=int, default=1000)
+    ap.add_argument("--batched-max", type=int, default=262144)
+    ap.add_argument("--served-requests", type=int, default=32, help="served leg: requests (0: no leg)")
+    ap.add_argument("--served-min", type=int, default=4096)
+    ap.add_argument("--served-max", type=int, default=1048576)
+    ap.add_argument("--slots", type=int, default=4, help="served: worker slots (streams)")
+    ap.add_argument("--budget-gib", type=float, default=32.0, help="served: HBM budget")
     ap.add_argument("--workers", type=int, default=8, help="host processes building the terrains")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -116,6 +302,7 @@ def main() -> int:
     n = w.n
     nnz = len(w.model.col)
     log(f"workload built: n={n} nnz={nnz} outlets={len(w.outlets())} in {time.time() - t0:.1f}s")
+    streams = build_streams(args)
     init = jitter_elevations(w.base, 0)
     t = DeviceTerrain.from_workload(w, init_elevation=init)
     L = _native.lib()
@@ -193,31 +380,29 @@ def main() -> int:
     avg_s = dms / dcnt / 1e3
     bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp) * dunits / dcnt
     achieved = bytes_per_launch / avg_s / 1e9 if bytes_per_launch else None
+    solve_gbs = iters * iteration_bytes(n, nnz, clamp) / (elapsed / args.steps) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
-                "launches_per_solve": dcnt, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None}
+                "launches_per_solve": dcnt, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None,
+                "solve_algorithmic_GBps": solve_gbs, "solve_frac": solve_gbs / HBM_PEAK_GBS,
+                "iteration_bytes": iteration_bytes(n, nnz, clamp), "kernel_ms_per_solve": total_kernel_ms}
     if args.traffic_kib is not None:
         # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
         # scaled by the factor the same counters read for a copy kernel of known size
         roofline["traffic"] = args.traffic_kib * 1024.0
     else:
         roofline["traffic"] = measured_traffic(dom, bytes_per_launch)
+    t.close()
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import flo
+    legs = {}
+    if "batched" in streams:
+        legs["batched"] = batched_leg(streams.pop("batched"))
+    if "served" in streams:
+        legs["served"] = served_leg(streams.pop("served"), args.slots, (1.0, float("inf")),
+                                    int(args.budget_gib * (1 << 30)))
 
-        rp, col, ln = w.model.graph()
-        g = flo.Graph(rp, col, ln)
-        ts = time.perf_counter()
-        _, cit, _ = flo.generate(g, w.model.areas(), w.erodibility, w.uplift, init, w.outlets(),
-                                 max_iteration=args.cpu_iters)
-        cs = time.perf_counter() - ts
-        cpu = {"value": float(n) * cit / cs, "unit": "site-iterations/s", "cores": 1, "kind": "port",
-               "sample": f"oracle generate() capped at max_iteration={cit} on the same {n}-site terrain "
-                         f"({cs:.1f} s, iteration 1 includes the lake heap)"}
-        log(f"cpu baseline: {cit} iterations in {cs:.2f}s")
+    cpu = None if args.no_cpu else cpu_baseline(w, init, args.cpu_iters)
 
     line = {
         "metric": "site-iterations/s (1M-site FP64 terrain, generate() loop to convergence)",
@@ -235,11 +420,15 @@ def main() -> int:
         "config": {"workload": f"fbm-terrain sites={n} box=[0,{args.width:g}]^2 lloyd=1 "
                                f"K=clip(10^fBm,0.1,10) ocean=fBm<-0.2 outlets={len(w.outlets())} nnz={nnz}",
                    "iterations": iters, "visit_order_ms": stats["visit_order_ms"],
+                   "visit_order_parts_ms": {"outlet_phase": stats["outlet_phase_ms"], "forest": stats["forest_ms"],
+                                            "ranks": stats["ranks_ms"]},
                    "host_syncs_per_solve": stats["host_syncs"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "host_buffers": host_buffers,
     }
+    line.update(legs)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -290,109 +479,20 @@ def copy_calibration() -> int:
 
 
 def multi_terrain(args) -> int:
-    """Batched (one union solve) or served (continuous-batching request loop)
-    against the plain path (one terrain at a time), on the same terrains."""
-    from fastlem_amd.workloads import terrain_stream
-
-    kind = "boundary" if args.mode == "batched" else "ocean"
-    t0 = time.time()
-    last = [time.time()]
-
-    def progress(done, count):
-        if done == count or time.time() - last[0] > 20:
-            last[0] = time.time()
-            log(f"built {done}/{count} terrains")
-
-    terrains = terrain_stream(args.terrains, args.min_sites, args.max_sites, 4242, kind, args.workers, progress)
-    sites = sum(len(t["row_ptr"]) - 1 for t in terrains)
-    log(f"{len(terrains)} terrains, {sites} sites built in {time.time() - t0:.1f}s")
-    from fastlem_amd.device import DeviceBatch, DeviceTerrain
-
-    # plain path: one terrain per solve, inputs resident
-    devs = [DeviceTerrain(t["row_ptr"], t["col"], t["length"], t["area"], t["erodibility"], t["uplift"],
-                          t["init_elevation"], t["outlets"]) for t in terrains]
-    devs[0].solve()  # warm-up
-    ts = time.perf_counter()
-    iters = [d.solve() for d in devs]
-    plain_s = time.perf_counter() - ts
-    work = float(sum((len(t["row_ptr"]) - 1) * i for t, i in zip(terrains, iters)))
-    plain = work / plain_s
-    log(f"plain: {plain_s:.3f}s, {plain:.3e} site-iterations/s")
-    for d in devs:
-        d.close()
-    line = {"mode": args.mode, "terrains": len(terrains), "sites": sites,
-            "site_range": [args.min_sites, args.max_sites], "plain_site_iter_per_s": plain,
-            "plain_seconds": plain_s, "unit": "site-iterations/s"}
+    """One multi-terrain form alone, on its full workload range: batched
+    (terrains log-uniform in [--batched-min, --batched-max], boundary outlets)
+    or served (requests log-uniform in [--served-min, --served-max], ocean
+    outlets, Poisson arrivals at several offered loads)."""
     if args.mode == "batched":
-        from fastlem_amd import _native
-
-        b = DeviceBatch(terrains)
-        b.solve()  # warm-up
-        _native.lib().fl_terrain_invalidate_cache(b._h)  # the timed solve recomputes the visit orders, as plain does
-        ts = time.perf_counter()
-        its = b.solve()
-        batch_s = time.perf_counter() - ts
-        assert list(its) == iters, "batched iteration counts differ from the plain runs"
-        line.update({"batched_site_iter_per_s": work / batch_s, "batched_seconds": batch_s,
-                     "speedup_vs_plain": plain_s / batch_s, "stats": b.last_stats()})
-        log(f"batched: {batch_s:.3f}s, {work / batch_s:.3e} site-iterations/s, x{plain_s / batch_s:.2f}")
+        args.served_requests = 0
     else:
-        import ctypes
-
-        from fastlem_amd import _native
-        from fastlem_amd.generator import context
-        from fastlem_amd.serving import Server, request
-
-        # the plain request path: the same generate() calls one after another
-        # (host buffers: validation, relabelling, upload, solve, download), as
-        # a caller without the server would make them
-        L = _native.lib()
-        ts = time.perf_counter()
-        gen_lat = []
-        for t, it_ref in zip(terrains, iters):
-            desc, keep, out = request(t)
-            it = ctypes.c_uint32()
-            _native.check(L.fl_generate_f64(context(), ctypes.byref(desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                            ctypes.byref(it)))
-            assert it.value == it_ref, "generate() iteration count differs from the resident solve"
-            gen_lat.append((time.perf_counter() - ts) * 1e3)
-        gen_s = time.perf_counter() - ts
-        plain_gen = work / gen_s
-        log(f"plain generate(): {gen_s:.3f}s, {plain_gen:.3e} site-iterations/s")
-        line.update({"plain_generate_site_iter_per_s": plain_gen, "plain_generate_seconds": gen_s,
-                     "plain_generate_p50_ms": float(np.percentile(gen_lat, 50)),
-                     "plain_generate_p99_ms": float(np.percentile(gen_lat, 99))})
-
-        rng = np.random.default_rng(7)
-        mean_service = gen_s / len(terrains)
-        results = []
-        for load in (0.5, 1.0, 2.0, 4.0, float("inf")):
-            log(f"served: offered load {load} x the plain service rate")
-            srv = Server(slots=args.slots, hbm_budget_bytes=32 << 30)
-            gaps = np.zeros(len(terrains)) if load == float("inf") else \
-                rng.exponential(mean_service / load, len(terrains))
-            ts = time.perf_counter()
-            tickets = []
-            for t, gap in zip(terrains, gaps):
-                if gap > 0:
-                    time.sleep(gap)
-                tickets.append(srv.submit(t))
-            lat = []
-            for k in tickets:
-                _, _, _, ms = srv.wait(k)
-                lat.append(ms)
-            span = time.perf_counter() - ts
-            peak = srv.stats()["peak_running"]
-            srv.close()
-            r = {"offered_load": load if load != float("inf") else "saturation",
-                 "site_iter_per_s": work / span, "p50_ms": float(np.percentile(lat, 50)),
-                 "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span, "peak_running": peak}
-            results.append(r)
-            log(f"served load={r['offered_load']}: {r['site_iter_per_s']:.3e} site-it/s "
-                f"p50={r['p50_ms']:.1f}ms p99={r['p99_ms']:.1f}ms peak={peak}")
-        line.update({"served": results, "slots": args.slots,
-                     "speedup_vs_plain_at_saturation": results[-1]["site_iter_per_s"] / plain,
-                     "speedup_vs_plain_generate_at_saturation": results[-1]["site_iter_per_s"] / plain_gen})
+        args.batched_terrains = 0
+    streams = build_streams(args)
+    if args.mode == "batched":
+        line = {"mode": "batched", **batched_leg(streams["batched"])}
+    else:
+        line = {"mode": "served", **served_leg(streams["served"], args.slots, (0.5, 1.0, 2.0, 4.0, float("inf")),
+                                               int(args.budget_gib * (1 << 30)))}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -567,15 +567,15 @@ int jump_to_fixpoint(hipStream_t s, uint32_t* d_flags, uint32_t nflags, uint32_t
 }  // namespace
 
 uint64_t fl_msf_scratch_bytes(uint32_t n, uint32_t nnz) {
-  // the length check's tables (at most 4 entries per undirected edge plus
-  // one table per terrain, bounded by n) and flags, then the rank buffers
+  // the length check's tables (terrain g's holds fewer than 2 * entries_g + 4
+  // slots, and there are at most n terrains) and flags, then the rank buffers
   fl_arena a;
   a.kind = fl_arena::COUNT;
   uint64_t check;
   {
     DevBuf buf(nullptr, &a);
     void* p;
-    buf.alloc(&p, 8ull * (2ull * nnz + 2ull * n + 2));
+    buf.alloc(&p, 8ull * (2ull * nnz + 4ull * n + 4));
     buf.alloc(&p, 3ull * 4 * (n + 1));
     check = a.peak;
   }
