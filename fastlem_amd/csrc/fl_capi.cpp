@@ -63,6 +63,7 @@ struct ProfScope {
 
 struct fl_ctx {
   int device = 0;
+  hipMemPool_t pool = nullptr;           // the library's memory pool of the device
   uint32_t compact_min_idle = 1u << 16;  // see solve_union
   int pipelined = 1;                     // one_iteration_fast after the first iteration (2: minimal
                                          // speculation, most iterations redone)
@@ -766,7 +767,8 @@ void fl_internal_set_error(const char* msg) { g_err = msg ? msg : ""; }
 
 hipError_t fl_arena_alloc(fl_arena* a, hipStream_t s, void** p, uint64_t bytes) {
   bytes = fl_round256(bytes);
-  if (!a || a->kind == fl_arena::POOL) return hipMallocAsync(p, bytes, s);
+  if (!a || a->kind == fl_arena::POOL)
+    return a && a->pool ? hipMallocFromPoolAsync(p, bytes, a->pool, s) : hipMallocAsync(p, bytes, s);
   if (a->kind == fl_arena::REGION && a->used + bytes > a->size) {
     *p = nullptr;
     return hipErrorOutOfMemory;
@@ -779,22 +781,63 @@ hipError_t fl_arena_alloc(fl_arena* a, hipStream_t s, void** p, uint64_t bytes) 
 
 extern "C" const char* fl_version(void) { return "fastlem_hip 0.1.0 (gfx950, f64)"; }
 
+namespace {
+
+// The library's own memory pool per device.  Freed device memory stays in it
+// for the next terrain (unbounded release threshold), without changing the
+// device's default pool that other HIP code in the process uses; it is
+// trimmed when the device's last context is destroyed.  The registry is
+// never destroyed (contexts may be destroyed during process exit).
+struct PoolRegistry {
+  std::mutex mu;
+  std::vector<std::pair<hipMemPool_t, int>> dev;  // per device: pool, live contexts
+};
+PoolRegistry& pools() {
+  static PoolRegistry* r = new PoolRegistry();
+  return *r;
+}
+
+int32_t acquire_pool(int device, hipMemPool_t* out) {
+  PoolRegistry& r = pools();
+  std::lock_guard<std::mutex> g(r.mu);
+  if ((int)r.dev.size() <= device) r.dev.resize(device + 1, {nullptr, 0});
+  auto& e = r.dev[device];
+  if (!e.first) {
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    HIPCHK(hipMemPoolCreate(&e.first, &props));
+    uint64_t keep = UINT64_MAX;
+    HIPCHK(hipMemPoolSetAttribute(e.first, hipMemPoolAttrReleaseThreshold, &keep));
+  }
+  e.second++;
+  *out = e.first;
+  return FL_OK;
+}
+
+void release_pool(int device) {
+  PoolRegistry& r = pools();
+  std::lock_guard<std::mutex> g(r.mu);
+  if ((int)r.dev.size() <= device || !r.dev[device].first) return;
+  auto& e = r.dev[device];
+  if (--e.second == 0) (void)hipMemPoolTrimTo(e.first, 0);
+}
+
+}  // namespace
+
 extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   if (!out) return set_err(FL_ERR_INVALID, "null output pointer");
   int count = 0;
   hipError_t e = hipGetDeviceCount(&count);
   if (e != hipSuccess || count == 0) return set_err(FL_ERR_NO_DEVICE, "no HIP device available");
   if (device < 0 || device >= count) return set_err(FL_ERR_INVALID, "device %d out of range", device);
-  std::unique_ptr<fl_ctx> c(new fl_ctx());
+  // a failure below releases what was set up so far
+  std::unique_ptr<fl_ctx, int32_t (*)(fl_ctx*)> c(new fl_ctx(), fl_ctx_destroy);
   c->device = device;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  // the device pool keeps freed memory for the next terrain instead of
-  // returning it at every synchronisation
-  hipMemPool_t pool;
-  HIPCHK(hipDeviceGetDefaultMemPool(&pool, device));
-  uint64_t keep = UINT64_MAX;
-  HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  FLCHK(acquire_pool(device, &c->pool));
   HIPCHK(hipHostMalloc((void**)&c->host_ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
   for (int q = 0; q < 2; q++) {
     HIPCHK(hipHostMalloc((void**)&c->host_slot[q], FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
@@ -814,6 +857,7 @@ extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
     if (c->host_slot[q]) hipHostFree(c->host_slot[q]);
     if (c->slot_ev[q]) hipEventDestroy(c->slot_ev[q]);
   }
+  if (c->pool) release_pool(c->device);
   delete c;
   return FL_OK;
 }
@@ -988,6 +1032,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   HIPCHK(hipSetDevice(c->device));
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> T(new fl_terrain(), fl_terrain_destroy);
   T->ctx = c;
+  T->arena.pool = c->pool;
   T->nterr = count;
   T->max_iteration = descs[0].max_iteration;
   uint32_t n = (uint32_t)N, nnz = (uint32_t)NNZ;
@@ -1123,6 +1168,7 @@ int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t*
   uint32_t G = (uint32_t)keep.size();
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> N(new fl_terrain(), fl_terrain_destroy);
   N->ctx = c;
+  N->arena.pool = c->pool;
   N->nterr = G;
   N->site_off.assign(G + 1, 0);
   N->entry_off.assign(G + 1, 0);

@@ -27,6 +27,7 @@ struct fl_arena {
   uint64_t size = 0;  // REGION: bytes available
   uint64_t used = 0;  // REGION / COUNT: bytes handed out
   uint64_t peak = 0;  // REGION / COUNT: largest `used` seen
+  hipMemPool_t pool = nullptr;  // POOL: the library's pool of the device (null: the default pool)
 };
 inline uint64_t fl_round256(uint64_t bytes) { return ((bytes < 8 ? 8 : bytes) + 255) & ~255ull; }
 // A buffer of `bytes` from `a` (null = POOL) on stream s.  A full REGION

@@ -8,7 +8,9 @@ GPU, ``generate()`` raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import threading
 from typing import Optional, Sequence
 
 import numpy as np
@@ -49,17 +51,38 @@ def jitter_elevations(base: np.ndarray, seed: int = 0) -> np.ndarray:
 
 
 class _Context:
-    """Process-wide device context (one HIP stream on device 0)."""
+    """The calling thread's device context (one HIP stream on device 0).
 
-    _ctx = None
+    Calls on one context must not overlap (its stream and host slots are
+    shared by everything the call enqueues), and ctypes releases the GIL
+    during foreign calls, so every thread gets a context of its own; their
+    work overlaps on the device.  Contexts live until the interpreter exits.
+    """
+
+    _local = threading.local()
+    _lock = threading.Lock()
+    _all: list = []
 
     @classmethod
     def get(cls):
-        if cls._ctx is None:
+        h = getattr(cls._local, "ctx", None)
+        if h is None:
             h = ctypes.c_void_p()
             check(_native.lib().fl_ctx_create(0, ctypes.byref(h)))
-            cls._ctx = h
-        return cls._ctx
+            with cls._lock:
+                cls._all.append(h)
+            cls._local.ctx = h
+        return h
+
+    @classmethod
+    def _destroy_all(cls):
+        with cls._lock:
+            for h in cls._all:
+                _native.lib().fl_ctx_destroy(h)
+            cls._all.clear()
+
+
+atexit.register(_Context._destroy_all)
 
 
 def context():

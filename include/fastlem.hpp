@@ -125,14 +125,26 @@ class Result {
   std::optional<GenerationError> error_;
 };
 
-// One process-wide device context (HIP device 0).
+// The calling thread's device context (HIP device 0), created on first use
+// and destroyed when the thread ends.  A context is one stream with its own
+// host slots and calls on one context must not overlap, so generate() calls
+// made at once from several threads each run on a context of their own (and
+// their work overlaps on the device), as separate reference generators run
+// independently.
 inline fl_ctx* default_context() {
-  static fl_ctx* ctx = [] {
+  struct Holder {
+    fl_ctx* ctx = nullptr;
+    ~Holder() {
+      if (ctx) fl_ctx_destroy(ctx);
+    }
+  };
+  thread_local Holder h;
+  if (!h.ctx) {
     fl_ctx* c = nullptr;
     if (fl_ctx_create(0, &c) != FL_OK) throw std::runtime_error(fl_last_error());
-    return c;
-  }();
-  return ctx;
+    h.ctx = c;
+  }
+  return h.ctx;
 }
 
 // src/lem/generator.rs:37-213

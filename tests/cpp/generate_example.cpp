@@ -6,12 +6,15 @@
 //                                       max_slope (NaN = None) as raw arrays;
 //                                       OUT: iterations (u32) + elevations;
 //                                       K: set_max_iteration(K)
+//   generate_example run2 IN OUT      -> the same generate() from two threads
+//                                       at once; OUT holds both results
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <thread>
 #include <vector>
 
 #include "fastlem.hpp"
@@ -45,7 +48,8 @@ int main(int argc, char** argv) {
     std::cout << "errors " << ok << "/4" << std::endl;
     return ok == 4 ? 0 : 1;
   }
-  if (argc >= 4 && std::strcmp(argv[1], "run") == 0) {
+  if (argc >= 4 && (std::strcmp(argv[1], "run") == 0 || std::strcmp(argv[1], "run2") == 0)) {
+    const bool two = std::strcmp(argv[1], "run2") == 0;
     std::ifstream f(argv[2], std::ios::binary);
     uint64_t hdr[2];
     f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
@@ -68,11 +72,27 @@ int main(int argc, char** argv) {
                            .set_max_slope(std::isnan(ms[i]) ? std::optional<double>() : std::optional<double>(ms[i])));
     auto gen = TerrainGenerator<>().set_model(model).set_parameters(params);
     if (argc >= 5) gen = gen.set_max_iteration((uint32_t)std::strtoul(argv[4], nullptr, 10));
-    auto terrain = gen.generate().unwrap();
     std::ofstream o(argv[3], std::ios::binary);
-    uint32_t it = gen.last_iterations();
-    o.write(reinterpret_cast<const char*>(&it), sizeof(it));
-    o.write(reinterpret_cast<const char*>(terrain.elevations().data()), n * sizeof(double));
+    if (!two) {
+      auto terrain = gen.generate().unwrap();
+      uint32_t it = gen.last_iterations();
+      o.write(reinterpret_cast<const char*>(&it), sizeof(it));
+      o.write(reinterpret_cast<const char*>(terrain.elevations().data()), n * sizeof(double));
+      return 0;
+    }
+    // two generators (copies) run at once on two threads
+    auto g2 = gen;
+    std::vector<double> e1, e2;
+    std::thread t1([&] { e1 = gen.generate().unwrap().elevations(); });
+    std::thread t2([&] { e2 = g2.generate().unwrap().elevations(); });
+    t1.join();
+    t2.join();
+    for (auto* g : {&gen, &g2}) {
+      uint32_t it = g->last_iterations();
+      o.write(reinterpret_cast<const char*>(&it), sizeof(it));
+    }
+    o.write(reinterpret_cast<const char*>(e1.data()), n * sizeof(double));
+    o.write(reinterpret_cast<const char*>(e2.data()), n * sizeof(double));
     return 0;
   }
   std::cerr << "usage: generate_example errors | run IN OUT" << std::endl;

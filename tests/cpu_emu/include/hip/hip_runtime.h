@@ -170,6 +170,32 @@ inline hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int) {
   return hipSuccess;
 }
 inline hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
+enum hipMemAllocationType { hipMemAllocationTypeInvalid = 0, hipMemAllocationTypePinned = 1 };
+enum hipMemAllocationHandleType { hipMemHandleTypeNone = 0 };
+enum hipMemLocationType { hipMemLocationTypeInvalid = 0, hipMemLocationTypeDevice = 1 };
+struct hipMemLocation {
+  hipMemLocationType type;
+  int id;
+};
+struct hipMemPoolProps {
+  hipMemAllocationType allocType;
+  hipMemAllocationHandleType handleTypes;
+  hipMemLocation location;
+  void* win32SecurityAttributes;
+  size_t maxSize;
+  unsigned char reserved[56];
+};
+// pools: allocations come from the emulation's heap
+inline hipError_t hipMemPoolCreate(hipMemPool_t* pool, const hipMemPoolProps*) {
+  static char tag;
+  *pool = &tag;
+  return hipSuccess;
+}
+inline hipError_t hipMemPoolDestroy(hipMemPool_t) { return hipSuccess; }
+inline hipError_t hipMemPoolTrimTo(hipMemPool_t, size_t) { return hipSuccess; }
+inline hipError_t hipMallocFromPoolAsync(void** p, size_t bytes, hipMemPool_t, hipStream_t) {
+  return hipMalloc(p, bytes);
+}
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned flags);
 hipError_t hipHostFree(void* p);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s = nullptr);

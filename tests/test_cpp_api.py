@@ -80,3 +80,53 @@ def test_cpp_generate_capped_oscillation(example_bin, plain10k, tmp_path):
                                      max_slope=ms, max_iteration=100)
     assert iters == ref_iters == 100
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def _build_example(out, libdir, libname):
+    cmd = ["g++", "-std=c++17", "-O2", "-pthread", "-I", os.path.join(ROOT, "include"), SRC, "-o", out, "-L",
+           libdir, f"-l{libname}", f"-Wl,-rpath,{libdir}"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return out
+
+
+def _run_two(binary, tmp_path, w, ms):
+    inp, out = tmp_path / "in2.bin", tmp_path / "out2.bin"
+    _write_input(inp, w, ms)
+    r = subprocess.run([binary, "run2", str(inp), str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    its = np.frombuffer(raw[:8], np.uint32)
+    h = np.frombuffer(raw[8:], np.float64).reshape(2, w.n)
+    return its, h
+
+
+def _check_two(its, h, w, ms):
+    from oracle import flo
+
+    ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets(),
+                               max_slope=ms)
+    for k in range(2):
+        assert its[k] == rit
+        assert np.array_equal(h[k].view(np.uint64), ref.view(np.uint64))
+
+
+def test_cpp_generate_from_two_threads_emulated(tmp_path):
+    """Two generate() calls at once from two threads (each thread its own
+    default context) through the CPU emulation of the library: both results
+    equal the oracle's bit for bit."""
+    from cpu_emu import build as emu_build
+    from fastlem_amd.workloads import config_plain
+
+    lib = emu_build.build()
+    binary = _build_example(str(tmp_path / "gen_emu"), os.path.dirname(lib), "fastlem_emu")
+    w = config_plain(1500, 39.0, 1.5)
+    ms = np.full(w.n, np.nan)
+    _check_two(*_run_two(binary, tmp_path, w, ms), w, None)
+
+
+@pytest.mark.gpu
+def test_cpp_generate_from_two_threads(example_bin, fbm20k, tmp_path):
+    w = fbm20k
+    ms = np.full(w.n, np.nan)
+    binary = _build_example(str(tmp_path / "gen2"), os.path.join(ROOT, "fastlem_amd", "lib"), "fastlem_hip")
+    _check_two(*_run_two(binary, tmp_path, w, ms), w, None)

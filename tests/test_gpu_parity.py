@@ -460,3 +460,33 @@ def test_visit_order_matches_heap_traversal():
     from fastlem_amd.generator import context
 
     visit_cases.check_all(_native.lib(), context())
+
+
+def test_generate_from_python_threads(plain10k, fbm20k):
+    """The drop-in generate() called from several Python threads at once
+    (ctypes releases the GIL): each thread runs on its own context and every
+    result equals the oracle's bit for bit."""
+    import threading
+
+    from oracle import flo
+    from fastlem_amd.generator import TerrainGenerator
+
+    ws = [plain10k, fbm20k, plain10k, fbm20k]
+    results = [None] * len(ws)
+
+    def run(k):
+        w = ws[k]
+        g = TerrainGenerator().set_model(w.model).set_parameters(w.parameters())
+        results[k] = (g.generate().elevations(), g.last_iterations)
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(len(ws))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for w, res in zip(ws, results):
+        assert res is not None
+        h, its = res
+        ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+        assert its == rit
+        assert _bits_equal(ref, h)
