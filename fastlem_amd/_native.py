@@ -71,6 +71,7 @@ def _declare(lib):
         "fl_ctx_create": (i32, [i32, ctypes.POINTER(vp)]),
         "fl_ctx_destroy": (i32, [vp]),
         "fl_generate_f64": (i32, [vp, ctypes.POINTER(TerrainDesc), c_f64p, c_u32p]),
+        "fl_generate_f64_noalloc": (i32, [vp, ctypes.POINTER(TerrainDesc), c_f64p, c_u32p]),
         "fl_terrain_create": (i32, [vp, ctypes.POINTER(TerrainDesc), ctypes.POINTER(vp)]),
         "fl_terrain_destroy": (i32, [vp]),
         "fl_terrain_solve": (i32, [vp, vp, u32, c_u32p]),

@@ -838,6 +838,7 @@ extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   FLCHK(acquire_pool(device, &c->pool));
+  FLCHK(fl_noalloc_scratch_init(device));  // the static scratch's address on this device
   HIPCHK(hipHostMalloc((void**)&c->host_ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
   for (int q = 0; q < 2; q++) {
     HIPCHK(hipHostMalloc((void**)&c->host_slot[q], FL_CTRL_WORDS * sizeof(uint32_t), hipHostMallocDefault));
@@ -999,7 +1000,8 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
 // Upload `count` terrains as one union graph: terrain g's sites occupy
 // [site_off[g], site_off[g+1]), each relabelled for locality inside its range;
 // no edge joins two terrains.
-int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl_terrain** out) {
+int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl_terrain** out,
+                     const fl_arena* region = nullptr) {
   if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
   if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
   uint64_t N = 0, NNZ = 0, K = 0;
@@ -1033,6 +1035,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> T(new fl_terrain(), fl_terrain_destroy);
   T->ctx = c;
   T->arena.pool = c->pool;
+  if (region) T->arena = *region;
   T->nterr = count;
   T->max_iteration = descs[0].max_iteration;
   uint32_t n = (uint32_t)N, nnz = (uint32_t)NNZ;
@@ -1488,6 +1491,44 @@ extern "C" int32_t fl_generate_f64(fl_ctx* c, const fl_terrain_desc* d, double* 
   int32_t rc = fl_terrain_solve(c, T, d->max_iteration, iterations_out);
   if (rc == FL_OK) rc = fl_terrain_download(c, T, elevations_out);
   fl_terrain_destroy(T);
+  return rc;
+}
+
+extern "C" int32_t fl_generate_f64_noalloc(fl_ctx* c, const fl_terrain_desc* d, double* elevations_out,
+                                           uint32_t* iterations_out) {
+  if (!c || !d) return set_err(FL_ERR_INVALID, "null context or description");
+  if (d->n >= 0xffffffffull || d->nnz >= 0xffffffffull)
+    return set_err(FL_ERR_TOO_LARGE, "n=%llu, nnz=%llu exceed 2^32-2", (unsigned long long)d->n,
+                   (unsigned long long)d->nnz);
+  uint8_t* base = nullptr;
+  uint64_t size = 0;
+  std::mutex* call_mu = nullptr;
+  FLCHK(fl_noalloc_scratch(c->device, &base, &size, &call_mu));
+  bool clamp = false;
+  if (d->has_max_slope)
+    for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;
+  const uint64_t need = union_device_bytes(d->n, d->nnz, 1, clamp) + fl_msf_scratch_bytes((uint32_t)d->n, (uint32_t)d->nnz);
+  if (need > size)
+    return set_err(FL_ERR_TOO_LARGE,
+                   "fl_generate_f64_noalloc: this terrain (%llu sites, %llu entries) needs %llu bytes of device "
+                   "scratch; the static scratch holds %llu",
+                   (unsigned long long)d->n, (unsigned long long)d->nnz, (unsigned long long)need,
+                   (unsigned long long)size);
+  // the array is the device's, shared by every call on it
+  std::lock_guard<std::mutex> serial(*call_mu);
+  fl_arena region;
+  region.kind = fl_arena::REGION;
+  region.base = base;
+  region.size = size;
+  // launches only: a captured graph's instantiation may allocate
+  const bool graphs = c->graphs;
+  c->graphs = false;
+  fl_terrain* T = nullptr;
+  int32_t rc = create_union(c, d, 1, &T, &region);
+  if (rc == FL_OK) rc = fl_terrain_solve(c, T, d->max_iteration, iterations_out);
+  if (rc == FL_OK) rc = fl_terrain_download(c, T, elevations_out);
+  if (T) fl_terrain_destroy(T);
+  c->graphs = graphs;
   return rc;
 }
 

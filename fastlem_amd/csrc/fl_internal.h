@@ -10,8 +10,16 @@ struct fl_rng;
 // Set the calling thread's fl_last_error() message.
 void fl_internal_set_error(const char* msg);
 
+#include <mutex>
 #include <string>
 #include <vector>
+
+// fl_noalloc.hip: the static device scratch of fl_generate_f64_noalloc.
+// fl_noalloc_scratch_init takes its address on `device` (which must be
+// current), once; fl_noalloc_scratch returns it, its size and the mutex that
+// serialises the calls using it.
+int fl_noalloc_scratch_init(int device);
+int fl_noalloc_scratch(int device, uint8_t** ptr, uint64_t* bytes, std::mutex** call_mu);
 
 // Where device buffers come from.
 //   POOL:   the device's memory pool, stream-ordered (hipMallocAsync /

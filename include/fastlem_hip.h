@@ -81,6 +81,23 @@ int32_t fl_ctx_destroy(fl_ctx *ctx);
 int32_t fl_generate_f64(fl_ctx *ctx, const fl_terrain_desc *desc, double *elevations_out,
                         uint32_t *iterations_out);
 
+/* Allocation-free form of fl_generate_f64 (same arguments, same results).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, code that must not call hipMalloc).
+ * Every device buffer of the call (the terrain's arrays and the visit order's
+ * scratch) lives in a static device array of FL_NOALLOC_SCRATCH_BYTES that
+ * each GPU running the library holds until the process ends; the call makes
+ * no device or pinned-host allocation and frees nothing (host heap memory is
+ * still used for staging).  A terrain that needs more than the array holds
+ * (fl_request_device_bytes; about 120,000 sites at 6 adjacency entries per
+ * site) is refused with FL_ERR_TOO_LARGE before anything runs.  All calls on
+ * one GPU share its array and would race on it: the library serialises them
+ * within the process.  The context's device must be the one it was created
+ * on (fl_ctx_create takes the array's device address). */
+#define FL_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int32_t fl_generate_f64_noalloc(fl_ctx *ctx, const fl_terrain_desc *desc, double *elevations_out,
+                                uint32_t *iterations_out);
+
 /* Device-resident form: upload once, solve many times. */
 int32_t fl_terrain_create(fl_ctx *ctx, const fl_terrain_desc *desc, fl_terrain **out);
 int32_t fl_terrain_destroy(fl_terrain *t);

@@ -185,6 +185,16 @@ struct hipMemPoolProps {
   size_t maxSize;
   unsigned char reserved[56];
 };
+#ifndef __align__
+#define __align__(n) __attribute__((aligned(n)))
+#endif
+// device symbols are host globals in the emulation
+#define HIP_SYMBOL(X) X
+template <typename T>
+inline hipError_t hipGetSymbolAddress(void** p, const T& symbol) {
+  *p = (void*)&symbol;
+  return hipSuccess;
+}
 // pools: allocations come from the emulation's heap
 inline hipError_t hipMemPoolCreate(hipMemPool_t* pool, const hipMemPoolProps*) {
   static char tag;

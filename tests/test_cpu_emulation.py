@@ -272,3 +272,16 @@ def test_emulated_visit_order_matches_heap_traversal(emu_lib):
         visit_cases.check_all(emu_lib, ctx)
     finally:
         emu_lib.fl_ctx_destroy(ctx)
+
+
+def test_emulated_noalloc_matches_generate(emu_lib):
+    """fl_generate_f64_noalloc (static scratch) equals fl_generate_f64, two
+    calls in a row with a smaller second terrain; too large is refused."""
+    import noalloc_cases
+
+    ctx = ctypes.c_void_p()
+    assert emu_lib.fl_ctx_create(0, ctypes.byref(ctx)) == 0
+    try:
+        noalloc_cases.check_noalloc(emu_lib, ctx, scale=0.05)
+    finally:
+        emu_lib.fl_ctx_destroy(ctx)
