@@ -101,6 +101,9 @@ def _declare(lib):
         "fl_stage_basin_order": (i32, [vp, vp, c_f64p, u32, c_u32p, c_u32p]),
         "fl_stage_visit_order": (i32, [vp, vp, c_u32p]),
         "fl_request_device_bytes": (i32, [ctypes.POINTER(TerrainDesc), ctypes.POINTER(u64)]),
+        "fl_interp_create": (i32, [vp, u64, c_f64p, c_f64p, u64, c_u32p, ctypes.POINTER(vp)]),
+        "fl_interp_query": (i32, [vp, vp, c_f64p, u64, c_f64p, c_f64p, c_f64p, c_u8p]),
+        "fl_interp_destroy": (i32, [vp]),
         "fl_jitter_elevations": (None, [u64, sz, c_f64p, c_f64p]),
         "fl_random_sites": (None, [sz, f64, f64, f64, f64, c_f64p, c_f64p]),
         "fl_chacha_block": (None, [ctypes.c_int, c_u32p, c_u32p]),

@@ -848,6 +848,13 @@ extern "C" int32_t fl_ctx_create(int32_t device, fl_ctx** out) {
   return FL_OK;
 }
 
+int32_t fl_ctx_stream(fl_ctx* c, hipStream_t* stream, int* device) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  *stream = c->stream;
+  *device = c->device;
+  return FL_OK;
+}
+
 extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
   if (!c) return FL_OK;
   hipSetDevice(c->device);

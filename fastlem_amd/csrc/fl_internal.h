@@ -10,6 +10,9 @@ struct fl_rng;
 // Set the calling thread's fl_last_error() message.
 void fl_internal_set_error(const char* msg);
 
+// The context's stream and device (for the library's other parts).
+int32_t fl_ctx_stream(fl_ctx* c, hipStream_t* stream, int* device);
+
 #include <mutex>
 #include <string>
 #include <vector>

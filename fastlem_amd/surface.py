@@ -122,9 +122,12 @@ def build_adjacency(xs, ys, tris) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 
 
 class TerrainModel2D:
-    """Mirror of ``TerrainModel2D`` (model.rs:18-69) with a CSR graph."""
+    """Mirror of ``TerrainModel2D`` (model.rs:18-69) with a CSR graph.
+    ``triangles`` (optional, [t, 3]): the sites' Delaunay triangles, kept
+    from the build for the natural-neighbour interpolator of its terrains."""
 
-    def __init__(self, xs, ys, areas, row_ptr, col, length, default_outlets):
+    def __init__(self, xs, ys, areas, row_ptr, col, length, default_outlets, triangles=None):
+        self.triangles = None if triangles is None else np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
         self.xs = np.ascontiguousarray(xs, np.float64)
         self.ys = np.ascontiguousarray(ys, np.float64)
         self._areas = np.ascontiguousarray(areas, np.float64)
@@ -154,16 +157,77 @@ class TerrainModel2D:
         return [(int(self.col[k]), float(self.length[k])) for k in range(b, e)]
 
     def create_terrain_from_result(self, elevations) -> "Terrain2D":
-        return Terrain2D(self.xs, self.ys, np.asarray(elevations, np.float64))
+        """model.rs:62-68.  The reference builds the interpolator here, inside
+        generate(); this one builds it on the first query (the same
+        triangulation of the same sites, when the build kept it)."""
+        return Terrain2D(self.xs, self.ys, np.asarray(elevations, np.float64), self.triangles)
+
+
+class TerrainInterpolator2D:
+    """Mirror of ``TerrainInterpolator2D`` (interpolator.rs:6-28):
+    natural-neighbour (Sibson) interpolation over the sites, on the GPU (C ABI
+    ``fl_interp_*``).  Its triangulation is the given one or, as the
+    reference's ``Interpolator::new(sites)`` does, a Delaunay triangulation of
+    the sites."""
+
+    def __init__(self, xs, ys, triangles=None):
+        from .generator import context
+
+        self.xs = np.ascontiguousarray(xs, np.float64)
+        self.ys = np.ascontiguousarray(ys, np.float64)
+        if triangles is None:
+            triangles, _ = delaunay_triangles(self.xs, self.ys)
+        tri = np.ascontiguousarray(triangles, np.uint32).reshape(-1)
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().fl_interp_create(context(), len(self.xs), ptr(self.xs, ctypes.c_double),
+                                                      ptr(self.ys, ctypes.c_double), len(tri) // 3,
+                                                      ptr(tri, ctypes.c_uint32), ctypes.byref(h)))
+        self._h = h
+
+    def interpolate_many(self, elevations, qx, qy) -> tuple[np.ndarray, np.ndarray]:
+        """Values at the points (qx[k], qy[k]) and whether each has one
+        (False outside the sites' convex hull, where the value is NaN)."""
+        from .generator import context
+
+        z = np.ascontiguousarray(elevations, np.float64)
+        qx = np.ascontiguousarray(qx, np.float64).reshape(-1)
+        qy = np.ascontiguousarray(qy, np.float64).reshape(-1)
+        if len(qx) != len(qy) or len(z) != len(self.xs):
+            raise ValueError("query coordinates or elevations of the wrong length")
+        out = np.empty(len(qx), np.float64)
+        found = np.empty(len(qx), np.uint8)
+        _native.check(_native.lib().fl_interp_query(context(), self._h, ptr(z, ctypes.c_double), len(qx),
+                                                     ptr(qx, ctypes.c_double), ptr(qy, ctypes.c_double),
+                                                     ptr(out, ctypes.c_double), ptr(found, ctypes.c_uint8)))
+        return out, found.astype(bool)
+
+    def interpolate(self, elevations, site: Site2D) -> Optional[float]:
+        """interpolator.rs:17-27: None outside the convex hull."""
+        v, ok = self.interpolate_many(elevations, [site.x], [site.y])
+        return float(v[0]) if ok[0] else None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().fl_interp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Terrain2D:
-    """Mirror of ``Terrain2D`` (terrain.rs:8-39): sites and elevations."""
+    """Mirror of ``Terrain2D`` (terrain.rs:8-39): sites, elevations and the
+    interpolated elevation anywhere in the sites' convex hull."""
 
-    def __init__(self, xs, ys, elevations):
+    def __init__(self, xs, ys, elevations, triangles=None):
         self.xs = xs
         self.ys = ys
         self._elevations = elevations
+        self._triangles = triangles
+        self._interp: Optional[TerrainInterpolator2D] = None
 
     def sites(self) -> list[Site2D]:
         return [Site2D(float(x), float(y)) for x, y in zip(self.xs, self.ys)]
@@ -171,9 +235,21 @@ class Terrain2D:
     def elevations(self) -> np.ndarray:
         return self._elevations
 
+    def _interpolator(self) -> TerrainInterpolator2D:
+        if self._interp is None:
+            self._interp = TerrainInterpolator2D(self.xs, self.ys, self._triangles)
+        return self._interp
+
     def get_elevation(self, site: Site2D) -> Optional[float]:
-        raise NotImplementedError(
-            "natural-neighbour interpolation (terrain.rs:36-38) is a post-solve query outside this library")
+        """terrain.rs:36-38: the natural-neighbour interpolation of the
+        elevations at `site`, None outside the sites' convex hull."""
+        return self._interpolator().interpolate(self._elevations, site)
+
+    def get_elevations(self, xs, ys) -> np.ndarray:
+        """get_elevation for many points at once (one GPU launch): NaN where
+        get_elevation returns None."""
+        v, _ = self._interpolator().interpolate_many(self._elevations, xs, ys)
+        return v
 
 
 class TerrainModel2DBulider:
@@ -258,4 +334,4 @@ class TerrainModel2DBulider:
         tris, hull = delaunay_triangles(xs, ys)
         areas, _, _ = voronoi_cells(xs, ys, tris, bmin, bmax)
         row_ptr, col, length = build_adjacency(xs, ys, tris)
-        return TerrainModel2D(xs, ys, areas, row_ptr, col, length, hull.astype(np.uint32))
+        return TerrainModel2D(xs, ys, areas, row_ptr, col, length, hull.astype(np.uint32), tris)

@@ -16,6 +16,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -55,16 +56,85 @@ struct Graph {
   size_t order() const { return row_ptr.empty() ? 0 : row_ptr.size() - 1; }
 };
 
+// The calling thread's device context (HIP device 0), created on first use
+// and destroyed when the thread ends.  A context is one stream with its own
+// host slots and calls on one context must not overlap, so generate() calls
+// made at once from several threads each run on a context of their own (and
+// their work overlaps on the device), as separate reference generators run
+// independently.
+inline fl_ctx* default_context() {
+  struct Holder {
+    fl_ctx* ctx = nullptr;
+    ~Holder() {
+      if (ctx) fl_ctx_destroy(ctx);
+    }
+  };
+  thread_local Holder h;
+  if (!h.ctx) {
+    fl_ctx* c = nullptr;
+    if (fl_ctx_create(0, &c) != FL_OK) throw std::runtime_error(fl_last_error());
+    h.ctx = c;
+  }
+  return h.ctx;
+}
+
+// src/models/surface/terrain.rs:8-39: sites, elevations and get_elevation,
+// the natural-neighbour interpolation of the elevations
+// (src/models/surface/interpolator.rs:11-27) on the GPU.  The interpolator
+// is built on the first query from the model's Delaunay triangles (the
+// reference builds it inside generate()); copies share it.
 class Terrain2D {
  public:
-  Terrain2D(std::vector<Site2D> sites, std::vector<double> elevations)
-      : sites_(std::move(sites)), elevations_(std::move(elevations)) {}
+  Terrain2D(std::vector<Site2D> sites, std::vector<double> elevations,
+            std::shared_ptr<const std::vector<uint32_t>> triangles = nullptr)
+      : sites_(std::move(sites)), elevations_(std::move(elevations)), triangles_(std::move(triangles)) {}
   const std::vector<Site2D>& sites() const { return sites_; }
   const std::vector<double>& elevations() const { return elevations_; }
 
+  // None (std::nullopt) outside the sites' convex hull.
+  std::optional<double> get_elevation(const Site2D& site) const {
+    double v = 0.0;
+    uint8_t found = 0;
+    if (fl_interp_query(default_context(), interpolator(), elevations_.data(), 1, &site.x, &site.y, &v, &found) !=
+        FL_OK)
+      throw std::runtime_error(fl_last_error());
+    return found ? std::optional<double>(v) : std::nullopt;
+  }
+  // get_elevation for many points in one launch: NaN where it is None.
+  std::vector<double> get_elevations(const std::vector<Site2D>& points) const {
+    std::vector<double> xs(points.size()), ys(points.size()), out(points.size());
+    std::vector<uint8_t> found(points.size());
+    for (size_t k = 0; k < points.size(); k++) {
+      xs[k] = points[k].x;
+      ys[k] = points[k].y;
+    }
+    if (fl_interp_query(default_context(), interpolator(), elevations_.data(), points.size(), xs.data(), ys.data(),
+                        out.data(), found.data()) != FL_OK)
+      throw std::runtime_error(fl_last_error());
+    return out;
+  }
+
  private:
+  fl_interp* interpolator() const {
+    if (!interp_) {
+      if (!triangles_) throw std::runtime_error("get_elevation needs the model's triangles (TerrainModel2D::set_triangles)");
+      std::vector<double> xs(sites_.size()), ys(sites_.size());
+      for (size_t i = 0; i < sites_.size(); i++) {
+        xs[i] = sites_[i].x;
+        ys[i] = sites_[i].y;
+      }
+      fl_interp* h = nullptr;
+      if (fl_interp_create(default_context(), sites_.size(), xs.data(), ys.data(), triangles_->size() / 3,
+                           triangles_->data(), &h) != FL_OK)
+        throw std::runtime_error(fl_last_error());
+      interp_ = std::shared_ptr<fl_interp>(h, [](fl_interp* p) { fl_interp_destroy(p); });
+    }
+    return interp_.get();
+  }
   std::vector<Site2D> sites_;
   std::vector<double> elevations_;
+  std::shared_ptr<const std::vector<uint32_t>> triangles_;
+  mutable std::shared_ptr<fl_interp> interp_;
 };
 
 // src/models/surface/model.rs:18-69 (Model implementation).
@@ -79,11 +149,19 @@ class TerrainModel2D {
   const std::vector<double>& areas() const { return areas_; }
   const std::vector<uint32_t>& default_outlets() const { return default_outlets_; }
   const Graph& graph() const { return graph_; }
+  // The sites' Delaunay triangles (3 site indices each), for get_elevation.
+  TerrainModel2D set_triangles(std::vector<uint32_t> triangles) const {
+    auto m = *this;
+    m.triangles_ = std::make_shared<const std::vector<uint32_t>>(std::move(triangles));
+    return m;
+  }
+  // src/models/surface/model.rs:62-68
   Terrain2D create_terrain_from_result(const std::vector<double>& elevations) const {
-    return Terrain2D(sites_, elevations);
+    return Terrain2D(sites_, elevations, triangles_);
   }
 
  private:
+  std::shared_ptr<const std::vector<uint32_t>> triangles_;
   std::vector<Site2D> sites_;
   std::vector<double> areas_;
   Graph graph_;
@@ -124,28 +202,6 @@ class Result {
   std::optional<T> value_;
   std::optional<GenerationError> error_;
 };
-
-// The calling thread's device context (HIP device 0), created on first use
-// and destroyed when the thread ends.  A context is one stream with its own
-// host slots and calls on one context must not overlap, so generate() calls
-// made at once from several threads each run on a context of their own (and
-// their work overlaps on the device), as separate reference generators run
-// independently.
-inline fl_ctx* default_context() {
-  struct Holder {
-    fl_ctx* ctx = nullptr;
-    ~Holder() {
-      if (ctx) fl_ctx_destroy(ctx);
-    }
-  };
-  thread_local Holder h;
-  if (!h.ctx) {
-    fl_ctx* c = nullptr;
-    if (fl_ctx_create(0, &c) != FL_OK) throw std::runtime_error(fl_last_error());
-    h.ctx = c;
-  }
-  return h.ctx;
-}
 
 // src/lem/generator.rs:37-213
 template <typename M = TerrainModel2D>

@@ -215,6 +215,22 @@ int32_t fl_stage_visit_order(fl_ctx *ctx, fl_terrain *t, uint32_t *rank_out);
  * estimate).  No device is needed. */
 int32_t fl_request_device_bytes(const fl_terrain_desc *desc, uint64_t *bytes);
 
+/* Natural-neighbour (Sibson) interpolation of a terrain's elevations:
+ * Terrain2D::get_elevation (src/models/surface/terrain.rs:36-38) through
+ * TerrainInterpolator2D (src/models/surface/interpolator.rs:11-27).
+ * fl_interp_create takes the sites and their Delaunay triangles (ntri x 3
+ * site indices, either orientation) once; fl_interp_query evaluates m
+ * points on the device: out[k] = the interpolated value of values[] (one per
+ * site) at (qx[k], qy[k]) with found[k] = 1, or NaN with found[k] = 0 outside
+ * the sites' convex hull (the reference returns None there).  A query equal
+ * to a site returns that site's value. */
+typedef struct fl_interp fl_interp;
+int32_t fl_interp_create(fl_ctx *ctx, uint64_t n, const double *xs, const double *ys, uint64_t ntri,
+                         const uint32_t *triangles, fl_interp **out);
+int32_t fl_interp_query(fl_ctx *ctx, fl_interp *interp, const double *values, uint64_t m, const double *qx,
+                        const double *qy, double *out, uint8_t *found);
+int32_t fl_interp_destroy(fl_interp *interp);
+
 /* ---- host setup helpers (one-time, CPU) ---- */
 
 /* StdRng (rand 0.8.5 ChaCha12) restatement used for the jitter of

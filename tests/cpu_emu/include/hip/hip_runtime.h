@@ -127,6 +127,12 @@ inline T atomicMax(T* p, U v) {
   if (o < (T)v) *p = (T)v;
   return o;
 }
+template <class T, class U>
+inline T atomicOr(T* p, U v) {
+  T o = *p;
+  *p = (T)(o | (T)v);
+  return o;
+}
 template <class T, class U, class V>
 inline T atomicCAS(T* p, U cmp, V v) {
   T o = *p;
