@@ -226,6 +226,7 @@ def served_leg(terrains, slots: int, loads, budget: int) -> dict:
         x = {"offered_load": "saturation" if sat else load, "site_iter_per_s": work / span,
              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span,
              "peak_running": st["peak_running"], "peak_bytes": st["peak_bytes"],
+             "server": {k: v for k, v in st.items() if k not in ("peak_running", "peak_bytes")},
              "vs_plain_resident": (work / span) / (work / plain_s), "vs_plain_generate": (work / span) / (work / gen_s)}
         r["loads"].append(x)
         log(f"served load={x['offered_load']}: {x['site_iter_per_s']:.3e} site-it/s p50={x['p50_ms']:.1f}ms "

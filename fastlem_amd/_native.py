@@ -52,6 +52,15 @@ class SolveStats(ctypes.Structure):
                 ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32)]
 
 
+class ServerCounts(ctypes.Structure):
+    """Mirror of ``fl_server_counts``."""
+
+    _fields_ = [("joins_idle", ctypes.c_uint64), ("joins_busy", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("max_lane_terrains", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
+                ("prep_workers", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("prep_ms", ctypes.c_double),
+                ("step_ms", ctypes.c_double), ("join_ms", ctypes.c_double), ("idle_ms", ctypes.c_double)]
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"fastlem_hip error {code}: {message}")
@@ -88,6 +97,7 @@ def _declare(lib):
         "fl_server_wait": (i32, [vp, u64, c_u32p, ctypes.POINTER(f64), ctypes.POINTER(f64)]),
         "fl_server_stats": (i32, [vp, ctypes.POINTER(u64), c_u32p]),
         "fl_server_destroy": (i32, [vp]),
+        "fl_server_counters": (i32, [vp, ctypes.POINTER(ServerCounts)]),
         "fl_terrain_last_stats": (i32, [vp, ctypes.POINTER(SolveStats)]),
         "fl_ctx_set_profiling": (i32, [vp, i32]),
         "fl_ctx_set_batch_compaction": (i32, [vp, u32]),

@@ -75,6 +75,12 @@ struct fl_ctx {
   uint32_t* host_slot[2] = {nullptr, nullptr};
   hipEvent_t slot_ev[2] = {nullptr, nullptr};
   std::vector<uint32_t> host_boff;
+  // pinned per-terrain slots of batches (changed flags, flags of the two
+  // flights, active mask), shared by the unions solved on this context (one
+  // at a time); grown by new blocks, the old ones kept until the context goes
+  uint8_t* pin = nullptr;
+  uint32_t pin_cap = 0;
+  std::vector<void*> pin_blocks;
 };
 
 struct fl_terrain {
@@ -122,7 +128,7 @@ struct fl_terrain {
   uint32_t* d_tid = nullptr;      // [n] terrain of each site (batches)
   uint8_t* d_active = nullptr;    // [nterr] terrains still iterating
   uint32_t* d_tchanged = nullptr; // [nterr] per-terrain changed flags
-  uint32_t* h_tchanged = nullptr; // pinned copy
+  uint32_t* h_tchanged = nullptr; // pinned copy (the context's slots)
 };
 
 namespace {
@@ -865,6 +871,7 @@ extern "C" int32_t fl_ctx_destroy(fl_ctx* c) {
     if (c->host_slot[q]) hipHostFree(c->host_slot[q]);
     if (c->slot_ev[q]) hipEventDestroy(c->slot_ev[q]);
   }
+  for (void* b : c->pin_blocks) hipHostFree(b);
   if (c->pool) release_pool(c->device);
   delete c;
   return FL_OK;
@@ -883,10 +890,6 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   } else {
     for (void* p : T->allocations) hipFree(p);
   }
-  if (T->h_tchanged) hipHostFree(T->h_tchanged);
-  for (uint32_t* q : T->h_tslot)
-    if (q) hipHostFree(q);
-  if (T->h_mask) hipHostFree(T->h_mask);
   delete T;
   return FL_OK;
 }
@@ -991,9 +994,18 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
   FLCHK(layout_device(P, n, nnz, count, clamp));
   fl_dev_work& w = P->w;
   if (count > 1) {
-    HIPCHK(hipHostMalloc((void**)&P->h_tchanged, count * sizeof(uint32_t), hipHostMallocDefault));
-    for (uint32_t*& q : P->h_tslot) HIPCHK(hipHostMalloc((void**)&q, count * sizeof(uint32_t), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&P->h_mask, count, hipHostMallocDefault));
+    if (count > c->pin_cap) {
+      const uint32_t cap = std::max<uint32_t>({count, 2 * c->pin_cap, 1024});
+      void* block = nullptr;
+      HIPCHK(hipHostMalloc(&block, (size_t)cap * 13, hipHostMallocDefault));
+      c->pin_blocks.push_back(block);
+      c->pin = (uint8_t*)block;
+      c->pin_cap = cap;
+    }
+    P->h_tchanged = (uint32_t*)c->pin;
+    P->h_tslot[0] = P->h_tchanged + c->pin_cap;
+    P->h_tslot[1] = P->h_tslot[0] + c->pin_cap;
+    P->h_mask = (uint8_t*)(P->h_tslot[1] + c->pin_cap);
   }
   hipStream_t s = c->stream;
   fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
@@ -1211,7 +1223,7 @@ int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t*
   FLCHK(dev_alloc(N.get(), &d_map, map.size()));
   HIPCHK(hipMemcpyAsync(d_map, map.data(), map.size() * 4, hipMemcpyHostToDevice, s));
   fl_compact_map m{G, d_map, d_map + G + 1, d_map + 2 * G + 1, d_map + 3 * G + 1};
-  fl_launch_compact(s, m, cur->t, cur->w, N->t, N->w, N->d_tid);
+  fl_launch_compact(s, m, cur->t, cur->w, N->t, N->w, N->d_tid, 0, n, 0);
   HIPCHK(hipMemcpyAsync(N->d_active, N->t_active.data(), G, hipMemcpyHostToDevice, s));
   N->have_order = true;
   N->jump_rounds = cur->jump_rounds;
@@ -1391,6 +1403,339 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- lanes ---
+//
+// A lane iterates a union whose terrains change at iteration boundaries (the
+// serving loop's continuous batching).  Prepared terrains join it: the
+// terrains still running and the joining ones are gathered into a new union
+// on the device, and the first iteration after a join runs synchronously
+// (the joining terrains' first iteration joins lakes).  A terrain that stops
+// (no elevation changed, or its max_iteration reached) has its elevations
+// copied out at once and leaves the union at the next gather, which also
+// happens when stopped terrains hold half of the union.  Each terrain sees
+// exactly the iterations of its own solve, so its result equals its solo run.
+
+struct fl_lane {
+  fl_ctx* c = nullptr;
+  fl_terrain* cur = nullptr;  // owned; null when idle
+  std::vector<fl_lane_member> mem;  // per terrain of cur
+  int hin = 0;
+  std::deque<Flight> q;
+  std::vector<uint32_t> enq;
+  std::vector<uint8_t> dev_mask;
+  int next_slot = 0;
+  bool sync_next = true;
+  bool drain = false;          // enqueue nothing ahead: the lane is to become quiet
+  bool compact_wanted = false;  // gather (drop the stopped terrains) once quiet
+  uint32_t active = 0;
+  uint64_t active_sites = 0;
+  uint32_t gathers = 0;
+};
+
+namespace {
+
+int32_t lane_upload_mask(fl_lane* L, const std::vector<uint8_t>& m) {
+  fl_terrain* cur = L->cur;
+  if (m == L->dev_mask) return FL_OK;
+  std::copy(m.begin(), m.end(), cur->h_mask);
+  HIPCHK(hipMemcpyAsync(cur->d_active, cur->h_mask, cur->nterr, hipMemcpyHostToDevice, L->c->stream));
+  L->dev_mask = m;
+  return FL_OK;
+}
+
+// The lane's running terrains and the joining ones gathered into a new
+// union (device to device); the old union and the joining terrains' own
+// objects are released.
+int32_t lane_gather(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_member>>& join) {
+  fl_ctx* c = L->c;
+  hipStream_t s = c->stream;
+  fl_terrain* cur = L->cur;
+  struct Part {
+    fl_terrain* src;
+    uint32_t g;  // terrain of src
+  };
+  std::vector<Part> parts;
+  std::vector<fl_lane_member> mem;
+  if (cur) {
+    cur->w.h = cur->hb[L->hin];  // the last confirmed elevations
+    for (uint32_t g = 0; g < cur->nterr; g++)
+      if (cur->t_active[g]) {
+        parts.push_back({cur, g});
+        mem.push_back(std::move(L->mem[g]));
+        mem.back().iterations = cur->t_iters[g];
+      }
+  }
+  for (auto& j : join) {
+    parts.push_back({j.first, 0});
+    mem.push_back(std::move(j.second));
+    mem.back().iterations = 0;
+  }
+  const uint32_t G = (uint32_t)parts.size();
+  std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> N(new fl_terrain(), fl_terrain_destroy);
+  N->ctx = c;
+  N->arena.pool = c->pool;
+  N->nterr = G;
+  N->site_off.assign(G + 1, 0);
+  N->entry_off.assign(G + 1, 0);
+  N->outlet_off.assign(G + 1, 0);
+  N->t_max_it.resize(G);
+  N->t_iters.resize(G);
+  N->t_active.assign(G, 1);
+  N->t_asym.assign(G, 0);
+  bool clamp = false;
+  for (uint32_t q = 0; q < G; q++) {
+    const Part& p = parts[q];
+    fl_terrain* S = p.src;
+    N->site_off[q + 1] = N->site_off[q] + (S->site_off[p.g + 1] - S->site_off[p.g]);
+    N->entry_off[q + 1] = N->entry_off[q] + (S->entry_off[p.g + 1] - S->entry_off[p.g]);
+    N->outlet_off[q + 1] = N->outlet_off[q] + (S->outlet_off[p.g + 1] - S->outlet_off[p.g]);
+    N->t_max_it[q] = mem[q].cap;
+    N->t_iters[q] = mem[q].iterations;
+    N->t_asym[q] = S->t_asym.empty() ? 0 : S->t_asym[p.g];
+    clamp |= S->t.any_max_slope;
+  }
+  const uint32_t n = N->site_off[G], nnz = N->entry_off[G];
+  if (G == 0 || n == 0) {
+    // nothing left to iterate (or only empty terrains, which stop at once)
+    if (cur) fl_terrain_destroy(cur);
+    L->cur = nullptr;
+    for (auto& j : join) fl_terrain_destroy(j.first);
+    L->mem.clear();
+    L->active = 0;
+    L->active_sites = 0;
+    return FL_OK;
+  }
+  FLCHK(alloc_device(c, N.get(), n, nnz, G > 1 ? G : 2, clamp));
+  // one map per source union: its parts are contiguous in the new union
+  std::vector<uint32_t> maps;
+  std::vector<std::pair<size_t, std::pair<uint32_t, uint32_t>>> launches;  // map offset, part range [q0, q1)
+  for (uint32_t q0 = 0; q0 < G;) {
+    uint32_t q1 = q0 + 1;
+    while (q1 < G && parts[q1].src == parts[q0].src) q1++;
+    const uint32_t k = q1 - q0;
+    const size_t off = maps.size();
+    maps.resize(off + 4 * (size_t)k + 1);
+    uint32_t* m = maps.data() + off;
+    fl_terrain* S = parts[q0].src;
+    for (uint32_t i = 0; i <= k; i++) m[i] = N->site_off[q0 + i];
+    for (uint32_t i = 0; i < k; i++) {
+      m[k + 1 + i] = S->site_off[parts[q0 + i].g];
+      m[2 * k + 1 + i] = N->entry_off[q0 + i];
+      m[3 * k + 1 + i] = S->entry_off[parts[q0 + i].g];
+    }
+    launches.push_back({off, {q0, q1}});
+    q0 = q1;
+  }
+  uint32_t* d_map = nullptr;
+  FLCHK(dev_alloc(N.get(), &d_map, maps.size()));
+  HIPCHK(hipMemcpyAsync(d_map, maps.data(), maps.size() * 4, hipMemcpyHostToDevice, s));
+  for (const auto& l : launches) {
+    const uint32_t q0 = l.second.first, q1 = l.second.second, k = q1 - q0;
+    uint32_t* m = d_map + l.first;
+    fl_compact_map cm{k, m, m + k + 1, m + 2 * k + 1, m + 3 * k + 1};
+    fl_terrain* S = parts[q0].src;
+    if (N->site_off[q1] > N->site_off[q0])
+      fl_launch_compact(s, cm, S->t, S->w, N->t, N->w, N->d_tid, N->site_off[q0], N->site_off[q1], q0);
+  }
+  HIPCHK(hipMemcpyAsync(N->d_active, N->t_active.data(), G, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // `maps` is pageable; the sources are released below
+  N->have_order = true;
+  N->jump_rounds = cur ? cur->jump_rounds : 4;
+  N->last_rounds = cur ? cur->last_rounds : 0;
+  if (cur) fl_terrain_destroy(cur);
+  for (auto& j : join) {
+    j.first->ctx = c;  // its buffers were last used on its own (synchronised) stream: free them on this one
+    fl_terrain_destroy(j.first);
+  }
+  L->cur = N.release();
+  L->mem = std::move(mem);
+  L->hin = 0;
+  L->enq = L->cur->t_iters;
+  L->dev_mask = L->cur->t_active;
+  L->sync_next = !join.empty();
+  L->drain = false;
+  L->compact_wanted = false;
+  L->active = G;
+  L->active_sites = n;
+  L->gathers++;
+  return FL_OK;
+}
+
+// Terrain g of the lane's union stopped: its elevations (device buffer
+// `buf`) into member g's host copy, in the caller's site order.
+int32_t lane_copy_out(fl_lane* L, uint32_t g, const double* buf) {
+  fl_terrain* cur = L->cur;
+  fl_lane_member& m = L->mem[g];
+  const uint32_t s0 = cur->site_off[g], ng = cur->site_off[g + 1] - s0;
+  m.result.resize(ng);
+  if (!ng) return FL_OK;
+  std::vector<double> tmp(ng);
+  HIPCHK(hipMemcpyAsync(tmp.data(), buf + s0, (uint64_t)ng * 8, hipMemcpyDeviceToHost, L->c->stream));
+  HIPCHK(hipStreamSynchronize(L->c->stream));
+  for (uint32_t i = 0; i < ng; i++) m.result[i] = tmp[m.new_of_old[i]];
+  return FL_OK;
+}
+
+}  // namespace
+
+fl_lane* fl_lane_create(fl_ctx* c) {
+  fl_lane* L = new fl_lane();
+  L->c = c;
+  return L;
+}
+
+void fl_lane_destroy(fl_lane* L) {
+  if (!L) return;
+  if (L->cur) {
+    hipStreamSynchronize(L->c->stream);
+    fl_terrain_destroy(L->cur);
+  }
+  delete L;
+}
+
+int32_t fl_lane_prepare(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out, std::vector<uint32_t>* new_of_old) {
+  *out = nullptr;
+  HIPCHK(hipSetDevice(c->device));
+  fl_terrain* T = nullptr;
+  FLCHK(create_union(c, d, 1, &T));
+  int32_t rc = T->t.n ? ensure_visit_order(c, T) : FL_OK;
+  if (rc != FL_OK) {
+    fl_terrain_destroy(T);
+    return rc;
+  }
+  *new_of_old = std::move(T->new_of_old);
+  *out = T;
+  return FL_OK;
+}
+
+uint64_t fl_lane_active_sites(const fl_lane* L) { return L->active_sites; }
+uint32_t fl_lane_terrains(const fl_lane* L) { return L->cur ? L->cur->nterr : 0; }
+bool fl_lane_idle(const fl_lane* L) { return L->active == 0; }
+bool fl_lane_quiet(const fl_lane* L) { return L->q.empty(); }
+void fl_lane_set_drain(fl_lane* L) { L->drain = true; }
+
+int32_t fl_lane_join(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_member>>& join) {
+  HIPCHK(hipSetDevice(L->c->device));
+  if (!L->q.empty()) return set_err(FL_ERR_INVALID, "a lane joins terrains only between iterations");
+  return lane_gather(L, join);
+}
+
+int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
+  fl_ctx* c = L->c;
+  hipStream_t s = c->stream;
+  HIPCHK(hipSetDevice(c->device));
+  if (!L->cur || L->active == 0) return FL_OK;
+  fl_terrain* cur = L->cur;
+  std::vector<uint32_t> stop_now;
+  // account one confirmed iteration of the terrains in `mask`
+  auto confirm = [&](const std::vector<uint8_t>& mask, const uint32_t* tflags) {
+    for (uint32_t g = 0; g < cur->nterr; g++) {
+      if (!mask[g] || !cur->t_active[g]) continue;
+      cur->t_iters[g]++;
+      const bool ch = tflags[g] != 0;
+      if (!ch || cur->t_iters[g] >= cur->t_max_it[g]) {
+        cur->t_active[g] = 0;
+        L->active--;
+        L->active_sites -= cur->site_off[g + 1] - cur->site_off[g];
+        stop_now.push_back(g);
+      }
+    }
+  };
+  const double* result = nullptr;  // the buffer holding the confirmed iteration's elevations
+  if (L->sync_next || !c->pipelined) {
+    std::vector<uint8_t> mask(cur->t_active);
+    FLCHK(lane_upload_mask(L, mask));
+    cur->w.h = cur->hb[L->hin];
+    cur->w.h2 = cur->hb[(L->hin + 1) % 3];
+    bool changed = false, lake = false;
+    FLCHK(one_iteration(c, cur, &changed, &lake));
+    for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
+    confirm(mask, cur->h_tchanged);
+    L->hin = (L->hin + 1) % 3;
+    result = cur->hb[L->hin];
+    L->sync_next = false;
+  } else {
+    // a drain for a join whose terrains went to another lane: go on
+    if (L->drain && L->q.empty() && !L->compact_wanted) L->drain = false;
+    while (!L->drain && L->q.size() < 2) {
+      std::vector<uint8_t> mask(cur->nterr);
+      bool any = false;
+      for (uint32_t g = 0; g < cur->nterr; g++) {
+        mask[g] = cur->t_active[g] && L->enq[g] < cur->t_max_it[g];
+        any |= mask[g] != 0;
+      }
+      if (!any) break;
+      FLCHK(lane_upload_mask(L, mask));
+      Flight f;
+      f.in = L->q.empty() ? L->hin : (L->q.back().in + 1) % 3;
+      f.slot = L->next_slot;
+      L->next_slot ^= 1;
+      f.mask = mask;
+      FLCHK(enqueue_fast(c, cur, f));
+      for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
+      L->q.push_back(std::move(f));
+    }
+    if (L->q.empty()) return FL_OK;
+    Flight f = std::move(L->q.front());
+    L->q.pop_front();
+    bool valid = false, changed = false;
+    FLCHK(finish_fast(c, cur, f, &valid, &changed));
+    const uint32_t* tflags = cur->h_tslot[f.slot];
+    if (!valid) {
+      // the flights after it started from a wrong input: let them land, then
+      // redo this iteration synchronously on its own input and mask
+      HIPCHK(hipStreamSynchronize(s));
+      for (const Flight& x : L->q)
+        for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] -= x.mask[g];
+      L->q.clear();
+      cur->stat_redone++;
+      bool lake = false;
+      FLCHK(lane_upload_mask(L, f.mask));
+      cur->w.h = cur->hb[f.in];
+      cur->w.h2 = cur->hb[(f.in + 1) % 3];
+      FLCHK(one_iteration(c, cur, &changed, &lake));
+      tflags = cur->h_tchanged;
+    }
+    confirm(f.mask, tflags);
+    L->hin = (f.in + 1) % 3;
+    result = cur->hb[L->hin];
+  }
+  // stopped terrains: their elevations out now (the buffer of the confirmed
+  // iteration is rewritten only two enqueues later)
+  for (uint32_t g : stop_now) {
+    FLCHK(lane_copy_out(L, g, result));
+    fl_lane_member m = std::move(L->mem[g]);
+    L->mem[g].tag = m.tag;  // the slot stays until the next gather
+    m.iterations = cur->t_iters[g];
+    stopped.push_back(std::move(m));
+  }
+  // gather when the stopped terrains hold half of the union
+  if (L->active && L->active_sites * 2 < cur->t.n && cur->t.n - L->active_sites >= c->compact_min_idle) {
+    L->drain = true;
+    L->compact_wanted = true;
+  }
+  if (L->active == 0) {
+    // nothing left: release the union once the last flights have landed
+    HIPCHK(hipStreamSynchronize(s));
+    L->q.clear();
+    fl_terrain_destroy(cur);
+    L->cur = nullptr;
+    L->mem.clear();
+    L->active_sites = 0;
+  } else if (L->compact_wanted && L->q.empty()) {
+    std::vector<std::pair<fl_terrain*, fl_lane_member>> none;
+    FLCHK(lane_gather(L, none));
+  }
+  return FL_OK;
+}
+
+uint64_t fl_terrain_arrays_bytes(const fl_terrain_desc* d) {
+  bool clamp = false;
+  if (d->has_max_slope)
+    for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;
+  return union_device_bytes(d->n, d->nnz, 2, clamp);
+}
 
 extern "C" int32_t fl_terrain_create(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out) {
   if (!d) return set_err(FL_ERR_INVALID, "null terrain description");

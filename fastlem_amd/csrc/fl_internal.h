@@ -79,6 +79,38 @@ void fl_device_order(hipStream_t s, uint32_t n, const uint32_t* rank, uint32_t* 
 // fl_device_scatter_u32 (up to n pairs) for n sites and nnz entries.
 uint64_t fl_msf_scratch_bytes(uint32_t n, uint32_t nnz);
 
+// fl_capi.cpp: lanes (continuous batching, used by the serving loop).  A
+// member is one request's terrain in a lane: its caller tag, max_iteration,
+// the relabelling of its sites and, once it stopped, its iteration count and
+// elevations in the caller's order.
+struct fl_lane_member {
+  uint64_t tag = 0;
+  uint32_t cap = 0xffffffffu;
+  uint32_t iterations = 0;
+  std::vector<uint32_t> new_of_old;
+  std::vector<double> result;
+};
+struct fl_lane;
+struct fl_terrain;
+fl_lane* fl_lane_create(fl_ctx* c);
+void fl_lane_destroy(fl_lane* lane);
+// A terrain ready to join a lane: validated, relabelled, uploaded, with its
+// visit order (on context c, synchronised).
+int32_t fl_lane_prepare(fl_ctx* c, const fl_terrain_desc* d, fl_terrain** out, std::vector<uint32_t>* new_of_old);
+// Between iterations only (fl_lane_quiet): the prepared terrains join the lane.
+int32_t fl_lane_join(fl_lane* lane, std::vector<std::pair<fl_terrain*, fl_lane_member>>& join);
+// One confirmed iteration; members that stopped in it are appended to `stopped`.
+int32_t fl_lane_step(fl_lane* lane, std::vector<fl_lane_member>& stopped);
+bool fl_lane_idle(const fl_lane* lane);
+bool fl_lane_quiet(const fl_lane* lane);
+// Stop enqueueing ahead, so that the lane becomes quiet (to join terrains).
+void fl_lane_set_drain(fl_lane* lane);
+uint64_t fl_lane_active_sites(const fl_lane* lane);
+uint32_t fl_lane_terrains(const fl_lane* lane);
+// HBM the union arrays of one terrain take (fl_request_device_bytes without
+// the visit order's scratch).
+uint64_t fl_terrain_arrays_bytes(const fl_terrain_desc* d);
+
 // fl_setup.cpp: per-terrain forms on a union graph whose terrain occupies
 // sites [s0, s0 + nl) (its rows reference only those sites).
 int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,

@@ -158,9 +158,12 @@ struct fl_compact_map {
   const uint32_t* old_eoff;  // [count]
 };
 // Copies the graph, parameters, visit ranks and order, current elevations
-// and drainage areas (shifting site and entry indices) and writes new_tid.
+// and drainage areas (shifting site and entry indices) and writes new_tid,
+// for the new sites [v0, v1): the terrains of the map, all from one source
+// union, which become terrains tid0, tid0 + 1, ... of the new union.
 void fl_launch_compact(hipStream_t s, const fl_compact_map& m, const fl_dev_terrain& ot, const fl_dev_work& ow,
-                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid);
+                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid, uint32_t v0, uint32_t v1,
+                       uint32_t tid0);
 // Verification: the reference's BFS traversal of the basin of `outlet`.
 void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t outlet,
                            uint32_t* order, uint32_t* len_out);

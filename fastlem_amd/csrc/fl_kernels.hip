@@ -1320,10 +1320,14 @@ __device__ __forceinline__ uint32_t shift_site(uint32_t x, const fl_compact_map&
 }
 }  // namespace
 
-__global__ void k_compact_sites(uint32_t n_new, uint32_t nnz_new, fl_compact_map m, fl_dev_terrain ot,
-                                fl_dev_work ow, fl_dev_terrain nt, fl_dev_work nw, uint32_t* __restrict__ new_tid) {
-  uint32_t v = gid();
-  if (v >= n_new) return;
+// New sites [v0, v1) of a union of n_new sites and nnz_new entries, taken
+// from the terrains of one source union (m, terrain q of the map becomes
+// terrain tid0 + q of the new union).
+__global__ void k_compact_sites(uint32_t v0, uint32_t v1, uint32_t n_new, uint32_t nnz_new, uint32_t tid0,
+                                fl_compact_map m, fl_dev_terrain ot, fl_dev_work ow, fl_dev_terrain nt,
+                                fl_dev_work nw, uint32_t* __restrict__ new_tid) {
+  uint32_t v = v0 + gid();
+  if (v >= v1) return;
   uint32_t q = terrain_of(m, v);
   uint32_t src = v - m.new_off[q] + m.old_off[q];
   nt.row_ptr[v] = ot.row_ptr[src] - m.old_eoff[q] + m.new_eoff[q];
@@ -1338,12 +1342,13 @@ __global__ void k_compact_sites(uint32_t n_new, uint32_t nnz_new, fl_compact_map
   nt.order[v] = shift_site(ot.order[src], m, q);  // rank positions share the terrain's site range
   nw.h[v] = ow.h[src];
   nw.A[v] = ow.A[src];
-  new_tid[v] = q;
+  new_tid[v] = tid0 + q;
 }
 
-__global__ void k_compact_entries(uint32_t n_new, fl_compact_map m, fl_dev_terrain ot, fl_dev_terrain nt) {
-  uint32_t v = gid();
-  if (v >= n_new) return;
+__global__ void k_compact_entries(uint32_t v0, uint32_t v1, fl_compact_map m, fl_dev_terrain ot,
+                                  fl_dev_terrain nt) {
+  uint32_t v = v0 + gid();
+  if (v >= v1) return;
   uint32_t q = terrain_of(m, v);
   uint32_t src = v - m.new_off[q] + m.old_off[q];
   uint32_t b = ot.row_ptr[src], e = ot.row_ptr[src + 1];
@@ -1609,7 +1614,8 @@ void fl_launch_basin_order(hipStream_t s, const fl_dev_terrain& t, const fl_dev_
 }
 
 void fl_launch_compact(hipStream_t s, const fl_compact_map& m, const fl_dev_terrain& ot, const fl_dev_work& ow,
-                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid) {
-  FL_LAUNCH(k_compact_sites, nt.n, s, nt.n, nt.nnz, m, ot, ow, nt, nw, new_tid);
-  FL_LAUNCH(k_compact_entries, nt.n, s, nt.n, m, ot, nt);
+                       const fl_dev_terrain& nt, const fl_dev_work& nw, uint32_t* new_tid, uint32_t v0, uint32_t v1,
+                       uint32_t tid0) {
+  FL_LAUNCH(k_compact_sites, v1 - v0, s, v0, v1, nt.n, nt.nnz, tid0, m, ot, ow, nt, nw, new_tid);
+  FL_LAUNCH(k_compact_entries, v1 - v0, s, v0, v1, m, ot, nt);
 }

@@ -1,12 +1,13 @@
 """Serving: generate() requests through the library's continuous-batching
 request loop (C ABI ``fl_server_*``).
 
-``Server(slots, hbm_budget_bytes)`` starts ``slots`` native worker threads,
-each with its own HIP stream.  ``submit`` enqueues a request and returns a
-ticket at once; a request is admitted as soon as a slot is free and its HBM
-estimate fits the budget, so work of requests arriving while others run
-overlaps on the device.  ``wait`` returns (elevations, iterations, queue_ms,
-latency_ms).
+``Server(slots, hbm_budget_bytes)`` starts native workers on up to four
+streams: preparation workers (validation, upload, visit order) and lanes,
+each iterating a union of terrains.  ``submit`` enqueues a request and
+returns a ticket at once; a request is admitted as soon as its HBM estimate
+fits the budget, and once prepared it joins a lane at the lane's next
+iteration boundary, sharing the launches of the terrains already running.
+``wait`` returns (elevations, iterations, queue_ms, latency_ms).
 """
 from __future__ import annotations
 
@@ -82,7 +83,13 @@ class Server:
         b = ctypes.c_uint64()
         r = ctypes.c_uint32()
         check(_native.lib().fl_server_stats(self._h, ctypes.byref(b), ctypes.byref(r)))
-        return {"peak_bytes": b.value, "peak_running": r.value}
+        c = _native.ServerCounts()
+        check(_native.lib().fl_server_counters(self._h, ctypes.byref(c)))
+        out = {"peak_bytes": b.value, "peak_running": r.value, "joins_idle_lane": c.joins_idle,
+               "joins_busy_lane": c.joins_busy, "max_lane_terrains": c.max_lane_terrains}
+        out.update({f: getattr(c, f) for f in ("steps", "lanes", "prep_workers", "prep_ms", "step_ms", "join_ms",
+                                                "idle_ms")})
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

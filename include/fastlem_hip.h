@@ -136,13 +136,18 @@ int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
  * graph.  Results do not depend on the mode. */
 int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
 
-/* Serving: a request loop on one GPU with continuous batching.  `slots`
- * worker threads each own a context (stream); a request is admitted to a free
- * slot as soon as the HBM it needs fits in hbm_budget_bytes (first come,
- * first served), so requests arriving while others run overlap with them on
- * the device.  Each request runs exactly fl_generate_f64.  The caller keeps
- * the description's arrays and elevations_out alive until fl_server_wait
- * returns for that ticket.  fl_server_destroy drains the queue. */
+/* Serving: a request loop on one GPU with continuous batching.  A request
+ * is admitted (first come, first served) as soon as the HBM it needs fits in
+ * hbm_budget_bytes (fl_request_device_bytes plus a second copy of its
+ * arrays, held while it joins a batch); a preparation worker validates,
+ * uploads and orders it; it then joins a lane at the lane's next iteration
+ * boundary.  A lane iterates a union of terrains in shared launches
+ * (fl_batch_* semantics: every terrain stops at its own convergence or
+ * max_iteration, so its result equals its solo fl_generate_f64), and a
+ * terrain leaves as soon as it stops.  `slots` streams are used in all (at
+ * most 4: half lanes, half preparation).  The caller keeps the description's
+ * arrays and elevations_out alive until fl_server_wait returns for that
+ * ticket.  fl_server_destroy drains the queue. */
 typedef struct fl_server fl_server;
 int32_t fl_server_create(int32_t device, uint32_t slots, uint64_t hbm_budget_bytes, fl_server **out);
 int32_t fl_server_submit(fl_server *server, const fl_terrain_desc *desc, double *elevations_out, uint64_t *ticket);
@@ -152,6 +157,16 @@ int32_t fl_server_wait(fl_server *server, uint64_t ticket, uint32_t *iterations_
                        double *latency_ms);
 int32_t fl_server_stats(fl_server *server, uint64_t *peak_bytes, uint32_t *peak_running);
 int32_t fl_server_destroy(fl_server *server);
+/* What the server did so far: terrains that joined an idle lane / a lane
+ * already iterating others, the most terrains one lane iterated together,
+ * and wall time summed over workers (preparation, lane iterations, joins,
+ * lanes waiting for prepared terrains). */
+typedef struct fl_server_counts {
+  uint64_t joins_idle, joins_busy, steps;
+  uint32_t max_lane_terrains, lanes, prep_workers, reserved;
+  double prep_ms, step_ms, join_ms, idle_ms;
+} fl_server_counts;
+int32_t fl_server_counters(fl_server *server, fl_server_counts *out);
 
 /* Statistics of the last fl_terrain_solve on a terrain. */
 typedef struct fl_solve_stats {

@@ -285,3 +285,66 @@ def test_emulated_noalloc_matches_generate(emu_lib):
         noalloc_cases.check_noalloc(emu_lib, ctx, scale=0.05)
     finally:
         emu_lib.fl_ctx_destroy(ctx)
+
+
+def test_emulated_server_continuous_batching(emu_lib):
+    """The serving loop through the CPU emulation: requests of mixed sizes,
+    caps (0 and small) and max slopes, submitted with gaps and under a budget
+    that holds only part of them, join lanes at iteration boundaries; every
+    result and iteration count equals the oracle's solo run."""
+    import time
+
+    from oracle import flo
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.serving import request
+    from fastlem_amd.workloads import config_fbm, config_plain, terrain_stream
+
+    from cpu_emu import emu
+
+    terrains = terrain_stream(8, 200, 1500, 77, "ocean", workers=1)
+    small = config_plain(300, 17.0, 1.0)
+    ms = np.full(small.n, np.nan)
+    ms[: small.n // 2] = 0.6
+    terrains.append(dict(DeviceBatch.terrain_of(small), max_slope=ms))
+    terrains.append(DeviceBatch.terrain_of(config_fbm(900, 30.0, seed=4), max_iteration=3))
+    terrains.append(DeviceBatch.terrain_of(config_plain(250, 16.0, 1.0), max_iteration=0))
+    L = emu_lib
+    need = []
+    reqs = []
+    for t in terrains:
+        desc, keep, out = request(t)
+        b = ctypes.c_uint64()
+        assert L.fl_request_device_bytes(ctypes.byref(desc), ctypes.byref(b)) == 0
+        need.append(b.value)
+        reqs.append((desc, keep, out))
+    budget = 3 * max(need) + 2 * sorted(need)[len(need) // 2]
+    srv = ctypes.c_void_p()
+    assert L.fl_server_create(0, 4, budget, ctypes.byref(srv)) == 0
+    tickets = []
+    try:
+        for k, (desc, keep, out) in enumerate(reqs):
+            if k % 3 == 2:
+                time.sleep(0.05)
+            tk = ctypes.c_uint64()
+            emu.check(L, L.fl_server_submit(srv, ctypes.byref(desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            ctypes.byref(tk)))
+            tickets.append(tk.value)
+        its = []
+        for tk in tickets:
+            it = ctypes.c_uint32()
+            qms = ctypes.c_double()
+            lat = ctypes.c_double()
+            emu.check(L, L.fl_server_wait(srv, tk, ctypes.byref(it), ctypes.byref(qms), ctypes.byref(lat)))
+            its.append(it.value)
+        pb = ctypes.c_uint64()
+        pr = ctypes.c_uint32()
+        assert L.fl_server_stats(srv, ctypes.byref(pb), ctypes.byref(pr)) == 0
+        assert pb.value <= budget and pr.value >= 2
+    finally:
+        L.fl_server_destroy(srv)
+    for t, (desc, keep, out), it in zip(terrains, reqs, its):
+        g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+        ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"],
+                                   max_iteration=t.get("max_iteration", 0xFFFFFFFF), max_slope=t.get("max_slope"))
+        assert it == rit
+        assert _bits_equal(out, ref)
