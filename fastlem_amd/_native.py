@@ -61,6 +61,17 @@ class ServerCounts(ctypes.Structure):
                 ("step_ms", ctypes.c_double), ("join_ms", ctypes.c_double), ("idle_ms", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    """Mirror of ``fl_tuning``."""
+
+    _fields_ = [("site_block", ctypes.c_uint32), ("long_chain", ctypes.c_uint32), ("tail_units", ctypes.c_uint32),
+                ("pipelining", ctypes.c_uint32), ("compact_min_idle", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"fastlem_hip error {code}: {message}")
@@ -105,6 +116,9 @@ def _declare(lib):
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64),
                                      ctypes.POINTER(u64)]),
         "fl_ctx_reset_profile": (i32, [vp]),
+        "fl_ctx_set_tuning": (i32, [vp, ctypes.POINTER(Tuning)]),
+        "fl_ctx_tuning_for": (i32, [vp, u64, u32, ctypes.POINTER(Tuning)]),
+        "fl_autotune": (i32, [vp, vp, u32, ctypes.POINTER(Tuning), ctypes.POINTER(f64), ctypes.POINTER(f64), c_u32p]),
         "fl_stage_receivers": (i32, [vp, vp, c_f64p, c_u32p]),
         "fl_stage_stream_tree": (i32, [vp, vp, c_f64p, c_u32p, c_u32p, ctypes.POINTER(i32)]),
         "fl_stage_iteration": (i32, [vp, vp, c_f64p, c_u32p, c_f64p, c_f64p, ctypes.POINTER(i32)]),

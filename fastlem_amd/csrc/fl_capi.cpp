@@ -21,6 +21,7 @@
 
 #include "fl_internal.h"
 #include "fl_kernels.h"
+#include "fl_tuning_table.h"
 
 namespace {
 
@@ -64,10 +65,15 @@ struct ProfScope {
 struct fl_ctx {
   int device = 0;
   hipMemPool_t pool = nullptr;           // the library's memory pool of the device
-  uint32_t compact_min_idle = 1u << 16;  // see solve_union
+  uint32_t compact_min_idle = 1u << 16;  // see solve_union (the call's tuning; set per call)
+  // tuning: `tune` holds the caller's settings, tune_set which of them apply
+  // (1 all: fl_ctx_set_tuning; 2 pipelining; 4 compaction)
+  fl_tuning tune{};
+  uint32_t tune_set = 0;
   int pipelined = 1;                     // one_iteration_fast after the first iteration (2: minimal
                                          // speculation, most iterations redone)
   bool graphs = false;                   // pipelined iterations replayed from captured graphs
+  bool forbid_graphs = false;            // no graph replay whatever the tuning (allocation-free calls)
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -93,6 +99,7 @@ struct fl_terrain {
   uint32_t last_rounds = 0;  // sweep rounds of the previous iteration
   std::vector<uint32_t> last_boff;  // its bucket offsets (boff[nb+1], bnoff[nb+1]): launch-size estimates
   uint32_t tail_split = 0;          // first round of the pipelined iterations' tail launches
+  uint32_t tail_units = FL_TAIL_WAVES;  // rounds of at most this many fold units go to the tail launches
   // captured pipelined iterations, one per elevation-buffer parity, for the
   // (jump rounds, sweep rounds) they were captured with
   // elevation buffers: an iteration reads hb[i] and writes hb[(i+1) % 3], so
@@ -306,6 +313,59 @@ int32_t validate(const fl_terrain_desc* d, bool* asym) {
   }
   return FL_OK;
 }
+
+// The tuning a call on `sites` sites (a batch when terrains > 1) takes: the
+// context's fixed tuning, else the table's row for the class, with the
+// context's explicit pipelining and compaction settings over it.
+fl_tuning effective_tuning(const fl_ctx* c, uint64_t sites, uint32_t terrains) {
+  if (c->tune_set & 1) return c->tune;
+  fl_tuning t{};
+  const uint32_t batch = terrains > 1 ? 1 : 0;
+  for (const fl_tuning_row& r : fl_tuning_table)
+    if (r.batch == batch && sites <= r.max_sites) {
+      t.site_block = r.site_block;
+      t.long_chain = r.long_chain;
+      t.tail_units = r.tail_units;
+      t.pipelining = r.pipelining;
+      t.compact_min_idle = r.compact_min_idle;
+      break;
+    }
+  if (!t.site_block) t = fl_tuning{256, 6, FL_TAIL_WAVES, 1, 1u << 16, {0, 0, 0}};
+  if (c->tune_set & 2) t.pipelining = c->tune.pipelining;
+  if (c->tune_set & 4) t.compact_min_idle = c->tune.compact_min_idle;
+  return t;
+}
+
+bool valid_tuning(const fl_tuning& t) {
+  return t.site_block >= 64 && t.site_block <= 1024 && t.site_block % 64 == 0 && t.long_chain >= 1 &&
+         (t.pipelining == 0 || t.pipelining == 1 || t.pipelining == 3);
+}
+
+// A call's tuning applied to the context, the calling thread and terrain T
+// (and the unions derived from it), restored when the call returns.
+struct TuneScope {
+  fl_ctx* c;
+  uint32_t block, compact;
+  int pipelined;
+  bool graphs;
+  TuneScope(fl_ctx* c_, fl_terrain* T, const fl_tuning& t)
+      : c(c_), block(fl_site_block), compact(c_->compact_min_idle), pipelined(c_->pipelined), graphs(c_->graphs) {
+    fl_site_block = t.site_block;
+    c->pipelined = t.pipelining == 3 ? 1 : (int)t.pipelining;
+    c->graphs = t.pipelining == 3 && !c->forbid_graphs;
+    c->compact_min_idle = t.compact_min_idle;
+    if (T) {
+      T->w.long_min = t.long_chain;
+      T->tail_units = t.tail_units;
+    }
+  }
+  ~TuneScope() {
+    fl_site_block = block;
+    c->compact_min_idle = compact;
+    c->pipelined = pipelined;
+    c->graphs = graphs;
+  }
+};
 
 int32_t read_ctrl(fl_ctx* c, fl_terrain* T) {
   HIPCHK(hipMemcpyAsync(c->host_ctrl, T->w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -523,11 +583,12 @@ fl_round_grid exact_grid(const std::vector<uint32_t>& off, uint32_t l) {
 }
 
 // First round of the tail launches: the rounds from it on each hold at most
-// FL_TAIL_WAVES fold units (one pass of the tail's wavefronts); `rounds`
-// (no tail) when fewer than two rounds qualify, as one round gains nothing.
-uint32_t tail_start(const std::vector<uint32_t>& off, uint32_t rounds) {
+// `units` fold units (tuned; the tail's FL_TAIL_WAVES wavefronts loop over
+// more); `rounds` (no tail) when fewer than two rounds qualify, as one round
+// gains nothing.
+uint32_t tail_start(const std::vector<uint32_t>& off, uint32_t rounds, uint32_t units) {
   uint32_t l = rounds;
-  while (l > 0 && exact_grid(off, l - 1).fold_blocks <= FL_TAIL_WAVES) l--;
+  while (l > 0 && exact_grid(off, l - 1).fold_blocks <= units) l--;
   return rounds - l >= 2 ? l : rounds;
 }
 
@@ -559,7 +620,7 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
   bool clamp = T->t.any_max_slope;
   uint32_t rounds = nb / 2;
   // the deep rounds, which hold few chains, go to one launch per sweep
-  uint32_t split = tail_start(off, rounds);
+  uint32_t split = tail_start(off, rounds, T->tail_units);
   uint64_t tail_units = tail_positions(off, split, rounds);
   if (split < rounds) fl_launch_da_tail(s, T->t, w, split, rounds - 1, tail_units, clamp);
   for (uint32_t l = split; l-- > 0;) fl_launch_da_round(s, T->t, w, l, exact_grid(off, l), clamp);
@@ -687,7 +748,7 @@ int32_t enqueue_fast(fl_ctx* c, fl_terrain* T, Flight& f) {
   // captured graphs stay valid)
   f.S = f.R;
   if (!T->last_boff.empty()) {
-    uint32_t want = tail_start(T->last_boff, f.R);
+    uint32_t want = tail_start(T->last_boff, f.R, T->tail_units);
     uint32_t keep = T->tail_split;
     f.S = (want < f.R && keep < f.R && want + 1 >= keep && want <= keep + 1) ? keep : want;
   }
@@ -1228,6 +1289,8 @@ int32_t compact_union(fl_ctx* c, fl_terrain* T, fl_terrain* cur, const uint32_t*
   N->have_order = true;
   N->jump_rounds = cur->jump_rounds;
   N->last_rounds = cur->last_rounds;
+  N->w.long_min = cur->w.long_min;
+  N->tail_units = cur->tail_units;
   HIPCHK(hipStreamSynchronize(s));  // `map` is pageable
   *out = N.release();
   return FL_OK;
@@ -1627,6 +1690,11 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
   HIPCHK(hipSetDevice(c->device));
   if (!L->cur || L->active == 0) return FL_OK;
   fl_terrain* cur = L->cur;
+  // a lane keeps its pipelining and graph settings (the mode of the
+  // iterations in flight), the rest follows the union's class
+  fl_tuning tt = effective_tuning(c, cur->t.n, std::max<uint32_t>(cur->nterr, 2));
+  tt.pipelining = c->graphs ? 3 : (uint32_t)c->pipelined;
+  TuneScope tune(c, cur, tt);
   std::vector<uint32_t> stop_now;
   // account one confirmed iteration of the terrains in `mask`
   auto confirm = [&](const std::vector<uint8_t>& mask, const uint32_t* tflags) {
@@ -1749,6 +1817,7 @@ extern "C" int32_t fl_batch_create(fl_ctx* c, const fl_terrain_desc* descs, uint
 extern "C" int32_t fl_batch_solve(fl_ctx* c, fl_terrain* T, uint32_t* iterations_out) {
   if (!c || !T) return set_err(FL_ERR_INVALID, "null context or batch");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   HIPCHK(hipSetDevice(c->device));
   FLCHK(solve_union(c, T, T->t_max_it.data()));
   if (iterations_out) std::copy(T->t_iters.begin(), T->t_iters.end(), iterations_out);
@@ -1764,6 +1833,7 @@ extern "C" int32_t fl_batch_count(const fl_terrain* T, uint32_t* count) {
 extern "C" int32_t fl_terrain_solve(fl_ctx* c, fl_terrain* T, uint32_t max_iteration, uint32_t* iterations_out) {
   if (!c || !T) return set_err(FL_ERR_INVALID, "null context or terrain");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   HIPCHK(hipSetDevice(c->device));
   std::vector<uint32_t> caps(T->nterr, max_iteration);
   FLCHK(solve_union(c, T, caps.data()));
@@ -1873,14 +1943,13 @@ extern "C" int32_t fl_generate_f64_noalloc(fl_ctx* c, const fl_terrain_desc* d, 
   region.base = base;
   region.size = size;
   // launches only: a captured graph's instantiation may allocate
-  const bool graphs = c->graphs;
-  c->graphs = false;
+  c->forbid_graphs = true;
   fl_terrain* T = nullptr;
   int32_t rc = create_union(c, d, 1, &T, &region);
   if (rc == FL_OK) rc = fl_terrain_solve(c, T, d->max_iteration, iterations_out);
   if (rc == FL_OK) rc = fl_terrain_download(c, T, elevations_out);
   if (T) fl_terrain_destroy(T);
-  c->graphs = graphs;
+  c->forbid_graphs = false;
   return rc;
 }
 
@@ -1889,6 +1958,7 @@ extern "C" int32_t fl_generate_f64_noalloc(fl_ctx* c, const fl_terrain_desc* d, 
 extern "C" int32_t fl_stage_receivers(fl_ctx* c, fl_terrain* T, const double* elev, uint32_t* next_out) {
   if (!c || !T || !elev || !next_out) return set_err(FL_ERR_INVALID, "null argument");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
@@ -1901,6 +1971,7 @@ extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* 
                                         uint32_t* subroot_out, int32_t* flag) {
   if (!c || !T || !elev) return set_err(FL_ERR_INVALID, "null argument");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
@@ -1923,6 +1994,7 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
                                       double* area_out, double* response_out, int32_t* flag) {
   if (!c || !T || !elev) return set_err(FL_ERR_INVALID, "null argument");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   if (T->t.n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
@@ -1938,15 +2010,113 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
 
 extern "C" int32_t fl_ctx_set_batch_compaction(fl_ctx* c, uint32_t min_idle_sites) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
-  c->compact_min_idle = min_idle_sites;
+  c->tune.compact_min_idle = min_idle_sites;
+  c->tune_set |= 4;
   return FL_OK;
 }
 
 extern "C" int32_t fl_ctx_set_pipelining(fl_ctx* c, int32_t enable) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
   if (enable < 0 || enable > 3) return set_err(FL_ERR_INVALID, "pipelining mode must be 0, 1, 2 or 3");
-  c->pipelined = enable == 3 ? 1 : enable;
-  c->graphs = enable == 3;
+  // mode 2 (minimal speculation, for tests) is held in the context's field
+  c->tune.pipelining = (uint32_t)enable;
+  c->tune_set |= 2;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_ctx_set_tuning(fl_ctx* c, const fl_tuning* t) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  if (!t) {
+    c->tune_set &= ~1u;
+    return FL_OK;
+  }
+  if (!valid_tuning(*t)) return set_err(FL_ERR_INVALID, "tuning out of range");
+  c->tune = *t;
+  c->tune_set |= 1;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_autotune(fl_ctx* c, fl_terrain* T, uint32_t repeats, fl_tuning* best, double* best_ms,
+                               double* table_ms, uint32_t* candidates) {
+  if (!c || !T || !best) return set_err(FL_ERR_INVALID, "null argument");
+  if (T->t.n == 0) return set_err(FL_ERR_INVALID, "empty terrain");
+  HIPCHK(hipSetDevice(c->device));
+  if (repeats == 0) repeats = 1;
+  const fl_tuning saved = c->tune;
+  const uint32_t saved_set = c->tune_set;
+  const uint32_t G = T->nterr;
+  const uint64_t n = T->t.n;
+  std::vector<uint64_t> ref_bits, bits(n);
+  std::vector<uint32_t> ref_iters;
+  uint32_t tried = 0;
+  // the fastest of `repeats` solves with tuning t, its results checked
+  // against the first tuning's bit for bit
+  auto run = [&](const fl_tuning& t, double* ms) -> int32_t {
+    double fastest = 1e300;
+    for (uint32_t r = 0; r < repeats; r++) {
+      const auto t0 = std::chrono::steady_clock::now();
+      {
+        TuneScope scope(c, T, t);
+        FLCHK(solve_union(c, T, T->t_max_it.data()));
+      }
+      fastest = std::min(fastest, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    HIPCHK(hipMemcpyAsync(bits.data(), T->w.h, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ref_bits.empty()) {
+      ref_bits = bits;
+      ref_iters = T->t_iters;
+    } else if (bits != ref_bits || T->t_iters != ref_iters) {
+      return set_err(FL_ERR_HIP, "tuning (block %u, long chains %u, tail %u, pipelining %u) changed the results",
+                     t.site_block, t.long_chain, t.tail_units, t.pipelining);
+    }
+    *ms = fastest;
+    tried++;
+    return FL_OK;
+  };
+  const fl_tuning table = effective_tuning(c, n, G);
+  double ms = 0.0, cur_ms = 0.0;
+  int32_t rc = run(table, &ms);  // warm-up: the visit order is computed and kept
+  if (rc == FL_OK) rc = run(table, &cur_ms);
+  fl_tuning cur = table;
+  const double table_time = cur_ms;
+  // coordinate descent, two passes; a candidate replaces the current one
+  // only when it is faster by more than 1% (timing noise)
+  const std::vector<uint32_t> chains = {3, 4, 6, 8, 12, 16, 24}, tails = {0, 8, 16, 32, 64},
+                              blocks = {128, 256, 512, 1024}, pipes = {1, 3},
+                              compacts = {16384, 65536, 262144, 0xffffffffu};
+  for (int pass = 0; pass < 2 && rc == FL_OK; pass++) {
+    auto sweep = [&](uint32_t fl_tuning::*field, const std::vector<uint32_t>& values) {
+      for (uint32_t v : values) {
+        if (rc != FL_OK || v == cur.*field) continue;
+        fl_tuning t = cur;
+        t.*field = v;
+        rc = run(t, &ms);
+        if (rc == FL_OK && ms < 0.99 * cur_ms) {
+          cur = t;
+          cur_ms = ms;
+        }
+      }
+    };
+    sweep(&fl_tuning::long_chain, chains);
+    sweep(&fl_tuning::tail_units, tails);
+    sweep(&fl_tuning::site_block, blocks);
+    sweep(&fl_tuning::pipelining, pipes);
+    if (G > 1) sweep(&fl_tuning::compact_min_idle, compacts);
+  }
+  c->tune = saved;
+  c->tune_set = saved_set;
+  if (rc != FL_OK) return rc;
+  *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+  if (table_ms) *table_ms = table_time;
+  if (candidates) *candidates = tried;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_ctx_tuning_for(fl_ctx* c, uint64_t sites, uint32_t terrains, fl_tuning* out) {
+  if (!c || !out) return set_err(FL_ERR_INVALID, "null argument");
+  *out = effective_tuning(c, sites, terrains);
   return FL_OK;
 }
 
@@ -2008,6 +2178,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
 extern "C" int32_t fl_stage_visit_order(fl_ctx* c, fl_terrain* T, uint32_t* rank_out) {
   if (!c || !T || (!rank_out && T->t.n)) return set_err(FL_ERR_INVALID, "null argument");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   const uint32_t n = T->t.n;
   if (n == 0) return FL_OK;
   HIPCHK(hipSetDevice(c->device));
@@ -2039,6 +2210,7 @@ extern "C" int32_t fl_stage_basin_order(fl_ctx* c, fl_terrain* T, const double* 
   if (!c || !T || !elev || !order_out || !len_out) return set_err(FL_ERR_INVALID, "null argument");
   if (outlet >= T->t.n) return set_err(FL_ERR_INVALID, "outlet out of range");
   ProfScope prof_scope(c->prof);
+  TuneScope tune(c, T, effective_tuning(c, T->t.n, T->nterr));
   HIPCHK(hipSetDevice(c->device));
   uint32_t n = T->t.n;
   FLCHK(upload_h(c, T, elev));

@@ -104,7 +104,14 @@ struct fl_dev_work {
   const uint32_t* guard = nullptr;
   int* flags = nullptr;                                      // = ctrl
   uint32_t* counters = nullptr;                              // = ctrl + FL_NUM_FLAGS
+  // chains of at least long_min sites fold in a wavefront, shorter ones in
+  // one lane (a tuning: results do not depend on it)
+  uint32_t long_min = 6;
 };
+
+// Workgroup size of the per-site kernels launched by the calling thread (a
+// tuning: results do not depend on it; 64..1024, a multiple of 64).
+extern thread_local uint32_t fl_site_block;
 
 void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);

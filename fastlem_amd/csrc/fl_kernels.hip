@@ -300,10 +300,10 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
   if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
 
-// Buckets: 2 * light depth + (chain length >= FL_LONG), so every round has a
-// range of short chains (one lane each) followed by long ones (one
-// wavefront each).
-#define FL_LONG 6
+// Buckets: 2 * light depth + (chain length >= long_min), so every round
+// has a range of short chains (one lane each) followed by long ones (one
+// wavefront each); long_min is a launch parameter (tuned; any value >= 1
+// gives the same results).
 #define FL_LDS_BUCKETS 128
 #define FL_BUCKET_BLOCK 1024
 #define FL_BUCKET_PER_THREAD 2
@@ -315,7 +315,8 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
 k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
-               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld) {
+               uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld,
+               uint32_t long_min) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
@@ -331,7 +332,7 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     if (v < n && hc[v] == FL_NONE) {
       unsigned long long r = rt[v];
       if (is_outlet[pair_ptr(r)]) {
-        uint32_t lw = pair_cnt(r), len = pair_cnt(cp[v]) + 1, l = 2u * lw + (len >= FL_LONG ? 1u : 0u);
+        uint32_t lw = pair_cnt(r), len = pair_cnt(cp[v]) + 1, l = 2u * lw + (len >= long_min ? 1u : 0u);
         if (l < FL_LDS_BUCKETS) {
           atomicAdd(s_c + l, 1u);
           atomicAdd(s_n + l, len);
@@ -430,7 +431,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
                const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
-               uint32_t* __restrict__ head_base) {
+               uint32_t* __restrict__ head_base, uint32_t long_min) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
@@ -448,7 +449,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
       unsigned long long r = rt[v];
       if (is_outlet[pair_ptr(r)]) {
         unsigned long long c = cp[v];
-        uint32_t len = pair_cnt(c) + 1, l = 2u * pair_cnt(r) + (len >= FL_LONG ? 1u : 0u);
+        uint32_t len = pair_cnt(c) + 1, l = 2u * pair_cnt(r) + (len >= long_min ? 1u : 0u);
         if (l < FL_LDS_BUCKETS) {
           lc[j] = atomicAdd(s_c + l, 1u);
           ln[j] = atomicAdd(s_n + l, len);
@@ -475,7 +476,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     unsigned long long c = cp[v];
     uint32_t len = pair_cnt(c) + 1;
-    uint32_t l = 2u * pair_cnt(rt[v]) + (len >= FL_LONG ? 1u : 0u);
+    uint32_t l = 2u * pair_cnt(rt[v]) + (len >= long_min ? 1u : 0u);
     uint32_t q = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
     chain_base[q] = pos;
@@ -1492,10 +1493,13 @@ fl_profiler::~fl_profiler() {
     }                                                                             \
   } while (0)
 
-#define FL_LAUNCH_U(kernel, count, units, stream, ...)                                    \
-  do {                                                                                      \
-    if ((count) > 0)                                                                        \
-      FL_LAUNCH_GRID(kernel, grid_for((count), 256), dim3(256), (units), (stream), __VA_ARGS__); \
+// Per-site kernels: one thread per item in workgroups of fl_site_block
+// threads (the tuned workgroup size of the call in progress).
+thread_local uint32_t fl_site_block = 256;
+#define FL_LAUNCH_U(kernel, count, units, stream, ...)                                                      \
+  do {                                                                                                        \
+    if ((count) > 0)                                                                                          \
+      FL_LAUNCH_GRID(kernel, grid_for((count), fl_site_block), dim3(fl_site_block), (units), (stream), __VA_ARGS__); \
   } while (0)
 #define FL_LAUNCH(kernel, count, stream, ...) FL_LAUNCH_U(kernel, count, count, stream, __VA_ARGS__)
 
@@ -1535,7 +1539,8 @@ void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_w
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
-                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD);
+                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD,
+                 w.long_min);
 }
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t rounds) {
@@ -1544,7 +1549,8 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
                  w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
-                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base);
+                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base,
+                 w.long_min);
   FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
             t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.pcnt, w.plist, w.parea);
 }

@@ -12,7 +12,8 @@
 // others run start within an iteration of a lane, and their work shares the
 // launches of the terrains already running.  An idle lane takes every
 // prepared terrain; a busy one takes them when no lane is idle.  Lanes and
-// preparation workers together use at most four streams.  Per request the
+// preparation workers together use at most four streams (one lane, the rest
+// preparing).  Per request the
 // server records queueing time (submission to admission) and latency
 // (submission to completion).
 #include <algorithm>
@@ -295,10 +296,12 @@ extern "C" int32_t fl_server_create(int32_t device, uint32_t slots, uint64_t hbm
   auto* S = new fl_server();
   S->device = device;
   S->budget = hbm_budget_bytes;
-  // at most four streams: lanes iterate, preparation workers upload and
-  // order; a single slot still gets one of each
+  // at most four streams: one lane iterates every prepared terrain in one
+  // union (the most sharing of launches); preparation (host validation and
+  // relabelling, upload, visit order) costs more per site than iterating, so
+  // the other streams prepare
   const uint32_t streams = std::min<uint32_t>(std::max<uint32_t>(slots, 2), 4);
-  S->n_lanes = std::max<uint32_t>(1, streams / 2);
+  S->n_lanes = 1;
   S->n_prep = streams - S->n_lanes;
   for (uint32_t i = 0; i < S->n_prep; i++) S->workers.emplace_back(prep_main, S);
   for (uint32_t i = 0; i < S->n_lanes; i++) S->workers.emplace_back(lane_main, S);

@@ -136,6 +136,40 @@ int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
  * graph.  Results do not depend on the mode. */
 int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
 
+/* Launch tuning.  None of these change any result (every setting gives the
+ * same bits); they shape the work:
+ *   site_block        workgroup size of the per-site kernels (64..1024, a
+ *                     multiple of 64);
+ *   long_chain        chains of at least this many sites fold in a wavefront
+ *                     with LDS-staged tiles, shorter ones in one lane (the
+ *                     switch between the two forms of a sweep round, >= 1);
+ *   tail_units        sweep rounds of at most this many fold units run
+ *                     together, one launch per sweep (levels per launch; 0:
+ *                     one launch per round);
+ *   pipelining        0, 1 or 3 as fl_ctx_set_pipelining (3: HIP graphs);
+ *   compact_min_idle  batches: as fl_ctx_set_batch_compaction.
+ * Without fl_ctx_set_tuning a solve takes the library's table, chosen by
+ * fl_autotune per workload class (site count, one terrain or a batch);
+ * fl_ctx_set_pipelining / fl_ctx_set_batch_compaction override those fields. */
+typedef struct fl_tuning {
+  uint32_t site_block, long_chain, tail_units, pipelining, compact_min_idle;
+  uint32_t reserved[3];
+} fl_tuning;
+/* A fixed tuning for every later call on the context; NULL returns to the
+ * table. */
+int32_t fl_ctx_set_tuning(fl_ctx *ctx, const fl_tuning *tuning);
+/* The tuning a solve of `sites` sites (a batch when terrains > 1) takes on
+ * this context. */
+int32_t fl_ctx_tuning_for(fl_ctx *ctx, uint64_t sites, uint32_t terrains, fl_tuning *out);
+/* Autotuner: coordinate descent over the tunings on a resident terrain or
+ * batch, each candidate timed over `repeats` full solves (the visit order
+ * cached) and checked to give the same elevations and iteration counts bit
+ * for bit.  *best: the fastest; *best_ms and *table_ms: its solve time and
+ * that of the table's tuning; *candidates: tunings tried.  The context's
+ * tuning is left as it was. */
+int32_t fl_autotune(fl_ctx *ctx, fl_terrain *t, uint32_t repeats, fl_tuning *best, double *best_ms,
+                    double *table_ms, uint32_t *candidates);
+
 /* Serving: a request loop on one GPU with continuous batching.  A request
  * is admitted (first come, first served) as soon as the HBM it needs fits in
  * hbm_budget_bytes (fl_request_device_bytes plus a second copy of its
@@ -145,7 +179,7 @@ int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
  * (fl_batch_* semantics: every terrain stops at its own convergence or
  * max_iteration, so its result equals its solo fl_generate_f64), and a
  * terrain leaves as soon as it stops.  `slots` streams are used in all (at
- * most 4: half lanes, half preparation).  The caller keeps the description's
+ * most 4: one lane, the rest preparing).  The caller keeps the description's
  * arrays and elevations_out alive until fl_server_wait returns for that
  * ticket.  fl_server_destroy drains the queue. */
 typedef struct fl_server fl_server;

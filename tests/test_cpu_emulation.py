@@ -348,3 +348,57 @@ def test_emulated_server_continuous_batching(emu_lib):
                                    max_iteration=t.get("max_iteration", 0xFFFFFFFF), max_slope=t.get("max_slope"))
         assert it == rit
         assert _bits_equal(out, ref)
+
+
+def _solve_with_tuning(L, ctx, h, tuning, n):
+    from fastlem_amd import _native
+
+    if tuning is None:
+        assert L.fl_ctx_set_tuning(ctx, None) == 0
+    else:
+        t = _native.Tuning(*tuning)
+        assert L.fl_ctx_set_tuning(ctx, ctypes.byref(t)) == 0, L.fl_last_error()
+    it = ctypes.c_uint32()
+    assert L.fl_terrain_solve(ctx, h, 0xFFFFFFFF, ctypes.byref(it)) == 0, L.fl_last_error()
+    out = np.empty(n)
+    assert L.fl_terrain_download(ctx, h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    return out, it.value
+
+
+def test_emulated_tunings_give_identical_results(emu_lib):
+    """Every launch tuning -- workgroup size, the short/long chain switch
+    (all chains short, all long), tail launches off and wide, pipelining
+    modes -- gives the same elevations bit for bit; the autotuner, which
+    checks exactly that for each candidate, completes."""
+    from fastlem_amd import _native
+    from fastlem_amd.device import batch_descs, DeviceBatch
+    from fastlem_amd.workloads import config_fbm
+
+    L = emu_lib
+    w = config_fbm(2500, 50.0, seed=8)
+    keep = []
+    d = batch_descs([DeviceBatch.terrain_of(w)], keep)
+    ctx = ctypes.c_void_p()
+    assert L.fl_ctx_create(0, ctypes.byref(ctx)) == 0
+    h = ctypes.c_void_p()
+    assert L.fl_terrain_create(ctx, ctypes.byref(d[0]), ctypes.byref(h)) == 0
+    try:
+        ref, rit = _solve_with_tuning(L, ctx, h, None, w.n)
+        for t in [(64, 1, 0, 1, 65536), (1024, 100000, 64, 1, 65536), (128, 3, 8, 0, 65536),
+                  (512, 12, 1000, 3, 65536), (256, 6, 16, 1, 65536)]:
+            got, it = _solve_with_tuning(L, ctx, h, t, w.n)
+            assert it == rit and _bits_equal(got, ref), t
+        assert L.fl_ctx_set_tuning(ctx, None) == 0
+        bad = _native.Tuning(100, 6, 16, 1, 65536)
+        assert L.fl_ctx_set_tuning(ctx, ctypes.byref(bad)) == -1  # block not a multiple of 64
+        best = _native.Tuning()
+        bms, tms = ctypes.c_double(), ctypes.c_double()
+        tried = ctypes.c_uint32()
+        assert L.fl_autotune(ctx, h, 1, ctypes.byref(best), ctypes.byref(bms), ctypes.byref(tms),
+                             ctypes.byref(tried)) == 0, L.fl_last_error()
+        assert tried.value >= 15 and bms.value <= tms.value
+        table = _native.Tuning()
+        assert L.fl_ctx_tuning_for(ctx, w.n, 1, ctypes.byref(table)) == 0 and table.site_block >= 64
+    finally:
+        L.fl_terrain_destroy(h)
+        L.fl_ctx_destroy(ctx)

@@ -490,3 +490,31 @@ def test_generate_from_python_threads(plain10k, fbm20k):
         ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
         assert its == rit
         assert _bits_equal(ref, h)
+
+
+def test_tunings_give_identical_results(fbm20k):
+    """On the GPU: every launch tuning gives the same elevations bit for bit
+    as the table's, and equals the oracle."""
+    import ctypes
+
+    from oracle import flo
+    from fastlem_amd import _native
+    from fastlem_amd.generator import context
+
+    w = fbm20k
+    t = _dev(w)
+    L = _native.lib()
+    ctx = context()
+    ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+    try:
+        for tun in [None, (64, 1, 0, 1, 65536), (1024, 100000, 64, 1, 65536), (128, 3, 8, 0, 65536),
+                    (512, 12, 1000, 3, 65536)]:
+            if tun is None:
+                assert L.fl_ctx_set_tuning(ctx, None) == 0
+            else:
+                tt = _native.Tuning(*tun)
+                assert L.fl_ctx_set_tuning(ctx, ctypes.byref(tt)) == 0
+            assert t.solve() == rit, tun
+            assert _bits_equal(ref, t.elevations()), tun
+    finally:
+        L.fl_ctx_set_tuning(ctx, None)
