@@ -4,9 +4,7 @@
 // max_sites and terrain kind match a solve gives its tuning.  A tuning
 // shapes launches only: every row gives the same results.
 //
-// Initial contents: the hand-swept values (per-site workgroups of 256,
-// wavefront chains from 6 sites, tail launches for rounds of up to 16 fold
-// units, pipelined iterations, batches compacted once 65536 sites idle).
+// Written by fastlem_amd/autotune.py on 2026-12-03 (one MI355X).
 #pragma once
 
 #include <stdint.h>
@@ -18,6 +16,10 @@ struct fl_tuning_row {
 };
 
 static const fl_tuning_row fl_tuning_table[] = {
-    {UINT64_MAX, 0, 256, 6, 16, 1, 65536},
-    {UINT64_MAX, 1, 256, 6, 16, 1, 65536},
+    {32768ull, 0, 256, 4, 16, 1, 65536u},  // terrain 16384 sites: 3.60 -> 3.57 ms
+    {131072ull, 0, 256, 6, 16, 1, 65536u},  // terrain 65536 sites: 7.66 -> 7.66 ms
+    {524288ull, 0, 256, 6, 32, 1, 65536u},  // terrain 262144 sites: 26.99 -> 26.71 ms
+    {UINT64_MAX, 0, 256, 6, 16, 1, 65536u},  // terrain 1048576 sites: 88.99 -> 88.99 ms
+    {1317794ull, 1, 256, 6, 16, 1, 65536u},  // batch of 32 terrains, 1000..65536 sites (540264 in all): 48.22 -> 48.22 ms
+    {UINT64_MAX, 1, 256, 6, 16, 1, 65536u},  // batch of 64 terrains, 1000..262144 sites (3214323 in all): 330.26 -> 330.26 ms
 };
