@@ -341,6 +341,8 @@ def main() -> int:
     value = site_iters / elapsed
 
     stats = t.last_stats()
+    tuning = _native.Tuning()
+    _native.check(L.fl_ctx_tuning_for(ctx, n, 1, ctypes.byref(tuning)))
     # the drop-in with host buffers (fl_generate_f64: validation, relabelling,
     # upload over PCIe, solve, download), outside the timed region
     out = np.empty(n, np.float64)
@@ -423,7 +425,7 @@ def main() -> int:
                    "iterations": iters, "visit_order_ms": stats["visit_order_ms"],
                    "visit_order_parts_ms": {"outlet_phase": stats["outlet_phase_ms"], "forest": stats["forest_ms"],
                                             "ranks": stats["ranks_ms"]},
-                   "host_syncs_per_solve": stats["host_syncs"]},
+                   "host_syncs_per_solve": stats["host_syncs"], "tuning": tuning.as_dict()},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
