@@ -49,7 +49,9 @@ class SolveStats(ctypes.Structure):
                 ("host_syncs", ctypes.c_uint32), ("order_from_forest", ctypes.c_uint32),
                 ("visit_order_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("outlet_phase_ms", ctypes.c_double), ("forest_ms", ctypes.c_double),
-                ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32)]
+                ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32),
+                ("create_validate_ms", ctypes.c_double), ("create_relabel_ms", ctypes.c_double),
+                ("create_upload_ms", ctypes.c_double)]
 
 
 class ServerCounts(ctypes.Structure):

@@ -116,6 +116,7 @@ struct fl_terrain {
            stat_compactions = 0, stat_redone = 0;
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
+  double stat_create[3] = {0.0, 0.0, 0.0};       // validation, relabelling, allocation and upload
   uint64_t bytes = 0;
   // where its device arrays come from (the device pool unless a caller
   // supplies a region); region buffers are never freed one by one
@@ -1085,6 +1086,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
   if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
   uint64_t N = 0, NNZ = 0, K = 0;
+  const auto c0 = std::chrono::steady_clock::now();
   std::vector<uint8_t> asym(count, 0);
   if (count == 1) {
     bool a = false;
@@ -1193,6 +1195,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
       is_out[o] = 1;
     }
   };
+  const auto c1 = std::chrono::steady_clock::now();
   if (count == 1)
     relabel(0, true);
   else
@@ -1203,6 +1206,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   fl_terrain* P = T.get();
   bool clamp = false;
   for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
+  const auto c2 = std::chrono::steady_clock::now();
   FLCHK(alloc_device(c, P, n, nnz, count, clamp));
   hipStream_t s = c->stream;
   T->entry_off.resize(count + 1);
@@ -1222,6 +1226,10 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   FLCHK(reset_state(c, P));
   // the pageable staging vectors above must outlive the copies
   HIPCHK(hipStreamSynchronize(s));
+  using ms_ = std::chrono::duration<double, std::milli>;
+  T->stat_create[0] = ms_(c1 - c0).count();
+  T->stat_create[1] = ms_(c2 - c1).count();
+  T->stat_create[2] = ms_(std::chrono::steady_clock::now() - c2).count();
   *out = T.release();
   return FL_OK;
 }
@@ -2172,6 +2180,9 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->solve_ms = T->stat_solve_ms;
   out->compactions = T->stat_compactions;
   out->redone = T->stat_redone;
+  out->create_validate_ms = T->stat_create[0];
+  out->create_relabel_ms = T->stat_create[1];
+  out->create_upload_ms = T->stat_create[2];
   return FL_OK;
 }
 

@@ -213,9 +213,12 @@ typedef struct fl_solve_stats {
   double solve_ms;          /* wall time of the solve */
   double outlet_phase_ms;   /* visit order parts: outlet phase (host, runs */
   double forest_ms;         /*   concurrently with) the GPU spanning forest, */
-  double ranks_ms;          /*   then the ranks from the forest (host) */
+  double ranks_ms;          /*   then the ranks from the forest (device) */
   uint32_t compactions;     /* batch: times the running terrains were gathered */
   uint32_t redone;          /* pipelined iterations redone synchronously */
+  /* fl_terrain_create / fl_batch_create: validation, relabelling (the
+   * locality order and the relabelled host arrays), allocation and upload */
+  double create_validate_ms, create_relabel_ms, create_upload_ms;
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 
