@@ -51,7 +51,8 @@ class SolveStats(ctypes.Structure):
                 ("outlet_phase_ms", ctypes.c_double), ("forest_ms", ctypes.c_double),
                 ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32),
                 ("create_validate_ms", ctypes.c_double), ("create_relabel_ms", ctypes.c_double),
-                ("create_upload_ms", ctypes.c_double)]
+                ("create_upload_ms", ctypes.c_double), ("create_on_device", ctypes.c_uint32),
+                ("create_clusters", ctypes.c_uint32)]
 
 
 class ServerCounts(ctypes.Structure):

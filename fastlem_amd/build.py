@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OUT = os.path.join(LIBDIR, "libfastlem_hip.so")
-SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_noalloc.hip", "fl_interp.hip", "fl_capi.cpp", "fl_setup.cpp", "fl_server.cpp"]
+SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_create.hip", "fl_noalloc.hip", "fl_interp.hip", "fl_capi.cpp", "fl_setup.cpp", "fl_server.cpp"]
 ARCH = os.environ.get("FASTLEM_ARCH", "gfx950")
 
 

@@ -117,13 +117,16 @@ struct fl_terrain {
   double stat_order_ms = 0.0, stat_solve_ms = 0.0;
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   double stat_create[3] = {0.0, 0.0, 0.0};       // validation, relabelling, allocation and upload
+  uint32_t stat_create_device = 0, stat_create_clusters = 0;
   uint64_t bytes = 0;
   // where its device arrays come from (the device pool unless a caller
   // supplies a region); region buffers are never freed one by one
   fl_arena arena;
-  // host copies (relabelled) needed for the one-time visit order
-  std::vector<uint32_t> h_row_ptr, h_col, h_outlets;
-  std::vector<double> h_len;
+  // outlets in resolution order (relabelled), and each outlet's row as the
+  // outlet phase of the visit order needs it: its entry count o_deg[q] and
+  // its neighbouring outlets o_nb[o_ptr[q] .. o_ptr[q+1]) (indices within
+  // its terrain's outlet list)
+  std::vector<uint32_t> h_outlets, o_deg, o_ptr, o_nb;
   // locality relabelling: device site u holds caller site old_of_new[u]
   std::vector<uint32_t> new_of_old, old_of_new;
   std::vector<void*> allocations;
@@ -220,12 +223,11 @@ RowError first_row_error(uint64_t n, C check_row) {
   return best;
 }
 
-// Checks a description; *asym is set when some edge's two entries carry
-// different length bits (legal input: the reference keys each heap push by
-// the popped site's own entry, stream_tree.rs:231-235, which the exact
-// traversal reproduces; the spanning-forest form needs one length per edge).
-int32_t validate(const fl_terrain_desc* d, bool* asym) {
-  *asym = false;
+// Checks of a description that need no look at its rows: null arrays,
+// sizes, and row offsets that start at 0, end at nnz and never decrease.
+// *max_row receives the longest row.
+int32_t check_header(const fl_terrain_desc* d, uint32_t* max_row) {
+  *max_row = 0;
   if (!d) return set_err(FL_ERR_INVALID, "null terrain description");
   if (d->n >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "n=%llu exceeds 2^32-2 sites", (unsigned long long)d->n);
   if (d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "nnz exceeds 2^32-2 entries");
@@ -235,14 +237,45 @@ int32_t validate(const fl_terrain_desc* d, bool* asym) {
     return set_err(FL_ERR_INVALID, "null array in terrain description");
   if (d->n_outlets && !d->outlets) return set_err(FL_ERR_INVALID, "null outlets");
   if (d->has_max_slope && !d->max_slope) return set_err(FL_ERR_INVALID, "has_max_slope without max_slope");
-  uint64_t n = d->n;
+  const uint64_t n = d->n;
+  const uint32_t* rp = d->row_ptr;
+  if (rp[0] != 0 || rp[n] != d->nnz) return set_err(FL_ERR_INVALID, "row_ptr must span [0, nnz]");
+  std::mutex mu;
+  uint64_t first_dec = UINT64_MAX;
+  uint32_t longest = 0;
+  host_parallel(n, 1 << 18, [&](uint64_t a, uint64_t b) {
+    uint32_t m = 0;
+    uint64_t dec = UINT64_MAX;
+    for (uint64_t i = a; i < b; i++) {
+      if (rp[i + 1] < rp[i]) {
+        dec = i;
+        break;
+      }
+      m = std::max(m, rp[i + 1] - rp[i]);
+    }
+    std::lock_guard<std::mutex> g(mu);
+    first_dec = std::min(first_dec, dec);
+    longest = std::max(longest, m);
+  });
+  if (first_dec != UINT64_MAX)
+    return set_err(FL_ERR_INVALID, "row_ptr decreases at %llu", (unsigned long long)first_dec);
+  *max_row = longest;
+  return FL_OK;
+}
+
+// The rows of a description whose header passed: neighbour indices,
+// self-loops and duplicates (the reference's BFS would visit such a site
+// twice), then symmetry (terrain-graph is undirected), each reported at the
+// first failing row.  *asym is set when some edge's two entries carry
+// different length bits (legal input: the reference keys each heap push by
+// the popped site's own entry, stream_tree.rs:231-235, which the exact
+// traversal reproduces; the spanning-forest form needs one length per edge).
+int32_t check_rows(const fl_terrain_desc* d, bool* asym) {
+  *asym = false;
+  const uint64_t n = d->n;
+  if (n == 0) return FL_OK;
   const uint32_t* rp = d->row_ptr;
   const uint32_t* col = d->col;
-  if (rp[0] != 0 || rp[n] != d->nnz) return set_err(FL_ERR_INVALID, "row_ptr must span [0, nnz]");
-  for (uint64_t i = 0; i < n; i++)
-    if (rp[i + 1] < rp[i]) return set_err(FL_ERR_INVALID, "row_ptr decreases at %llu", (unsigned long long)i);
-  // neighbour indices, self-loops and duplicates (the reference's BFS would
-  // visit such a site twice), in entry order within each row
   RowError e = first_row_error(n, [&](uint64_t i) {
     RowError r;
     const uint32_t b = rp[i], deg = rp[i + 1] - b;
@@ -276,8 +309,6 @@ int32_t validate(const fl_terrain_desc* d, bool* asym) {
   if (e.kind == 1) return set_err(FL_ERR_INVALID, "col[%u]=%u out of range", e.k, e.j);
   if (e.kind == 2) return set_err(FL_ERR_INVALID, "self-loop at site %llu", (unsigned long long)e.row);
   if (e.kind == 3) return set_err(FL_ERR_INVALID, "duplicate edge %llu-%u", (unsigned long long)e.row, e.j);
-  // symmetry (terrain-graph is undirected), and whether both entries of
-  // every edge carry the same length bits
   std::atomic<bool> len_differs{false};
   e = first_row_error(n, [&](uint64_t i) {
     bool differs = false;
@@ -306,11 +337,29 @@ int32_t validate(const fl_terrain_desc* d, bool* asym) {
   });
   *asym = len_differs.load();
   if (e.kind == 4) return set_err(FL_ERR_INVALID, "edge %llu-%u is not symmetric", (unsigned long long)e.row, e.j);
-  std::vector<uint8_t> seen(n, 0);
-  for (uint64_t k = 0; k < d->n_outlets; k++) {
-    if (d->outlets[k] >= n) return set_err(FL_ERR_INVALID, "outlet %u out of range", d->outlets[k]);
-    if (seen[d->outlets[k]]) return set_err(FL_ERR_INVALID, "duplicate outlet %u", d->outlets[k]);
-    seen[d->outlets[k]] = 1;
+  return FL_OK;
+}
+
+// The message check_rows gives for its first failing entry k (terrain-local)
+// of the first kind (1..3: index, self-loop, duplicate) or symmetry (4).
+int32_t row_error(const fl_terrain_desc* d, uint32_t k, bool symmetry) {
+  const uint32_t* rp = d->row_ptr;
+  const uint64_t row = (uint64_t)(std::upper_bound(rp, rp + d->n + 1, k) - rp) - 1;
+  const uint32_t j = d->col[k];
+  if (symmetry) return set_err(FL_ERR_INVALID, "edge %llu-%u is not symmetric", (unsigned long long)row, j);
+  if (j >= d->n) return set_err(FL_ERR_INVALID, "col[%u]=%u out of range", k, j);
+  if (j == row) return set_err(FL_ERR_INVALID, "self-loop at site %llu", (unsigned long long)row);
+  return set_err(FL_ERR_INVALID, "duplicate edge %llu-%u", (unsigned long long)row, j);
+}
+
+// Outlets in range and listed once; slot[i] = q + 1 for outlet q at site i
+// (slot: n zeroed entries).
+int32_t check_outlets(const fl_terrain_desc* d, uint32_t* slot) {
+  for (uint64_t q = 0; q < d->n_outlets; q++) {
+    const uint32_t o = d->outlets[q];
+    if (o >= d->n) return set_err(FL_ERR_INVALID, "outlet %u out of range", o);
+    if (slot[o]) return set_err(FL_ERR_INVALID, "duplicate outlet %u", o);
+    slot[o] = (uint32_t)q + 1;
   }
   return FL_OK;
 }
@@ -403,17 +452,40 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
       n_exact += how[g] == 1;
     }
   }
+  // the exact terrains' graphs (relabelled) from the device, at their own
+  // offsets of union-sized host arrays
+  std::unique_ptr<uint32_t[]> x_rp, x_col;
+  std::unique_ptr<double[]> x_len;
+  if (n_exact) {
+    x_rp.reset(new uint32_t[(size_t)n + 1]);
+    x_col.reset(new uint32_t[std::max<uint32_t>(T->t.nnz, 1)]);
+    x_len.reset(new double[std::max<uint32_t>(T->t.nnz, 1)]);
+    for (uint32_t g = 0; g < G; g++) {
+      if (how[g] != 1) continue;
+      const uint64_t s0 = T->site_off[g], s1 = T->site_off[g + 1], e0 = T->entry_off[g], e1 = T->entry_off[g + 1];
+      HIPCHK(hipMemcpyAsync(x_rp.get() + s0, T->t.row_ptr + s0, (s1 - s0 + 1) * 4, hipMemcpyDeviceToHost, s));
+      if (e1 > e0) {
+        HIPCHK(hipMemcpyAsync(x_col.get() + e0, T->t.col + e0, (e1 - e0) * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(x_len.get() + e0, T->t.len + e0, (e1 - e0) * 8, hipMemcpyDeviceToHost, s));
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  }
   // host threads: the outlet phase of the fast terrains (exact std heap tie
-  // handling), and the whole traversal of the exact ones
+  // handling, from each outlet's row), and the whole traversal of the exact
+  // ones
   std::vector<uint32_t> first(K), exact_rank(n_exact ? n : 0);
   std::future<int> outlet_phase = std::async(std::launch::async, [&]() {
     auto p0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> pops;
     for (uint32_t g = 0; g < G; g++) {
       if (how[g] != 0) continue;
-      uint32_t o0 = T->outlet_off[g], o1 = T->outlet_off[g + 1], s0 = T->site_off[g];
-      int rc = fl_outlet_pop_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
-                                         T->h_outlets.data() + o0, o1 - o0, first.data() + o0);
+      const uint32_t o0 = T->outlet_off[g], k = T->outlet_off[g + 1] - o0;
+      pops.resize(k);
+      int rc = fl_outlet_pop_order_compact(k, T->o_deg.data() + o0, T->o_ptr.data() + o0, T->o_nb.data(),
+                                           pops.data());
       if (rc != FL_OK) return rc;
+      for (uint32_t q = 0; q < k; q++) first[o0 + q] = T->h_outlets[o0 + pops[q]];
     }
     T->stat_order_parts[0] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
@@ -427,9 +499,9 @@ int32_t ensure_visit_order(fl_ctx* c, fl_terrain* T) {
         [&](uint32_t g) {
           if (how[g] != 1) return;
           uint32_t s0 = T->site_off[g], o0 = T->outlet_off[g];
-          rcs[g] = fl_lake_visit_order_range(s0, T->site_off[g + 1] - s0, T->h_row_ptr.data(), T->h_col.data(),
-                                             T->h_len.data(), T->h_outlets.data() + o0, T->outlet_off[g + 1] - o0,
-                                             s0, exact_rank.data());
+          rcs[g] = fl_lake_visit_order_range(s0, T->site_off[g + 1] - s0, x_rp.get(), x_col.get(), x_len.get(),
+                                             T->h_outlets.data() + o0, T->outlet_off[g + 1] - o0, s0,
+                                             exact_rank.data());
         });
     for (int rc : rcs)
       if (rc != FL_OK) return rc;
@@ -945,10 +1017,10 @@ extern "C" int32_t fl_terrain_destroy(fl_terrain* T) {
   for (hipGraphExec_t& e : T->gexec)
     if (e) hipGraphExecDestroy(e);
   // every call that used the terrain has returned (they end synchronised),
-  // so the frees are ordered after its last work on any stream
+  // so the frees are ordered after its last work on any stream; the pool
+  // reuses the memory in stream order
   if (T->ctx) {
-    for (void* p : T->allocations) hipFreeAsync(p, T->ctx->stream);
-    hipStreamSynchronize(T->ctx->stream);
+    for (void* p : T->allocations) (void)hipFreeAsync(p, T->ctx->stream);
   } else {
     for (void* p : T->allocations) hipFree(p);
   }
@@ -1053,7 +1125,27 @@ uint64_t union_device_bytes(uint64_t n, uint64_t nnz, uint32_t count, bool clamp
 // (layout_device), pinned host slots for a batch; the bucket histograms
 // start at zero.
 int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, bool clamp) {
-  FLCHK(layout_device(P, n, nnz, count, clamp));
+  if (P->arena.kind == fl_arena::POOL) {
+    // one pool block for all the arrays (one allocation and one free: the
+    // pool's cost grows with the number of its blocks), cut as a region
+    const uint64_t bytes = union_device_bytes(n, nnz, count, clamp);
+    void* block = nullptr;
+    hipError_t e = fl_arena_alloc(&P->arena, c->stream, &block, bytes);
+    if (e != hipSuccess)
+      return set_err(e == hipErrorOutOfMemory ? FL_ERR_OUT_OF_MEMORY : FL_ERR_HIP, "device arrays (%llu bytes): %s",
+                     (unsigned long long)bytes, hipGetErrorString(e));
+    P->allocations.push_back(block);
+    const fl_arena pool = P->arena;
+    P->arena = fl_arena{};
+    P->arena.kind = fl_arena::REGION;
+    P->arena.base = (uint8_t*)block;
+    P->arena.size = fl_round256(bytes);
+    const int32_t rc = layout_device(P, n, nnz, count, clamp);
+    P->arena = pool;
+    FLCHK(rc);
+  } else {
+    FLCHK(layout_device(P, n, nnz, count, clamp));
+  }
   fl_dev_work& w = P->w;
   if (count > 1) {
     if (count > c->pin_cap) {
@@ -1078,35 +1170,135 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
   return FL_OK;
 }
 
+// Each outlet's row as the outlet phase needs it (fl_terrain::o_deg, o_ptr,
+// o_nb) from the caller's graphs, which passed their checks; slot: union site
+// -> 1 + its index in its terrain's outlet list (0: not an outlet).
+void outlet_rows(fl_terrain* T, const fl_terrain_desc* descs, uint32_t count, const uint32_t* slot) {
+  const uint32_t K = T->outlet_off[count];
+  T->o_deg.assign(K, 0);
+  T->o_ptr.assign((size_t)K + 1, 0);
+  // f(g, q, row of outlet q, its terrain's first union site) for outlets
+  // [a, b) of the union, terrain by terrain
+  auto each = [&](auto f) {
+    host_parallel(K, 1 << 14, [&](uint64_t a, uint64_t b) {
+      uint32_t g = (uint32_t)(std::upper_bound(T->outlet_off.begin(), T->outlet_off.end(), (uint32_t)a) -
+                              T->outlet_off.begin()) - 1;
+      for (uint64_t q = a; q < b; q++) {
+        while (T->outlet_off[g + 1] <= q) g++;
+        const fl_terrain_desc* d = &descs[g];
+        const uint32_t o = d->outlets[q - T->outlet_off[g]];
+        f(q, d->row_ptr[o], d->row_ptr[o + 1], d->col, T->site_off[g]);
+      }
+    });
+  };
+  each([&](uint64_t q, uint32_t ka, uint32_t kb, const uint32_t* col, uint32_t s0) {
+    uint32_t m = 0;
+    for (uint32_t k = ka; k < kb; k++) m += slot[s0 + col[k]] != 0;
+    T->o_deg[q] = kb - ka;
+    T->o_ptr[q + 1] = m;
+  });
+  for (uint32_t q = 0; q < K; q++) T->o_ptr[q + 1] += T->o_ptr[q];
+  T->o_nb.resize(T->o_ptr[K]);
+  each([&](uint64_t q, uint32_t ka, uint32_t kb, const uint32_t* col, uint32_t s0) {
+    uint32_t e = T->o_ptr[q];
+    for (uint32_t k = ka; k < kb; k++)
+      if (const uint32_t z = slot[s0 + col[k]]) T->o_nb[e++] = z - 1;
+  });
+}
+
+// Staging of the device path of create_union: the caller's arrays as given
+// and the relabelling maps (the per-site arrays stage in work arrays).
+struct Staging {
+  uint32_t *meta = nullptr, *rp = nullptr, *col = nullptr, *idx = nullptr, *o2n = nullptr, *n2o = nullptr;
+  double* len = nullptr;
+};
+hipError_t alloc_staging(fl_scratch& S, uint64_t n, uint64_t nnz, uint64_t K, uint32_t G, Staging& st) {
+  hipError_t e;
+  if ((e = S.alloc(&st.meta, 2ull * (G + 1))) != hipSuccess) return e;
+  if ((e = S.alloc(&st.rp, n + G)) != hipSuccess) return e;
+  if ((e = S.alloc(&st.col, nnz)) != hipSuccess) return e;
+  if ((e = S.alloc(&st.len, nnz)) != hipSuccess) return e;
+  if ((e = S.alloc(&st.idx, K)) != hipSuccess) return e;
+  if ((e = S.alloc(&st.o2n, n)) != hipSuccess) return e;
+  return S.alloc(&st.n2o, n);
+}
+
+uint64_t staging_bytes(uint64_t n, uint64_t nnz, uint64_t K, uint32_t G) {
+  fl_arena a;
+  a.kind = fl_arena::COUNT;
+  {
+    fl_scratch S(nullptr, &a);
+    Staging st;
+    alloc_staging(S, n, nnz, K, G, st);
+  }
+  return a.peak;
+}
+
+// Device bytes beyond a union's arrays that its creation and first visit
+// order need at their peak.
+uint64_t scratch_bytes(uint64_t n, uint64_t nnz, uint64_t K, uint32_t G) {
+  const uint64_t create = staging_bytes(n, nnz, K, G) + fl_create_scratch_bytes((uint32_t)n, (uint32_t)nnz, G);
+  return std::max(create, fl_msf_scratch_bytes((uint32_t)n, (uint32_t)nnz));
+}
+
 // Upload `count` terrains as one union graph: terrain g's sites occupy
 // [site_off[g], site_off[g+1]), each relabelled for locality inside its range;
-// no edge joins two terrains.
+// no edge joins two terrains.  The graphs are checked and relabelled on the
+// device when every row holds at most FL_DEVICE_MAX_ROW entries, else on host
+// threads; both report the same first error.
 int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl_terrain** out,
                      const fl_arena* region = nullptr) {
   if (!c || !out || (!descs && count)) return set_err(FL_ERR_INVALID, "null context, description or output");
   if (count == 0) return set_err(FL_ERR_INVALID, "no terrain");
-  uint64_t N = 0, NNZ = 0, K = 0;
+  using ms_ = std::chrono::duration<double, std::milli>;
   const auto c0 = std::chrono::steady_clock::now();
+  // a terrain's error, named as the first failing terrain of a batch
+  auto fail = [&](uint32_t g, int32_t rc, const std::string& msg) {
+    return count > 1 ? set_err(rc, "terrain %u: %s", g, msg.c_str()) : set_err(rc, "%s", msg.c_str());
+  };
+  // checks that need no look at the rows, on host threads
+  std::vector<int32_t> rc(count, FL_OK);
+  std::vector<std::string> msg(count);
+  std::vector<uint32_t> longest(count, 0);
+  auto each_terrain = [&](auto f) {
+    if (count == 1)
+      f(0u);
+    else
+      host_each(count, [&](uint32_t g) { return descs[g].nnz; }, f);
+  };
+  each_terrain([&](uint32_t g) {
+    rc[g] = check_header(&descs[g], &longest[g]);
+    if (rc[g] != FL_OK) msg[g] = g_err;
+  });
+  bool device_path = true;
+  for (uint32_t g = 0; g < count; g++) device_path &= rc[g] == FL_OK && longest[g] <= FL_DEVICE_MAX_ROW;
+  // host path: every check of every terrain on host threads
   std::vector<uint8_t> asym(count, 0);
-  if (count == 1) {
-    bool a = false;
-    FLCHK(validate(&descs[0], &a));
-    asym[0] = a;
-  } else {
-    // terrains on host threads; the first failing terrain's error is reported
-    std::vector<int32_t> rc(count, FL_OK);
-    std::vector<std::string> msg(count);
-    host_each(
-        count, [&](uint32_t g) { return descs[g].nnz; },
-        [&](uint32_t g) {
-          bool a = false;
-          rc[g] = validate(&descs[g], &a);
-          asym[g] = a;
-          if (rc[g] != FL_OK) msg[g] = g_err;
-        });
+  std::unique_ptr<uint32_t[]> slot;
+  if (!device_path) {
+    std::vector<std::unique_ptr<uint32_t[]>> tslot(count);
+    each_terrain([&](uint32_t g) {
+      if (rc[g] != FL_OK) return;
+      bool a = false;
+      rc[g] = check_rows(&descs[g], &a);
+      asym[g] = a;
+      if (rc[g] == FL_OK) {
+        tslot[g].reset(new uint32_t[descs[g].n + 1]());
+        rc[g] = check_outlets(&descs[g], tslot[g].get());
+      }
+      if (rc[g] != FL_OK) msg[g] = g_err;
+    });
     for (uint32_t g = 0; g < count; g++)
-      if (rc[g] != FL_OK) return set_err(rc[g], "terrain %u: %s", g, msg[g].c_str());
+      if (rc[g] != FL_OK) return fail(g, rc[g], msg[g]);
+    uint64_t N = 0;
+    for (uint32_t g = 0; g < count; g++) N += descs[g].n;
+    if (N < 0xffffffffull) {
+      slot.reset(new uint32_t[N + 1]);
+      for (uint32_t g = 0, s0 = 0; g < count; s0 += (uint32_t)descs[g].n, g++)
+        std::copy(tslot[g].get(), tslot[g].get() + descs[g].n, slot.get() + s0);
+    }
   }
+  uint64_t N = 0, NNZ = 0, K = 0;
   for (uint32_t g = 0; g < count; g++) {
     N += descs[g].n;
     NNZ += descs[g].nnz;
@@ -1125,6 +1317,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   T->t.nnz = nnz;
   T->site_off.assign(count + 1, 0);
   T->outlet_off.assign(count + 1, 0);
+  T->entry_off.assign(count + 1, 0);
   T->t_max_it.resize(count);
   T->t_iters.assign(count, 0);
   T->t_active.assign(count, 0);
@@ -1132,104 +1325,237 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
   for (uint32_t g = 0; g < count; g++) {
     T->site_off[g + 1] = T->site_off[g] + (uint32_t)descs[g].n;
     T->outlet_off[g + 1] = T->outlet_off[g] + (uint32_t)descs[g].n_outlets;
+    T->entry_off[g + 1] = T->entry_off[g] + (uint32_t)descs[g].nnz;
     T->t_max_it[g] = descs[g].max_iteration;
   }
   if (n == 0) {
     *out = T.release();
     return FL_OK;
   }
-  T->old_of_new.resize(n);
-  T->new_of_old.resize(n);
-  T->h_row_ptr.assign((size_t)n + 1, 0);
-  T->h_col.resize(nnz);
-  T->h_len.resize(nnz);
-  T->h_outlets.resize(K);
-  std::vector<double> area(n), erod(n), uplift(n), init(n), tan_ms(n);
-  std::vector<uint8_t> is_out(n, 0);
-  std::vector<uint32_t> tid(count > 1 ? n : 0);
-  // entry offset of each terrain in the union
-  std::vector<uint64_t> q0(count + 1, 0);
-  for (uint32_t g = 0; g < count; g++) q0[g + 1] = q0[g] + descs[g].nnz;
-  // terrain g relabelled into its site and entry ranges (disjoint between
-  // terrains): breadth-first order, then its rows and site values copied,
-  // over host threads when `inner`
-  auto relabel = [&](uint32_t g, bool inner) {
-    const fl_terrain_desc* d = &descs[g];
-    const uint32_t s0 = T->site_off[g], ng = (uint32_t)d->n;
-    if (ng == 0) return;
-    std::vector<uint32_t> o2, n2;
-    locality_order(ng, d->row_ptr, d->col, o2, n2);
-    uint32_t q = (uint32_t)q0[g];
-    for (uint32_t u = 0; u < ng; u++) {
-      const uint32_t i = o2[u];
-      q += d->row_ptr[i + 1] - d->row_ptr[i];
-      T->h_row_ptr[s0 + u + 1] = q;
-    }
-    auto fill = [&](uint64_t lo, uint64_t hi) {
-      for (uint64_t u = lo; u < hi; u++) {
-        const uint32_t i = o2[u];
-        T->old_of_new[s0 + u] = s0 + i;
-        T->new_of_old[s0 + i] = s0 + (uint32_t)u;
-        // (the start of the terrain's first row is its own offset: the entry
-        // before it belongs to the previous terrain)
-        uint32_t e = u == 0 ? (uint32_t)q0[g] : T->h_row_ptr[s0 + u];
-        for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++, e++) {
-          T->h_col[e] = s0 + n2[d->col[k]];
-          T->h_len[e] = d->length[k];
+  if (device_path) {
+    // outlets checked on host threads (reported after the terrain's row errors)
+    slot.reset(new uint32_t[n]());
+    each_terrain([&](uint32_t g) {
+      rc[g] = check_outlets(&descs[g], slot.get() + T->site_off[g]);
+      if (rc[g] != FL_OK) msg[g] = g_err;
+    });
+  }
+  // tan(max slope) on the host, with the reference's libm; NaN: no clamp
+  std::unique_ptr<double[]> tan_ms;
+  bool clamp = false;
+  for (uint32_t g = 0; g < count; g++) clamp |= descs[g].n > 0 && descs[g].has_max_slope != nullptr;
+  if (clamp) {
+    tan_ms.reset(new double[n]);
+    std::atomic<bool> any{false};
+    each_terrain([&](uint32_t g) {
+      const fl_terrain_desc* d = &descs[g];
+      double* dst = tan_ms.get() + T->site_off[g];
+      auto run = [&](uint64_t a, uint64_t b) {
+        bool some = false;
+        for (uint64_t i = a; i < b; i++) {
+          dst[i] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
+          some |= !std::isnan(dst[i]);
         }
-        area[s0 + u] = d->area[i];
-        erod[s0 + u] = d->erodibility[i];
-        uplift[s0 + u] = d->uplift_rate[i];
-        init[s0 + u] = d->init_elevation[i];
-        tan_ms[s0 + u] = (d->has_max_slope && d->has_max_slope[i]) ? std::tan(d->max_slope[i]) : std::nan("");
-        if (count > 1) tid[s0 + u] = g;
+        if (some) any.store(true, std::memory_order_relaxed);
+      };
+      if (count == 1)
+        host_parallel(d->n, 1 << 16, run);
+      else
+        run(0, d->n);
+    });
+    clamp = any.load();
+  }
+  fl_terrain* P = T.get();
+  fl_dev_terrain& t = T->t;
+  hipStream_t s = c->stream;
+  if (!device_path) {
+    // host path: breadth-first order and copies per terrain (rows and site
+    // values over host threads for one terrain)
+    const auto c1 = std::chrono::steady_clock::now();
+    T->old_of_new.resize(n);
+    T->new_of_old.resize(n);
+    T->h_outlets.resize(K);
+    std::vector<uint32_t> h_row_ptr((size_t)n + 1, 0), h_col(nnz);
+    std::vector<double> h_len(nnz), area(n), erod(n), uplift(n), init(n);
+    std::vector<uint8_t> is_out(n, 0);
+    auto relabel = [&](uint32_t g, bool inner) {
+      const fl_terrain_desc* d = &descs[g];
+      const uint32_t s0 = T->site_off[g], ng = (uint32_t)d->n;
+      if (ng == 0) return;
+      std::vector<uint32_t> o2, n2;
+      locality_order(ng, d->row_ptr, d->col, o2, n2);
+      uint32_t q = T->entry_off[g];
+      for (uint32_t u = 0; u < ng; u++) {
+        const uint32_t i = o2[u];
+        q += d->row_ptr[i + 1] - d->row_ptr[i];
+        h_row_ptr[s0 + u + 1] = q;
+      }
+      auto fill = [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t u = lo; u < hi; u++) {
+          const uint32_t i = o2[u];
+          T->old_of_new[s0 + u] = s0 + i;
+          T->new_of_old[s0 + i] = s0 + (uint32_t)u;
+          // (the start of the terrain's first row is its own offset: the
+          // entry before it belongs to the previous terrain)
+          uint32_t e = u == 0 ? T->entry_off[g] : h_row_ptr[s0 + u];
+          for (uint32_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; k++, e++) {
+            h_col[e] = s0 + n2[d->col[k]];
+            h_len[e] = d->length[k];
+          }
+          area[s0 + u] = d->area[i];
+          erod[s0 + u] = d->erodibility[i];
+          uplift[s0 + u] = d->uplift_rate[i];
+          init[s0 + u] = d->init_elevation[i];
+        }
+      };
+      if (inner)
+        host_parallel(ng, 65536, fill);
+      else
+        fill(0, ng);
+      for (uint64_t k = 0; k < d->n_outlets; k++) {
+        uint32_t o = s0 + n2[d->outlets[k]];
+        T->h_outlets[T->outlet_off[g] + k] = o;
+        is_out[o] = 1;
       }
     };
-    if (inner)
-      host_parallel(ng, 65536, fill);
+    if (count == 1)
+      relabel(0, true);
     else
-      fill(0, ng);
-    for (uint64_t k = 0; k < d->n_outlets; k++) {
-      uint32_t o = s0 + n2[d->outlets[k]];
-      T->h_outlets[T->outlet_off[g] + k] = o;
-      is_out[o] = 1;
+      host_each(
+          count, [&](uint32_t g) { return descs[g].nnz; }, [&](uint32_t g) { relabel(g, false); });
+    std::vector<double> tan_new;
+    if (clamp) {
+      tan_new.resize(n);
+      for (uint32_t u = 0; u < n; u++) tan_new[u] = tan_ms[T->old_of_new[u]];
     }
-  };
-  const auto c1 = std::chrono::steady_clock::now();
-  if (count == 1)
-    relabel(0, true);
-  else
-    host_each(
-        count, [&](uint32_t g) { return descs[g].nnz; }, [&](uint32_t g) { relabel(g, false); });
-
-  fl_dev_terrain& t = T->t;
-  fl_terrain* P = T.get();
-  bool clamp = false;
-  for (uint32_t u = 0; u < n; u++) clamp |= !std::isnan(tan_ms[u]);
-  const auto c2 = std::chrono::steady_clock::now();
-  FLCHK(alloc_device(c, P, n, nnz, count, clamp));
-  hipStream_t s = c->stream;
-  T->entry_off.resize(count + 1);
-  for (uint32_t g = 0; g <= count; g++) T->entry_off[g] = T->h_row_ptr[T->site_off[g]];
-  if (count > 1) HIPCHK(hipMemcpyAsync(T->d_tid, tid.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.row_ptr, T->h_row_ptr.data(), ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
-  if (nnz) {
-    HIPCHK(hipMemcpyAsync(t.col, T->h_col.data(), (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(t.len, T->h_len.data(), (uint64_t)nnz * 8, hipMemcpyHostToDevice, s));
+    outlet_rows(P, descs, count, slot.get());
+    const auto c2 = std::chrono::steady_clock::now();
+    FLCHK(alloc_device(c, P, n, nnz, count, clamp));
+    HIPCHK(hipMemcpyAsync(t.row_ptr, h_row_ptr.data(), ((uint64_t)n + 1) * 4, hipMemcpyHostToDevice, s));
+    if (nnz) {
+      HIPCHK(hipMemcpyAsync(t.col, h_col.data(), (uint64_t)nnz * 4, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(t.len, h_len.data(), (uint64_t)nnz * 8, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipMemcpyAsync(t.area, area.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.erod, erod.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.uplift, uplift.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    if (clamp)
+      HIPCHK(hipMemcpyAsync(t.tan_ms, tan_new.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    else
+      fl_launch_fill_u32(s, (uint32_t*)t.tan_ms, 2ull * n, 0xffffffffu);  // NaN
+    HIPCHK(hipMemcpyAsync(t.init_h, init.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.is_outlet, is_out.data(), n, hipMemcpyHostToDevice, s));
+    {
+      fl_scratch S(s, &T->arena);
+      uint32_t* d_off = nullptr;
+      HIPCHK(S.alloc(&d_off, (uint64_t)count + 1));
+      HIPCHK(hipMemcpyAsync(d_off, T->site_off.data(), (count + 1ull) * 4, hipMemcpyHostToDevice, s));
+      if (count > 1) fl_device_fill_tid(s, n, count, d_off, T->d_tid);
+      FLCHK(reset_state(c, P));
+      // the pageable staging vectors above must outlive the copies
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    T->stat_create[0] = ms_(c1 - c0).count();
+    T->stat_create[1] = ms_(c2 - c1).count();
+    T->stat_create[2] = ms_(std::chrono::steady_clock::now() - c2).count();
+    *out = T.release();
+    return FL_OK;
   }
-  HIPCHK(hipMemcpyAsync(t.area, area.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.erod, erod.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.uplift, uplift.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.tan_ms, tan_ms.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.init_h, init.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(t.is_outlet, is_out.data(), n, hipMemcpyHostToDevice, s));
-  FLCHK(reset_state(c, P));
-  // the pageable staging vectors above must outlive the copies
-  HIPCHK(hipStreamSynchronize(s));
-  using ms_ = std::chrono::duration<double, std::milli>;
-  T->stat_create[0] = ms_(c1 - c0).count();
-  T->stat_create[1] = ms_(c2 - c1).count();
-  T->stat_create[2] = ms_(std::chrono::steady_clock::now() - c2).count();
+  // device path: the caller's arrays uploaded as given, checked, ordered and
+  // relabelled on the device
+  const auto c1 = std::chrono::steady_clock::now();
+  FLCHK(alloc_device(c, P, n, nnz, count, clamp));
+  fl_dev_work& w = T->w;
+  std::string err;
+  double t_check = 0.0, t_relabel = 0.0;
+  {
+    fl_scratch S(s, &T->arena);
+    Staging st;
+    hipError_t e = S.reserve(staging_bytes(n, nnz, K, count));
+    if (e == hipSuccess) e = alloc_staging(S, n, nnz, K, count, st);
+    if (e != hipSuccess) {
+      if (T->arena.kind == fl_arena::REGION)
+        return set_err(FL_ERR_TOO_LARGE, "device region of %llu bytes exhausted", (unsigned long long)T->arena.size);
+      return set_err(e == hipErrorOutOfMemory ? FL_ERR_OUT_OF_MEMORY : FL_ERR_HIP, "staging: %s", hipGetErrorString(e));
+    }
+    std::vector<uint32_t> meta(2ull * (count + 1));
+    std::copy(T->site_off.begin(), T->site_off.end(), meta.begin());
+    std::copy(T->entry_off.begin(), T->entry_off.end(), meta.begin() + count + 1);
+    HIPCHK(hipMemcpyAsync(st.meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s));
+    // per-site arrays stage in work arrays, which the solve writes before it
+    // reads them
+    double* raw[5] = {w.xp, w.taup, w.tau, w.dnext, w.parea};
+    for (uint32_t g = 0; g < count; g++) {
+      const fl_terrain_desc* d = &descs[g];
+      const uint64_t s0 = T->site_off[g], e0 = T->entry_off[g], ng = d->n, eg = d->nnz;
+      if (ng == 0) continue;
+      HIPCHK(hipMemcpyAsync(st.rp + s0 + g, d->row_ptr, (ng + 1) * 4, hipMemcpyHostToDevice, s));
+      if (eg) {
+        HIPCHK(hipMemcpyAsync(st.col + e0, d->col, eg * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(st.len + e0, d->length, eg * 8, hipMemcpyHostToDevice, s));
+      }
+      HIPCHK(hipMemcpyAsync(raw[0] + s0, d->area, ng * 8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(raw[1] + s0, d->erodibility, ng * 8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(raw[2] + s0, d->uplift_rate, ng * 8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(raw[3] + s0, d->init_elevation, ng * 8, hipMemcpyHostToDevice, s));
+    }
+    if (clamp) HIPCHK(hipMemcpyAsync(raw[4], tan_ms.get(), (uint64_t)n * 8, hipMemcpyHostToDevice, s));
+    const uint32_t* d_site_off = st.meta;
+    if (count > 1) fl_device_fill_tid(s, n, count, d_site_off, T->d_tid);
+    fl_raw_union R;
+    R.n = n;
+    R.nnz = nnz;
+    R.G = count;
+    R.site_off = d_site_off;
+    R.entry_off = st.meta + count + 1;
+    R.tid = count > 1 ? T->d_tid : nullptr;
+    R.rp = st.rp;
+    R.col = st.col;
+    const auto c2 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> bad, unpaired;
+    std::vector<uint8_t> differs;
+    int r = fl_device_check_graph(s, R, st.len, bad, unpaired, differs, &err, &T->arena);
+    if (r != FL_OK) return set_err(r, "graph checks: %s", err.c_str());
+    for (uint32_t g = 0; g < count; g++) {
+      if (bad[g] != 0xffffffffu || unpaired[g] != 0xffffffffu) {
+        row_error(&descs[g], bad[g] != 0xffffffffu ? bad[g] : unpaired[g], bad[g] == 0xffffffffu);
+        return fail(g, FL_ERR_INVALID, g_err);
+      }
+      if (rc[g] != FL_OK) return fail(g, rc[g], msg[g]);
+      T->t_asym[g] = differs[g];
+    }
+    const auto c3 = std::chrono::steady_clock::now();
+    t_check = ms_(c3 - c2).count();
+    // the outlets' rows on a host thread while the device orders and relabels
+    std::future<void> rows = std::async(std::launch::async, [&] { outlet_rows(P, descs, count, slot.get()); });
+    r = fl_device_locality(s, R, st.o2n, st.n2o, &T->stat_create_clusters, &err, &T->arena);
+    if (r == FL_OK) {
+      const double* src[5] = {raw[0], raw[1], raw[2], raw[3], clamp ? raw[4] : nullptr};
+      double* const dst[5] = {t.area, t.erod, t.uplift, t.init_h, t.tan_ms};
+      r = fl_device_relabel(s, R, st.len, st.o2n, st.n2o, t.row_ptr, t.col, t.len, src, dst, &err, &T->arena);
+    }
+    rows.get();
+    if (r != FL_OK) return set_err(r, "relabelling: %s", err.c_str());
+    T->old_of_new.resize(n);
+    T->new_of_old.resize(n);
+    HIPCHK(hipMemcpyAsync(T->old_of_new.data(), st.o2n, (uint64_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(T->new_of_old.data(), st.n2o, (uint64_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    T->h_outlets.resize(K);
+    for (uint32_t g = 0; g < count; g++)
+      for (uint64_t q = 0; q < descs[g].n_outlets; q++)
+        T->h_outlets[T->outlet_off[g] + q] = T->new_of_old[T->site_off[g] + descs[g].outlets[q]];
+    if (K) HIPCHK(hipMemcpyAsync(st.idx, T->h_outlets.data(), K * 4, hipMemcpyHostToDevice, s));
+    fl_device_mark_u8(s, n, (uint32_t)K, st.idx, t.is_outlet);
+    FLCHK(reset_state(c, P));
+    // the host arrays above must outlive the copies
+    HIPCHK(hipStreamSynchronize(s));
+    t_relabel = ms_(std::chrono::steady_clock::now() - c3).count();
+    T->stat_create[2] = ms_(c2 - c1).count();
+  }
+  T->stat_create[0] = ms_(c1 - c0).count() + t_check;
+  T->stat_create[1] = t_relabel;
+  T->stat_create_device = 1;
   *out = T.release();
   return FL_OK;
 }
@@ -1937,7 +2263,7 @@ extern "C" int32_t fl_generate_f64_noalloc(fl_ctx* c, const fl_terrain_desc* d, 
   bool clamp = false;
   if (d->has_max_slope)
     for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;
-  const uint64_t need = union_device_bytes(d->n, d->nnz, 1, clamp) + fl_msf_scratch_bytes((uint32_t)d->n, (uint32_t)d->nnz);
+  const uint64_t need = union_device_bytes(d->n, d->nnz, 1, clamp) + scratch_bytes(d->n, d->nnz, d->n_outlets, 1);
   if (need > size)
     return set_err(FL_ERR_TOO_LARGE,
                    "fl_generate_f64_noalloc: this terrain (%llu sites, %llu entries) needs %llu bytes of device "
@@ -2183,6 +2509,8 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->create_validate_ms = T->stat_create[0];
   out->create_relabel_ms = T->stat_create[1];
   out->create_upload_ms = T->stat_create[2];
+  out->create_on_device = T->stat_create_device;
+  out->create_clusters = T->stat_create_clusters;
   return FL_OK;
 }
 
@@ -2212,7 +2540,7 @@ extern "C" int32_t fl_request_device_bytes(const fl_terrain_desc* d, uint64_t* b
   bool clamp = false;
   if (d->has_max_slope)
     for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;
-  *bytes = union_device_bytes(d->n, d->nnz, 1, clamp) + fl_msf_scratch_bytes((uint32_t)d->n, (uint32_t)d->nnz);
+  *bytes = union_device_bytes(d->n, d->nnz, 1, clamp) + scratch_bytes(d->n, d->nnz, d->n_outlets, 1);
   return FL_OK;
 }
 

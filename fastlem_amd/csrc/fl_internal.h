@@ -44,6 +44,43 @@ inline uint64_t fl_round256(uint64_t bytes) { return ((bytes < 8 ? 8 : bytes) + 
 // A buffer of `bytes` from `a` (null = POOL) on stream s.  A full REGION
 // returns hipErrorOutOfMemory and sets *p to null.
 hipError_t fl_arena_alloc(fl_arena* a, hipStream_t s, void** p, uint64_t bytes);
+// Scratch taken from an arena and given back when the scope ends: pool
+// buffers are freed in stream order, region arenas rewind to where the scope
+// began.  reserve(bytes) on a pool takes one block of that size, from which
+// the scope's later buffers are cut (one pool allocation and one free for
+// the whole scope: the pool's cost grows with the number of its blocks).
+struct fl_scratch {
+  hipStream_t s;
+  fl_arena* arena;
+  uint64_t mark;
+  std::vector<void*> ptrs;
+  fl_arena block;  // the reserved block, as a region
+  fl_scratch(hipStream_t s_, fl_arena* a) : s(s_), arena(a), mark(a ? a->used : 0) {}
+  fl_scratch(const fl_scratch&) = delete;
+  fl_scratch& operator=(const fl_scratch&) = delete;
+  hipError_t reserve(uint64_t bytes) {
+    if ((arena && arena->kind != fl_arena::POOL) || !ptrs.empty() || block.base) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = fl_arena_alloc(arena, s, &p, bytes);
+    if (e != hipSuccess) return e;
+    ptrs.push_back(p);
+    block.kind = fl_arena::REGION;
+    block.base = (uint8_t*)p;
+    block.size = fl_round256(bytes);
+    return hipSuccess;
+  }
+  template <class T>
+  hipError_t alloc(T** p, uint64_t count) {
+    if (block.base) return fl_arena_alloc(&block, s, (void**)p, count * sizeof(T));
+    hipError_t e = fl_arena_alloc(arena, s, (void**)p, count * sizeof(T));
+    if (e == hipSuccess && (!arena || arena->kind == fl_arena::POOL)) ptrs.push_back((void*)*p);
+    return e;
+  }
+  ~fl_scratch() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+    if (arena && arena->kind != fl_arena::POOL) arena->used = mark;
+  }
+};
 
 // fl_order.hip, the device part of the lake-traversal visit order.  Scratch
 // comes from `arena` (null: the device pool) and is released before return.
@@ -78,6 +115,49 @@ void fl_device_order(hipStream_t s, uint32_t n, const uint32_t* rank, uint32_t* 
 // Peak scratch bytes of fl_device_len_check, fl_device_ranks and
 // fl_device_scatter_u32 (up to n pairs) for n sites and nnz entries.
 uint64_t fl_msf_scratch_bytes(uint32_t n, uint32_t nnz);
+
+// fl_create.hip, the device part of an upload.  The caller's terrains as
+// given ("raw"), packed as a union: terrain g holds union sites
+// [site_off[g], site_off[g+1]) and union entries [entry_off[g],
+// entry_off[g+1]); its n_g + 1 row offsets (terrain-local entry indices) sit
+// at rp[site_off[g] + g ...] and its neighbour indices are terrain-local.
+// Every pointer is a device pointer; tid (terrain of each site) is null for
+// one terrain.  Row offsets must be checked (spanning each terrain's entries,
+// non-decreasing) and rows at most FL_DEVICE_MAX_ROW long before any of
+// these run; the locality order and the relabelling need graphs that passed
+// fl_device_check_graph.
+enum { FL_DEVICE_MAX_ROW = 64 };
+struct fl_raw_union {
+  uint32_t n = 0, nnz = 0, G = 1;
+  const uint32_t* site_off = nullptr;   // [G + 1]
+  const uint32_t* entry_off = nullptr;  // [G + 1]
+  const uint32_t* tid = nullptr;        // [n] or null
+  const uint32_t* rp = nullptr;         // [n + G]
+  const uint32_t* col = nullptr;        // [nnz]
+};
+// tid[i] = the terrain holding union site i.
+void fl_device_fill_tid(hipStream_t s, uint32_t n, uint32_t G, const uint32_t* site_off, uint32_t* tid);
+// dst[0..n) = 0, then dst[idx[q]] = 1 for the `count` device indices.
+void fl_device_mark_u8(hipStream_t s, uint32_t n, uint32_t count, const uint32_t* idx, uint8_t* dst);
+// Per terrain: the smallest terrain-local entry index that is out of range,
+// a self-loop or a repeat within its row (UINT32_MAX: none); the smallest
+// entry i->j whose row j holds no i (among entries that pass the first
+// check); whether the two entries of some edge hold different length bits.
+int fl_device_check_graph(hipStream_t s, const fl_raw_union& R, const double* len, std::vector<uint32_t>& bad_entry,
+                          std::vector<uint32_t>& asym_entry, std::vector<uint8_t>& len_differs, std::string* err,
+                          fl_arena* arena);
+// A locality order of the union's sites that keeps every terrain in its own
+// site range (old_of_new, new_of_old: device, union labels).
+int fl_device_locality(hipStream_t s, const fl_raw_union& R, uint32_t* old_of_new, uint32_t* new_of_old,
+                       uint32_t* clusters, std::string* err, fl_arena* arena);
+// The relabelled graph (row_ptr [n+1], col and len [nnz], union labels) and
+// five per-site arrays dst[a][u] = src[a][old_of_new[u]] (a null source
+// gives NaN).
+int fl_device_relabel(hipStream_t s, const fl_raw_union& R, const double* len_raw, const uint32_t* old_of_new,
+                      const uint32_t* new_of_old, uint32_t* row_ptr, uint32_t* col, double* len,
+                      const double* const src[5], double* const dst[5], std::string* err, fl_arena* arena);
+// Peak scratch of the three calls above for n sites, nnz entries, G terrains.
+uint64_t fl_create_scratch_bytes(uint32_t n, uint32_t nnz, uint32_t G);
 
 // fl_capi.cpp: lanes (continuous batching, used by the serving loop).  A
 // member is one request's terrain in a lane: its caller tag, max_iteration,
@@ -115,6 +195,11 @@ uint64_t fl_terrain_arrays_bytes(const fl_terrain_desc* d);
 // sites [s0, s0 + nl) (its rows reference only those sites).
 int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
                               const uint32_t* outlets, size_t k, uint32_t* order_out);
+// The outlet phase from the outlets' own rows: outlet q (of k, in listed
+// order) has deg[q] entries, of which nb[nb_ptr[q] .. nb_ptr[q+1]) lead to
+// other outlets (their indices q'); order_out receives outlet indices.
+int fl_outlet_pop_order_compact(size_t k, const uint32_t* deg, const uint32_t* nb_ptr, const uint32_t* nb,
+                                uint32_t* order_out);
 int fl_lake_visit_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
                               const double* len, const uint32_t* outlets, size_t k, uint32_t rank_base,
                               uint32_t* rank_out);

@@ -454,21 +454,10 @@ inline dim3 grid_for(uint64_t n) { return dim3((uint32_t)((n + 255) / 256)); }
 // Scratch of one visit-order build from an arena (the device pool:
 // stream-ordered, no device-wide synchronisation); released when the build
 // returns (pool buffers freed, region buffers rewound).
-struct DevBuf {
-  hipStream_t s;
-  fl_arena* arena;
-  uint64_t mark;
-  std::vector<void*> ptrs;
-  DevBuf(hipStream_t s_, fl_arena* a) : s(s_), arena(a), mark(a ? a->used : 0) {}
-  hipError_t alloc(void** p, uint64_t bytes) {
-    hipError_t e = fl_arena_alloc(arena, s, p, bytes);
-    if (e == hipSuccess && (!arena || arena->kind == fl_arena::POOL)) ptrs.push_back(*p);
-    return e;
-  }
-  ~DevBuf() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-    if (arena && arena->kind != fl_arena::POOL) arena->used = mark;
-  }
+// Scratch of one call (fl_scratch) with byte-sized allocations.
+struct DevBuf : fl_scratch {
+  using fl_scratch::fl_scratch;
+  hipError_t alloc(void** p, uint64_t bytes) { return fl_scratch::alloc((uint8_t**)p, bytes); }
 };
 
 #define HIPTRY(call)                                               \
@@ -543,6 +532,20 @@ hipError_t rank_bufs(DevBuf& buf, uint32_t n, RankBufs& r) {
   return hipSuccess;
 }
 
+// Bytes of rank_bufs for n sites plus the terrain bases (G entries).
+uint64_t rank_scratch_bytes(uint32_t n, uint64_t G) {
+  fl_arena a;
+  a.kind = fl_arena::COUNT;
+  {
+    DevBuf buf(nullptr, &a);
+    RankBufs r;
+    rank_bufs(buf, n, r);
+    void* p;
+    buf.alloc(&p, 4 * G);
+  }
+  return a.peak;
+}
+
 // Pointer-jumping rounds launched `batch` at a time, each flagging movement
 // in its own word of r.flags (cleared beforehand); returns after the first
 // batch whose last round moved nothing, or an error past kMaxFlags rounds.
@@ -602,6 +605,7 @@ int fl_device_len_check(hipStream_t s, uint32_t n, const uint32_t* rp, const uin
   uint64_t entries = len_tables(G, entry_off, toff, tmask);
   unsigned long long* table = nullptr;
   uint32_t *d_meta = nullptr, *d_bad = nullptr;
+  HIPTRY(buf.reserve(fl_round256(entries * 8) + fl_round256(3ull * 4 * G)));
   HIPTRY(buf.alloc((void**)&table, entries * 8));
   HIPTRY(buf.alloc((void**)&d_meta, 3ull * 4 * G));
   uint32_t *d_toff = d_meta, *d_mask = d_meta + G;
@@ -625,6 +629,7 @@ int fl_device_ranks(hipStream_t s, uint32_t n, const uint32_t* rp, const uint32_
   if (n == 0) return FL_OK;
   auto t0 = std::chrono::steady_clock::now();
   DevBuf buf(s, arena);
+  HIPTRY(buf.reserve(rank_scratch_bytes(n, base.size())));
   RankBufs r;
   HIPTRY(rank_bufs(buf, n, r));
   uint32_t* d_base = nullptr;

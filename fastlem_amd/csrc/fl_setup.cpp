@@ -417,13 +417,16 @@ extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint
 // the vacated lowest Z position of that path.  None of this depends on the
 // values of P elements, so only their number (the unvisited neighbours of
 // each popped outlet) is needed.
-int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
-                              const uint32_t* outlets, size_t k, uint32_t* order_out) {
+namespace {
+
+// The simulation over k outlets named ids[0..k) in push order (their
+// listed order).  pushes(id) returns how many entries the popped outlet id
+// pushes (its neighbours not popped before it) and records it as popped;
+// ahead(id, rows) fetches its row (rows) or its row offsets (!rows) early.
+template <class Pushes, class Ahead>
+int outlet_pop_core(const uint32_t* ids, size_t k, Pushes pushes, Ahead ahead, uint32_t* order_out) {
   const uint32_t NONE = 0xffffffffu;
-  std::vector<uint32_t> zid(outlets, outlets + k);
-  std::vector<uint8_t> visited(nl, 0);
-  for (size_t q = 0; q < k; q++)
-    if (outlets[q] < s0 || outlets[q] - s0 >= nl) return FL_ERR_INVALID;
+  std::vector<uint32_t> zid(ids, ids + k);
   uint64_t len = k;
   for (size_t step = 0; step < k; step++) {
     uint64_t L = len;
@@ -431,11 +434,11 @@ int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
     // the next pop is one of the root's children and the one after it one
     // of their children (each sift-down moves one child up): fetch their
     // adjacency two pops ahead (row offsets) and one pop ahead (entries)
-    __builtin_prefetch(col + row_ptr[result]);
+    ahead(result, true);
     for (uint64_t q = 1; q <= 2 && q < k; q++)
-      if (zid[q] != NONE) __builtin_prefetch(col + row_ptr[zid[q]]);
+      if (zid[q] != NONE) ahead(zid[q], true);
     for (uint64_t q = 3; q <= 6 && q < k; q++)
-      if (zid[q] != NONE) __builtin_prefetch(row_ptr + zid[q]);
+      if (zid[q] != NONE) ahead(zid[q], false);
     uint64_t last = L - 1;
     uint32_t last_z = last < k ? zid[last] : NONE;
     if (last < k) zid[last] = NONE;
@@ -462,12 +465,55 @@ int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
     }
     if (result == NONE) return FL_ERR_INVALID;  // cannot happen with distinct outlets
     order_out[step] = result;
-    visited[result - s0] = 1;
-    uint64_t pushes = 0;
-    for (uint32_t e = row_ptr[result]; e < row_ptr[result + 1]; e++) pushes += visited[col[e] - s0] ? 0 : 1;
-    len += pushes;
+    len += pushes(result);
   }
   return FL_OK;
+}
+
+}  // namespace
+
+int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr, const uint32_t* col,
+                              const uint32_t* outlets, size_t k, uint32_t* order_out) {
+  for (size_t q = 0; q < k; q++)
+    if (outlets[q] < s0 || outlets[q] - s0 >= nl) return FL_ERR_INVALID;
+  std::vector<uint8_t> visited(nl, 0);
+  return outlet_pop_core(
+      outlets, k,
+      [&](uint32_t v) {
+        visited[v - s0] = 1;
+        uint64_t pushes = 0;
+        for (uint32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) pushes += visited[col[e] - s0] ? 0 : 1;
+        return pushes;
+      },
+      [&](uint32_t v, bool rows) {
+        if (rows)
+          __builtin_prefetch(col + row_ptr[v]);
+        else
+          __builtin_prefetch(row_ptr + v);
+      },
+      order_out);
+}
+
+int fl_outlet_pop_order_compact(size_t k, const uint32_t* deg, const uint32_t* nb_ptr, const uint32_t* nb,
+                                uint32_t* order_out) {
+  std::vector<uint32_t> ids(k);
+  for (size_t q = 0; q < k; q++) ids[q] = (uint32_t)q;
+  std::vector<uint8_t> popped(k, 0);
+  return outlet_pop_core(
+      ids.data(), k,
+      [&](uint32_t q) {
+        popped[q] = 1;
+        uint64_t before = 0;
+        for (uint32_t e = nb_ptr[q]; e < nb_ptr[q + 1]; e++) before += popped[nb[e]];
+        return (uint64_t)deg[q] - before;
+      },
+      [&](uint32_t q, bool rows) {
+        if (rows)
+          __builtin_prefetch(nb + nb_ptr[q]);
+        else
+          __builtin_prefetch(nb_ptr + q);
+      },
+      order_out);
 }
 
 extern "C" int fl_outlet_pop_order(size_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* outlets,

@@ -217,8 +217,12 @@ typedef struct fl_solve_stats {
   uint32_t compactions;     /* batch: times the running terrains were gathered */
   uint32_t redone;          /* pipelined iterations redone synchronously */
   /* fl_terrain_create / fl_batch_create: validation, relabelling (the
-   * locality order and the relabelled host arrays), allocation and upload */
+   * locality order and the relabelled arrays), allocation and upload */
   double create_validate_ms, create_relabel_ms, create_upload_ms;
+  uint32_t create_on_device;  /* 1: graphs checked and relabelled on the
+                                 device, 0: on host threads (rows longer
+                                 than 64 entries, or a header error) */
+  uint32_t create_clusters;   /* device path: clusters of the locality order */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

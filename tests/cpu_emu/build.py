@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "fastlem_amd", "csrc")
-SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_noalloc.hip", "fl_interp.hip", "fl_capi.cpp", "fl_setup.cpp", "fl_server.cpp"]
+SOURCES = ["fl_kernels.hip", "fl_order.hip", "fl_create.hip", "fl_noalloc.hip", "fl_interp.hip", "fl_capi.cpp", "fl_setup.cpp", "fl_server.cpp"]
 
 
 def lib_path(asan: bool = False) -> str:

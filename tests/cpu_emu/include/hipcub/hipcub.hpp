@@ -1,4 +1,5 @@
-// CPU emulation of hipcub::DeviceRadixSort::SortPairs (test infrastructure):
+// CPU emulation of hipcub::DeviceRadixSort::SortPairs and DeviceScan (test
+// infrastructure). The sort:
 // a stable sort on the keys' radix order (IEEE floats ordered as the radix
 // sort orders them: sign-flipped bit patterns).
 #pragma once
@@ -52,6 +53,41 @@ struct DeviceRadixSort {
     }
     std::copy(k2.begin(), k2.end(), keys_out);
     std::copy(v2.begin(), v2.end(), vals_out);
+    return hipSuccess;
+  }
+};
+
+// Prefix sums (inclusive and exclusive) over n items, sequentially.
+struct DeviceScan {
+  template <class T>
+  static hipError_t InclusiveSum(void* temp, size_t& temp_bytes, const T* in, T* out, int n,
+                                 hipStream_t stream = nullptr) {
+    (void)stream;
+    if (!temp) {
+      temp_bytes = 16;
+      return hipSuccess;
+    }
+    T acc = T(0);
+    for (int i = 0; i < n; i++) {
+      acc = acc + in[i];
+      out[i] = acc;
+    }
+    return hipSuccess;
+  }
+  template <class T>
+  static hipError_t ExclusiveSum(void* temp, size_t& temp_bytes, const T* in, T* out, int n,
+                                 hipStream_t stream = nullptr) {
+    (void)stream;
+    if (!temp) {
+      temp_bytes = 16;
+      return hipSuccess;
+    }
+    T acc = T(0);
+    for (int i = 0; i < n; i++) {
+      const T v = in[i];
+      out[i] = acc;
+      acc = acc + v;
+    }
     return hipSuccess;
   }
 };

@@ -88,6 +88,7 @@ def _emu_solve(L, terrs, compact=None):
         emu.check(L, L.fl_terrain_last_stats(b, ctypes.byref(st)))
         L.fl_terrain_destroy(b)
         _emu_solve.last_compactions = st.compactions
+        _emu_solve.last_stats = st
     finally:
         L.fl_ctx_destroy(ctx)
     return outs, its
@@ -105,6 +106,38 @@ def test_emulated_comb_many_light_children(emu_lib):
     ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
     assert it == rit
     assert _bits_equal(got, ref)
+
+
+def test_emulated_create_paths(emu_lib):
+    """Terrains whose rows hold at most 64 entries are checked, put in the
+    clustered locality order and relabelled on the device; a longer row (a
+    comb spine site with 63 leaves and two spine neighbours) sends the upload
+    to the host path, for a whole batch too.  Every path gives the oracle's
+    bits."""
+    from conftest import comb_terrain
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm
+    from oracle import flo
+
+    def oracle(t):
+        g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+        ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
+        return ref, rit
+
+    fbm = DeviceBatch.terrain_of(config_fbm(3000, 55.0, seed=4))
+    at64, at65 = comb_terrain(spine=6, leaves=62, seed=1), comb_terrain(spine=6, leaves=63, seed=2)
+    assert np.diff(at64["row_ptr"]).max() == 64 and np.diff(at65["row_ptr"]).max() == 65
+    for terrs, on_device in (([fbm], 1), ([at64], 1), ([at65], 0), ([fbm, at64], 1), ([fbm, at65, at64], 0)):
+        outs, its = _emu_solve(emu_lib, terrs)
+        st = _emu_solve.last_stats
+        assert st.create_on_device == on_device, [len(t["row_ptr"]) - 1 for t in terrs]
+        if on_device:
+            n = sum(len(t["row_ptr"]) - 1 for t in terrs)
+            assert n / 20 <= st.create_clusters <= n / 2, st.create_clusters
+        for t, got, it in zip(terrs, outs, its):
+            ref, rit = oracle(t)
+            assert it == rit
+            assert _bits_equal(got, ref)
 
 
 def test_emulated_batch_matches_solo(emu_lib):
