@@ -119,6 +119,14 @@ def build_streams(args) -> dict:
         out["served"] = terrain_stream(args.served_requests, args.served_min, args.served_max, 5151, "ocean",
                                        args.workers)
     if out:
+        # the spawn pool started multiprocessing's resource tracker; stop it
+        # so that no helper process outlives the run
+        from multiprocessing import resource_tracker
+
+        try:
+            resource_tracker._resource_tracker._stop()
+        except AttributeError:
+            pass
         log(f"streams built in {time.time() - t0:.1f}s: "
             + ", ".join(f"{k} {len(v)} terrains / {sum(len(t['row_ptr']) - 1 for t in v)} sites"
                         for k, v in out.items()))

@@ -74,6 +74,7 @@ struct fl_ctx {
                                          // speculation, most iterations redone)
   bool graphs = false;                   // pipelined iterations replayed from captured graphs
   bool forbid_graphs = false;            // no graph replay whatever the tuning (allocation-free calls)
+  bool recompute_all = false;            // every basin in every iteration (no dirty-basin selection)
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -105,8 +106,17 @@ struct fl_terrain {
   // elevation buffers: an iteration reads hb[i] and writes hb[(i+1) % 3], so
   // an iteration in flight never overwrites the input of the one before it
   double* hb[3] = {nullptr, nullptr, nullptr};
+  // the tree of the iteration that wrote hb[i] (receivers, distances, basin
+  // roots) is nxb[i], dxb[i], rtb[i]: an iteration reading hb[i] compares
+  // its tree with them; prev_valid: they hold the last iteration's tree (not
+  // so before a solve's first iteration or a union's first after a gather)
+  uint32_t* nxb[3] = {nullptr, nullptr, nullptr};
+  double* dxb[3] = {nullptr, nullptr, nullptr};
+  uint32_t* rtb[3] = {nullptr, nullptr, nullptr};
+  bool prev_valid = false;
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
+  bool gF[3] = {false, false, false};
   uint32_t* h_tslot[2] = {nullptr, nullptr};  // pinned per-terrain changed flags of iterations in flight
   uint8_t* h_mask = nullptr;                  // pinned staging of the active mask
   bool graphs_failed = false;
@@ -118,6 +128,7 @@ struct fl_terrain {
   double stat_order_parts[3] = {0.0, 0.0, 0.0};  // outlet phase, spanning forest, ranks
   double stat_create[3] = {0.0, 0.0, 0.0};       // validation, relabelling, allocation and upload
   uint32_t stat_create_device = 0, stat_create_clusters = 0;
+  uint64_t stat_recomputed = 0;  // sites of the recomputed basins, over the solve's iterations
   uint64_t bytes = 0;
   // where its device arrays come from (the device pool unless a caller
   // supplies a region); region buffers are never freed one by one
@@ -143,6 +154,21 @@ struct fl_terrain {
 };
 
 namespace {
+
+// The buffers of an iteration reading slot `in`: elevations hb[in] ->
+// hb[in+1], and its tree nxb/dxb/rtb[in+1] compared with nxb/dxb/rtb[in].
+void set_slots(fl_terrain* T, int in) {
+  const int out = (in + 1) % 3;
+  fl_dev_work& w = T->w;
+  w.h = T->hb[in];
+  w.h2 = T->hb[out];
+  w.next = T->nxb[out];
+  w.dnext = T->dxb[out];
+  w.next_prev = T->nxb[in];
+  w.dnext_prev = T->dxb[in];
+  w.root_prev = T->rtb[in];
+  w.root = T->rtb[out];
+}
 
 template <typename T>
 int32_t dev_alloc(fl_terrain* T_, T** p, uint64_t count) {
@@ -597,10 +623,11 @@ int32_t join_lakes(fl_ctx* c, fl_terrain* T) {
 
 // Decomposition of the receiver forest into heavy chains laid out by light
 // depth; ends with the control block on the host (bucket offsets, flags).
-int32_t decompose(fl_ctx* c, fl_terrain* T) {
+int32_t decompose(fl_ctx* c, fl_terrain* T, bool full) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
+  fl_launch_active(s, T->t, w, full, T->d_tid, T->d_active);
   fl_launch_children(s, T->t, w);
   uint32_t round = 0;
   uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
@@ -665,6 +692,13 @@ uint32_t tail_start(const std::vector<uint32_t>& off, uint32_t rounds, uint32_t 
   return rounds - l >= 2 ? l : rounds;
 }
 
+// Chain positions of all rounds: the sites of the basins an iteration
+// recomputed (0 when the offsets were not read back).
+uint64_t all_positions(const std::vector<uint32_t>& off) {
+  if (off.size() < 2) return 0;
+  return off.back();
+}
+
 uint64_t tail_positions(const std::vector<uint32_t>& off, uint32_t from, uint32_t rounds) {
   uint64_t units = 0;
   for (uint32_t l = from; l < rounds; l++) units += exact_grid(off, l).positions;
@@ -674,19 +708,20 @@ uint64_t tail_positions(const std::vector<uint32_t>& off, uint32_t from, uint32_
 // One outer iteration (generator.rs:141-209), synchronous form: the host
 // confirms pointer-jumping convergence and reads the bucket offsets before
 // the sweeps.  The decomposition is built speculatively before the lake test
-// (lakes occur in the first iteration only in practice).
-int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
+// (lakes occur in the first iteration only in practice).  `full`: every
+// basin is recomputed (no previous tree), else only the dirty ones.
+int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool* changed, bool* had_lake) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
   fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
   fl_launch_receivers(s, T->t, w);
-  FLCHK(decompose(c, T));
+  FLCHK(decompose(c, T, full));
   *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
   if (*had_lake) {
     FLCHK(join_lakes(c, T));
     fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-    FLCHK(decompose(c, T));
+    FLCHK(decompose(c, T, full));
   }
   uint32_t nb = T->prev_nb;
   const std::vector<uint32_t>& off = c->host_boff;
@@ -713,8 +748,9 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool* changed, bool* had_lake) {
 // rounds S..R-1 in the tail launches), enqueued on the context's stream or
 // recorded by a capture.  rec_* receive the profiler records of the launches
 // (when profiling).
-void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, std::vector<size_t>* rec_da,
-                       std::vector<size_t>* rec_tau, size_t* rec_tail, size_t* rec_scatter) {
+void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full,
+                       std::vector<size_t>* rec_da, std::vector<size_t>* rec_tau, size_t* rec_tail,
+                       size_t* rec_scatter) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
@@ -722,6 +758,7 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
   // the previous iteration used 2 * (R - 1) bucket entries at most
   fl_launch_clear_iter(s, w, std::min<uint32_t>(2 * R + 1, 2 * n + 2));
   fl_launch_receivers(s, T->t, w);
+  fl_launch_active(s, T->t, w, full, T->d_tid, T->d_active);
   fl_launch_children(s, T->t, w);
   for (uint32_t r = 0; r < J; r++) fl_launch_chain_jump(s, T->t, w, r);
   w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
@@ -761,10 +798,10 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
 // The pipelined iteration for (J, R) and the current buffer parity as a
 // graph, captured on first use.  False when capture is not possible (the
 // caller launches directly).
-bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S) {
+bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full) {
   if (T->graphs_failed) return false;
   int p = T->w.h == T->hb[0] ? 0 : (T->w.h == T->hb[1] ? 1 : 2);
-  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S) {
+  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S || T->gF[p] != full) {
     if (T->gexec[p]) {
       hipGraphExecDestroy(T->gexec[p]);
       T->gexec[p] = nullptr;
@@ -774,7 +811,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
       T->graphs_failed = true;
       return false;
     }
-    enqueue_fast_body(c, T, J, R, S, nullptr, nullptr, nullptr, nullptr);
+    enqueue_fast_body(c, T, J, R, S, full, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&T->gexec[p], graph, nullptr, nullptr, 0);
     if (graph) hipGraphDestroy(graph);
@@ -787,6 +824,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
     T->gJ[p] = J;
     T->gR[p] = R;
     T->gS[p] = S;
+    T->gF[p] = full;
   }
   return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
 }
@@ -795,6 +833,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
 struct Flight {
   int in = 0;
   int slot = 0;
+  bool full = false;  // every basin recomputed (no previous tree)
   uint32_t J = 0, R = 0, S = 0;
   std::vector<uint8_t> mask;  // terrains iterating in it
   std::vector<size_t> rec_da, rec_tau;
@@ -826,13 +865,13 @@ int32_t enqueue_fast(fl_ctx* c, fl_terrain* T, Flight& f) {
     f.S = (want < f.R && keep < f.R && want + 1 >= keep && want <= keep + 1) ? keep : want;
   }
   T->tail_split = f.S;
-  w.h = T->hb[f.in];
-  w.h2 = T->hb[(f.in + 1) % 3];
+  set_slots(T, f.in);
   f.rec_da.assign(f.R, SIZE_MAX);
   f.rec_tau.assign(f.R, SIZE_MAX);
   f.rec_tail[0] = f.rec_tail[1] = SIZE_MAX;
-  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R, f.S);
-  if (!graphed) enqueue_fast_body(c, T, f.J, f.R, f.S, &f.rec_da, &f.rec_tau, f.rec_tail, &f.rec_scatter);
+  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R, f.S, f.full);
+  if (!graphed)
+    enqueue_fast_body(c, T, f.J, f.R, f.S, f.full, &f.rec_da, &f.rec_tau, f.rec_tail, &f.rec_scatter);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tslot[f.slot], T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->host_slot[f.slot], w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -892,6 +931,7 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
 
 int32_t reset_state(fl_ctx* c, fl_terrain* T) {
   uint32_t n = T->t.n;
+  T->prev_valid = false;
   HIPCHK(hipMemcpyAsync(T->w.h, T->t.init_h, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(T->w.A, T->t.area, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return FL_OK;
@@ -1085,14 +1125,22 @@ int32_t layout_device(fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, b
   FLCHK(dev_alloc(P, &t.rank, n));
   FLCHK(dev_alloc(P, &t.order, n));
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.dnext, &w.xp, &w.taup, &w.parea}) FLCHK(dev_alloc(P, p, n));
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.xp, &w.taup, &w.parea}) FLCHK(dev_alloc(P, p, n));
+  for (int k = 0; k < 3; k++) {
+    FLCHK(dev_alloc(P, &P->dxb[k], n));
+    FLCHK(dev_alloc(P, &P->nxb[k], n));
+    FLCHK(dev_alloc(P, &P->rtb[k], n));
+  }
+  FLCHK(dev_alloc(P, &w.dirty, n));
+  FLCHK(dev_alloc(P, &w.act, n));
   FLCHK(dev_alloc(P, &w.plist, 4 * (uint64_t)n));
   FLCHK(dev_alloc(P, &P->hb[2], n));
   P->hb[0] = w.h;
   P->hb[1] = w.h2;
+  set_slots(P, 0);
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.next, &w.sub, &w.sub2, &w.hc, &w.pcnt, &w.chain_base, &w.chain_len, &w.head_base,
+  for (uint32_t** p : {&w.sub, &w.sub2, &w.hc, &w.pcnt, &w.chain_base, &w.chain_len, &w.head_base,
                        &w.perm, &w.ppos, &w.rpos})
     FLCHK(dev_alloc(P, p, n));
   FLCHK(dev_alloc(P, &w.cp, n));
@@ -1162,6 +1210,9 @@ int32_t alloc_device(fl_ctx* c, fl_terrain* P, uint32_t n, uint32_t nnz, uint32_
     P->h_mask = (uint8_t*)(P->h_tslot[1] + c->pin_cap);
   }
   hipStream_t s = c->stream;
+  // basin roots are site indices at all times (dirty-basin marks index by them)
+  for (int k = 0; k < 3; k++) fl_launch_fill_u32(s, P->rtb[k], n, 0);
+  P->prev_valid = false;
   fl_launch_fill_u32(s, w.bcnt, 2 * (uint64_t)n + 4, 0);
   fl_launch_fill_u32(s, w.bnodes, 2 * (uint64_t)n + 4, 0);
   fl_launch_fill_u32(s, w.bcur, 2 * (uint64_t)n + 4, 0);
@@ -1484,7 +1535,7 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
     HIPCHK(hipMemcpyAsync(st.meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s));
     // per-site arrays stage in work arrays, which the solve writes before it
     // reads them
-    double* raw[5] = {w.xp, w.taup, w.tau, w.dnext, w.parea};
+    double* raw[5] = {w.xp, w.taup, w.tau, T->dxb[0], w.parea};
     for (uint32_t g = 0; g < count; g++) {
       const fl_terrain_desc* d = &descs[g];
       const uint64_t s0 = T->site_off[g], e0 = T->entry_off[g], ng = d->n, eg = d->nnz;
@@ -1639,6 +1690,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   auto t0 = std::chrono::steady_clock::now();
   T->stat_syncs = 0;
   T->stat_order_ms = 0.0;
+  T->stat_recomputed = 0;
   T->stat_lake_iterations = 0;
   T->stat_compactions = 0;
   T->stat_redone = 0;
@@ -1654,8 +1706,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     if (T->t_active[g]) active_sites += T->site_off[g + 1] - T->site_off[g];
   }
   if (T->t.n == 0) return FL_OK;
-  T->w.h = T->hb[0];
-  T->w.h2 = T->hb[1];
+  set_slots(T, 0);
   FLCHK(reset_state(c, T));  // elevations start from the initial state even if nothing iterates
   if (active == 0) {
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1704,12 +1755,14 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       bool changed = false, lake = false;
       std::vector<uint8_t> mask(cur->t_active);
       FLCHK(upload_mask(mask));
-      cur->w.h = cur->hb[hin];
-      cur->w.h2 = cur->hb[(hin + 1) % 3];
-      FLCHK(one_iteration(c, cur, &changed, &lake));
+      set_slots(cur, hin);
+      const bool full = !cur->prev_valid || c->recompute_all;
+      cur->prev_valid = true;
+      FLCHK(one_iteration(c, cur, full, &changed, &lake));
       T->stat_lake_iterations += lake ? 1 : 0;
       for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
       confirm(mask, cur->d_tchanged ? cur->h_tchanged : nullptr, changed);
+      T->stat_recomputed += all_positions(cur->last_boff);
       hin = (hin + 1) % 3;
       it++;
     } else {
@@ -1729,6 +1782,8 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
         f.slot = next_slot;
         next_slot ^= 1;
         f.mask = mask;
+        f.full = !cur->prev_valid || c->recompute_all;
+        cur->prev_valid = true;
         FLCHK(enqueue_fast(c, cur, f));
         for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
         q.push_back(std::move(f));
@@ -1749,13 +1804,13 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
         T->stat_redone++;
         bool lake = false;
         FLCHK(upload_mask(f.mask));
-        cur->w.h = cur->hb[f.in];
-        cur->w.h2 = cur->hb[(f.in + 1) % 3];
-        FLCHK(one_iteration(c, cur, &changed, &lake));
+        set_slots(cur, f.in);
+        FLCHK(one_iteration(c, cur, f.full, &changed, &lake));
         T->stat_lake_iterations += lake ? 1 : 0;
         tflags = cur->d_tchanged ? cur->h_tchanged : nullptr;
       }
       confirm(f.mask, tflags, changed);
+      T->stat_recomputed += all_positions(cur->last_boff);
       hin = (f.in + 1) % 3;
       it++;
     }
@@ -1764,8 +1819,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     bool want = cur->nterr > 1 && active_sites * 2 < cur->t.n && cur->t.n - active_sites >= c->compact_min_idle;
     if (want) drain = true;
     if (drain && q.empty()) {
-      cur->w.h = cur->hb[hin];
-      cur->w.h2 = cur->hb[(hin + 1) % 3];
+      set_slots(cur, hin);
       fl_terrain* next_union = nullptr;
       std::vector<uint32_t> next_orig;
       FLCHK(compact_union(c, T, cur, caps, orig, &next_union, next_orig));
@@ -1783,8 +1837,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   // iterations still in flight after the last one that counted recompute a
   // fixed point (or masked terrains): wait for them, their results are unused
   HIPCHK(hipStreamSynchronize(s));
-  cur->w.h = cur->hb[hin];
-  cur->w.h2 = cur->hb[(hin + 1) % 3];
+  set_slots(cur, hin);
   if (cur != T) {
     for (uint32_t g = 0; g < cur->nterr; g++) {
       uint32_t s0 = cur->site_off[g], ng = cur->site_off[g + 1] - s0;
@@ -1855,7 +1908,7 @@ int32_t lane_gather(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_membe
   std::vector<Part> parts;
   std::vector<fl_lane_member> mem;
   if (cur) {
-    cur->w.h = cur->hb[L->hin];  // the last confirmed elevations
+    set_slots(cur, L->hin);  // the last confirmed elevations
     for (uint32_t g = 0; g < cur->nterr; g++)
       if (cur->t_active[g]) {
         parts.push_back({cur, g});
@@ -2048,10 +2101,11 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
   if (L->sync_next || !c->pipelined) {
     std::vector<uint8_t> mask(cur->t_active);
     FLCHK(lane_upload_mask(L, mask));
-    cur->w.h = cur->hb[L->hin];
-    cur->w.h2 = cur->hb[(L->hin + 1) % 3];
+    set_slots(cur, L->hin);
+    const bool full = !cur->prev_valid || c->recompute_all;
+    cur->prev_valid = true;
     bool changed = false, lake = false;
-    FLCHK(one_iteration(c, cur, &changed, &lake));
+    FLCHK(one_iteration(c, cur, full, &changed, &lake));
     for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
     confirm(mask, cur->h_tchanged);
     L->hin = (L->hin + 1) % 3;
@@ -2074,6 +2128,8 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
       f.slot = L->next_slot;
       L->next_slot ^= 1;
       f.mask = mask;
+      f.full = !cur->prev_valid || c->recompute_all;
+      cur->prev_valid = true;
       FLCHK(enqueue_fast(c, cur, f));
       for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
       L->q.push_back(std::move(f));
@@ -2094,9 +2150,8 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
       cur->stat_redone++;
       bool lake = false;
       FLCHK(lane_upload_mask(L, f.mask));
-      cur->w.h = cur->hb[f.in];
-      cur->w.h2 = cur->hb[(f.in + 1) % 3];
-      FLCHK(one_iteration(c, cur, &changed, &lake));
+      set_slots(cur, f.in);
+      FLCHK(one_iteration(c, cur, f.full, &changed, &lake));
       tflags = cur->h_tchanged;
     }
     confirm(f.mask, tflags);
@@ -2202,6 +2257,7 @@ int32_t upload_h(fl_ctx* c, fl_terrain* T, const double* elev) {
   for (uint32_t u = 0; u < n; u++) tmp[u] = elev[T->old_of_new[u]];
   HIPCHK(hipMemcpyAsync(T->w.h, tmp.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  T->prev_valid = false;  // given elevations: no previous tree
   return FL_OK;
 }
 }  // namespace
@@ -2333,7 +2389,7 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   bool changed = false, lake = false;
-  FLCHK(one_iteration(c, T, &changed, &lake));
+  FLCHK(one_iteration(c, T, true, &changed, &lake));
   FLCHK(download_f64_old(c, T, elev, T->w.h));
   FLCHK(download_idx_old(c, T, next_out, T->w.next));
   FLCHK(download_f64_old(c, T, area_out, T->w.A));
@@ -2355,6 +2411,12 @@ extern "C" int32_t fl_ctx_set_pipelining(fl_ctx* c, int32_t enable) {
   // mode 2 (minimal speculation, for tests) is held in the context's field
   c->tune.pipelining = (uint32_t)enable;
   c->tune_set |= 2;
+  return FL_OK;
+}
+
+extern "C" int32_t fl_ctx_set_recompute_all(fl_ctx* c, int32_t all) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  c->recompute_all = all != 0;
   return FL_OK;
 }
 
@@ -2511,6 +2573,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->create_upload_ms = T->stat_create[2];
   out->create_on_device = T->stat_create_device;
   out->create_clusters = T->stat_create_clusters;
+  out->recomputed_sites = T->stat_recomputed;
   return FL_OK;
 }
 
@@ -2558,6 +2621,7 @@ extern "C" int32_t fl_stage_basin_order(fl_ctx* c, fl_terrain* T, const double* 
   fl_launch_receivers(s, T->t, T->w);
   FLCHK(read_ctrl(c, T));
   if (host_flag(c, FL_FLAG_LAKE)) FLCHK(join_lakes(c, T));
+  fl_launch_active(s, T->t, T->w, true, nullptr, nullptr);
   fl_launch_children(s, T->t, T->w);  // child masks of the final tree
   // scratch: the chain-major permutation buffer and one counter
   uint32_t* d_order = T->w.perm;

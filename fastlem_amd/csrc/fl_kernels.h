@@ -107,6 +107,17 @@ struct fl_dev_work {
   // chains of at least long_min sites fold in a wavefront, shorter ones in
   // one lane (a tuning: results do not depend on it)
   uint32_t long_min = 6;
+  // Basins recomputed in an iteration.  A new elevation is a function of the
+  // drainage tree alone (and the static parameters), so a basin of the
+  // previous iteration's tree in which no site's receiver changed -- nor any
+  // receiver changed to or from one of its sites -- keeps its tree and its
+  // elevations bit for bit; only the other basins' sites are active.
+  const uint32_t* next_prev = nullptr;  // [n] the previous iteration's tree
+  const double* dnext_prev = nullptr;   // [n]
+  const uint32_t* root_prev = nullptr;  // [n] its basin (root) of every site
+  uint32_t* root = nullptr;             // [n] this iteration's, for the next one
+  uint8_t* dirty = nullptr;             // [n] by root: basin to recompute
+  uint8_t* act = nullptr;               // [n] site recomputed in this iteration
 };
 
 // Workgroup size of the per-site kernels launched by the calling thread (a
@@ -117,6 +128,11 @@ void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
 void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+// The active sites of an iteration: every site of the terrains still
+// iterating (`full`, or no previous tree), else those of the dirty basins
+// (tid, t_active: per-site terrain and per-terrain mask of a union, or null).
+void fl_launch_active(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
+                      const uint8_t* t_active);
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);

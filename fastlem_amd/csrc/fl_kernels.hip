@@ -222,11 +222,42 @@ __global__ void k_lake_flip(uint32_t n, const uint32_t* __restrict__ rp, const u
 // rows are rescanned by the consumers).  The heavy child (first with the
 // largest previous drainage area) continues u's chain; the choice only
 // shapes the work, never the results.
+// Basins of the previous iteration's tree that are recomputed: those holding
+// a site whose receiver or receiver distance changed, or that site's old or
+// new receiver.  Any other basin keeps every parent pointer and distance of
+// its tree, and no site of another basin attaches to it, so its drainage
+// areas, response times and elevations come out bit for bit as before.
+__global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const double* __restrict__ dnext,
+                        const uint32_t* __restrict__ next_prev, const double* __restrict__ dnext_prev,
+                        const uint32_t* __restrict__ root_prev, uint8_t* __restrict__ dirty) {
+  uint32_t v = gid();
+  if (v >= n) return;
+  const uint32_t a = next[v], b = next_prev[v];
+  if (a != b || __double_as_longlong(dnext[v]) != __double_as_longlong(dnext_prev[v])) {
+    dirty[root_prev[v]] = 1;
+    dirty[root_prev[a]] = 1;
+    dirty[root_prev[b]] = 1;
+  }
+}
+
+// act[v]: v belongs to a dirty basin (every site when `dirty` is null) of a
+// terrain still iterating.
+__global__ void k_active(uint32_t n, const uint32_t* __restrict__ root_prev, const uint8_t* __restrict__ dirty,
+                         const uint32_t* __restrict__ tid, const uint8_t* __restrict__ t_active,
+                         uint8_t* __restrict__ act) {
+  uint32_t v = gid();
+  if (v >= n) return;
+  bool a = !dirty || dirty[root_prev[v]];
+  if (tid) a = a && t_active[tid[v]];
+  act[v] = a ? 1 : 0;
+}
+
 __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const double* __restrict__ prev_area,
-                           unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc) {
+                           unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc,
+                           const uint8_t* __restrict__ act) {
   uint32_t u = gid();
-  if (u >= n) return;
+  if (u >= n || !act[u]) return;
   uint32_t b = rp[u], e = rp[u + 1];
   uint32_t heavy = FL_NONE;
   unsigned long long m = 0;
@@ -246,13 +277,20 @@ __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const ui
   hc[u] = heavy;
 }
 
-// Chain and root pointers for pointer jumping.
+// Chain and root pointers for pointer jumping (active sites).
 //   cp: (chain parent, 1), (v, 0) at a chain head
 //   rt: (receiver, 1 at a head that is not a root: a light edge), (v, 0) at a root
+// An inactive site keeps its basin: its root is carried to this iteration's.
 __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ hc,
-                             unsigned long long* __restrict__ cp, unsigned long long* __restrict__ rt) {
+                             unsigned long long* __restrict__ cp, unsigned long long* __restrict__ rt,
+                             const uint8_t* __restrict__ act, const uint32_t* __restrict__ root_prev,
+                             uint32_t* __restrict__ root_out) {
   uint32_t v = gid();
   if (v >= n) return;
+  if (!act[v]) {
+    root_out[v] = root_prev[v];
+    return;
+  }
   uint32_t p = next[v];
   bool root = (p == v);
   bool head = root || hc[p] != v;
@@ -273,18 +311,23 @@ __global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, cons
 // movement proves convergence (a pointer at a fixed point never moves), and
 // the rounds launched after it return at once.
 __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long long* rt,
-                             uint32_t* __restrict__ flag_set, const uint32_t* __restrict__ prev_flag) {
+                             uint32_t* __restrict__ flag_set, const uint32_t* __restrict__ prev_flag,
+                             const uint8_t* __restrict__ act) {
   uint32_t v = gid();
   // the site's own pairs are read together with the previous round's flag
+  // (an inactive site's pairs are stale and unused: no active site points
+  // at one, as a dirty basin holds its sites' whole paths)
   unsigned long long c = 0, r = 0;
+  bool on = false;
   if (v < n) {
+    on = act[v] != 0;
     c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
     r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
   }
   // a round after a round without movement has nothing to do
   if (prev_flag && *prev_flag == 0) return;
   bool moved = false;
-  if (v < n) {
+  if (on) {
     uint32_t a = pair_ptr(c), b = pair_ptr(r);
     unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
     unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
@@ -316,7 +359,7 @@ __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
 k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld,
-               uint32_t long_min) {
+               uint32_t long_min, const uint8_t* __restrict__ act) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
@@ -329,7 +372,7 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
   uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    if (v < n && hc[v] == FL_NONE) {
+    if (v < n && act[v] && hc[v] == FL_NONE) {
       unsigned long long r = rt[v];
       if (is_outlet[pair_ptr(r)]) {
         uint32_t lw = pair_cnt(r), len = pair_cnt(cp[v]) + 1, l = 2u * lw + (len >= long_min ? 1u : 0u);
@@ -431,7 +474,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
                const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
-               uint32_t* __restrict__ head_base, uint32_t long_min) {
+               uint32_t* __restrict__ head_base, uint32_t long_min, const uint8_t* __restrict__ act) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
@@ -445,7 +488,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     lc[j] = FL_NONE;
     ln[j] = 0;
-    if (v < n && hc[v] == FL_NONE) {
+    if (v < n && act[v] && hc[v] == FL_NONE) {
       unsigned long long r = rt[v];
       if (is_outlet[pair_ptr(r)]) {
         unsigned long long c = cp[v];
@@ -501,7 +544,8 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
                              uint32_t* __restrict__ bncur, const uint32_t* __restrict__ rp,
                              const uint32_t* __restrict__ col, const unsigned long long* __restrict__ cmask,
                              const uint32_t* __restrict__ hc, uint32_t* __restrict__ pcnt,
-                             uint32_t* __restrict__ plist, double* __restrict__ parea) {
+                             uint32_t* __restrict__ plist, double* __restrict__ parea,
+                             const uint8_t* __restrict__ act, uint32_t* __restrict__ root) {
   uint32_t v = gid();
   // read together with the guard (one load latency)
   const uint32_t mld = *maxld;
@@ -516,8 +560,9 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
     bcur[i] = 0;
     bncur[i] = 0;
   }
-  if (v >= n) return;
+  if (v >= n || !act[v]) return;
   uint32_t r = pair_ptr(rv);
+  root[v] = r;
   if (!is_outlet[r]) {
     A[v] = area[v];
     tau[v] = 0.0;
@@ -1272,33 +1317,34 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
-                              const uint8_t* __restrict__ active, uint32_t* __restrict__ tchanged) {
+                              const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged) {
   const uint32_t v = gid();
   // the site's pairs are read together with the guard (one load latency)
   const unsigned long long rv = v < n ? rt[v] : 0ull, cv = v < n ? cp[v] : 0ull;
+  const bool on = v < n && act[v];
   if (*guard != 0) return;  // pointer jumping has not converged: the iteration is redone
   bool changed = false;
   const uint32_t o = pair_ptr(rv);
-  if (v < n && is_outlet[o]) {
+  if (v < n && !on) {
+    // a site of a basin that keeps its tree (or of a terrain that stopped)
+    // keeps its elevation
+    h2[v] = h[v];
+  } else if (v < n && is_outlet[o]) {
     const uint32_t g = tid ? tid[v] : 0;
-    if (tid && !active[g]) {
-      h2[v] = h[v];
+    const uint32_t P = head_base[pair_ptr(cv)] + pair_cnt(cv);
+    const double tk = taup[P];
+    double nh;
+    if (clamp) {
+      nh = hnp[P];
     } else {
-      const uint32_t P = head_base[pair_ptr(cv)] + pair_cnt(cv);
-      const double tk = taup[P];
-      double nh;
-      if (clamp) {
-        nh = hnp[P];
-      } else {
-        const double to = taup[head_base[o]];
-        const double ho = h[o] + U[o] * rust_max(to - to, 0.0);
-        nh = (v == o) ? ho : ho + U[v] * rust_max(tk - to, 0.0);
-      }
-      tau[v] = tk;
-      h2[v] = nh;
-      changed = nh != h[v];
-      if (changed && tchanged) tchanged[g] = 1;
+      const double to = taup[head_base[o]];
+      const double ho = h[o] + U[o] * rust_max(to - to, 0.0);
+      nh = (v == o) ? ho : ho + U[v] * rust_max(tk - to, 0.0);
     }
+    tau[v] = tk;
+    h2[v] = nh;
+    changed = nh != h[v];
+    if (changed && tchanged) tchanged[g] = 1;
   }
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
@@ -1524,23 +1570,34 @@ void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
             t.is_outlet, w.next, w.dnext);
 }
 
+void fl_launch_active(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
+                      const uint8_t* t_active) {
+  if (!full) {
+    FL_LAUNCH(k_fill_u32, (t.n + 3) / 4, s, (t.n + 3) / 4, (uint32_t*)w.dirty, 0u);
+    FL_LAUNCH(k_dirty, t.n, s, t.n, w.next, w.dnext, w.next_prev, w.dnext_prev, w.root_prev, w.dirty);
+  }
+  FL_LAUNCH(k_active, t.n, s, t.n, w.root_prev, full ? (const uint8_t*)nullptr : (const uint8_t*)w.dirty, tid,
+            t_active, w.act);
+}
+
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc);
-  FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, (unsigned long long*)w.cp, (unsigned long long*)w.rt);
+  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc, w.act);
+  FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, (unsigned long long*)w.cp, (unsigned long long*)w.rt, w.act,
+            w.root_prev, w.root);
 }
 
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round) {
   uint32_t r = round < FL_MAX_JUMP_ROUNDS ? round : FL_MAX_JUMP_ROUNDS - 1;
   uint32_t* fset = w.ctrl + FL_CTRL_JUMPS + r;
   const uint32_t* prev = r > 0 ? w.ctrl + FL_CTRL_JUMPS + r - 1 : nullptr;
-  FL_LAUNCH(k_chain_jump, t.n, s, t.n, (unsigned long long*)w.cp, (unsigned long long*)w.rt, fset, prev);
+  FL_LAUNCH(k_chain_jump, t.n, s, t.n, (unsigned long long*)w.cp, (unsigned long long*)w.rt, fset, prev, w.act);
 }
 
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_count, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.bcnt, w.bnodes, w.counters + FL_CNT_MAXLD,
-                 w.long_min);
+                 w.long_min, w.act);
 }
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t rounds) {
@@ -1550,9 +1607,9 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
                  (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base,
-                 w.long_min);
+                 w.long_min, w.act);
   FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
-            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.pcnt, w.plist, w.parea);
+            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.pcnt, w.plist, w.parea, w.act, w.root);
 }
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
@@ -1600,7 +1657,7 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
   FL_LAUNCH(k_tau_scatter, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
-            t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, active,
+            t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, w.act,
             tchanged);
 }
 

@@ -135,6 +135,13 @@ int32_t fl_ctx_set_batch_compaction(fl_ctx *ctx, uint32_t min_idle_sites);
  * testing it); 3 is mode 1 with each iteration replayed from a captured HIP
  * graph.  Results do not depend on the mode. */
 int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
+/* Which basins an iteration recomputes.  A new elevation depends on the
+ * drainage tree alone, so by default (0) an iteration recomputes only the
+ * basins of the previous iteration's tree in which some receiver changed
+ * (or that a changed receiver leaves or enters); the others keep their
+ * elevations bit for bit.  1 recomputes every basin in every iteration, as
+ * the reference does (for measurement).  Results do not depend on it. */
+int32_t fl_ctx_set_recompute_all(fl_ctx *ctx, int32_t all);
 
 /* Launch tuning.  None of these change any result (every setting gives the
  * same bits); they shape the work:
@@ -223,6 +230,9 @@ typedef struct fl_solve_stats {
                                  device, 0: on host threads (rows longer
                                  than 64 entries, or a header error) */
   uint32_t create_clusters;   /* device path: clusters of the locality order */
+  uint64_t recomputed_sites;  /* sites whose basin was recomputed, summed over
+                                 the solve's iterations (n x iterations when
+                                 every basin is; see fl_ctx_set_recompute_all) */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

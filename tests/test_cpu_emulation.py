@@ -62,7 +62,7 @@ def test_emulated_capped_oscillation(emu_lib):
     assert _bits_equal(got, ref)
 
 
-def _emu_solve(L, terrs, compact=None):
+def _emu_solve(L, terrs, compact=None, recompute_all=None):
     """fl_batch_create/solve/download through the emulated library."""
     from cpu_emu import emu
     from fastlem_amd import _native
@@ -78,6 +78,8 @@ def _emu_solve(L, terrs, compact=None):
         emu.check(L, L.fl_batch_create(ctx, descs, len(terrs), ctypes.byref(b)))
         if compact is not None:
             emu.check(L, L.fl_ctx_set_batch_compaction(ctx, compact))
+        if recompute_all is not None:
+            emu.check(L, L.fl_ctx_set_recompute_all(ctx, recompute_all))
         its = np.zeros(len(terrs), np.uint32)
         emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
         for i, t in enumerate(terrs):
@@ -138,6 +140,30 @@ def test_emulated_create_paths(emu_lib):
             ref, rit = oracle(t)
             assert it == rit
             assert _bits_equal(got, ref)
+
+
+def test_emulated_dirty_basins_only(emu_lib):
+    """By default an iteration recomputes only the basins whose drainage tree
+    changed; recomputing every basin (the reference's work) gives the same
+    bits and iteration counts, alone and in a batch with caps, and counts
+    every reachable site in every iteration."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm, config_plain
+
+    fbm = DeviceBatch.terrain_of(config_fbm(4000, 63.0, seed=3))
+    plain = DeviceBatch.terrain_of(config_plain(1500), max_iteration=7)
+    for terrs in ([fbm], [fbm, plain]):
+        outs_all, its_all = _emu_solve(emu_lib, terrs, recompute_all=1)
+        every = _emu_solve.last_stats.recomputed_sites
+        outs, its = _emu_solve(emu_lib, terrs, recompute_all=0)
+        some = _emu_solve.last_stats.recomputed_sites
+        assert list(its) == list(its_all)
+        for a, b in zip(outs, outs_all):
+            assert _bits_equal(a, b)
+        if len(terrs) == 1:
+            n = len(fbm["row_ptr"]) - 1
+            assert every <= n * its[0] and every > 0.9 * n * its[0], (every, n, its[0])
+        assert some < 0.8 * every, (some, every)
 
 
 def test_emulated_batch_matches_solo(emu_lib):
