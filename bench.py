@@ -10,6 +10,10 @@ A step is one full solve: elevations reset to the jittered initial state in
 HBM, then outer iterations until no elevation changes (src/lem/generator.rs:
 140-210), including the lake-traversal visit order that a fresh generate()
 call computes.  value = sites * iterations / seconds, over the timed steps.
+Each iteration recomputes the basins whose drainage tree changed (the others
+keep their elevations bit for bit); `every_basin` times the same solve
+recomputing every basin in every iteration, as the reference does, and
+checks that it gives the same bits.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
 (algorithmic bytes per launch / HIP-event-timed launch duration) and of the
@@ -263,6 +267,47 @@ def cpu_baseline(w, init, cap=None) -> dict:
     return r
 
 
+def profiled_roofline(L, ctx, t, n, nnz, clamp):
+    """The roofline of the dominant kernel in one more (profiled) solve: HIP
+    events stamped by every dispatch on the library's stream.  Returns the
+    roofline object and the kernel time of the solve in ms."""
+    import ctypes
+
+    L.fl_ctx_set_profiling(ctx, 1)
+    L.fl_ctx_reset_profile(ctx)
+    L.fl_terrain_invalidate_cache(t._h)
+    t.solve()
+    names = ["k_receivers", "k_dirty", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count",
+             "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
+             "k_tau_fold<true>", "k_da_tail", "k_tau_tail<false>", "k_tau_tail<true>", "k_tau_scatter",
+             "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
+    prof = {}
+    for nm in names:
+        cnt = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        units = ctypes.c_uint64()
+        L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(units))
+        if cnt.value:
+            prof[nm] = (cnt.value, ms.value, units.value)
+    L.fl_ctx_set_profiling(ctx, 0)
+    total_kernel_ms = sum(v[1] for v in prof.values())
+    for nm, (cnt, ms, units) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        bpu = kernel_bytes_per_unit(nm, n, nnz, clamp)
+        gbs = bpu * units / (ms / 1e3) / 1e9 if bpu and ms else 0.0
+        log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us "
+            f"units/launch={units / cnt:11.1f} {gbs:7.1f} GB/s")
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    dcnt, dms, dunits = prof[dom]
+    avg_s = dms / dcnt / 1e3
+    bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp) * dunits / dcnt
+    achieved = bytes_per_launch / avg_s / 1e9 if bytes_per_launch else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+                "launches_per_solve": dcnt, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None}
+    return roofline, total_kernel_ms
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,49 +406,40 @@ def main() -> int:
     host_s = time.perf_counter() - ts
     host_buffers = {"seconds": host_s, "site_iter_per_s": float(n) * hit.value / host_s, "iterations": hit.value}
     log(f"host buffers (fl_generate_f64): {hit.value} iterations in {host_s:.3f}s")
-    # roofline of the dominant kernel: HIP events around every launch on the
-    # library's stream, one extra solve
-    L.fl_ctx_set_profiling(ctx, 1)
-    L.fl_ctx_reset_profile(ctx)
-    t.solve()
-    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count", "k_bucket_scan",
-             "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
-             "k_tau_fold<true>", "k_da_tail", "k_tau_tail<false>", "k_tau_tail<true>", "k_tau_scatter", "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32",
-             "k_fill_u64", "k_copy_u32"]
-    prof = {}
-    for nm in names:
-        cnt = ctypes.c_uint64()
-        ms = ctypes.c_double()
-        units = ctypes.c_uint64()
-        L.fl_ctx_kernel_time(ctx, nm.encode(), ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(units))
-        if cnt.value:
-            prof[nm] = (cnt.value, ms.value, units.value)
-    L.fl_ctx_set_profiling(ctx, 0)
+    # the same solve recomputing every basin in every iteration (the
+    # reference's work per iteration), outside the timed region: same bits
+    h_default = t.elevations()
+    L.fl_ctx_set_recompute_all(ctx, 1)
+    every = []
+    for _ in range(2):
+        L.fl_terrain_invalidate_cache(t._h)
+        ts = time.perf_counter()
+        it_all = t.solve()
+        every.append(time.perf_counter() - ts)
+    assert it_all == iters and np.array_equal(t.elevations().view(np.uint64), h_default.view(np.uint64)), \
+        "recomputing every basin changed the result"
+    every_stats = t.last_stats()
+    L.fl_ctx_set_recompute_all(ctx, 0)
+    every_basin = {"ms_per_step": min(every) * 1e3, "site_iter_per_s": float(n) * iters / min(every),
+                   "recomputed_fraction": every_stats["recomputed_sites"] / (float(n) * iters)}
+    log(f"every basin recomputed: {min(every) * 1e3:.1f} ms per solve")
     clamp = bool(w.max_slope is not None and np.any(~np.isnan(w.max_slope)))
-    total_kernel_ms = sum(v[1] for v in prof.values())
-    for nm, (cnt, ms, units) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-        bpu = kernel_bytes_per_unit(nm, n, nnz, clamp)
-        gbs = bpu * units / (ms / 1e3) / 1e9 if bpu and ms else 0.0
-        log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us "
-            f"units/launch={units / cnt:11.1f} {gbs:7.1f} GB/s")
-    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
-    dcnt, dms, dunits = prof[dom]
-    avg_s = dms / dcnt / 1e3
-    bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp) * dunits / dcnt
-    achieved = bytes_per_launch / avg_s / 1e9 if bytes_per_launch else None
+    roofline, total_kernel_ms = profiled_roofline(L, ctx, t, n, nnz, clamp)
     solve_gbs = iters * iteration_bytes(n, nnz, clamp) / (elapsed / args.steps) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
-                "launches_per_solve": dcnt, "kernel_share": dms / total_kernel_ms if total_kernel_ms else None,
-                "solve_algorithmic_GBps": solve_gbs, "solve_frac": solve_gbs / HBM_PEAK_GBS,
-                "iteration_bytes": iteration_bytes(n, nnz, clamp), "kernel_ms_per_solve": total_kernel_ms}
+    roofline.update({"solve_algorithmic_GBps": solve_gbs, "solve_frac": solve_gbs / HBM_PEAK_GBS,
+                     "iteration_bytes": iteration_bytes(n, nnz, clamp), "kernel_ms_per_solve": total_kernel_ms})
     if args.traffic_kib is not None:
         # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
         # scaled by the factor the same counters read for a copy kernel of known size
         roofline["traffic"] = args.traffic_kib * 1024.0
     else:
-        roofline["traffic"] = measured_traffic(dom, bytes_per_launch)
+        roofline["traffic"] = measured_traffic(roofline["kernel"], roofline["algorithmic_bytes_per_launch"])
+    L.fl_ctx_set_recompute_all(ctx, 1)
+    every_roof, every_kernel_ms = profiled_roofline(L, ctx, t, n, nnz, clamp)
+    L.fl_ctx_set_recompute_all(ctx, 0)
+    every_roof["kernel_ms_per_solve"] = every_kernel_ms
+    every_roof["traffic"] = measured_traffic(every_roof["kernel"], every_roof["algorithmic_bytes_per_launch"])
+    every_basin["roofline"] = every_roof
     t.close()
 
     legs = {}
@@ -433,11 +469,13 @@ def main() -> int:
                    "iterations": iters, "visit_order_ms": stats["visit_order_ms"],
                    "visit_order_parts_ms": {"outlet_phase": stats["outlet_phase_ms"], "forest": stats["forest_ms"],
                                             "ranks": stats["ranks_ms"]},
-                   "host_syncs_per_solve": stats["host_syncs"], "tuning": tuning.as_dict()},
+                   "host_syncs_per_solve": stats["host_syncs"], "tuning": tuning.as_dict(),
+                   "recomputed_fraction": stats["recomputed_sites"] / (float(n) * iters)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "host_buffers": host_buffers,
+        "every_basin": every_basin,
     }
     line.update(legs)
     print(json.dumps(line), flush=True)

@@ -113,10 +113,12 @@ struct fl_terrain {
   uint32_t* nxb[3] = {nullptr, nullptr, nullptr};
   double* dxb[3] = {nullptr, nullptr, nullptr};
   uint32_t* rtb[3] = {nullptr, nullptr, nullptr};
+  uint8_t* chgb[3] = {nullptr, nullptr, nullptr};  // per-site "elevation changed" of that iteration
   bool prev_valid = false;
+  bool after_lake = true;  // the last iteration joined lakes (its tree is not raw receivers)
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
-  bool gF[3] = {false, false, false};
+  int gF[3] = {0, 0, 0};
   uint32_t* h_tslot[2] = {nullptr, nullptr};  // pinned per-terrain changed flags of iterations in flight
   uint8_t* h_mask = nullptr;                  // pinned staging of the active mask
   bool graphs_failed = false;
@@ -168,6 +170,8 @@ void set_slots(fl_terrain* T, int in) {
   w.dnext_prev = T->dxb[in];
   w.root_prev = T->rtb[in];
   w.root = T->rtb[out];
+  w.chg_prev = T->chgb[in];
+  w.chg = T->chgb[out];
 }
 
 template <typename T>
@@ -627,8 +631,8 @@ int32_t decompose(fl_ctx* c, fl_terrain* T, bool full) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
-  fl_launch_active(s, T->t, w, full, T->d_tid, T->d_active);
-  fl_launch_children(s, T->t, w);
+  if (!full) fl_launch_dirty(s, T->t, w);
+  fl_launch_children(s, T->t, w, full, T->d_tid, T->d_active);
   uint32_t round = 0;
   uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
   uint32_t quiet = FL_NONE_ROUND;  // first round in which no pointer moved
@@ -710,12 +714,14 @@ uint64_t tail_positions(const std::vector<uint32_t>& off, uint32_t from, uint32_
 // the sweeps.  The decomposition is built speculatively before the lake test
 // (lakes occur in the first iteration only in practice).  `full`: every
 // basin is recomputed (no previous tree), else only the dirty ones.
-int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool* changed, bool* had_lake) {
+// `keep`: sites whose neighbourhood the previous iteration left unchanged
+// keep its receivers (its tree had no lake joins).
+int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* changed, bool* had_lake) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
   fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, w);
+  fl_launch_receivers(s, T->t, w, keep);
   FLCHK(decompose(c, T, full));
   *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
   if (*had_lake) {
@@ -748,7 +754,7 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool* changed, bool* 
 // rounds S..R-1 in the tail launches), enqueued on the context's stream or
 // recorded by a capture.  rec_* receive the profiler records of the launches
 // (when profiling).
-void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full,
+void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full, bool keep,
                        std::vector<size_t>* rec_da, std::vector<size_t>* rec_tau, size_t* rec_tail,
                        size_t* rec_scatter) {
   hipStream_t s = c->stream;
@@ -757,9 +763,9 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
   bool clamp = T->t.any_max_slope;
   // the previous iteration used 2 * (R - 1) bucket entries at most
   fl_launch_clear_iter(s, w, std::min<uint32_t>(2 * R + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, w);
-  fl_launch_active(s, T->t, w, full, T->d_tid, T->d_active);
-  fl_launch_children(s, T->t, w);
+  fl_launch_receivers(s, T->t, w, keep);
+  if (!full) fl_launch_dirty(s, T->t, w);
+  fl_launch_children(s, T->t, w, full, T->d_tid, T->d_active);
   for (uint32_t r = 0; r < J; r++) fl_launch_chain_jump(s, T->t, w, r);
   w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
   fl_launch_chain_count(s, T->t, w);
@@ -798,10 +804,11 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
 // The pipelined iteration for (J, R) and the current buffer parity as a
 // graph, captured on first use.  False when capture is not possible (the
 // caller launches directly).
-bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full) {
+bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_t S, bool full, bool keep) {
   if (T->graphs_failed) return false;
   int p = T->w.h == T->hb[0] ? 0 : (T->w.h == T->hb[1] ? 1 : 2);
-  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S || T->gF[p] != full) {
+  const int mode = (full ? 1 : 0) | (keep ? 2 : 0);
+  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S || T->gF[p] != mode) {
     if (T->gexec[p]) {
       hipGraphExecDestroy(T->gexec[p]);
       T->gexec[p] = nullptr;
@@ -811,7 +818,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
       T->graphs_failed = true;
       return false;
     }
-    enqueue_fast_body(c, T, J, R, S, full, nullptr, nullptr, nullptr, nullptr);
+    enqueue_fast_body(c, T, J, R, S, full, keep, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&T->gexec[p], graph, nullptr, nullptr, 0);
     if (graph) hipGraphDestroy(graph);
@@ -824,7 +831,7 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
     T->gJ[p] = J;
     T->gR[p] = R;
     T->gS[p] = S;
-    T->gF[p] = full;
+    T->gF[p] = mode;
   }
   return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
 }
@@ -834,6 +841,7 @@ struct Flight {
   int in = 0;
   int slot = 0;
   bool full = false;  // every basin recomputed (no previous tree)
+  bool keep = false;  // receivers kept where the neighbourhood did not change
   uint32_t J = 0, R = 0, S = 0;
   std::vector<uint8_t> mask;  // terrains iterating in it
   std::vector<size_t> rec_da, rec_tau;
@@ -869,9 +877,9 @@ int32_t enqueue_fast(fl_ctx* c, fl_terrain* T, Flight& f) {
   f.rec_da.assign(f.R, SIZE_MAX);
   f.rec_tau.assign(f.R, SIZE_MAX);
   f.rec_tail[0] = f.rec_tail[1] = SIZE_MAX;
-  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R, f.S, f.full);
+  bool graphed = !c->prof && c->graphs && launch_fast_graph(c, T, f.J, f.R, f.S, f.full, f.keep);
   if (!graphed)
-    enqueue_fast_body(c, T, f.J, f.R, f.S, f.full, &f.rec_da, &f.rec_tau, f.rec_tail, &f.rec_scatter);
+    enqueue_fast_body(c, T, f.J, f.R, f.S, f.full, f.keep, &f.rec_da, &f.rec_tau, f.rec_tail, &f.rec_scatter);
   if (T->d_tchanged)
     HIPCHK(hipMemcpyAsync(T->h_tslot[f.slot], T->d_tchanged, T->nterr * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->host_slot[f.slot], w.ctrl, FL_CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -932,6 +940,7 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
 int32_t reset_state(fl_ctx* c, fl_terrain* T) {
   uint32_t n = T->t.n;
   T->prev_valid = false;
+  T->after_lake = true;
   HIPCHK(hipMemcpyAsync(T->w.h, T->t.init_h, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(T->w.A, T->t.area, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return FL_OK;
@@ -1130,6 +1139,7 @@ int32_t layout_device(fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, b
     FLCHK(dev_alloc(P, &P->dxb[k], n));
     FLCHK(dev_alloc(P, &P->nxb[k], n));
     FLCHK(dev_alloc(P, &P->rtb[k], n));
+    FLCHK(dev_alloc(P, &P->chgb[k], n));
   }
   FLCHK(dev_alloc(P, &w.dirty, n));
   FLCHK(dev_alloc(P, &w.act, n));
@@ -1757,8 +1767,10 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       FLCHK(upload_mask(mask));
       set_slots(cur, hin);
       const bool full = !cur->prev_valid || c->recompute_all;
+      const bool keep = !full && !cur->after_lake;
       cur->prev_valid = true;
-      FLCHK(one_iteration(c, cur, full, &changed, &lake));
+      FLCHK(one_iteration(c, cur, full, keep, &changed, &lake));
+      cur->after_lake = lake;
       T->stat_lake_iterations += lake ? 1 : 0;
       for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
       confirm(mask, cur->d_tchanged ? cur->h_tchanged : nullptr, changed);
@@ -1783,7 +1795,9 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
         next_slot ^= 1;
         f.mask = mask;
         f.full = !cur->prev_valid || c->recompute_all;
+        f.keep = !f.full && !cur->after_lake;
         cur->prev_valid = true;
+        cur->after_lake = false;  // a flight that finds lakes is redone synchronously
         FLCHK(enqueue_fast(c, cur, f));
         for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
         q.push_back(std::move(f));
@@ -1805,7 +1819,8 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
         bool lake = false;
         FLCHK(upload_mask(f.mask));
         set_slots(cur, f.in);
-        FLCHK(one_iteration(c, cur, f.full, &changed, &lake));
+        FLCHK(one_iteration(c, cur, f.full, f.keep, &changed, &lake));
+        cur->after_lake = lake;
         T->stat_lake_iterations += lake ? 1 : 0;
         tflags = cur->d_tchanged ? cur->h_tchanged : nullptr;
       }
@@ -2103,9 +2118,11 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
     FLCHK(lane_upload_mask(L, mask));
     set_slots(cur, L->hin);
     const bool full = !cur->prev_valid || c->recompute_all;
+    const bool keep = !full && !cur->after_lake;
     cur->prev_valid = true;
     bool changed = false, lake = false;
-    FLCHK(one_iteration(c, cur, full, &changed, &lake));
+    FLCHK(one_iteration(c, cur, full, keep, &changed, &lake));
+    cur->after_lake = lake;
     for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
     confirm(mask, cur->h_tchanged);
     L->hin = (L->hin + 1) % 3;
@@ -2129,7 +2146,9 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
       L->next_slot ^= 1;
       f.mask = mask;
       f.full = !cur->prev_valid || c->recompute_all;
+      f.keep = !f.full && !cur->after_lake;
       cur->prev_valid = true;
+      cur->after_lake = false;  // a flight that finds lakes is redone synchronously
       FLCHK(enqueue_fast(c, cur, f));
       for (uint32_t g = 0; g < cur->nterr; g++) L->enq[g] += mask[g];
       L->q.push_back(std::move(f));
@@ -2151,7 +2170,8 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
       bool lake = false;
       FLCHK(lane_upload_mask(L, f.mask));
       set_slots(cur, f.in);
-      FLCHK(one_iteration(c, cur, f.full, &changed, &lake));
+      FLCHK(one_iteration(c, cur, f.full, f.keep, &changed, &lake));
+      cur->after_lake = lake;
       tflags = cur->h_tchanged;
     }
     confirm(f.mask, tflags);
@@ -2258,6 +2278,7 @@ int32_t upload_h(fl_ctx* c, fl_terrain* T, const double* elev) {
   HIPCHK(hipMemcpyAsync(T->w.h, tmp.data(), (uint64_t)n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   T->prev_valid = false;  // given elevations: no previous tree
+  T->after_lake = true;
   return FL_OK;
 }
 }  // namespace
@@ -2353,7 +2374,7 @@ extern "C" int32_t fl_stage_receivers(fl_ctx* c, fl_terrain* T, const double* el
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
-  fl_launch_receivers(c->stream, T->t, T->w);
+  fl_launch_receivers(c->stream, T->t, T->w, false);
   return download_idx_old(c, T, next_out, T->w.next);
 }
 
@@ -2366,7 +2387,7 @@ extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* 
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
-  fl_launch_receivers(c->stream, T->t, T->w);
+  fl_launch_receivers(c->stream, T->t, T->w, false);
   FLCHK(read_ctrl(c, T));
   bool lake = host_flag(c, FL_FLAG_LAKE) != 0;
   FLCHK(label_chain_ends(c, T));
@@ -2389,7 +2410,7 @@ extern "C" int32_t fl_stage_iteration(fl_ctx* c, fl_terrain* T, double* elev, ui
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   bool changed = false, lake = false;
-  FLCHK(one_iteration(c, T, true, &changed, &lake));
+  FLCHK(one_iteration(c, T, true, false, &changed, &lake));
   FLCHK(download_f64_old(c, T, elev, T->w.h));
   FLCHK(download_idx_old(c, T, next_out, T->w.next));
   FLCHK(download_f64_old(c, T, area_out, T->w.A));
@@ -2618,11 +2639,10 @@ extern "C" int32_t fl_stage_basin_order(fl_ctx* c, fl_terrain* T, const double* 
   FLCHK(upload_h(c, T, elev));
   hipStream_t s = c->stream;
   fl_launch_clear_iter(s, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, T->w);
+  fl_launch_receivers(s, T->t, T->w, false);
   FLCHK(read_ctrl(c, T));
   if (host_flag(c, FL_FLAG_LAKE)) FLCHK(join_lakes(c, T));
-  fl_launch_active(s, T->t, T->w, true, nullptr, nullptr);
-  fl_launch_children(s, T->t, T->w);  // child masks of the final tree
+  fl_launch_children(s, T->t, T->w, true, nullptr, nullptr);  // child masks of the final tree
   // scratch: the chain-major permutation buffer and one counter
   uint32_t* d_order = T->w.perm;
   uint32_t* d_len = T->w.ctrl + FL_CTRL_JUMPS;

@@ -118,22 +118,30 @@ struct fl_dev_work {
   uint32_t* root = nullptr;             // [n] this iteration's, for the next one
   uint8_t* dirty = nullptr;             // [n] by root: basin to recompute
   uint8_t* act = nullptr;               // [n] site recomputed in this iteration
+  const uint8_t* chg_prev = nullptr;    // [n] elevation changed by the previous iteration
+  uint8_t* chg = nullptr;               // [n] ... by this one
 };
 
 // Workgroup size of the per-site kernels launched by the calling thread (a
 // tuning: results do not depend on it; 64..1024, a multiple of 64).
 extern thread_local uint32_t fl_site_block;
 
-void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+// keep_unchanged: sites whose neighbourhood's elevations the previous
+// iteration did not change keep its receivers (valid when that iteration's
+// tree had no lake joins).
+void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged);
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
 void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-// The active sites of an iteration: every site of the terrains still
-// iterating (`full`, or no previous tree), else those of the dirty basins
-// (tid, t_active: per-site terrain and per-terrain mask of a union, or null).
-void fl_launch_active(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
-                      const uint8_t* t_active);
-void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+// Marks the dirty basins (after the receivers, which clear the marks, and
+// any lake joins).
+void fl_launch_dirty(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
+// Children, heavy children and chain pointers of the iteration's active
+// sites: every site of the terrains still iterating (`full`, or no previous
+// tree), else those of the dirty basins (tid, t_active: per-site terrain and
+// per-terrain mask of a union, or null).
+void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
+                        const uint8_t* t_active);
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round);
 void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 // Bucket offsets (scan), chain placement and the chain-major permutation;

@@ -102,21 +102,39 @@ __device__ __forceinline__ double first_len(const uint32_t* __restrict__ rp, con
 // next[i]: the neighbour with the strictly largest positive slope
 // (h_i - h_j)/len, earliest slot on ties; i itself for outlets and pits.
 // flags[FL_FLAG_LAKE] is raised when a non-outlet site has no receiver.
+// A receiver depends on the site's and its neighbours' elevations only: with
+// chg_prev (per-site flags of the elevations the previous iteration changed,
+// whose tree had no lake joins), a site whose neighbourhood did not change
+// keeps its previous receiver and distance.
 __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                             const double* __restrict__ len, const double* __restrict__ h,
                             const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ next,
-                            double* __restrict__ dnext, int* __restrict__ flags) {
+                            double* __restrict__ dnext, int* __restrict__ flags,
+                            const uint8_t* __restrict__ chg_prev, const uint32_t* __restrict__ next_prev,
+                            const double* __restrict__ dnext_prev, uint8_t* __restrict__ dirty) {
   uint32_t i = gid();
   if (i >= n) return;
+  if (dirty) dirty[i] = 0;  // the basin marks of this iteration start clear
   if (is_outlet[i]) {
     next[i] = i;
     dnext[i] = 1.0;
     return;
   }
+  const uint32_t b = rp[i], e = rp[i + 1];
+  if (chg_prev) {
+    bool same = chg_prev[i] == 0;
+    for (uint32_t k = b; k < e && same; ++k) same = chg_prev[col[k]] == 0;
+    if (same) {
+      const uint32_t pj = next_prev[i];
+      if (pj == i) flags[FL_FLAG_LAKE] = 1;
+      next[i] = pj;
+      dnext[i] = dnext_prev[i];
+      return;
+    }
+  }
   const double hi = h[i];
   double best = 0.0, d = 1.0;
   uint32_t bj = i;
-  const uint32_t b = rp[i], e = rp[i + 1];
   for (uint32_t k = b; k < e; ++k) {
     uint32_t j = col[k];
     double hj = h[j];
@@ -240,24 +258,22 @@ __global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const dou
   }
 }
 
-// act[v]: v belongs to a dirty basin (every site when `dirty` is null) of a
-// terrain still iterating.
-__global__ void k_active(uint32_t n, const uint32_t* __restrict__ root_prev, const uint8_t* __restrict__ dirty,
-                         const uint32_t* __restrict__ tid, const uint8_t* __restrict__ t_active,
-                         uint8_t* __restrict__ act) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  bool a = !dirty || dirty[root_prev[v]];
-  if (tid) a = a && t_active[tid[v]];
-  act[v] = a ? 1 : 0;
-}
-
+// Child slots and heavy child of every active site; act[u] (read by the
+// iteration's later kernels): u belongs to a dirty basin (every site when
+// `dirty` is null) of a terrain still iterating (tid, t_active: per-site
+// terrain and per-terrain mask of a union, or null).
 __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const double* __restrict__ prev_area,
                            unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc,
-                           const uint8_t* __restrict__ act) {
+                           uint8_t* __restrict__ act, const uint8_t* __restrict__ dirty,
+                           const uint32_t* __restrict__ root_prev, const uint32_t* __restrict__ tid,
+                           const uint8_t* __restrict__ t_active) {
   uint32_t u = gid();
-  if (u >= n || !act[u]) return;
+  if (u >= n) return;
+  bool a = !dirty || dirty[root_prev[u]];
+  if (tid) a = a && t_active[tid[u]];
+  act[u] = a ? 1 : 0;
+  if (!a) return;
   uint32_t b = rp[u], e = rp[u + 1];
   uint32_t heavy = FL_NONE;
   unsigned long long m = 0;
@@ -317,10 +333,9 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
   // the site's own pairs are read together with the previous round's flag
   // (an inactive site's pairs are stale and unused: no active site points
   // at one, as a dirty basin holds its sites' whole paths)
+  const bool on = v < n && act[v] != 0;
   unsigned long long c = 0, r = 0;
-  bool on = false;
-  if (v < n) {
-    on = act[v] != 0;
+  if (on) {
     c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
     r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
   }
@@ -328,15 +343,20 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
   if (prev_flag && *prev_flag == 0) return;
   bool moved = false;
   if (on) {
-    uint32_t a = pair_ptr(c), b = pair_ptr(r);
-    unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
-    unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
-    if (pair_ptr(ca) != a) {
-      __atomic_store_n(cp + v, pair(pair_ptr(ca), pair_cnt(c) + pair_cnt(ca)), __ATOMIC_RELAXED);
+    // two hops a round: (a, d) <- (a'', d + d' + d'') with (a', d') the
+    // pair of a and (a'', d'') that of a' (the reach triples per round)
+    const uint32_t a = pair_ptr(c), b = pair_ptr(r);
+    const unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
+    const unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
+    const bool mc = pair_ptr(ca) != a, mr = pair_ptr(rb) != b;
+    const unsigned long long ca2 = mc ? __atomic_load_n(cp + pair_ptr(ca), __ATOMIC_RELAXED) : 0ull;
+    const unsigned long long rb2 = mr ? __atomic_load_n(rt + pair_ptr(rb), __ATOMIC_RELAXED) : 0ull;
+    if (mc) {
+      __atomic_store_n(cp + v, pair(pair_ptr(ca2), pair_cnt(c) + pair_cnt(ca) + pair_cnt(ca2)), __ATOMIC_RELAXED);
       moved = true;
     }
-    if (pair_ptr(rb) != b) {
-      __atomic_store_n(rt + v, pair(pair_ptr(rb), pair_cnt(r) + pair_cnt(rb)), __ATOMIC_RELAXED);
+    if (mr) {
+      __atomic_store_n(rt + v, pair(pair_ptr(rb2), pair_cnt(r) + pair_cnt(rb) + pair_cnt(rb2)), __ATOMIC_RELAXED);
       moved = true;
     }
   }
@@ -549,7 +569,8 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
   uint32_t v = gid();
   // read together with the guard (one load latency)
   const uint32_t mld = *maxld;
-  const unsigned long long rv = v < n ? rt[v] : 0ull, cv = v < n ? cp[v] : 0ull;
+  const bool on = v < n && act[v];
+  const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   // the bucket histograms are used up (count, scan, place): clear the
   // entries this decomposition touched for the next one
@@ -560,7 +581,7 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
     bcur[i] = 0;
     bncur[i] = 0;
   }
-  if (v >= n || !act[v]) return;
+  if (!on) return;
   uint32_t r = pair_ptr(rv);
   root[v] = r;
   if (!is_outlet[r]) {
@@ -1317,11 +1338,12 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
                               const double* __restrict__ hnp, const double* __restrict__ U,
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
-                              const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged) {
+                              const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged,
+                              uint8_t* __restrict__ chg) {
   const uint32_t v = gid();
   // the site's pairs are read together with the guard (one load latency)
-  const unsigned long long rv = v < n ? rt[v] : 0ull, cv = v < n ? cp[v] : 0ull;
   const bool on = v < n && act[v];
+  const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
   if (*guard != 0) return;  // pointer jumping has not converged: the iteration is redone
   bool changed = false;
   const uint32_t o = pair_ptr(rv);
@@ -1329,7 +1351,7 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
     // a site of a basin that keeps its tree (or of a terrain that stopped)
     // keeps its elevation
     h2[v] = h[v];
-  } else if (v < n && is_outlet[o]) {
+  } else if (on && is_outlet[o]) {
     const uint32_t g = tid ? tid[v] : 0;
     const uint32_t P = head_base[pair_ptr(cv)] + pair_cnt(cv);
     const double tk = taup[P];
@@ -1346,6 +1368,9 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
     changed = nh != h[v];
     if (changed && tchanged) tchanged[g] = 1;
   }
+  // per-site change flags for the next iteration's receivers (sites outside
+  // every basin kept their elevation in k_chain_perm)
+  if (v < n) chg[v] = changed ? 1 : 0;
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
 
@@ -1549,8 +1574,9 @@ thread_local uint32_t fl_site_block = 256;
   } while (0)
 #define FL_LAUNCH(kernel, count, stream, ...) FL_LAUNCH_U(kernel, count, count, stream, __VA_ARGS__)
 
-void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_receivers, t.n, s, t.n, t.row_ptr, t.col, t.len, w.h, t.is_outlet, w.next, w.dnext, w.flags);
+void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged) {
+  FL_LAUNCH(k_receivers, t.n, s, t.n, t.row_ptr, t.col, t.len, w.h, t.is_outlet, w.next, w.dnext, w.flags,
+            keep_unchanged ? w.chg_prev : (const uint8_t*)nullptr, w.next_prev, w.dnext_prev, w.dirty);
 }
 
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
@@ -1570,18 +1596,14 @@ void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
             t.is_outlet, w.next, w.dnext);
 }
 
-void fl_launch_active(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
-                      const uint8_t* t_active) {
-  if (!full) {
-    FL_LAUNCH(k_fill_u32, (t.n + 3) / 4, s, (t.n + 3) / 4, (uint32_t*)w.dirty, 0u);
-    FL_LAUNCH(k_dirty, t.n, s, t.n, w.next, w.dnext, w.next_prev, w.dnext_prev, w.root_prev, w.dirty);
-  }
-  FL_LAUNCH(k_active, t.n, s, t.n, w.root_prev, full ? (const uint8_t*)nullptr : (const uint8_t*)w.dirty, tid,
-            t_active, w.act);
+void fl_launch_dirty(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
+  FL_LAUNCH(k_dirty, t.n, s, t.n, w.next, w.dnext, w.next_prev, w.dnext_prev, w.root_prev, w.dirty);
 }
 
-void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
-  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc, w.act);
+void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
+                        const uint8_t* t_active) {
+  FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc, w.act,
+            full ? (const uint8_t*)nullptr : (const uint8_t*)w.dirty, w.root_prev, tid, t_active);
   FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, (unsigned long long*)w.cp, (unsigned long long*)w.rt, w.act,
             w.root_prev, w.root);
 }
@@ -1658,7 +1680,7 @@ void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
   FL_LAUNCH(k_tau_scatter, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
             t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, w.act,
-            tchanged);
+            tchanged, w.chg);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {
