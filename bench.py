@@ -318,6 +318,9 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=None,
                     help="cap the CPU baseline's iterations (default: run it to convergence)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-comparisons", action="store_true",
+                    help="no host-buffer drop-in and no every-basin solves (profiler runs that must see only "
+                         "the timed solve's kind of work)")
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="measured HBM KiB per launch of the dominant kernel (rocprofv3 --pmc), for the roofline")
     ap.add_argument("--mode", choices=["single", "batched", "served", "copy"], default="single",
@@ -396,33 +399,35 @@ def main() -> int:
     stats = t.last_stats()
     tuning = _native.Tuning()
     _native.check(L.fl_ctx_tuning_for(ctx, n, 1, ctypes.byref(tuning)))
-    # the drop-in with host buffers (fl_generate_f64: validation, relabelling,
-    # upload over PCIe, solve, download), outside the timed region
-    out = np.empty(n, np.float64)
-    hit = ctypes.c_uint32()
-    ts = time.perf_counter()
-    _native.check(L.fl_generate_f64(ctx, ctypes.byref(t._desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                    ctypes.byref(hit)))
-    host_s = time.perf_counter() - ts
-    host_buffers = {"seconds": host_s, "site_iter_per_s": float(n) * hit.value / host_s, "iterations": hit.value}
-    log(f"host buffers (fl_generate_f64): {hit.value} iterations in {host_s:.3f}s")
-    # the same solve recomputing every basin in every iteration (the
-    # reference's work per iteration), outside the timed region: same bits
-    h_default = t.elevations()
-    L.fl_ctx_set_recompute_all(ctx, 1)
-    every = []
-    for _ in range(2):
-        L.fl_terrain_invalidate_cache(t._h)
+    host_buffers, every_basin = None, None
+    if not args.skip_comparisons:
+        # the drop-in with host buffers (fl_generate_f64: validation, relabelling,
+        # upload over PCIe, solve, download), outside the timed region
+        out = np.empty(n, np.float64)
+        hit = ctypes.c_uint32()
         ts = time.perf_counter()
-        it_all = t.solve()
-        every.append(time.perf_counter() - ts)
-    assert it_all == iters and np.array_equal(t.elevations().view(np.uint64), h_default.view(np.uint64)), \
-        "recomputing every basin changed the result"
-    every_stats = t.last_stats()
-    L.fl_ctx_set_recompute_all(ctx, 0)
-    every_basin = {"ms_per_step": min(every) * 1e3, "site_iter_per_s": float(n) * iters / min(every),
-                   "recomputed_fraction": every_stats["recomputed_sites"] / (float(n) * iters)}
-    log(f"every basin recomputed: {min(every) * 1e3:.1f} ms per solve")
+        _native.check(L.fl_generate_f64(ctx, ctypes.byref(t._desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        ctypes.byref(hit)))
+        host_s = time.perf_counter() - ts
+        host_buffers = {"seconds": host_s, "site_iter_per_s": float(n) * hit.value / host_s, "iterations": hit.value}
+        log(f"host buffers (fl_generate_f64): {hit.value} iterations in {host_s:.3f}s")
+        # the same solve recomputing every basin in every iteration (the
+        # reference's work per iteration), outside the timed region: same bits
+        h_default = t.elevations()
+        L.fl_ctx_set_recompute_all(ctx, 1)
+        every = []
+        for _ in range(2):
+            L.fl_terrain_invalidate_cache(t._h)
+            ts = time.perf_counter()
+            it_all = t.solve()
+            every.append(time.perf_counter() - ts)
+        assert it_all == iters and np.array_equal(t.elevations().view(np.uint64), h_default.view(np.uint64)), \
+            "recomputing every basin changed the result"
+        every_stats = t.last_stats()
+        L.fl_ctx_set_recompute_all(ctx, 0)
+        every_basin = {"ms_per_step": min(every) * 1e3, "site_iter_per_s": float(n) * iters / min(every),
+                       "recomputed_fraction": every_stats["recomputed_sites"] / (float(n) * iters)}
+        log(f"every basin recomputed: {min(every) * 1e3:.1f} ms per solve")
     clamp = bool(w.max_slope is not None and np.any(~np.isnan(w.max_slope)))
     roofline, total_kernel_ms = profiled_roofline(L, ctx, t, n, nnz, clamp)
     solve_gbs = iters * iteration_bytes(n, nnz, clamp) / (elapsed / args.steps) / 1e9
@@ -434,12 +439,13 @@ def main() -> int:
         roofline["traffic"] = args.traffic_kib * 1024.0
     else:
         roofline["traffic"] = measured_traffic(roofline["kernel"], roofline["algorithmic_bytes_per_launch"])
-    L.fl_ctx_set_recompute_all(ctx, 1)
-    every_roof, every_kernel_ms = profiled_roofline(L, ctx, t, n, nnz, clamp)
-    L.fl_ctx_set_recompute_all(ctx, 0)
-    every_roof["kernel_ms_per_solve"] = every_kernel_ms
-    every_roof["traffic"] = measured_traffic(every_roof["kernel"], every_roof["algorithmic_bytes_per_launch"])
-    every_basin["roofline"] = every_roof
+    if every_basin is not None:
+        L.fl_ctx_set_recompute_all(ctx, 1)
+        every_roof, every_kernel_ms = profiled_roofline(L, ctx, t, n, nnz, clamp)
+        L.fl_ctx_set_recompute_all(ctx, 0)
+        every_roof["kernel_ms_per_solve"] = every_kernel_ms
+        every_roof["traffic"] = measured_traffic(every_roof["kernel"], every_roof["algorithmic_bytes_per_launch"])
+        every_basin["roofline"] = every_roof
     t.close()
 
     legs = {}
