@@ -1,0 +1,182 @@
+"""On the GPU: the paths that skip or move work give the reference's bits.
+
+* Recomputing only the basins whose drainage tree changed against
+  recomputing every basin (fl_ctx_set_recompute_all) and against the oracle,
+  with and without max slopes, in the synchronous, pipelined and redo-heavy
+  modes.
+* The upload checked and relabelled on the device against the host path
+  (rows of 64 and 65 entries), alone and in batches, and the device checks'
+  first-error messages.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import graph_of, init_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def strip():
+    from fastlem_amd.workloads import config_strip
+
+    return config_strip(256, 128)
+
+
+def _create_error(terrains):
+    """(status, message) of fl_batch_create on the terrain dicts."""
+    from fastlem_amd import _native
+    from fastlem_amd.device import batch_descs
+
+    L = _native.lib()
+    keep = []
+    d = batch_descs(terrains, keep)
+    ctx = ctypes.c_void_p()
+    assert L.fl_ctx_create(0, ctypes.byref(ctx)) == 0
+    h = ctypes.c_void_p()
+    rc = L.fl_batch_create(ctx, d, len(terrains), ctypes.byref(h))
+    msg = L.fl_last_error().decode()
+    if rc == 0:
+        L.fl_terrain_destroy(h)
+    L.fl_ctx_destroy(ctx)
+    return rc, msg
+
+
+def _symmetric_edit(t, i, j_old, j_new):
+    """Terrain dict t with row i's entry j_old replaced by j_new (one-sided)."""
+    t = dict(t)
+    col = np.array(t["col"], np.uint32)
+    rp = t["row_ptr"]
+    k = rp[i] + int(np.nonzero(col[rp[i]:rp[i + 1]] == j_old)[0][0])
+    col[k] = j_new
+    t["col"] = col
+    return t
+
+
+def _max_slope_terrain(w0, rad):
+    from fastlem_amd.workloads import Workload
+
+    ms = np.full(w0.n, np.nan)
+    ms[: w0.n // 2] = rad
+    return Workload(w0.name, w0.model, w0.erodibility, w0.uplift, w0.is_outlet, w0.base, ms)
+
+
+@pytest.mark.parametrize("kind", ["fbm", "plain-max-slope", "strip"])
+def test_dirty_basins_match_every_basin(kind, plain10k, fbm20k, strip):
+    from oracle import flo
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceTerrain
+    from fastlem_amd.generator import context
+
+    w = {"fbm": fbm20k, "plain-max-slope": _max_slope_terrain(plain10k, 0.6), "strip": strip}[kind]
+    ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets(),
+                               max_slope=w.max_slope)
+    L = _native.lib()
+    ctx = context()
+    t = DeviceTerrain.from_workload(w)
+    try:
+        counted = {}
+        for mode in (1, 0, 2):
+            assert L.fl_ctx_set_pipelining(ctx, mode) == 0
+            for every in (1, 0):
+                assert L.fl_ctx_set_recompute_all(ctx, every) == 0
+                L.fl_terrain_invalidate_cache(t._h)
+                assert t.solve() == rit, (mode, every)
+                assert _bits_equal(ref, t.elevations()), (mode, every)
+                counted[(mode, every)] = t.last_stats()["recomputed_sites"]
+        # every reachable site in every iteration, or fewer
+        assert counted[(1, 1)] <= w.n * rit
+        assert counted[(1, 0)] < counted[(1, 1)]
+    finally:
+        L.fl_ctx_set_pipelining(ctx, 1)
+        L.fl_ctx_set_recompute_all(ctx, 0)
+        t.close()
+
+
+def test_batch_dirty_basins_with_caps(plain10k, fbm20k):
+    """A batch whose terrains stop at different iterations (one capped):
+    each terrain's bits and count equal its solo oracle run in both modes."""
+    from oracle import flo
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.generator import context
+
+    ws = [plain10k, fbm20k, plain10k]
+    caps = [0xFFFFFFFF, 0xFFFFFFFF, 9]
+    terrs = [DeviceBatch.terrain_of(w, max_iteration=c) for w, c in zip(ws, caps)]
+    L = _native.lib()
+    ctx = context()
+    b = DeviceBatch(terrs)
+    try:
+        for every in (0, 1):
+            assert L.fl_ctx_set_recompute_all(ctx, every) == 0
+            L.fl_terrain_invalidate_cache(b._h)
+            its = b.solve(compact_min_idle=1)
+            for i, (w, cap) in enumerate(zip(ws, caps)):
+                ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w),
+                                           w.outlets(), max_iteration=cap)
+                assert its[i] == rit, (every, i)
+                assert _bits_equal(ref, b.elevations(i)), (every, i)
+    finally:
+        L.fl_ctx_set_recompute_all(ctx, 0)
+        b.close()
+
+
+def test_upload_paths_match_oracle():
+    """Rows of at most 64 entries are checked and relabelled on the device,
+    a row of 65 sends the upload to host threads; every path gives the
+    oracle's bits, for single terrains and batches."""
+    from conftest import comb_terrain
+    from oracle import flo
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm
+
+    fbm = DeviceBatch.terrain_of(config_fbm(6000, 77.0, seed=12))
+    at64, at65 = comb_terrain(spine=6, leaves=62, seed=1), comb_terrain(spine=6, leaves=63, seed=2)
+    for terrs, on_device in (([fbm], 1), ([at64], 1), ([at65], 0), ([fbm, at64], 1), ([at65, fbm], 0)):
+        b = DeviceBatch(terrs)
+        try:
+            its = b.solve()
+            st = b.last_stats()
+            assert st["create_on_device"] == on_device
+            for i, t in enumerate(terrs):
+                g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+                ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"],
+                                           t["outlets"])
+                assert its[i] == rit
+                assert _bits_equal(ref, b.elevations(i))
+        finally:
+            b.close()
+
+
+def test_device_checks_report_the_first_error():
+    """The device checks name the entry a sequential scan of the rows meets
+    first, with the host path's message (a 300k-site terrain)."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_plain
+
+    big = DeviceBatch.terrain_of(config_plain(300_000, 548.0))
+    rp, col = big["row_ptr"], big["col"]
+    assert _create_error([big]) == (0, "")
+    a = _symmetric_edit(big, 250_000, int(col[rp[250_000]]), 250_000)
+    r1 = 120_000
+    a = _symmetric_edit(a, r1, int(col[rp[r1] + 1]), int(col[rp[r1]]))
+    assert _create_error([a]) == (-1, f"duplicate edge {r1}-{int(col[rp[r1]])}")
+    b = _symmetric_edit(big, 200_000, int(col[rp[200_000]]), 300_000)
+    assert _create_error([b]) == (-1, f"col[{rp[200_000]}]=300000 out of range")
+    c = _symmetric_edit(big, 180_000, int(col[rp[180_000]]), 5)
+    c = _symmetric_edit(c, 90_000, int(col[rp[90_000]]), 5)
+    n = len(rp) - 1
+    cc = np.asarray(c["col"], np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    keys = np.sort(rows * n + cc)
+    rev = cc * n + rows
+    pos = np.minimum(np.searchsorted(keys, rev), len(keys) - 1)
+    bad_k = int(np.nonzero(keys[pos] != rev)[0][0])
+    assert _create_error([c]) == (-1, f"edge {int(rows[bad_k])}-{int(cc[bad_k])} is not symmetric")
