@@ -52,7 +52,8 @@ class SolveStats(ctypes.Structure):
                 ("ranks_ms", ctypes.c_double), ("compactions", ctypes.c_uint32), ("redone", ctypes.c_uint32),
                 ("create_validate_ms", ctypes.c_double), ("create_relabel_ms", ctypes.c_double),
                 ("create_upload_ms", ctypes.c_double), ("create_on_device", ctypes.c_uint32),
-                ("create_clusters", ctypes.c_uint32), ("recomputed_sites", ctypes.c_uint64)]
+                ("create_clusters", ctypes.c_uint32), ("recomputed_sites", ctypes.c_uint64),
+                ("first_sweep_rounds", ctypes.c_uint32), ("max_sweep_rounds", ctypes.c_uint32)]
 
 
 class ServerCounts(ctypes.Structure):

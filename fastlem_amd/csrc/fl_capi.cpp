@@ -116,6 +116,10 @@ struct fl_terrain {
   uint8_t* chgb[3] = {nullptr, nullptr, nullptr};  // per-site "elevation changed" of that iteration
   bool prev_valid = false;
   bool after_lake = true;  // the last iteration joined lakes (its tree is not raw receivers)
+  // w.A holds no drainage area of a recent tree (a solve's start, new
+  // terrains in the union): the next iteration takes subtree sizes as its
+  // heavy-child hint
+  bool hint_stale = false;
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
   int gF[3] = {0, 0, 0};
@@ -131,6 +135,7 @@ struct fl_terrain {
   double stat_create[3] = {0.0, 0.0, 0.0};       // validation, relabelling, allocation and upload
   uint32_t stat_create_device = 0, stat_create_clusters = 0;
   uint64_t stat_recomputed = 0;  // sites of the recomputed basins, over the solve's iterations
+  uint32_t stat_rounds_first = 0, stat_rounds_max = 0;  // sweep rounds: the solve's first iteration, any
   uint64_t bytes = 0;
   // where its device arrays come from (the device pool unless a caller
   // supplies a region); region buffers are never freed one by one
@@ -722,12 +727,29 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
   uint32_t n = T->t.n;
   fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
   fl_launch_receivers(s, T->t, w, keep);
-  FLCHK(decompose(c, T, full));
-  *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
-  if (*had_lake) {
-    FLCHK(join_lakes(c, T));
-    fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+  if (T->hint_stale) {
+    // the heavy children follow subtree sizes of the final tree: with the
+    // sites' own areas as the hint the chains are short and the light depth
+    // (sweep rounds) runs to ten times the usual
+    FLCHK(read_ctrl(c, T));
+    *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
+    if (*had_lake) {
+      FLCHK(join_lakes(c, T));
+      fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+    }
+    std::string err;
+    if (fl_device_subtree_sizes(s, n, w.next, T->d_tid, T->d_active, w.A, &err, &T->arena) != FL_OK)
+      return set_err(FL_ERR_HIP, "%s", err.c_str());
+    T->hint_stale = false;
     FLCHK(decompose(c, T, full));
+  } else {
+    FLCHK(decompose(c, T, full));
+    *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
+    if (*had_lake) {
+      FLCHK(join_lakes(c, T));
+      fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+      FLCHK(decompose(c, T, full));
+    }
   }
   uint32_t nb = T->prev_nb;
   const std::vector<uint32_t>& off = c->host_boff;
@@ -937,10 +959,17 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
   return FL_OK;
 }
 
+// Sweep rounds of iteration `it` of a solve into its statistics.
+void note_rounds(fl_terrain* T, uint32_t rounds, uint32_t it) {
+  if (it == 0) T->stat_rounds_first = rounds;
+  T->stat_rounds_max = std::max(T->stat_rounds_max, rounds);
+}
+
 int32_t reset_state(fl_ctx* c, fl_terrain* T) {
   uint32_t n = T->t.n;
   T->prev_valid = false;
   T->after_lake = true;
+  T->hint_stale = true;
   HIPCHK(hipMemcpyAsync(T->w.h, T->t.init_h, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(T->w.A, T->t.area, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return FL_OK;
@@ -1299,7 +1328,7 @@ uint64_t staging_bytes(uint64_t n, uint64_t nnz, uint64_t K, uint32_t G) {
 // order need at their peak.
 uint64_t scratch_bytes(uint64_t n, uint64_t nnz, uint64_t K, uint32_t G) {
   const uint64_t create = staging_bytes(n, nnz, K, G) + fl_create_scratch_bytes((uint32_t)n, (uint32_t)nnz, G);
-  return std::max(create, fl_msf_scratch_bytes((uint32_t)n, (uint32_t)nnz));
+  return std::max({create, fl_msf_scratch_bytes((uint32_t)n, (uint32_t)nnz), fl_subtree_scratch_bytes((uint32_t)n)});
 }
 
 // Upload `count` terrains as one union graph: terrain g's sites occupy
@@ -1701,6 +1730,8 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   T->stat_syncs = 0;
   T->stat_order_ms = 0.0;
   T->stat_recomputed = 0;
+  T->stat_rounds_first = 0;
+  T->stat_rounds_max = 0;
   T->stat_lake_iterations = 0;
   T->stat_compactions = 0;
   T->stat_redone = 0;
@@ -1760,8 +1791,11 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   };
   while (active) {
     if (cur != T) cur->stat_syncs = 0;
-    if (it == 0 || !c->pipelined) {
-      // the first iteration (lakes in practice) and the unpipelined form
+    // the first iteration (lakes in practice) and the unpipelined form; the
+    // second too: the first iteration's heavy children came from the sites'
+    // own areas (no drainage area yet), so its round counts (often ten times
+    // the usual) would make a poor speculation for the second
+    if (it < 2 || !c->pipelined) {
       bool changed = false, lake = false;
       std::vector<uint8_t> mask(cur->t_active);
       FLCHK(upload_mask(mask));
@@ -1775,6 +1809,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       for (uint32_t g = 0; g < cur->nterr; g++) enq[g] += mask[g];
       confirm(mask, cur->d_tchanged ? cur->h_tchanged : nullptr, changed);
       T->stat_recomputed += all_positions(cur->last_boff);
+      note_rounds(T, cur->last_rounds, it);
       hin = (hin + 1) % 3;
       it++;
     } else {
@@ -1826,6 +1861,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       }
       confirm(f.mask, tflags, changed);
       T->stat_recomputed += all_positions(cur->last_boff);
+      note_rounds(T, cur->last_rounds, it);
       hin = (f.in + 1) % 3;
       it++;
     }
@@ -1890,7 +1926,7 @@ struct fl_lane {
   std::vector<uint32_t> enq;
   std::vector<uint8_t> dev_mask;
   int next_slot = 0;
-  bool sync_next = true;
+  uint32_t sync_next = 2;  // iterations to run synchronously (after a join: its first and second)
   bool drain = false;          // enqueue nothing ahead: the lane is to become quiet
   bool compact_wanted = false;  // gather (drop the stopped terrains) once quiet
   uint32_t active = 0;
@@ -2018,7 +2054,8 @@ int32_t lane_gather(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_membe
   L->hin = 0;
   L->enq = L->cur->t_iters;
   L->dev_mask = L->cur->t_active;
-  L->sync_next = !join.empty();
+  L->sync_next = join.empty() ? 0u : 2u;
+  L->cur->hint_stale = !join.empty();
   L->drain = false;
   L->compact_wanted = false;
   L->active = G;
@@ -2127,7 +2164,7 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
     confirm(mask, cur->h_tchanged);
     L->hin = (L->hin + 1) % 3;
     result = cur->hb[L->hin];
-    L->sync_next = false;
+    if (L->sync_next) L->sync_next--;
   } else {
     // a drain for a join whose terrains went to another lane: go on
     if (L->drain && L->q.empty() && !L->compact_wanted) L->drain = false;
@@ -2595,6 +2632,8 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->create_on_device = T->stat_create_device;
   out->create_clusters = T->stat_create_clusters;
   out->recomputed_sites = T->stat_recomputed;
+  out->first_sweep_rounds = T->stat_rounds_first;
+  out->max_sweep_rounds = T->stat_rounds_max;
   return FL_OK;
 }
 

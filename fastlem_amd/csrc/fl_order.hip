@@ -316,6 +316,63 @@ __global__ void k_list_jump(uint32_t M, unsigned long long* list, uint32_t* __re
   if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
 
+// ------------------------------------------------ subtree sizes ----
+//
+// Sizes of the subtrees of a forest given by its parent pointers (next; a
+// root points at itself), from an Euler tour ranked by list ranking: arc 2v
+// enters non-root site v from its parent, arc 2v + 1 leaves it, and the
+// 2 (size(v) - 1) arcs between them are those of v's subtree.  The sites of
+// a terrain that is not active (tid, t_active) count as roots: their
+// pointers may be stale.
+__device__ __forceinline__ bool st_root(uint32_t v, uint32_t p, const uint32_t* tid, const uint8_t* t_active) {
+  return p == v || (tid && !t_active[tid[v]]);
+}
+
+// Children lists built with exchanges: a site's children come in no
+// particular order, which sizes do not depend on.
+__global__ void k_st_link(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ tid,
+                          const uint8_t* __restrict__ t_active, uint32_t* __restrict__ head,
+                          uint32_t* __restrict__ sib) {
+  uint32_t v = gid();
+  if (v >= n) return;
+  const uint32_t p = next[v];
+  sib[v] = st_root(v, p, tid, t_active) ? FL_NONE : atomicExch(head + p, v);
+}
+
+// (successor, 1) per arc: into v, then v's first child or back out of v;
+// out of v, then v's next sibling or out of v's parent (the tour's end when
+// the parent is a root: (FL_NONE, 1)).  A root's two slots are (FL_NONE, 0).
+__global__ void k_st_tour(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ tid,
+                          const uint8_t* __restrict__ t_active, const uint32_t* __restrict__ head,
+                          const uint32_t* __restrict__ sib, unsigned long long* __restrict__ tour) {
+  uint32_t v = gid();
+  if (v >= n) return;
+  const uint32_t p = next[v];
+  if (st_root(v, p, tid, t_active)) {
+    tour[2ull * v] = pr(FL_NONE, 0);
+    tour[2ull * v + 1] = pr(FL_NONE, 0);
+    return;
+  }
+  const uint32_t c = head[v], sb = sib[v];
+  tour[2ull * v] = pr(c != FL_NONE ? 2u * c : 2u * v + 1u, 1);
+  tour[2ull * v + 1] = pr(sb != FL_NONE ? 2u * sb : (st_root(p, next[p], tid, t_active) ? FL_NONE : 2u * p + 1u), 1);
+}
+
+// size(v) = (arcs to the end from the arc into v - those from the arc out of
+// v + 1) / 2; a root gets n.
+__global__ void k_st_size(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ tid,
+                          const uint8_t* __restrict__ t_active, const unsigned long long* __restrict__ tour,
+                          double* __restrict__ size) {
+  uint32_t v = gid();
+  if (v >= n) return;
+  if (st_root(v, next[v], tid, t_active)) {
+    size[v] = (double)n;
+    return;
+  }
+  const uint32_t a = pr_cnt(tour[2ull * v]), b = pr_cnt(tour[2ull * v + 1]);
+  size[v] = (double)((a - b + 1u) / 2u);
+}
+
 // Parent and weight of every forest site: of an edge's two arcs the one
 // earlier in the tour (more arcs after it) points away from S.  The weight
 // is the edge's position in length order; S weighs FL_NONE (infinite).
@@ -751,4 +808,49 @@ int fl_device_scatter_u32(hipStream_t s, uint32_t count, const uint32_t* idx, co
 
 void fl_device_order(hipStream_t s, uint32_t n, const uint32_t* rank, uint32_t* order) {
   if (n) hipLaunchKernelGGL(k_order_of_rank, grid_for(n), dim3(256), 0, s, n, rank, order);
+}
+
+uint64_t fl_subtree_scratch_bytes(uint32_t n) {
+  fl_arena a;
+  a.kind = fl_arena::COUNT;
+  {
+    DevBuf buf(nullptr, &a);
+    void* p;
+    buf.alloc(&p, 4ull * n);
+    buf.alloc(&p, 4ull * n);
+    buf.alloc(&p, 16ull * n);
+    buf.alloc(&p, 4ull * kMaxFlags);
+  }
+  return a.peak;
+}
+
+int fl_device_subtree_sizes(hipStream_t s, uint32_t n, const uint32_t* next, const uint32_t* tid,
+                            const uint8_t* t_active, double* size, std::string* err, fl_arena* arena) {
+  if (n == 0 || n >= (1u << 31)) return FL_OK;  // arc indices must fit 32 bits
+  DevBuf buf(s, arena);
+  HIPTRY(buf.reserve(fl_subtree_scratch_bytes(n)));
+  uint32_t *head = nullptr, *sib = nullptr, *flags = nullptr;
+  unsigned long long* tour = nullptr;
+  HIPTRY(buf.alloc((void**)&head, 4ull * n));
+  HIPTRY(buf.alloc((void**)&sib, 4ull * n));
+  HIPTRY(buf.alloc((void**)&tour, 16ull * n));
+  HIPTRY(buf.alloc((void**)&flags, 4ull * kMaxFlags));
+  const dim3 B(256), Gn = grid_for(n), G2 = grid_for(2ull * n);
+  hipLaunchKernelGGL(k_fill32, Gn, B, 0, s, n, head, FL_NONE);
+  hipLaunchKernelGGL(k_fill32, dim3(1), B, 0, s, kMaxFlags, flags, 0u);
+  hipLaunchKernelGGL(k_st_link, Gn, B, 0, s, n, next, tid, t_active, head, sib);
+  hipLaunchKernelGGL(k_st_tour, Gn, B, 0, s, n, next, tid, t_active, head, sib, tour);
+  // a list of L arcs is ranked after ceil(log2 L) rounds; one more proves it
+  uint32_t lg = 1;
+  while ((1ull << lg) < 2ull * n) lg++;
+  int rc = jump_to_fixpoint(
+      s, flags, kMaxFlags, lg + 1,
+      [&](uint32_t* fset, const uint32_t* prev) {
+        hipLaunchKernelGGL(k_list_jump, G2, B, 0, s, 2u * n, tour, fset, prev);
+      },
+      err, "subtree sizes");
+  if (rc != FL_OK) return rc;
+  hipLaunchKernelGGL(k_st_size, Gn, B, 0, s, n, next, tid, t_active, tour, size);
+  HIPTRY(hipGetLastError());
+  return FL_OK;
 }

@@ -233,6 +233,9 @@ typedef struct fl_solve_stats {
   uint64_t recomputed_sites;  /* sites whose basin was recomputed, summed over
                                  the solve's iterations (n x iterations when
                                  every basin is; see fl_ctx_set_recompute_all) */
+  uint32_t first_sweep_rounds; /* sweep rounds (light depth of the heavy-chain
+                                  decomposition, + 1) of the first iteration */
+  uint32_t max_sweep_rounds;   /*   and the most of any iteration */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

@@ -133,6 +133,12 @@ inline T atomicOr(T* p, U v) {
   *p = (T)(o | (T)v);
   return o;
 }
+template <class T, class U>
+inline T atomicExch(T* p, U v) {
+  T o = *p;
+  *p = (T)v;
+  return o;
+}
 template <class T, class U, class V>
 inline T atomicCAS(T* p, U cmp, V v) {
   T o = *p;

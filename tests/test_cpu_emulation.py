@@ -166,6 +166,24 @@ def test_emulated_dirty_basins_only(emu_lib):
         assert some < 0.8 * every, (some, every)
 
 
+def test_emulated_first_iteration_light_depth(emu_lib):
+    """A solve's first iteration picks heavy children by subtree size (no
+    drainage area exists yet), so every light edge at least halves the
+    subtree: its sweep rounds (light depth + 1) are at most floor(log2 n) + 1,
+    alone and in a batch with a frozen terrain's stale pointers."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm, config_plain
+
+    fbm = DeviceBatch.terrain_of(config_fbm(4000, 63.0, seed=5))
+    plain = DeviceBatch.terrain_of(config_plain(1500), max_iteration=2)
+    for terrs in ([fbm], [plain, fbm]):
+        n = sum(len(t["row_ptr"]) - 1 for t in terrs)
+        _emu_solve(emu_lib, terrs, compact=0)
+        st = _emu_solve.last_stats
+        assert 2 <= st.first_sweep_rounds <= int(np.log2(n)) + 1, (st.first_sweep_rounds, n)
+        assert st.max_sweep_rounds >= st.first_sweep_rounds
+
+
 def test_emulated_batch_matches_solo(emu_lib):
     """The batched union (per-terrain convergence, caps and freeze masks)
     against each terrain's oracle run."""
