@@ -417,55 +417,122 @@ extern "C" int fl_lake_visit_order(size_t n, const uint32_t* row_ptr, const uint
 // the vacated lowest Z position of that path.  None of this depends on the
 // values of P elements, so only their number (the unvisited neighbours of
 // each popped outlet) is needed.
+//
+// While the end element is P ("pure" pops) the process has a closed form: a
+// pop outputs the root and refills it from the right subtree while that
+// holds Z elements, then from the left one, so the outputs follow the
+// right-first preorder of the Z set, and the position a pop vacates (the end
+// of its path) is the first remaining one in right-first postorder.  One
+// depth-first walk therefore yields every pure pop; the walk is replayed
+// until a pop would find a Z element at the heap's end (the count of pushes
+// decides where the end is), and from the state there -- positions vacated
+// so far freed, the remaining values laid out in preorder -- the pops are
+// simulated one by one again until the end element is P.  After 16 such
+// returns the simulation goes on pop by pop.
 namespace {
+
+struct OutletHeap {
+  static constexpr uint32_t NONE = 0xffffffffu;
+  uint64_t k = 0;
+  std::vector<uint32_t> z;     // Z element at each heap position, NONE elsewhere (2k + 3 entries)
+  std::vector<uint32_t> val;   // walk: Z elements in right-first preorder
+  std::vector<uint32_t> vac;   // walk: right-first postorder index of each position
+  std::vector<uint32_t> post;  // walk: positions in right-first postorder
+  uint32_t pc = 0, qc = 0;
+
+  // Heap positions hold at most k Z elements, all in [0, k); the depth of the
+  // recursion is the heap's, at most 64.
+  void walk(uint64_t h) {
+    val[pc++] = z[h];
+    if (z[2 * h + 2] != NONE) walk(2 * h + 2);
+    if (z[2 * h + 1] != NONE) walk(2 * h + 1);
+    vac[h] = qc;
+    post[qc++] = (uint32_t)h;
+  }
+  void place(uint64_t h, const uint32_t* v) {
+    z[h] = v[pc++];
+    if (z[2 * h + 2] != NONE) place(2 * h + 2, v);
+    if (z[2 * h + 1] != NONE) place(2 * h + 1, v);
+  }
+  // One pop of a heap of `len` elements (len updated); returns the popped Z
+  // element.
+  uint32_t pop(uint64_t& len) {
+    const uint32_t result = z[0];
+    const uint64_t last = len - 1;
+    uint32_t last_z = NONE;
+    if (last < k) {
+      last_z = z[last];
+      z[last] = NONE;
+    }
+    len = last;
+    uint64_t h = 0;
+    for (;;) {
+      const uint32_t a = z[2 * h + 1], b = z[2 * h + 2];
+      if ((a & b) == NONE) break;  // neither child holds a Z element
+      z[h] = b != NONE ? b : a;
+      h = b != NONE ? 2 * h + 2 : 2 * h + 1;
+    }
+    z[h] = last_z;
+    return result;
+  }
+};
 
 // The simulation over k outlets named ids[0..k) in push order (their
 // listed order).  pushes(id) returns how many entries the popped outlet id
 // pushes (its neighbours not popped before it) and records it as popped;
-// ahead(id, rows) fetches its row (rows) or its row offsets (!rows) early.
+// ahead(id) fetches its adjacency early.
 template <class Pushes, class Ahead>
 int outlet_pop_core(const uint32_t* ids, size_t k, Pushes pushes, Ahead ahead, uint32_t* order_out) {
-  const uint32_t NONE = 0xffffffffu;
-  std::vector<uint32_t> zid(ids, ids + k);
+  const uint32_t NONE = OutletHeap::NONE;
+  OutletHeap H;
+  H.k = k;
+  H.z.assign(2 * k + 3, NONE);
+  std::copy(ids, ids + k, H.z.begin());
   uint64_t len = k;
-  for (size_t step = 0; step < k; step++) {
-    uint64_t L = len;
-    uint32_t result = zid[0];
-    // the next pop is one of the root's children and the one after it one
-    // of their children (each sift-down moves one child up): fetch their
-    // adjacency two pops ahead (row offsets) and one pop ahead (entries)
-    ahead(result, true);
-    for (uint64_t q = 1; q <= 2 && q < k; q++)
-      if (zid[q] != NONE) ahead(zid[q], true);
-    for (uint64_t q = 3; q <= 6 && q < k; q++)
-      if (zid[q] != NONE) ahead(zid[q], false);
-    uint64_t last = L - 1;
-    uint32_t last_z = last < k ? zid[last] : NONE;
-    if (last < k) zid[last] = NONE;
-    len = L - 1;
-    if (len > 0) {
-      uint64_t end = L - 1, h = 0;
-      for (;;) {
-        uint64_t c1 = 2 * h + 1, c2 = c1 + 1, c;
-        if (c1 >= end) break;
-        bool z1 = c1 < k && zid[c1] != NONE;
-        if (c2 < end) {
-          bool z2 = c2 < k && zid[c2] != NONE;
-          if (z2) c = c2;
-          else if (z1) c = c1;
-          else break;
-        } else {
-          if (!z1) break;
-          c = c1;
-        }
-        zid[h] = zid[c];
-        h = c;
-      }
-      zid[h] = last_z;
+  size_t t = 0;
+  int returns = 0;
+  auto one_pop = [&]() -> bool {
+    for (uint64_t q = 1; q <= 2; q++)
+      if (H.z[q] != NONE) ahead(H.z[q]);
+    const uint32_t r = H.pop(len);
+    if (r == NONE) return false;  // cannot happen with distinct outlets
+    order_out[t++] = r;
+    len += pushes(r);
+    return true;
+  };
+  while (t < k) {
+    // a Z element at the end moves into the path: simulate
+    while (t < k && len - 1 < k && H.z[len - 1] != NONE)
+      if (!one_pop()) return FL_ERR_INVALID;
+    if (t == k) break;
+    if (returns == 16) {
+      while (t < k)
+        if (!one_pop()) return FL_ERR_INVALID;
+      break;
     }
-    if (result == NONE) return FL_ERR_INVALID;  // cannot happen with distinct outlets
-    order_out[step] = result;
-    len += pushes(result);
+    returns++;
+    const size_t m = k - t;
+    H.val.resize(m);
+    H.post.resize(m);
+    H.vac.resize(k);
+    H.pc = H.qc = 0;
+    if (H.z[0] == NONE) return FL_ERR_INVALID;
+    H.walk(0);
+    if (H.pc != m) return FL_ERR_INVALID;
+    size_t r = 0;
+    for (; r < m; r++) {
+      const uint64_t last = len - 1;
+      if (last < k && H.z[last] != NONE && H.vac[last] >= r) break;  // still a Z element there
+      if (r + 4 < m) ahead(H.val[r + 4]);
+      const uint32_t res = H.val[r];
+      order_out[t + r] = res;
+      len = last + pushes(res);
+    }
+    if (r == m) break;
+    for (size_t q = 0; q < r; q++) H.z[H.post[q]] = NONE;
+    H.pc = 0;
+    H.place(0, H.val.data() + r);
+    t += r;
   }
   return FL_OK;
 }
@@ -485,12 +552,7 @@ int fl_outlet_pop_order_range(uint32_t s0, uint32_t nl, const uint32_t* row_ptr,
         for (uint32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) pushes += visited[col[e] - s0] ? 0 : 1;
         return pushes;
       },
-      [&](uint32_t v, bool rows) {
-        if (rows)
-          __builtin_prefetch(col + row_ptr[v]);
-        else
-          __builtin_prefetch(row_ptr + v);
-      },
+      [&](uint32_t v) { __builtin_prefetch(col + row_ptr[v]); },
       order_out);
 }
 
@@ -507,12 +569,7 @@ int fl_outlet_pop_order_compact(size_t k, const uint32_t* deg, const uint32_t* n
         for (uint32_t e = nb_ptr[q]; e < nb_ptr[q + 1]; e++) before += popped[nb[e]];
         return (uint64_t)deg[q] - before;
       },
-      [&](uint32_t q, bool rows) {
-        if (rows)
-          __builtin_prefetch(nb + nb_ptr[q]);
-        else
-          __builtin_prefetch(nb_ptr + q);
-      },
+      [&](uint32_t q) { __builtin_prefetch(nb + nb_ptr[q]); },
       order_out);
 }
 

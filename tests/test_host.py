@@ -252,7 +252,8 @@ def _outlet_order(g, outlets):
 
 
 @pytest.mark.parametrize("seed,k,grid", [(0, 1, False), (1, 5, False), (2, 40, False), (3, 150, False),
-                                         (4, 60, True), (5, 399, False)])
+                                         (4, 60, True), (5, 399, False), (6, 400, False), (7, 380, True),
+                                         (8, 250, False), (9, 320, True)])
 def test_outlet_phase_matches_heap_simulation(seed, k, grid):
     """The first k pops of the full Rust-heap simulation are the outlets; the
     shape-only simulation must give the same order (outlets adjacent to each
@@ -263,6 +264,27 @@ def test_outlet_phase_matches_heap_simulation(seed, k, grid):
     g, _ = random_delaunay_graph(400, 300 + seed, grid=grid)
     rng = np.random.default_rng(seed)
     outlets = rng.choice(g.n, k, replace=False).astype(np.uint32)
+    ref = flo.visit_order(g, outlets)[:k]
+    assert np.array_equal(_outlet_order(g, outlets), ref)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_outlet_phase_with_isolated_outlets(seed):
+    """Outlets that push nothing (isolated sites) shrink the heap until its
+    end holds outlets again: the order must follow the heap simulation
+    through every switch between the walked and the simulated pops."""
+    from oracle import flo
+    from test_oracle_pinned import random_delaunay_graph
+
+    g0, _ = random_delaunay_graph(120, 900 + seed)
+    iso = 60 + 40 * seed
+    n = g0.n + iso
+    rp = np.concatenate([g0.row_ptr.astype(np.int64), np.full(iso, g0.row_ptr[-1], np.int64)])
+    g = flo.Graph(rp, g0.col, g0.length)
+    rng = np.random.default_rng(seed)
+    outlets = np.concatenate([np.arange(g0.n, n), rng.choice(g0.n, 3 + 5 * seed, replace=False)])
+    outlets = rng.permutation(outlets).astype(np.uint32)
+    k = len(outlets)
     ref = flo.visit_order(g, outlets)[:k]
     assert np.array_equal(_outlet_order(g, outlets), ref)
 
