@@ -632,11 +632,13 @@ int32_t join_lakes(fl_ctx* c, fl_terrain* T) {
 
 // Decomposition of the receiver forest into heavy chains laid out by light
 // depth; ends with the control block on the host (bucket offsets, flags).
-int32_t decompose(fl_ctx* c, fl_terrain* T, bool full) {
+// `joined`: lakes were joined since the receivers marked the dirty basins
+// (the joins' changes are marked too).
+int32_t decompose(fl_ctx* c, fl_terrain* T, bool full, bool joined) {
   hipStream_t s = c->stream;
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
-  if (!full) fl_launch_dirty(s, T->t, w);
+  if (!full && joined) fl_launch_dirty(s, T->t, w);
   fl_launch_children(s, T->t, w, full, T->d_tid, T->d_active);
   uint32_t round = 0;
   uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
@@ -726,7 +728,7 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
   fl_dev_work& w = T->w;
   uint32_t n = T->t.n;
   fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, w, keep);
+  fl_launch_receivers(s, T->t, w, keep, !full);
   if (T->hint_stale) {
     // the heavy children follow subtree sizes of the final tree: with the
     // sites' own areas as the hint the chains are short and the light depth
@@ -741,14 +743,14 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
     if (fl_device_subtree_sizes(s, n, w.next, T->d_tid, T->d_active, w.A, &err, &T->arena) != FL_OK)
       return set_err(FL_ERR_HIP, "%s", err.c_str());
     T->hint_stale = false;
-    FLCHK(decompose(c, T, full));
+    FLCHK(decompose(c, T, full, *had_lake));
   } else {
-    FLCHK(decompose(c, T, full));
+    FLCHK(decompose(c, T, full, false));
     *had_lake = host_flag(c, FL_FLAG_LAKE) != 0;
     if (*had_lake) {
       FLCHK(join_lakes(c, T));
       fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-      FLCHK(decompose(c, T, full));
+      FLCHK(decompose(c, T, full, true));
     }
   }
   uint32_t nb = T->prev_nb;
@@ -785,8 +787,7 @@ void enqueue_fast_body(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
   bool clamp = T->t.any_max_slope;
   // the previous iteration used 2 * (R - 1) bucket entries at most
   fl_launch_clear_iter(s, w, std::min<uint32_t>(2 * R + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, w, keep);
-  if (!full) fl_launch_dirty(s, T->t, w);
+  fl_launch_receivers(s, T->t, w, keep, !full);
   fl_launch_children(s, T->t, w, full, T->d_tid, T->d_active);
   for (uint32_t r = 0; r < J; r++) fl_launch_chain_jump(s, T->t, w, r);
   w.guard = w.ctrl + FL_CTRL_JUMPS + (J - 1);
@@ -1163,7 +1164,7 @@ int32_t layout_device(fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, b
   FLCHK(dev_alloc(P, &t.rank, n));
   FLCHK(dev_alloc(P, &t.order, n));
   t.any_max_slope = clamp;
-  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.xp, &w.taup, &w.parea}) FLCHK(dev_alloc(P, p, n));
+  for (double** p : {&w.h, &w.h2, &w.A, &w.tau, &w.xp, &w.taup, &w.parea, &w.pK, &w.pdn}) FLCHK(dev_alloc(P, p, n));
   for (int k = 0; k < 3; k++) {
     FLCHK(dev_alloc(P, &P->dxb[k], n));
     FLCHK(dev_alloc(P, &P->nxb[k], n));
@@ -1172,16 +1173,16 @@ int32_t layout_device(fl_terrain* P, uint32_t n, uint32_t nnz, uint32_t count, b
   }
   FLCHK(dev_alloc(P, &w.dirty, n));
   FLCHK(dev_alloc(P, &w.act, n));
-  FLCHK(dev_alloc(P, &w.plist, 4 * (uint64_t)n));
+  FLCHK(dev_alloc(P, &w.cval, 4 * (uint64_t)n));
   FLCHK(dev_alloc(P, &P->hb[2], n));
   P->hb[0] = w.h;
   P->hb[1] = w.h2;
   set_slots(P, 0);
   if (clamp)
     for (double** p : {&w.hnp, &w.up, &w.tanp, &w.dp, &w.hop}) FLCHK(dev_alloc(P, p, n));
-  for (uint32_t** p : {&w.sub, &w.sub2, &w.hc, &w.pcnt, &w.chain_base, &w.chain_len, &w.head_base,
-                       &w.perm, &w.ppos, &w.rpos})
+  for (uint32_t** p : {&w.sub, &w.sub2, &w.hc, &w.pcnt, &w.pdest, &w.head_chain, &w.head_base, &w.perm})
     FLCHK(dev_alloc(P, p, n));
+  FLCHK(dev_alloc(P, &w.cmeta, n));
   FLCHK(dev_alloc(P, &w.cp, n));
   FLCHK(dev_alloc(P, &w.rt, n));
   FLCHK(dev_alloc(P, &w.cmask, n));
@@ -2411,7 +2412,7 @@ extern "C" int32_t fl_stage_receivers(fl_ctx* c, fl_terrain* T, const double* el
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
-  fl_launch_receivers(c->stream, T->t, T->w, false);
+  fl_launch_receivers(c->stream, T->t, T->w, false, false);
   return download_idx_old(c, T, next_out, T->w.next);
 }
 
@@ -2424,7 +2425,7 @@ extern "C" int32_t fl_stage_stream_tree(fl_ctx* c, fl_terrain* T, const double* 
   HIPCHK(hipSetDevice(c->device));
   FLCHK(upload_h(c, T, elev));
   fl_launch_clear_iter(c->stream, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * T->t.n + 2));
-  fl_launch_receivers(c->stream, T->t, T->w, false);
+  fl_launch_receivers(c->stream, T->t, T->w, false, false);
   FLCHK(read_ctrl(c, T));
   bool lake = host_flag(c, FL_FLAG_LAKE) != 0;
   FLCHK(label_chain_ends(c, T));
@@ -2678,7 +2679,7 @@ extern "C" int32_t fl_stage_basin_order(fl_ctx* c, fl_terrain* T, const double* 
   FLCHK(upload_h(c, T, elev));
   hipStream_t s = c->stream;
   fl_launch_clear_iter(s, T->w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
-  fl_launch_receivers(s, T->t, T->w, false);
+  fl_launch_receivers(s, T->t, T->w, false, false);
   FLCHK(read_ctrl(c, T));
   if (host_flag(c, FL_FLAG_LAKE)) FLCHK(join_lakes(c, T));
   fl_launch_children(s, T->t, T->w, true, nullptr, nullptr);  // child masks of the final tree

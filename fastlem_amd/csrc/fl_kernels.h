@@ -69,8 +69,6 @@ struct fl_dev_work {
   double* tanp = nullptr;   // [n] tan(max slope), chain-major (max-slope form)
   double* dp = nullptr;     // [n] receiver distance, chain-major (max-slope form)
   double* hop = nullptr;    // [n] old elevation, chain-major (max-slope form)
-  uint32_t* ppos = nullptr; // [n] at a chain's base position: position of the head's receiver
-  uint32_t* rpos = nullptr; // [n] at a chain's base position: position of the basin outlet
   double* A = nullptr;      // [n] drainage area (kept: heavy-child hint)
   double* tau = nullptr;    // [n] response time
   double* dnext = nullptr;  // [n] distance to the receiver
@@ -83,10 +81,16 @@ struct fl_dev_work {
   uint64_t* cmask = nullptr;  // [n] child slots of each site
   uint32_t* hc = nullptr;     // [n] heavy child
   // drainage-area inputs in chain-major order (written by k_chain_perm):
-  // light children (FL_LIST per position, fold order), counts, own area
-  uint32_t* plist = nullptr;  // [4n]
+  // list counts, own area, erodibility, receiver distance; the listed light
+  // children's drainage areas (FL_LIST slots per position, fold order,
+  // filled by the children's rounds: pdest at a chain's base position names
+  // the slot of its head)
   uint32_t* pcnt = nullptr;   // [n] npre | npost << 8, or the overflow flag
   double* parea = nullptr;    // [n]
+  double* pK = nullptr;       // [n]
+  double* pdn = nullptr;      // [n]
+  double* cval = nullptr;     // [FL_LIST n]
+  uint32_t* pdest = nullptr;  // [n]
 
   // pointer jumping, two (pointer, count) pairs per site, pointer in the low
   // word: cp = (chain ancestor, distance to it), rt = (tree ancestor, light
@@ -95,8 +99,11 @@ struct fl_dev_work {
   uint64_t* rt = nullptr;
   uint32_t *bcnt = nullptr, *bnodes = nullptr, *boff = nullptr, *bnoff = nullptr, *bcur = nullptr,
            *bncur = nullptr;                                  // [n+2] light-depth buckets
-  uint32_t *chain_base = nullptr, *chain_len = nullptr;      // [n] chains by bucket
-  uint32_t* head_base = nullptr;                             // [n]
+  // chains by bucket: (base position, length, position of the head's
+  // receiver or FL_NONE, position of the basin outlet)
+  uint4* cmeta = nullptr;                                    // [n]
+  uint32_t* head_chain = nullptr;                            // [n] at a chain head: its chain
+  uint32_t* head_base = nullptr;                             // [n] at a chain head: its base position
   uint32_t* perm = nullptr;                                  // [n] chain-major site order
   uint32_t* ctrl = nullptr;                                  // [FL_CTRL_WORDS]
   // movement flag of the last pointer-jumping round launched: the kernels
@@ -129,12 +136,16 @@ extern thread_local uint32_t fl_site_block;
 // keep_unchanged: sites whose neighbourhood's elevations the previous
 // iteration did not change keep its receivers (valid when that iteration's
 // tree had no lake joins).
-void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged);
+// mark: flag the previous basins that the changed receivers touch (w.dirty,
+// cleared by the previous iteration's k_tau_scatter).
+void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged,
+                         bool mark);
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 void fl_launch_sub_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool flip);
 void fl_launch_lake_join(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
-// Marks the dirty basins (after the receivers, which clear the marks, and
-// any lake joins).
+// Marks the dirty basins from the final receivers after a lake join (the
+// receivers marked those of their own changes; k_tau_scatter clears the
+// marks for the next iteration).
 void fl_launch_dirty(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w);
 // Children, heavy children and chain pointers of the iteration's active
 // sites: every site of the terrains still iterating (`full`, or no previous

@@ -106,15 +106,21 @@ __device__ __forceinline__ double first_len(const uint32_t* __restrict__ rp, con
 // chg_prev (per-site flags of the elevations the previous iteration changed,
 // whose tree had no lake joins), a site whose neighbourhood did not change
 // keeps its previous receiver and distance.
+//
+// With `dirty` (cleared by the previous iteration's k_tau_scatter), a site
+// whose receiver or receiver distance differs from the previous tree's
+// marks the previous basins of itself, its new and its old receiver as
+// recomputed (the marks of k_dirty, made here without a launch of their
+// own).
 __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                             const double* __restrict__ len, const double* __restrict__ h,
                             const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ next,
                             double* __restrict__ dnext, int* __restrict__ flags,
                             const uint8_t* __restrict__ chg_prev, const uint32_t* __restrict__ next_prev,
-                            const double* __restrict__ dnext_prev, uint8_t* __restrict__ dirty) {
+                            const double* __restrict__ dnext_prev, const uint32_t* __restrict__ root_prev,
+                            uint8_t* __restrict__ dirty) {
   uint32_t i = gid();
   if (i >= n) return;
-  if (dirty) dirty[i] = 0;  // the basin marks of this iteration start clear
   if (is_outlet[i]) {
     next[i] = i;
     dnext[i] = 1.0;
@@ -151,6 +157,14 @@ __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const u
   if (bj == i) flags[FL_FLAG_LAKE] = 1;
   next[i] = bj;
   dnext[i] = d;
+  if (dirty) {
+    const uint32_t pj = next_prev[i];
+    if (bj != pj || __double_as_longlong(d) != __double_as_longlong(dnext_prev[i])) {
+      dirty[root_prev[i]] = 1;
+      dirty[root_prev[bj]] = 1;
+      dirty[root_prev[pj]] = 1;
+    }
+  }
 }
 
 // ------------------------------------------------------------ lake join ---
@@ -242,7 +256,8 @@ __global__ void k_lake_flip(uint32_t n, const uint32_t* __restrict__ rp, const u
 // shapes the work, never the results.
 // Basins of the previous iteration's tree that are recomputed: those holding
 // a site whose receiver or receiver distance changed, or that site's old or
-// new receiver.  Any other basin keeps every parent pointer and distance of
+// new receiver (k_receivers marks those of raw receivers; this kernel runs
+// after a lake join changed receivers).  Any other basin keeps every parent pointer and distance of
 // its tree, and no site of another basin attaches to it, so its drainage
 // areas, response times and elevations come out bit for bit as before.
 __global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const double* __restrict__ dnext,
@@ -487,13 +502,24 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ 
 // Give every chain (through its tail) a slot in its light-depth bucket and a
 // contiguous range of positions: LDS atomics rank the chains of a workgroup,
 // one global atomic per (workgroup, bucket) reserves the ranges.  Order
-// inside a bucket is immaterial to the results.
+// inside a bucket is immaterial to the results.  Chain q's record starts
+// (base, len) (k_chain_perm adds the receiver's and the outlet's positions),
+// its head knows q and its base, and the base's receiver slot starts empty.
+__device__ __forceinline__ void place_chain(uint32_t q, uint32_t pos, uint32_t len, uint32_t hd,
+                                            uint4* __restrict__ cmeta, uint32_t* __restrict__ head_chain,
+                                            uint32_t* __restrict__ head_base, uint32_t* __restrict__ pdest) {
+  reinterpret_cast<uint2*>(cmeta)[2ull * q] = make_uint2(pos, len);
+  head_chain[hd] = q;
+  head_base[hd] = pos;
+  pdest[pos] = FL_NONE;
+}
+
 __global__ void __launch_bounds__(FL_BUCKET_BLOCK)
 k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* __restrict__ hc, const unsigned long long* __restrict__ cp,
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                const uint32_t* __restrict__ boff,
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
-               uint32_t* __restrict__ chain_base, uint32_t* __restrict__ chain_len,
+               uint4* __restrict__ cmeta, uint32_t* __restrict__ head_chain, uint32_t* __restrict__ pdest,
                uint32_t* __restrict__ head_base, uint32_t long_min, const uint8_t* __restrict__ act) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
@@ -519,9 +545,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
         } else {
           uint32_t q = boff[l] + atomicAdd(bcur + l, 1u);
           uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
-          chain_base[q] = pos;
-          chain_len[q] = len;
-          head_base[pair_ptr(c)] = pos;
+          place_chain(q, pos, len, pair_ptr(c), cmeta, head_chain, head_base, pdest);
         }
       }
     }
@@ -542,35 +566,53 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     uint32_t l = 2u * pair_cnt(rt[v]) + (len >= long_min ? 1u : 0u);
     uint32_t q = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
-    chain_base[q] = pos;
-    chain_len[q] = len;
-    head_base[pair_ptr(c)] = pos;
+    place_chain(q, pos, len, pair_ptr(c), cmeta, head_chain, head_base, pdest);
   }
 }
 
-// Scatter sites to their chain-major positions.  At each chain's base
-// position record the position of the head's receiver (FL_NONE for an
+// Scatter sites to their chain-major positions with the inputs the sweeps
+// read there: the drainage-area fold's list counts, area, erodibility and
+// receiver distance (with max slopes also uplift, tan(max slope), distance
+// and old elevation), and for each listed light child the slot its chain's
+// head fills with its drainage area (pdest at the child chain's base).  A
+// chain's record gets the positions of the head's receiver (FL_NONE for an
 // outlet) and of the basin's outlet.  Sites outside every basin keep their
 // elevation, drainage area = own area and response time 0 (generator.rs:
 // 144-145), as the reference leaves them.
+struct fl_perm_out {
+  uint32_t* perm;
+  uint32_t* pcnt;
+  uint32_t* pdest;
+  double* parea;
+  double* pK;
+  double* pdn;
+  double* up;
+  double* tanp;
+  double* dp;
+  double* hop;
+  uint4* cmeta;
+};
+
 __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
                              const unsigned long long* __restrict__ rt, const uint32_t* __restrict__ next,
                              const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
-                             uint32_t* __restrict__ perm, uint32_t* __restrict__ ppos, uint32_t* __restrict__ rpos,
-                             const double* __restrict__ area, double* __restrict__ A, double* __restrict__ tau,
-                             const double* __restrict__ h, double* __restrict__ h2,
+                             const uint32_t* __restrict__ head_chain, fl_perm_out o,
+                             const double* __restrict__ area, const double* __restrict__ K,
+                             const double* __restrict__ dnext, const double* __restrict__ U,
+                             const double* __restrict__ tan_ms, int clamp, double* __restrict__ A,
+                             double* __restrict__ tau, const double* __restrict__ h, double* __restrict__ h2,
                              const uint32_t* __restrict__ maxld, uint32_t* __restrict__ bcnt,
                              uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
                              uint32_t* __restrict__ bncur, const uint32_t* __restrict__ rp,
                              const uint32_t* __restrict__ col, const unsigned long long* __restrict__ cmask,
-                             const uint32_t* __restrict__ hc, uint32_t* __restrict__ pcnt,
-                             uint32_t* __restrict__ plist, double* __restrict__ parea,
-                             const uint8_t* __restrict__ act, uint32_t* __restrict__ root) {
+                             const uint32_t* __restrict__ hc, const uint8_t* __restrict__ act,
+                             uint32_t* __restrict__ root) {
   uint32_t v = gid();
   // read together with the guard (one load latency)
   const uint32_t mld = *maxld;
   const bool on = v < n && act[v];
   const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
+  const uint32_t hq = on ? head_chain[v] : 0u;  // meaningful at a chain head only
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   // the bucket histograms are used up (count, scan, place): clear the
   // entries this decomposition touched for the next one
@@ -594,11 +636,20 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
   uint32_t hd = pair_ptr(c);
   uint32_t base = head_base[hd];
   const uint32_t P = base + pair_cnt(c);
-  perm[P] = v;
-  // the drainage-area inputs of the site, in chain-major order: its area and
-  // its light children in fold order (descending slot), those before the
-  // heavy child first; more than FL_LIST of them set the overflow bit and
-  // the fold reads the graph instead
+  o.perm[P] = v;
+  const double dv = dnext[v];
+  o.pK[P] = K[v];
+  o.pdn[P] = dv;
+  if (clamp) {
+    o.up[P] = U[v];
+    o.tanp[P] = tan_ms[v];
+    o.dp[P] = dv;
+    o.hop[P] = h[v];
+  }
+  // the light children in fold order (descending slot), those before the
+  // heavy child first: the first FL_LIST get slots FL_LIST * P + i for
+  // their chains' heads to fill; more set the overflow bit and the fold
+  // reads the graph instead
   {
     const uint32_t b = rp[v], deg = rp[v + 1] - b, hk = hc[v];
     const unsigned long long m = cmask[v];
@@ -612,19 +663,19 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
         continue;
       }
       uint32_t idx = npre + npost;
-      if (idx < FL_LIST) plist[(size_t)FL_LIST * P + idx] = ch;
+      if (idx < FL_LIST) o.pdest[head_base[ch]] = FL_LIST * P + idx;
       if (after) ++npost;
       else ++npre;
     }
     bool over = npre + npost > FL_LIST;
-    pcnt[P] = over ? FL_LIST_OVERFLOW : (npre | (npost << 8));
-    parea[P] = area[v];
+    o.pcnt[P] = over ? FL_LIST_OVERFLOW : (npre | (npost << 8));
+    o.parea[P] = area[v];
   }
   if (hd == v) {
     uint32_t p = next[v];
     unsigned long long cpp = cp[p];
-    ppos[base] = (p == v) ? FL_NONE : head_base[pair_ptr(cpp)] + pair_cnt(cpp);
-    rpos[base] = head_base[r];
+    const uint32_t pp = (p == v) ? FL_NONE : head_base[pair_ptr(cpp)] + pair_cnt(cpp);
+    reinterpret_cast<uint2*>(o.cmeta)[2ull * hq + 1] = make_uint2(pp, head_base[r]);
   }
 }
 
@@ -718,13 +769,9 @@ __device__ __forceinline__ double da_tail_terms(double r, uint32_t np, double c,
   return r;
 }
 
-// The partial fold of chain position P from its list (written by
-// k_chain_perm): pre = area + the light children before the heavy one (in
-// fold order), posts p0..p3 after it, np posts; absent terms are -0.0.  One
-// level of gathers (the children's A).  A site with more light children than
-// listed folds from the graph instead.
-// The partial folds of a listed position from its gathered values (v_i =
-// the i-th listed child's A, or anything when the list has fewer entries).
+// The partial folds of a listed position from the values of its listed light
+// children (v_i = the i-th listed child's A, or anything when the list has
+// fewer entries: such slots are replaced by -0.0 before any arithmetic).
 __device__ __forceinline__ void da_combine(uint32_t cn, double area_k, double v0, double v1, double v2, double v3,
                                            double& pre, double& p0, double& p1, double& p2, double& p3,
                                            uint32_t& np) {
@@ -744,74 +791,65 @@ __device__ __forceinline__ void da_combine(uint32_t cn, double area_k, double v0
   np = nl - npre;
 }
 
-__device__ __forceinline__ void da_inputs(uint32_t P, uint32_t k, uint32_t cn, uint4 ids, double area_k,
-                                          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                                          const uint32_t* __restrict__ next,
-                                          const unsigned long long* __restrict__ cmask,
-                                          const uint32_t* __restrict__ hc, const double* __restrict__ area,
-                                          const double* __restrict__ A, double& pre, double& p0, double& p1,
-                                          double& p2, double& p3, uint32_t& np) {
-  (void)P;
-  if (cn & FL_LIST_OVERFLOW) {
-    da_partial(k, rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
-    return;
-  }
-  const uint32_t nl = (cn & 0xffu) + ((cn >> 8) & 0xffu);
-  const double v0 = nl > 0 ? A[ids.x] : FL_NEG0, v1 = nl > 1 ? A[ids.y] : FL_NEG0;
-  const double v2 = nl > 2 ? A[ids.z] : FL_NEG0, v3 = nl > 3 ? A[ids.w] : FL_NEG0;
-  da_combine(cn, area_k, v0, v1, v2, v3, pre, p0, p1, p2, p3, np);
-}
+#define FL_DA_BLOCK 4
 
-#define FL_DA_BLOCK 8
+// Everything the drainage-area fold reads at chain position P, in
+// chain-major order (written by k_chain_perm, the children's values by the
+// deeper rounds): one level of loads, no gathers.
+//   k: the site; cn: npre | npost << 8, or the overflow flag;
+//   v0..v3: the listed light children's A (cval[FL_LIST * P + i]);
+//   area, K, d: the site's area, erodibility and receiver distance.
+struct fl_da_pos {
+  uint32_t k, cn;
+  double v0, v1, v2, v3, area, K, d;
+};
 
-// Inputs of the response-time sweep for chain position P of site k, written
-// as soon as k's drainage area a is final: x = 1.0 / (K * a^0.5) * d
-// (generator.rs:172-173, the reference's association); with max slopes the
-// recurrence's inputs gathered into chain-major order.
-struct fl_tau_in {
-  const double* K;
-  const double* dnext;
-  const double* U;
-  const double* tan_ms;
-  const double* h;
+// The arrays of a drainage-area round.
+struct fl_da_args {
+  const uint32_t* perm;
+  const uint32_t* pcnt;
+  const double* cval;   // [FL_LIST n] light children's A by (position, list slot)
+  const double* parea;
+  const double* pK;
+  const double* pdn;
+  const uint32_t* pdest;  // at a chain's base position: its head's slot in cval, or FL_NONE
+  double* cval_out;
   double* xp;
-  double* up;
-  double* tanp;
-  double* dp;
-  double* hop;
-  int clamp;
+  // the graph, for positions with more light children than listed
+  const uint32_t* rp;
+  const uint32_t* col;
+  const uint32_t* next;
+  const unsigned long long* cmask;
+  const uint32_t* hc;
+  const double* area;
+  double* A;
 };
 
-// The site values tau_inputs reads, loaded together with a position's
-// gathers: after the store of A the compiler cannot move loads above it.
-struct fl_tau_site {
-  double K, d;
-};
-__device__ __forceinline__ fl_tau_site tau_site(const fl_tau_in& ti, uint32_t k) {
-  return fl_tau_site{ti.K[k], ti.dnext[k]};
+__device__ __forceinline__ fl_da_pos da_load(const fl_da_args& g, uint32_t P) {
+  const double2* __restrict__ cv = reinterpret_cast<const double2*>(g.cval);
+  const double2 a = cv[2ull * P], b = cv[2ull * P + 1];
+  return fl_da_pos{g.perm[P], g.pcnt[P], a.x, a.y, b.x, b.y, g.parea[P], g.pK[P], g.pdn[P]};
 }
-__device__ __forceinline__ void tau_inputs(const fl_tau_in& ti, uint32_t P, uint32_t k, double a,
-                                           fl_tau_site v) {
-  double celerity = v.K * rust_pow_half(a);
-  ti.xp[P] = 1.0 / celerity * v.d;
-  if (ti.clamp) {
-    ti.up[P] = ti.U[k];
-    ti.tanp[P] = ti.tan_ms[k];
-    ti.dp[P] = v.d;
-    ti.hop[P] = ti.h[k];
-  }
+
+// A site's drainage area is final: store it and the response-time input of
+// its position, x = 1.0 / (K * a^0.5) * d (generator.rs:172-173, the
+// reference's association).
+__device__ __forceinline__ void da_store(const fl_da_args& g, uint32_t P, uint32_t k, double a, double K, double d) {
+  g.A[k] = a;
+  double celerity = K * rust_pow_half(a);
+  g.xp[P] = 1.0 / celerity * d;
 }
 
 // Drainage area of one light depth: the sequential fold along each chain,
 // tail to head, A = ((pre + A_heavy) + p0) + p1 (+ p2 + p3 ...), with the
-// partial folds of each position made from its list (da_inputs).  Long
-// chains [c1, c2) take one 64-lane wavefront each: tiles of 64 positions are
-// prepared by the whole wavefront and staged in LDS, lane 0 folds them eight
-// at a time (reads first, then the dependent adds; positions with three or
-// more posts marked by a ballot), the wavefront stores the results.  Short
-// chains [c0, c1) take one lane each, FL_DA_BLOCK positions at a time.  As
-// each site's A is final the response-time inputs of its position are
-// written (tau_inputs).  Bounds from the bucket offsets of round l.
+// partial folds of each position made from its listed children's values.
+// Long chains [c1, c2) take one 64-lane wavefront each: tiles of 64
+// positions are prepared by the whole wavefront and staged in LDS, lane 0
+// folds them eight at a time (reads first, then the dependent adds;
+// positions with three or more posts marked by a ballot), the wavefront
+// stores the results.  Short chains [c0, c1) take one lane each,
+// FL_DA_BLOCK positions at a time.  A chain's head hands its A to its
+// receiver's list slot (pdest).  Bounds from the bucket offsets of round l.
 // One wavefront's LDS tile of a drainage-area round.
 struct fl_da_lds {
   double p[64], a[64], b[64], c[64], d[64], r[64];
@@ -828,55 +866,31 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Wavefront `wave` of `nwaves` in drainage-area round l (short chains
-// [c0, c1), long chains [c1, c2)).
-__device__ __forceinline__ void
-da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, uint32_t lane, fl_da_lds& L,
-         const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-         const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
-         const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-         const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-         const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
-         const fl_tau_in& ti) {
-  const uint4* __restrict__ lists = reinterpret_cast<const uint4*>(plist);
+// [c0, c1), long chains [c1, c2)); cmeta: (base, len, ...) per chain.
+__device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
+                                         uint32_t lane, fl_da_lds& L, const uint4* __restrict__ cmeta,
+                                         const fl_da_args& g) {
+  const uint2* __restrict__ cm2 = reinterpret_cast<const uint2*>(cmeta);
   // long chains: one per block (uniform per block).  Tiles of 64 positions
-  // from the tail, software-pipelined: the gathers of the next tile and the
-  // list loads of the one after are issued before lane 0 folds the current
-  // tile, with in-chain indices and no branches, so nothing waits for them
-  // before the fold; their values are combined when their tile is staged.
-  // No tile input depends on this round's results (light children head
-  // chains of deeper rounds; a listed slot past the list reads the site's
-  // own A and is dropped).
+  // from the tail, software-pipelined: the loads of the next tile are
+  // issued before lane 0 folds the current one, with in-chain indices and
+  // no branches, so nothing waits for them before the fold.  No tile input
+  // depends on this round's results (light children head chains of deeper
+  // rounds).
   for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
-    const uint32_t base = chain_base[c], len = chain_len[c];
-    struct Lists {
-      uint32_t k, cn;
-      uint4 ids;
-      double area;
-    };
-    struct Raw {
-      double g0, g1, g2, g3, area;
-      uint32_t k, cn;
-      fl_tau_site ts;
-    };
+    const uint2 m = cm2[2ull * c];
+    const uint32_t base = m.x, len = m.y;
+    const uint32_t pd = g.pdest[base];
     // position of this lane in the tile ending at `end` (lanes past the
     // tile read its first position)
     auto pos_of = [&](uint32_t end) {
       const uint32_t cs = end > 64 ? end - 64 : 0;
       return base + (lane < end - cs ? cs + lane : cs);
     };
-    auto lists_of = [&](uint32_t end) {
-      const uint32_t P = pos_of(end);
-      return Lists{perm[P], pcnt[P], lists[P], parea[P]};
-    };
-    auto gather = [&](const Lists& q) {
-      const uint32_t nl = (q.cn & FL_LIST_OVERFLOW) ? 0u : (q.cn & 0xffu) + ((q.cn >> 8) & 0xffu);
-      return Raw{A[nl > 0 ? q.ids.x : q.k], A[nl > 1 ? q.ids.y : q.k], A[nl > 2 ? q.ids.z : q.k],
-                 A[nl > 3 ? q.ids.w : q.k], q.area, q.k, q.cn, tau_site(ti, q.k)};
-    };
     uint32_t end = len;
     uint32_t end1 = end > 64 ? end - 64 : 0;
-    Raw cur = gather(lists_of(end));
-    Lists nxt = lists_of(end1 > 0 ? end1 : end);
+    fl_da_pos cur = da_load(g, pos_of(end));
+    fl_da_pos nxt = da_load(g, pos_of(end1 > 0 ? end1 : end));
     double r = FL_NEG0;  // lane 0's running value
     while (end > 0) {
       const uint32_t cs = end > 64 ? end - 64 : 0, cnt = end - cs;
@@ -885,9 +899,9 @@ da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, 
       uint32_t np = 0;
       if (mine) {
         if (cur.cn & FL_LIST_OVERFLOW)
-          da_partial(cur.k, rp, col, next, cmask, hc, area, A, p, a, b, x, y, np);
+          da_partial(cur.k, g.rp, g.col, g.next, g.cmask, g.hc, g.area, g.A, p, a, b, x, y, np);
         else
-          da_combine(cur.cn, cur.area, cur.g0, cur.g1, cur.g2, cur.g3, p, a, b, x, y, np);
+          da_combine(cur.cn, cur.area, cur.v0, cur.v1, cur.v2, cur.v3, p, a, b, x, y, np);
       }
       L.p[lane] = p;
       L.a[lane] = a;
@@ -902,11 +916,11 @@ da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, 
       // so the loads in flight are not waited for during the fold
       const bool deep = __any(np > FL_POSTS);
       const uint32_t kk = cur.k;
-      const fl_tau_site ts = cur.ts;
+      const double Kk = cur.K, dk = cur.d;
       wave_sync();
       const uint32_t end2 = end1 > 64 ? end1 - 64 : 0;
-      if (end1 > 0) cur = gather(nxt);
-      if (end2 > 0) nxt = lists_of(end2);
+      if (end1 > 0) cur = nxt;
+      if (end2 > 0) nxt = da_load(g, pos_of(end2));
       if (lane == 0 && !deep) {
         // the terms of da_tail_terms for np <= FL_POSTS: p2 (and p3)
         int e = (int)cnt - 1;
@@ -951,30 +965,31 @@ da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, 
           for (int u = 0; u < 8; ++u) {
             r = ((tp[u] + r) + ta[u]) + tb[u];
             if ((rare >> (e - u)) & 1ull)
-              r = da_tail_terms(r, L.n[e - u], L.c[e - u], L.d[e - u], L.k[e - u], rp, col, next, cmask, hc, A);
+              r = da_tail_terms(r, L.n[e - u], L.c[e - u], L.d[e - u], L.k[e - u], g.rp, g.col, g.next, g.cmask,
+                                g.hc, g.A);
             L.r[e - u] = r;
           }
         }
         for (; e >= 0; --e) {
           r = ((L.p[e] + r) + L.a[e]) + L.b[e];
-          if ((rare >> e) & 1ull) r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], rp, col, next, cmask, hc, A);
+          if ((rare >> e) & 1ull)
+            r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], g.rp, g.col, g.next, g.cmask, g.hc, g.A);
           L.r[e] = r;
         }
       }
       wave_sync();
-      if (mine) {
-        A[kk] = L.r[lane];
-        tau_inputs(ti, base + cs + lane, kk, L.r[lane], ts);
-      }
+      if (mine) da_store(g, base + cs + lane, kk, L.r[lane], Kk, dk);
       wave_sync();  // the tile arrays are rewritten next
       end = end1;
       end1 = end2;
     }
+    // lane 0's value is the head's A
+    if (lane == 0 && pd != FL_NONE) g.cval_out[pd] = r;
   }
   // short chains: one lane each, the last blocks first (they hold no long
   // chain).  When one pass covers them, the 64-chain groups of the wavefronts
   // that share an XCD (workgroup index mod 8) are contiguous: chains placed
-  // together hold nearby sites, whose site-ordered reads then share an L2.
+  // together hold nearby sites, whose site-ordered stores then share an L2.
   const uint32_t groups = (c1 - c0 + 63u) >> 6;
   uint32_t sw = nwaves - 1 - wave;
   if (groups <= nwaves && sw < groups) {
@@ -982,59 +997,47 @@ da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, 
     sw = x * q + (x < rq ? x : rq) + (sw >> 3);
   }
   for (uint32_t c = c0 + sw * 64u + lane; c < c1; c += nwaves * 64u) {
-    const uint32_t base = chain_base[c], len = chain_len[c];
+    const uint2 m = cm2[2ull * c];
+    const uint32_t base = m.x, len = m.y;
+    const uint32_t pd = g.pdest[base];
     double r = FL_NEG0;
     uint32_t end = len;
     while (end > 0) {
       const uint32_t cs = end > FL_DA_BLOCK ? end - FL_DA_BLOCK : 0, cnt = end - cs;
-      uint32_t vk[FL_DA_BLOCK], vc[FL_DA_BLOCK];
-      uint4 vl[FL_DA_BLOCK];
-      double va[FL_DA_BLOCK];
+      fl_da_pos q[FL_DA_BLOCK];
+#pragma unroll
+      for (uint32_t u = 0; u < FL_DA_BLOCK; ++u)
+        if (u < cnt) q[u] = da_load(g, base + end - 1 - u);
 #pragma unroll
       for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
         if (u < cnt) {
-          const uint32_t P = base + end - 1 - u;
-          vk[u] = perm[P];
-          vc[u] = pcnt[P];
-          vl[u] = lists[P];
-          va[u] = parea[P];
-        }
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < FL_DA_BLOCK; ++u) {
-        if (u < cnt) {
-          const uint32_t P = base + end - 1 - u;
           double pre, p0, p1, p2, p3;
           uint32_t np;
-          const fl_tau_site ts = tau_site(ti, vk[u]);
-          da_inputs(P, vk[u], vc[u], vl[u], va[u], rp, col, next, cmask, hc, area, A, pre, p0, p1, p2, p3, np);
+          if (q[u].cn & FL_LIST_OVERFLOW)
+            da_partial(q[u].k, g.rp, g.col, g.next, g.cmask, g.hc, g.area, g.A, pre, p0, p1, p2, p3, np);
+          else
+            da_combine(q[u].cn, q[u].area, q[u].v0, q[u].v1, q[u].v2, q[u].v3, pre, p0, p1, p2, p3, np);
           r = ((pre + r) + p0) + p1;
-          if (np >= 3) r = da_tail_terms(r, np, p2, p3, vk[u], rp, col, next, cmask, hc, A);
-          A[vk[u]] = r;
-          tau_inputs(ti, P, vk[u], r, ts);
+          if (np >= 3) r = da_tail_terms(r, np, p2, p3, q[u].k, g.rp, g.col, g.next, g.cmask, g.hc, g.A);
+          da_store(g, base + end - 1 - u, q[u].k, r, q[u].K, q[u].d);
         }
       }
       end = cs;
     }
+    if (pd != FL_NONE) g.cval_out[pd] = r;
   }
 }
 
 __global__ void __launch_bounds__(64)
 k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff,
-          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist,
-          const double* __restrict__ parea, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-          const uint32_t* __restrict__ next, const unsigned long long* __restrict__ cmask,
-          const uint32_t* __restrict__ hc, const double* __restrict__ area, double* __restrict__ A,
-          fl_tau_in ti) {
+          const uint4* __restrict__ cmeta, fl_da_args g) {
   // the round's bounds are read together with the guard (one load latency;
   // the bounds enter the test so that their loads stay above it: offsets are
   // ascending, c2 < c0 never holds once the guard is clear)
   const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ fl_da_lds L;
-  da_round(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, chain_base, chain_len, perm, pcnt, plist, parea, rp,
-           col, next, cmask, hc, area, A, ti);
+  da_round(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, cmeta, g);
 }
 
 // Drainage-area rounds maxld, maxld - 1, ..., lsplit (the small deep ones) in
@@ -1043,12 +1046,7 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
 // launch instead of one per round.  Rounds above `cap` are left out.
 __global__ void __launch_bounds__(64 * FL_TAIL_WAVES)
 k_da_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, const uint32_t* __restrict__ maxld,
-          const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
-          const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ perm,
-          const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ plist, const double* __restrict__ parea,
-          const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col, const uint32_t* __restrict__ next,
-          const unsigned long long* __restrict__ cmask, const uint32_t* __restrict__ hc,
-          const double* __restrict__ area, double* __restrict__ A, fl_tau_in ti) {
+          const uint32_t* __restrict__ boff, const uint4* __restrict__ cmeta, fl_da_args g) {
   uint32_t top = *maxld;
   if (*guard != 0 || top > 0xfffffffeu) return;  // pointer jumping has not converged: the iteration is redone
   top = top < cap ? top : cap;
@@ -1056,8 +1054,7 @@ k_da_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, con
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
   for (uint32_t l = top + 1; l-- > lsplit;) {
     const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
-    da_round(c0, c1, c2, wave, nwaves, lane, L[wave], chain_base, chain_len, perm, pcnt, plist, parea, rp, col,
-             next, cmask, hc, area, A, ti);
+    da_round(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, g);
     __syncthreads();
   }
 }
@@ -1081,20 +1078,22 @@ struct fl_tau_lds {
 template <bool clamp>
 __device__ __forceinline__ void
 tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, uint32_t lane, fl_tau_lds& L,
-          const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-          const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+          const uint4* __restrict__ cmeta, const double* __restrict__ xp,
           double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
           const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp) {
   // long chains: one per block (uniform per block)
   for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
-    const uint32_t base = chain_base[c], len = chain_len[c];
+    // base, length, position of the head's receiver (FL_NONE at the basin
+    // outlet) and of the basin outlet
+    const uint4 m = cmeta[c];
+    const uint32_t base = m.x, len = m.y;
     // lane 0's recurrence state
     double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
-    const uint32_t pp = ppos[base];
+    const uint32_t pp = m.z;
     if (lane == 0 && pp != FL_NONE) {
       tau = taup[pp];
       if (clamp) {
-        uint32_t ro = rpos[base];
+        uint32_t ro = m.w;
         tau_o = taup[ro];
         h_o = hnp[ro];
         h_p = hnp[pp];
@@ -1209,8 +1208,8 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
     sw = x * q + (x < rq ? x : rq) + (sw >> 3);
   }
   for (uint32_t c = c0 + sw * 64u + lane; c < c1; c += nwaves * 64u) {
-    uint32_t base = chain_base[c], len = chain_len[c];
-    uint32_t pp = ppos[base];
+    const uint4 m = cmeta[c];
+    const uint32_t base = m.x, len = m.y, pp = m.z;
     double tau, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
     uint32_t t = 0;
     if (pp == FL_NONE) {
@@ -1231,7 +1230,7 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
     } else {
       tau = taup[pp];
       if (clamp) {
-        uint32_t ro = rpos[base];
+        uint32_t ro = m.w;
         tau_o = taup[ro];
         h_o = hnp[ro];
         h_p = hnp[pp];
@@ -1286,9 +1285,8 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
 
 template <bool clamp>
 __global__ void __launch_bounds__(64)
-k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base,
-           const uint32_t* __restrict__ chain_len, const uint32_t* __restrict__ ppos,
-           const uint32_t* __restrict__ rpos, const double* __restrict__ xp, double* __restrict__ taup,
+k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __restrict__ boff,
+           const uint4* __restrict__ cmeta, const double* __restrict__ xp, double* __restrict__ taup,
            const double* __restrict__ up, const double* __restrict__ tanp, const double* __restrict__ dp,
            const double* __restrict__ hop, double* __restrict__ hnp) {
   // the round's bounds are read together with the guard (one load latency;
@@ -1297,7 +1295,7 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
   const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ fl_tau_lds L;
-  tau_round<clamp>(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, chain_base, chain_len, ppos, rpos, xp, taup, up, tanp, dp, hop, hnp);
+  tau_round<clamp>(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, cmeta, xp, taup, up, tanp, dp, hop, hnp);
 }
 
 // Response-time rounds lsplit, lsplit + 1, ..., maxld (the small deep ones)
@@ -1307,8 +1305,7 @@ k_tau_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __res
 template <bool clamp>
 __global__ void __launch_bounds__(64 * FL_TAIL_WAVES)
 k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, const uint32_t* __restrict__ maxld,
-           const uint32_t* __restrict__ boff, const uint32_t* __restrict__ chain_base, const uint32_t* __restrict__ chain_len,
-          const uint32_t* __restrict__ ppos, const uint32_t* __restrict__ rpos, const double* __restrict__ xp,
+           const uint32_t* __restrict__ boff, const uint4* __restrict__ cmeta, const double* __restrict__ xp,
           double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
           const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp) {
   uint32_t top = *maxld;
@@ -1318,7 +1315,7 @@ k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, co
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
   for (uint32_t l = lsplit; l <= top; ++l) {
     const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
-    tau_round<clamp>(c0, c1, c2, wave, nwaves, lane, L[wave], chain_base, chain_len, ppos, rpos, xp, taup, up, tanp, dp, hop, hnp);
+    tau_round<clamp>(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, xp, taup, up, tanp, dp, hop, hnp);
     __syncthreads();
   }
 }
@@ -1339,8 +1336,12 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged,
-                              uint8_t* __restrict__ chg) {
+                              uint8_t* __restrict__ chg, uint8_t* __restrict__ dirty) {
   const uint32_t v = gid();
+  // the basin marks were read by this iteration's k_children: clear them for
+  // the next iteration's receivers (also when the iteration is redone: a
+  // redo marks again from its own receivers)
+  if (v < n) dirty[v] = 0;
   // the site's pairs are read together with the guard (one load latency)
   const bool on = v < n && act[v];
   const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
@@ -1574,9 +1575,11 @@ thread_local uint32_t fl_site_block = 256;
   } while (0)
 #define FL_LAUNCH(kernel, count, stream, ...) FL_LAUNCH_U(kernel, count, count, stream, __VA_ARGS__)
 
-void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged) {
+void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool keep_unchanged,
+                         bool mark) {
   FL_LAUNCH(k_receivers, t.n, s, t.n, t.row_ptr, t.col, t.len, w.h, t.is_outlet, w.next, w.dnext, w.flags,
-            keep_unchanged ? w.chg_prev : (const uint8_t*)nullptr, w.next_prev, w.dnext_prev, w.dirty);
+            keep_unchanged ? w.chg_prev : (const uint8_t*)nullptr, w.next_prev, w.dnext_prev, w.root_prev,
+            mark ? w.dirty : (uint8_t*)nullptr);
 }
 
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
@@ -1628,27 +1631,35 @@ void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev
                  w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
-                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff, w.bcur, w.bncur, w.chain_base, w.chain_len, w.head_base,
-                 w.long_min, w.act);
-  FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, w.next, t.is_outlet, w.head_base, w.perm, w.ppos, w.rpos,
-            t.area, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur, w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.pcnt, w.plist, w.parea, w.act, w.root);
+                 (const unsigned long long*)w.cp, (const unsigned long long*)w.rt, t.is_outlet, w.boff, w.bnoff,
+                 w.bcur, w.bncur, w.cmeta, w.head_chain, w.pdest, w.head_base, w.long_min, w.act);
+  fl_perm_out o{w.perm, w.pcnt, w.pdest, w.parea, w.pK, w.pdn, w.up, w.tanp, w.dp, w.hop, w.cmeta};
+  FL_LAUNCH(k_chain_perm, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
+            w.next, t.is_outlet, w.head_base, w.head_chain, o, t.area, t.erod, w.dnext, t.uplift, t.tan_ms,
+            t.any_max_slope ? 1 : 0, w.A, w.tau, w.h, w.h2, w.counters + FL_CNT_MAXLD, w.bcnt, w.bnodes, w.bcur,
+            w.bncur, t.row_ptr, t.col, (const unsigned long long*)w.cmask, w.hc, w.act, w.root);
 }
+
+namespace {
+fl_da_args da_args(const fl_dev_terrain& t, const fl_dev_work& w) {
+  return fl_da_args{w.perm, w.pcnt,   w.cval, w.parea, w.pK, w.pdn, w.pdest, w.cval, w.xp,
+                    t.row_ptr, t.col, w.next, (const unsigned long long*)w.cmask, w.hc, t.area, w.A};
+}
+}  // namespace
 
 void fl_launch_da_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
                         const fl_round_grid& g, bool clamp) {
-  fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
+  (void)clamp;  // the max-slope inputs are laid out by k_chain_perm
   if (g.fold_blocks)
-    FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
-                   w.chain_len, w.perm, w.pcnt, w.plist, w.parea, t.row_ptr, t.col, w.next,
-                   (const unsigned long long*)w.cmask, w.hc, t.area, w.A, ti);
+    FL_LAUNCH_GRID(k_da_fold, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.cmeta,
+                   da_args(t, w));
 }
 
 void fl_launch_da_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
                        uint64_t units, bool clamp) {
-  fl_tau_in ti{t.erod, w.dnext, t.uplift, t.tan_ms, w.h, w.xp, w.up, w.tanp, w.dp, w.hop, clamp ? 1 : 0};
-  FL_LAUNCH_GRID(k_da_tail, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap, w.counters + FL_CNT_MAXLD,
-                 w.boff, w.chain_base, w.chain_len, w.perm, w.pcnt, w.plist, w.parea, t.row_ptr, t.col, w.next,
-                 (const unsigned long long*)w.cmask, w.hc, t.area, w.A, ti);
+  (void)clamp;
+  FL_LAUNCH_GRID(k_da_tail, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap,
+                 w.counters + FL_CNT_MAXLD, w.boff, w.cmeta, da_args(t, w));
 }
 
 void fl_launch_tau_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t lsplit, uint32_t cap,
@@ -1656,12 +1667,10 @@ void fl_launch_tau_tail(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wor
   (void)t;
   if (clamp)
     FL_LAUNCH_GRID(k_tau_tail<true>, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap,
-                   w.counters + FL_CNT_MAXLD, w.boff, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up,
-                   w.tanp, w.dp, w.hop, w.hnp);
+                   w.counters + FL_CNT_MAXLD, w.boff, w.cmeta, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
   else
     FL_LAUNCH_GRID(k_tau_tail<false>, dim3(1), dim3(64 * FL_TAIL_WAVES), units, s, w.guard, lsplit, cap,
-                   w.counters + FL_CNT_MAXLD, w.boff, w.chain_base, w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up,
-                   w.tanp, w.dp, w.hop, w.hnp);
+                   w.counters + FL_CNT_MAXLD, w.boff, w.cmeta, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
 }
 
 void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t l,
@@ -1669,18 +1678,18 @@ void fl_launch_tau_round(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
   (void)t;
   if (!g.fold_blocks) return;
   if (clamp)
-    FL_LAUNCH_GRID(k_tau_fold<true>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
-                   w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
+    FL_LAUNCH_GRID(k_tau_fold<true>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.cmeta,
+                   w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
   else
-    FL_LAUNCH_GRID(k_tau_fold<false>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.chain_base,
-                   w.chain_len, w.ppos, w.rpos, w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
+    FL_LAUNCH_GRID(k_tau_fold<false>, dim3(g.fold_blocks), dim3(64), g.positions, s, w.guard, l, w.boff, w.cmeta,
+                   w.xp, w.taup, w.up, w.tanp, w.dp, w.hop, w.hnp);
 }
 
 void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool clamp,
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
   FL_LAUNCH(k_tau_scatter, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
             t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, w.act,
-            tchanged, w.chg);
+            tchanged, w.chg, w.dirty);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {

@@ -61,6 +61,13 @@ struct alignas(16) uint4 {
   unsigned int x, y, z, w;
 };
 inline uint4 make_uint4(unsigned int x, unsigned int y, unsigned int z, unsigned int w) { return uint4{x, y, z, w}; }
+struct alignas(8) uint2 {
+  unsigned int x, y;
+};
+inline uint2 make_uint2(unsigned int x, unsigned int y) { return uint2{x, y}; }
+struct alignas(16) double2 {
+  double x, y;
+};
 
 struct dim3 {
   uint32_t x, y, z;
