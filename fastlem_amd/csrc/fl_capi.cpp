@@ -116,9 +116,10 @@ struct fl_terrain {
   uint8_t* chgb[3] = {nullptr, nullptr, nullptr};  // per-site "elevation changed" of that iteration
   bool prev_valid = false;
   bool after_lake = true;  // the last iteration joined lakes (its tree is not raw receivers)
-  // w.A holds no drainage area of a recent tree (a solve's start, new
-  // terrains in the union): the next iteration takes subtree sizes as its
-  // heavy-child hint
+  // w.A holds no drainage area of a tree like the next one (a solve's
+  // start, new terrains in the union, the iteration after lake joins, whose
+  // drainage changes wholesale where lakes were): the next iteration runs
+  // synchronously and takes subtree sizes as its heavy-child hint
   bool hint_stale = false;
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
@@ -739,6 +740,8 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
       FLCHK(join_lakes(c, T));
       fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
     }
+    // sizes of the whole forest: sites of basins kept from the previous tree
+    // get them too (their drainage areas are read by no recomputed basin)
     std::string err;
     if (fl_device_subtree_sizes(s, n, w.next, T->d_tid, T->d_active, w.A, &err, &T->arena) != FL_OK)
       return set_err(FL_ERR_HIP, "%s", err.c_str());
@@ -771,6 +774,7 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
   FLCHK(read_ctrl(c, T));
   *changed = host_flag(c, FL_FLAG_CHANGED) != 0;
   std::swap(w.h, w.h2);
+  if (*had_lake) T->hint_stale = true;
   return FL_OK;
 }
 
@@ -1792,11 +1796,10 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   };
   while (active) {
     if (cur != T) cur->stat_syncs = 0;
-    // the first iteration (lakes in practice) and the unpipelined form; the
-    // second too: the first iteration's heavy children came from the sites'
-    // own areas (no drainage area yet), so its round counts (often ten times
-    // the usual) would make a poor speculation for the second
-    if (it < 2 || !c->pipelined) {
+    // the first iteration, those whose heavy-child hint is stale (after lake
+    // joins: a deep decomposition there would also make a poor speculation
+    // for the iterations after it) and the unpipelined form
+    if (it == 0 || cur->hint_stale || !c->pipelined) {
       bool changed = false, lake = false;
       std::vector<uint8_t> mask(cur->t_active);
       FLCHK(upload_mask(mask));
@@ -1927,7 +1930,7 @@ struct fl_lane {
   std::vector<uint32_t> enq;
   std::vector<uint8_t> dev_mask;
   int next_slot = 0;
-  uint32_t sync_next = 2;  // iterations to run synchronously (after a join: its first and second)
+  uint32_t sync_next = 1;  // iterations to run synchronously (after a join: its first)
   bool drain = false;          // enqueue nothing ahead: the lane is to become quiet
   bool compact_wanted = false;  // gather (drop the stopped terrains) once quiet
   uint32_t active = 0;
@@ -2055,7 +2058,7 @@ int32_t lane_gather(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_membe
   L->hin = 0;
   L->enq = L->cur->t_iters;
   L->dev_mask = L->cur->t_active;
-  L->sync_next = join.empty() ? 0u : 2u;
+  L->sync_next = join.empty() ? 0u : 1u;
   L->cur->hint_stale = !join.empty();
   L->drain = false;
   L->compact_wanted = false;
@@ -2151,7 +2154,7 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
     }
   };
   const double* result = nullptr;  // the buffer holding the confirmed iteration's elevations
-  if (L->sync_next || !c->pipelined) {
+  if (L->sync_next || cur->hint_stale || !c->pipelined) {
     std::vector<uint8_t> mask(cur->t_active);
     FLCHK(lane_upload_mask(L, mask));
     set_slots(cur, L->hin);

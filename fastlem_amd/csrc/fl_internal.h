@@ -113,9 +113,11 @@ int fl_device_scatter_u32(hipStream_t s, uint32_t count, const uint32_t* idx, co
 // order[rank[v]] = v for every site v with a rank (rank != UINT32_MAX).
 void fl_device_order(hipStream_t s, uint32_t n, const uint32_t* rank, uint32_t* order);
 // size[v] = the number of sites in v's subtree of the forest `next` (a root
-// points at itself; roots get n): the heavy-child hint of a first iteration,
-// when no drainage area exists yet.  Sites of terrains that are not active
-// (tid and t_active, device; null for one terrain) count as roots.
+// points at itself; roots get n): the heavy-child hint of an iteration with
+// no drainage area of a similar tree (a solve's first, the one after lake
+// joins).  Sites of terrains that are not active (tid and t_active, device;
+// null for one terrain) count as roots.  Stream-ordered: nothing is read
+// back (convergence is not checked: the sizes are only a hint).
 int fl_device_subtree_sizes(hipStream_t s, uint32_t n, const uint32_t* next, const uint32_t* tid,
                             const uint8_t* t_active, double* size, std::string* err, fl_arena* arena);
 uint64_t fl_subtree_scratch_bytes(uint32_t n);

@@ -840,16 +840,13 @@ int fl_device_subtree_sizes(hipStream_t s, uint32_t n, const uint32_t* next, con
   hipLaunchKernelGGL(k_fill32, dim3(1), B, 0, s, kMaxFlags, flags, 0u);
   hipLaunchKernelGGL(k_st_link, Gn, B, 0, s, n, next, tid, t_active, head, sib);
   hipLaunchKernelGGL(k_st_tour, Gn, B, 0, s, n, next, tid, t_active, head, sib, tour);
-  // a list of L arcs is ranked after ceil(log2 L) rounds; one more proves it
+  // a list of L arcs is ranked after ceil(log2 L) rounds (each round at
+  // least doubles every pointer's reach), rounds after a quiet one return
+  // at once; nothing is read back: the sizes are a hint, the stream goes on
   uint32_t lg = 1;
   while ((1ull << lg) < 2ull * n) lg++;
-  int rc = jump_to_fixpoint(
-      s, flags, kMaxFlags, lg + 1,
-      [&](uint32_t* fset, const uint32_t* prev) {
-        hipLaunchKernelGGL(k_list_jump, G2, B, 0, s, 2u * n, tour, fset, prev);
-      },
-      err, "subtree sizes");
-  if (rc != FL_OK) return rc;
+  for (uint32_t r = 0; r < lg && r < kMaxFlags; r++)
+    hipLaunchKernelGGL(k_list_jump, G2, B, 0, s, 2u * n, tour, flags + r, r ? flags + r - 1 : nullptr);
   hipLaunchKernelGGL(k_st_size, Gn, B, 0, s, n, next, tid, t_active, tour, size);
   HIPTRY(hipGetLastError());
   return FL_OK;
