@@ -866,33 +866,39 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Wavefront `wave` of `nwaves` in drainage-area round l (short chains
-// [c0, c1), long chains [c1, c2)); cmeta: (base, len, ...) per chain.
+// [c0, c1), long chains [c1, c2)); cmeta: (base, len, ...) per chain.  A
+// long chain's tiles are loaded D - 1 ahead of the fold (D = 2 or 3: the
+// registers a workgroup of 1024 lanes has allow two).
+template <int D>
 __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
                                          uint32_t lane, fl_da_lds& L, const uint4* __restrict__ cmeta,
                                          const fl_da_args& g) {
   const uint2* __restrict__ cm2 = reinterpret_cast<const uint2*>(cmeta);
   // long chains: one per block (uniform per block).  Tiles of 64 positions
-  // from the tail, software-pipelined: the loads of the next tile are
-  // issued before lane 0 folds the current one, with in-chain indices and
-  // no branches, so nothing waits for them before the fold.  No tile input
-  // depends on this round's results (light children head chains of deeper
-  // rounds).
+  // from the tail, software-pipelined D deep: a tile's loads are issued
+  // D - 1 tiles before its fold, unconditionally and at positions clamped
+  // into the chain, into registers of their own (the loop is unrolled D
+  // times: moving a register a load is still writing would wait for it), so
+  // waiting for one tile never waits for the tiles loaded after it.  No tile
+  // input depends on this round's results (light children head chains of
+  // deeper rounds).
   for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
     const uint2 m = cm2[2ull * c];
     const uint32_t base = m.x, len = m.y;
     const uint32_t pd = g.pdest[base];
-    // position of this lane in the tile ending at `end` (lanes past the
-    // tile read its first position)
-    auto pos_of = [&](uint32_t end) {
-      const uint32_t cs = end > 64 ? end - 64 : 0;
-      return base + (lane < end - cs ? cs + lane : cs);
+    // position of this lane in the tile ending at `e` (e > 0; lanes past
+    // the tile read its first position)
+    auto pos_of = [&](uint32_t e) {
+      const uint32_t cs = e > 64 ? e - 64 : 0;
+      return base + (lane < e - cs ? cs + lane : cs);
     };
+    // the tile ending `ahead` tiles below `e`, or the tail tile past the head
+    auto ahead_of = [&](uint32_t e, uint32_t ahead) { return e > 64 * ahead ? e - 64 * ahead : len; };
     uint32_t end = len;
-    uint32_t end1 = end > 64 ? end - 64 : 0;
-    fl_da_pos cur = da_load(g, pos_of(end));
-    fl_da_pos nxt = da_load(g, pos_of(end1 > 0 ? end1 : end));
     double r = FL_NEG0;  // lane 0's running value
-    while (end > 0) {
+    // fold the tile ending at `end` from cur, then load into cur the tile D
+    // tiles further towards the head
+    auto step = [&](fl_da_pos& cur) {
       const uint32_t cs = end > 64 ? end - 64 : 0, cnt = end - cs;
       const bool mine = lane < cnt;
       double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
@@ -910,17 +916,18 @@ __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, 
       L.d[lane] = y;
       L.n[lane] = np;
       L.k[lane] = mine ? cur.k : FL_NONE;
+      // positions with posts: absent posts are -0.0 and x + (-0.0) == x, so
+      // lane 0 adds a position's posts only where it has them (a scalar
+      // branch on these masks), one dependent add per position otherwise
+      const unsigned long long post1 = __ballot(np >= 1), post2 = __ballot(np >= 2);
       const unsigned long long rare = __ballot(np >= 3);
       // a position with more posts than staged reads the graph in the fold
-      // (rare): such tiles take the fold with loads, the others one without,
-      // so the loads in flight are not waited for during the fold
+      // (rare): such tiles take the fold with loads, the others one without
       const bool deep = __any(np > FL_POSTS);
       const uint32_t kk = cur.k;
       const double Kk = cur.K, dk = cur.d;
       wave_sync();
-      const uint32_t end2 = end1 > 64 ? end1 - 64 : 0;
-      if (end1 > 0) cur = nxt;
-      if (end2 > 0) nxt = da_load(g, pos_of(end2));
+      cur = da_load(g, pos_of(ahead_of(end, D)));
       if (lane == 0 && !deep) {
         // the terms of da_tail_terms for np <= FL_POSTS: p2 (and p3)
         int e = (int)cnt - 1;
@@ -934,54 +941,73 @@ __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, 
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
-            r = ((tp[u] + r) + ta[u]) + tb[u];
-            if ((rare >> (e - u)) & 1ull) {
-              r = r + L.c[e - u];
-              if (L.n[e - u] >= 4) r = r + L.d[e - u];
+            r = tp[u] + r;
+            if ((post1 >> (e - u)) & 1ull) {
+              r = r + ta[u];
+              if ((post2 >> (e - u)) & 1ull) {
+                r = r + tb[u];
+                if ((rare >> (e - u)) & 1ull) {
+                  r = r + L.c[e - u];
+                  if (L.n[e - u] >= 4) r = r + L.d[e - u];
+                }
+              }
             }
             L.r[e - u] = r;
           }
         }
         for (; e >= 0; --e) {
-          r = ((L.p[e] + r) + L.a[e]) + L.b[e];
-          if ((rare >> e) & 1ull) {
-            r = r + L.c[e];
-            if (L.n[e] >= 4) r = r + L.d[e];
+          r = L.p[e] + r;
+          if ((post1 >> e) & 1ull) {
+            r = r + L.a[e];
+            if ((post2 >> e) & 1ull) {
+              r = r + L.b[e];
+              if ((rare >> e) & 1ull) {
+                r = r + L.c[e];
+                if (L.n[e] >= 4) r = r + L.d[e];
+              }
+            }
           }
           L.r[e] = r;
         }
       }
       if (lane == 0 && deep) {
-        int e = (int)cnt - 1;
-        for (; e >= 7; e -= 8) {
-          double tp[8], ta[8], tb[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            tp[u] = L.p[e - u];
-            ta[u] = L.a[e - u];
-            tb[u] = L.b[e - u];
+        for (int e = (int)cnt - 1; e >= 0; --e) {
+          r = L.p[e] + r;
+          if ((post1 >> e) & 1ull) {
+            r = r + L.a[e];
+            if ((post2 >> e) & 1ull) {
+              r = r + L.b[e];
+              if ((rare >> e) & 1ull)
+                r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], g.rp, g.col, g.next, g.cmask, g.hc, g.A);
+            }
           }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            r = ((tp[u] + r) + ta[u]) + tb[u];
-            if ((rare >> (e - u)) & 1ull)
-              r = da_tail_terms(r, L.n[e - u], L.c[e - u], L.d[e - u], L.k[e - u], g.rp, g.col, g.next, g.cmask,
-                                g.hc, g.A);
-            L.r[e - u] = r;
-          }
-        }
-        for (; e >= 0; --e) {
-          r = ((L.p[e] + r) + L.a[e]) + L.b[e];
-          if ((rare >> e) & 1ull)
-            r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], g.rp, g.col, g.next, g.cmask, g.hc, g.A);
           L.r[e] = r;
         }
       }
       wave_sync();
       if (mine) da_store(g, base + cs + lane, kk, L.r[lane], Kk, dk);
       wave_sync();  // the tile arrays are rewritten next
-      end = end1;
-      end1 = end2;
+      end = cs;
+    };
+    fl_da_pos q0 = da_load(g, pos_of(end));
+    fl_da_pos q1 = da_load(g, pos_of(ahead_of(end, 1)));
+    if (D >= 3) {
+      fl_da_pos q2 = da_load(g, pos_of(ahead_of(end, 2)));
+      for (;;) {
+        step(q0);
+        if (end == 0) break;
+        step(q1);
+        if (end == 0) break;
+        step(q2);
+        if (end == 0) break;
+      }
+    } else {
+      for (;;) {
+        step(q0);
+        if (end == 0) break;
+        step(q1);
+        if (end == 0) break;
+      }
     }
     // lane 0's value is the head's A
     if (lane == 0 && pd != FL_NONE) g.cval_out[pd] = r;
@@ -1037,7 +1063,7 @@ k_da_fold(const uint32_t* __restrict__ guard, uint32_t l, const uint32_t* __rest
   const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
   if (*guard != 0 || c2 < c0) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ fl_da_lds L;
-  da_round(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, cmeta, g);
+  da_round<3>(c0, c1, c2, blockIdx.x, gridDim.x, threadIdx.x, L, cmeta, g);
 }
 
 // Drainage-area rounds maxld, maxld - 1, ..., lsplit (the small deep ones) in
@@ -1054,7 +1080,7 @@ k_da_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, con
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
   for (uint32_t l = top + 1; l-- > lsplit;) {
     const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
-    da_round(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, g);
+    da_round<2>(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, g);
     __syncthreads();
   }
 }
@@ -1075,19 +1101,32 @@ struct fl_tau_lds {
 
 // Wavefront `wave` of `nwaves` in response-time round l (short chains
 // [c0, c1), long chains [c1, c2)).
+//
+// The reference's step rt = 0.0 + (rt_next + x) adds 0.0, which changes
+// nothing unless y = rt_next + x is -0.0 (0.0 + -0.0 = +0.0; a NaN y is
+// already quiet and passes unchanged).  A sum is -0.0 only when both terms
+// are, and no running value is: a basin's outlet starts at 0.0 + (...),
+// never -0.0, and every other chain starts from its receiver's value.  So
+// the chains take the single add (one dependent operation per position
+// instead of two, bit for bit the same).
 template <bool clamp>
 __device__ __forceinline__ void
 tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves, uint32_t lane, fl_tau_lds& L,
           const uint4* __restrict__ cmeta, const double* __restrict__ xp,
           double* __restrict__ taup, const double* __restrict__ up, const double* __restrict__ tanp,
           const double* __restrict__ dp, const double* __restrict__ hop, double* __restrict__ hnp) {
-  // long chains: one per block (uniform per block)
+  // long chains: one per block (uniform per block).  Tiles of 64 positions
+  // from the head, loaded three tiles ahead of their fold, unconditionally
+  // at positions clamped into the chain and into registers of their own
+  // (the loop is unrolled three times), so waiting for one tile never waits
+  // for the tiles loaded after it.
   for (uint32_t c = c1 + wave; c < c2; c += nwaves) {
     // base, length, position of the head's receiver (FL_NONE at the basin
     // outlet) and of the basin outlet
     const uint4 m = cmeta[c];
     const uint32_t base = m.x, len = m.y;
-    // lane 0's recurrence state
+    // lane 0's recurrence state (at a basin outlet 0.0 + x_o, the
+    // reference's 0.0 + (0.0 + x_o): the same value, see above)
     double tau = 0.0, tau_o = 0.0, h_p = 0.0, h_o = 0.0;
     const uint32_t pp = m.z;
     if (lane == 0 && pp != FL_NONE) {
@@ -1099,41 +1138,53 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
         h_p = hnp[pp];
       }
     }
-    double x = 0.0, u = 0.0, tn = 0.0, d = 0.0;
-    if (lane < len) {
-      x = xp[base + lane];
+    struct TauIn {
+      double x, u, t, d;
+    };
+    auto tile_in = [&](uint32_t t0) {
+      const uint32_t P = base + (t0 + lane < len ? t0 + lane : len - 1);
+      TauIn q{xp[P], 0.0, 0.0, 0.0};
       if (clamp) {
-        u = up[base + lane];
-        tn = tanp[base + lane];
-        d = dp[base + lane];
+        q.u = up[P];
+        q.t = tanp[P];
+        q.d = dp[P];
       }
-    }
-    for (uint32_t t = 0; t < len; t += 64) {
+      return q;
+    };
+    uint32_t t = 0;
+    // fold the tile starting at t from q, then load into q the tile three
+    // tiles further
+    auto step = [&](TauIn& q) {
       const uint32_t cnt = (len - t) < 64 ? (len - t) : 64;
-      L.x[lane] = x;
+      const bool in = lane < cnt;
+      L.x[lane] = in ? q.x : 0.0;
       if (clamp) {
-        L.u[lane] = u;
-        L.t[lane] = tn;
-        L.d[lane] = d;
+        L.u[lane] = in ? q.u : 0.0;
+        L.t[lane] = in ? q.t : 0.0;
+        L.d[lane] = in ? q.d : 0.0;
       }
       wave_sync();
-      x = u = tn = d = 0.0;
-      if (t + 64 + lane < len) {
-        uint32_t P = base + t + 64 + lane;
-        x = xp[P];
-        if (clamp) {
-          u = up[P];
-          tn = tanp[P];
-          d = dp[P];
-        }
-      }
+      q = tile_in(t + 192);
       if (lane == 0) {
-        uint32_t j = 0;
-        if (t == 0 && pp == FL_NONE) {
-          // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
-          tau = 0.0 + (0.0 + L.x[0]);
-          L.tau[0] = tau;
-          if (clamp) {
+        if (!clamp) {
+          // batches of eight, all within the tile (positions past the chain
+          // fold zeros into values nothing reads)
+          for (uint32_t j = 0; j < cnt; j += 8) {
+            double tx[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) tx[u] = L.x[j + u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+              tau = tau + tx[u];  // 0.0 + y == y here (see above)
+              L.tau[j + u] = tau;
+            }
+          }
+        } else {
+          uint32_t j = 0;
+          if (t == 0 && pp == FL_NONE) {
+            // the outlet: rt_o = 0.0 + (0.0 + x_o); h_o' = h_o + U_o * max(0, 0)
+            tau = 0.0 + (0.0 + L.x[0]);
+            L.tau[0] = tau;
             double ho = hop[base];
             double nh = ho + L.u[0] * rust_max(tau - tau, 0.0);
             double slope = (nh - ho) / L.d[0];
@@ -1142,61 +1193,53 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
             h_p = nh;
             h_o = nh;
             tau_o = tau;
+            j = 1;
           }
-          j = 1;
-        }
-        if (!clamp) {
           while (j < cnt) {
-            const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
-            double tx[8];
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q)
-              if (q < m) tx[q] = L.x[j + q];
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-              if (q < m) {
-                tau = 0.0 + (tau + tx[q]);
-                L.tau[j + q] = tau;
-              }
-            }
-            j += m;
-          }
-        } else {
-          while (j < cnt) {
-            const uint32_t m = (cnt - j) < 8 ? (cnt - j) : 8;
+            const uint32_t mm = (cnt - j) < 8 ? (cnt - j) : 8;
             double tx[8], tu[8], tt[8], td[8];
 #pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-              if (q < m) {
-                tx[q] = L.x[j + q];
-                tu[q] = L.u[j + q];
-                tt[q] = L.t[j + q];
-                td[q] = L.d[j + q];
+            for (uint32_t u = 0; u < 8; ++u) {
+              if (u < mm) {
+                tx[u] = L.x[j + u];
+                tu[u] = L.u[j + u];
+                tt[u] = L.t[j + u];
+                td[u] = L.d[j + u];
               }
             }
 #pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-              if (q < m) {
-                tau = 0.0 + (tau + tx[q]);
-                double nh = h_o + tu[q] * rust_max(tau - tau_o, 0.0);
-                double slope = (nh - h_p) / td[q];
-                if (slope > tt[q]) nh = h_p + tt[q] * td[q];
-                L.tau[j + q] = tau;
-                L.h[j + q] = nh;
+            for (uint32_t u = 0; u < 8; ++u) {
+              if (u < mm) {
+                tau = tau + tx[u];  // 0.0 + y == y here (see above)
+                double nh = h_o + tu[u] * rust_max(tau - tau_o, 0.0);
+                double slope = (nh - h_p) / td[u];
+                if (slope > tt[u]) nh = h_p + tt[u] * td[u];
+                L.tau[j + u] = tau;
+                L.h[j + u] = nh;
                 h_p = nh;
               }
             }
-            j += m;
+            j += mm;
           }
         }
       }
       wave_sync();
-      if (lane < cnt) {
+      if (in) {
         taup[base + t + lane] = L.tau[lane];
         if (clamp) hnp[base + t + lane] = L.h[lane];
       }
+      wave_sync();  // the tile arrays are rewritten next
+      t += 64;
+    };
+    TauIn q0 = tile_in(0), q1 = tile_in(64), q2 = tile_in(128);
+    for (;;) {
+      step(q0);
+      if (t >= len) break;
+      step(q1);
+      if (t >= len) break;
+      step(q2);
+      if (t >= len) break;
     }
-    wave_sync();  // the tile arrays are reused by the next chain
   }
   // short chains: one lane each, the last blocks first (they hold no long
   // chain); the groups of the wavefronts that share an XCD are contiguous
@@ -1247,7 +1290,7 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
 #pragma unroll
         for (uint32_t q = 0; q < FL_BATCH; ++q) {
           if (q < cnt) {
-            tau = 0.0 + (tau + vx[q]);
+            tau = tau + vx[q];  // 0.0 + y == y here (see tau_round)
             taup[base + t + q] = tau;
           }
         }
@@ -1268,7 +1311,7 @@ tau_round(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t wave, uint32_t nwaves,
 #pragma unroll
         for (uint32_t q = 0; q < FL_BATCH; ++q) {
           if (q < cnt) {
-            tau = 0.0 + (tau + vx[q]);
+            tau = tau + vx[q];  // 0.0 + y == y here (see tau_round)
             double nh = h_o + vu[q] * rust_max(tau - tau_o, 0.0);
             double slope = (nh - h_p) / vd[q];
             if (slope > vt[q]) nh = h_p + vt[q] * vd[q];

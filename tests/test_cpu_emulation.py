@@ -96,13 +96,16 @@ def _emu_solve(L, terrs, compact=None, recompute_all=None):
     return outs, its
 
 
-def test_emulated_comb_many_light_children(emu_lib):
-    """Every site of a 24-site heavy chain (the long-chain path) folds six
-    light children after its heavy child, beyond the four kept per site."""
+@pytest.mark.parametrize("spine,leaves", [(24, 6), (300, 1), (200, 3)])
+def test_emulated_comb_many_light_children(emu_lib, spine, leaves):
+    """Every site of a heavy chain (the long-chain path) folds its light
+    children after its heavy child: six, beyond the four kept per site, on a
+    24-site chain; one or three on chains of several 64-position tiles (the
+    tiles loaded ahead of the fold)."""
     from conftest import comb_terrain
     from oracle import flo
 
-    t = comb_terrain()
+    t = comb_terrain(spine=spine, leaves=leaves)
     (got,), (it,) = _emu_solve(emu_lib, [t])
     g = flo.Graph(t["row_ptr"], t["col"], t["length"])
     ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"], t["outlets"])
