@@ -903,11 +903,21 @@ __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, 
       const bool mine = lane < cnt;
       double p = 0.0, a = 0.0, b = 0.0, x = 0.0, y = 0.0;
       uint32_t np = 0;
-      if (mine) {
-        if (cur.cn & FL_LIST_OVERFLOW)
-          da_partial(cur.k, g.rp, g.col, g.next, g.cmask, g.hc, g.area, g.A, p, a, b, x, y, np);
-        else
-          da_combine(cur.cn, cur.area, cur.v0, cur.v1, cur.v2, cur.v3, p, a, b, x, y, np);
+      if (__any(mine && (cur.cn & FL_LIST_OVERFLOW))) {
+        // a position with more light children than listed (rare) reads the
+        // graph; kept apart so that the usual tiles issue no load here
+        if (mine) {
+          if (cur.cn & FL_LIST_OVERFLOW)
+            da_partial(cur.k, g.rp, g.col, g.next, g.cmask, g.hc, g.area, g.A, p, a, b, x, y, np);
+          else
+            da_combine(cur.cn, cur.area, cur.v0, cur.v1, cur.v2, cur.v3, p, a, b, x, y, np);
+        }
+      } else {
+        da_combine(cur.cn, cur.area, cur.v0, cur.v1, cur.v2, cur.v3, p, a, b, x, y, np);
+        if (!mine) {
+          p = a = b = x = y = 0.0;
+          np = 0;
+        }
       }
       L.p[lane] = p;
       L.a[lane] = a;
@@ -916,20 +926,22 @@ __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, 
       L.d[lane] = y;
       L.n[lane] = np;
       L.k[lane] = mine ? cur.k : FL_NONE;
-      // positions with posts: absent posts are -0.0 and x + (-0.0) == x, so
-      // lane 0 adds a position's posts only where it has them (a scalar
-      // branch on these masks), one dependent add per position otherwise
-      const unsigned long long post1 = __ballot(np >= 1), post2 = __ballot(np >= 2);
-      const unsigned long long rare = __ballot(np >= 3);
-      // a position with more posts than staged reads the graph in the fold
-      // (rare): such tiles take the fold with loads, the others one without
+      // positions with a second post, and with three or more (those add p2,
+      // p3 and, past FL_POSTS posts, read the graph: rare, and such "deep"
+      // tiles take a fold of their own, so that no other fold holds a load)
+      const unsigned long long post2 = __ballot(np >= 2), rare = __ballot(np >= 3), post4 = __ballot(np >= 4);
       const bool deep = __any(np > FL_POSTS);
       const uint32_t kk = cur.k;
       const double Kk = cur.K, dk = cur.d;
       wave_sync();
-      cur = da_load(g, pos_of(ahead_of(end, D)));
       if (lane == 0 && !deep) {
-        // the terms of da_tail_terms for np <= FL_POSTS: p2 (and p3)
+        // x + (-0.0) == x: positions add the -0.0 pads of the posts they do
+        // not have, so a batch of eight folds without branches, with as
+        // many dependent adds a position as the most posts in the batch
+        // require (two for none or one post, up to five for four).  Every
+        // operand of a batch is read from the tile before its adds: no wait
+        // inside them, and no load from global memory in this loop (waiting
+        // for one would wait for the tiles in flight too)
         int e = (int)cnt - 1;
         for (; e >= 7; e -= 8) {
           double tp[8], ta[8], tb[8];
@@ -939,52 +951,69 @@ __device__ __forceinline__ void da_round(uint32_t c0, uint32_t c1, uint32_t c2, 
             ta[u] = L.a[e - u];
             tb[u] = L.b[e - u];
           }
+          const uint32_t sh = (uint32_t)(e - 7);
+          const uint32_t m2 = (uint32_t)(post2 >> sh) & 0xffu, m3 = (uint32_t)(rare >> sh) & 0xffu;
+          if (m3 == 0) {
+            if (m2 == 0) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            r = tp[u] + r;
-            if ((post1 >> (e - u)) & 1ull) {
-              r = r + ta[u];
-              if ((post2 >> (e - u)) & 1ull) {
-                r = r + tb[u];
-                if ((rare >> (e - u)) & 1ull) {
-                  r = r + L.c[e - u];
-                  if (L.n[e - u] >= 4) r = r + L.d[e - u];
+              for (int u = 0; u < 8; ++u) {
+                r = (tp[u] + r) + ta[u];
+                L.r[e - u] = r;
+              }
+            } else {
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                r = ((tp[u] + r) + ta[u]) + tb[u];
+                L.r[e - u] = r;
+              }
+            }
+          } else {
+            // (the third and fourth posts in two halves of four positions:
+            // registers)
+            const bool four = ((uint32_t)(post4 >> sh) & 0xffu) != 0;
+#pragma unroll
+            for (int h = 0; h < 8; h += 4) {
+              double tc[4], td[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                tc[q] = L.c[e - h - q];
+                td[q] = L.d[e - h - q];
+              }
+              if (!four) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  r = (((tp[h + q] + r) + ta[h + q]) + tb[h + q]) + tc[q];
+                  L.r[e - h - q] = r;
+                }
+              } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  r = ((((tp[h + q] + r) + ta[h + q]) + tb[h + q]) + tc[q]) + td[q];
+                  L.r[e - h - q] = r;
                 }
               }
             }
-            L.r[e - u] = r;
           }
         }
         for (; e >= 0; --e) {
-          r = L.p[e] + r;
-          if ((post1 >> e) & 1ull) {
-            r = r + L.a[e];
-            if ((post2 >> e) & 1ull) {
-              r = r + L.b[e];
-              if ((rare >> e) & 1ull) {
-                r = r + L.c[e];
-                if (L.n[e] >= 4) r = r + L.d[e];
-              }
-            }
-          }
+          r = ((((L.p[e] + r) + L.a[e]) + L.b[e]) + L.c[e]) + L.d[e];
           L.r[e] = r;
         }
       }
       if (lane == 0 && deep) {
+        // a tile with a position of more posts than staged: those read the
+        // graph (da_tail_terms)
         for (int e = (int)cnt - 1; e >= 0; --e) {
-          r = L.p[e] + r;
-          if ((post1 >> e) & 1ull) {
-            r = r + L.a[e];
-            if ((post2 >> e) & 1ull) {
-              r = r + L.b[e];
-              if ((rare >> e) & 1ull)
-                r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], g.rp, g.col, g.next, g.cmask, g.hc, g.A);
-            }
-          }
+          r = ((L.p[e] + r) + L.a[e]) + L.b[e];
+          if ((rare >> e) & 1ull)
+            r = da_tail_terms(r, L.n[e], L.c[e], L.d[e], L.k[e], g.rp, g.col, g.next, g.cmask, g.hc, g.A);
           L.r[e] = r;
         }
       }
       wave_sync();
+      // the tile D tiles further, loaded after the fold: a fold that waits
+      // for a load of its own (a deep tile) then waits for no tile's loads
+      cur = da_load(g, pos_of(ahead_of(end, D)));
       if (mine) da_store(g, base + cs + lane, kk, L.r[lane], Kk, dk);
       wave_sync();  // the tile arrays are rewritten next
       end = cs;

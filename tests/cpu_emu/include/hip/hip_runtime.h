@@ -103,6 +103,7 @@ int readlane(int v, int lane);
 // a wave barrier waits for every live lane of the wavefront; fences order
 // nothing further here (one fiber runs at a time)
 #define __builtin_amdgcn_wave_barrier() ((void)::flemu::wave_any(0))
+#define __builtin_amdgcn_s_setprio(p) ((void)(p))
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
   ::flemu::launch(dim3(grid), dim3(block), [=]() { kernel(__VA_ARGS__); })
