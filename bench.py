@@ -44,41 +44,48 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool) -> float:
+def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool, active: float = 1.0) -> float:
     """Algorithmic (compulsory) bytes per work item of one kernel: every array
     it must touch, read or written once; gathers that re-read an array are not
     counted.  Units are sites for the per-site kernels and chain positions
     (sites inside basins, in chain-major order) for the sweeps; the profiler
-    accumulates the units of every launch.  deg = nnz / n adjacency entries."""
+    accumulates the units of every launch.  deg = nnz / n adjacency entries.
+    `active`: the fraction of the sites whose basins are recomputed (the
+    per-site kernels of the decomposition read one flag for the others)."""
     deg = nnz / max(n, 1)
+    a = active
     table = {
-        # row_ptr 4 + h 8 + is_outlet 1, writes next 4 + dnext 8; col 4 + len 8 per entry
-        "k_receivers": 25 + 12 * deg,
+        # row_ptr 4 + h 8 + is_outlet 1 + the previous receiver and distance 12, writes
+        # next 4 + dnext 8; col 4 + len 8 per entry (the basin marks of changed sites aside)
+        "k_receivers": 37 + 12 * deg,
+        # every site: its previous basin 4 and that basin's mark 1, writes act 1; active:
         # row_ptr 4 + next 4 + previous area 8, writes cmask 8 + hc 4; col 4 per entry
-        "k_children": 28 + 4 * deg,
-        # next 4 + hc 4, writes jc pc jn lw 16
-        "k_chain_init": 24,
-        # jc pc jn lw read 16 + written 16
-        "k_chain_jump": 32,
-        # hc 4 per site (the tails' extra reads are not counted)
-        "k_bucket_count": 4,
-        "k_bucket_place": 4,
-        # rt 8 + is_outlet 1 + cp 8 + head_base 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4,
-        # writes perm 4 + list 16 + count 4 + area 8
-        "k_chain_perm": 81,
-        # perm 4 + count 4 + list 16 + area 8 + a child's A 8, writes A 8; then K 8 + dnext 8,
-        # writes x 8 (+ U, tan, h read and four chain-major copies written with max slopes)
-        "k_da_fold": 128 if clamp else 72,
+        "k_children": 6 + a * (28 + 4 * deg),
+        # act 1; active: next 4 + hc 4, writes the two pairs 16; others copy their root 8
+        "k_chain_init": 1 + a * 24 + (1 - a) * 8,
+        # act 1; active: the two pairs read 16 + written 16
+        "k_chain_jump": 1 + a * 32,
+        # act 1; active: hc 4 (the tails' extra reads are not counted)
+        "k_bucket_count": 1 + a * 4,
+        "k_bucket_place": 1 + a * 4,
+        # rt 8 + is_outlet 1 + cp 8 + head_base 4 + row_ptr 4 + cmask 8 + hc 4 + area 8 + col 4
+        # + K 8 + dnext 8, writes perm 4 + count 4 + area 8 + K 8 + dnext 8 + root 4 (+ U, tan,
+        # h read and four chain-major copies written with max slopes)
+        "k_chain_perm": 1 + a * (157 if clamp else 101),
+        # perm 4 + count 4 + area 8 + K 8 + dnext 8 + a listed child's A 8 (chain-major), writes
+        # A 8 + x 8
+        "k_da_fold": 56,
         # x 8, writes tau 8 (+ U, tan, d read and h written)
         "k_tau_fold<false>": 16,
         "k_tau_fold<true>": 48,
         # the deep rounds of both sweeps, one launch each: as their round kernels
-        "k_da_tail": 128 if clamp else 72,
+        "k_da_tail": 56,
         "k_tau_tail<false>": 16,
         "k_tau_tail<true>": 48,
-        # per site: rt 8 + cp 8 + head_base 4 + tau 8 + h 8 + U 8 (or the chain-major h 8),
-        # writes tau 8 + h 8 (the roots' values are shared)
-        "k_tau_scatter": 60,
+        # every site: act 1, writes the changed flag 1 and clears a basin mark 1; active:
+        # rt 8 + cp 8 + head_base 4 + tau 8 + h 8 + U 8 (or the chain-major h 8), writes
+        # tau 8 + h 8 (the roots' values are shared); others copy h 16
+        "k_tau_scatter": 3 + a * 60 + (1 - a) * 16,
     }
     return float(table.get(name, 0.0))
 
@@ -277,7 +284,9 @@ def profiled_roofline(L, ctx, t, n, nnz, clamp):
     L.fl_ctx_reset_profile(ctx)
     L.fl_terrain_invalidate_cache(t._h)
     t.solve()
-    names = ["k_receivers", "k_dirty", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count",
+    st = t.last_stats()
+    active = min(1.0, st["recomputed_sites"] / (float(n) * max(st["iterations"], 1))) if n else 1.0
+    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count",
              "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
              "k_tau_fold<true>", "k_da_tail", "k_tau_tail<false>", "k_tau_tail<true>", "k_tau_scatter",
              "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]
@@ -292,14 +301,14 @@ def profiled_roofline(L, ctx, t, n, nnz, clamp):
     L.fl_ctx_set_profiling(ctx, 0)
     total_kernel_ms = sum(v[1] for v in prof.values())
     for nm, (cnt, ms, units) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-        bpu = kernel_bytes_per_unit(nm, n, nnz, clamp)
+        bpu = kernel_bytes_per_unit(nm, n, nnz, clamp, active)
         gbs = bpu * units / (ms / 1e3) / 1e9 if bpu and ms else 0.0
         log(f"  {nm:16s} launches={cnt:7d} total={ms:9.3f} ms avg={ms / cnt * 1e3:8.2f} us "
             f"units/launch={units / cnt:11.1f} {gbs:7.1f} GB/s")
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
     dcnt, dms, dunits = prof[dom]
     avg_s = dms / dcnt / 1e3
-    bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp) * dunits / dcnt
+    bytes_per_launch = kernel_bytes_per_unit(dom, n, nnz, clamp, active) * dunits / dcnt
     achieved = bytes_per_launch / avg_s / 1e9 if bytes_per_launch else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
