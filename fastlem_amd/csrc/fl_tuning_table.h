@@ -16,10 +16,10 @@ struct fl_tuning_row {
 };
 
 static const fl_tuning_row fl_tuning_table[] = {
-    {32768ull, 0, 256, 4, 16, 1, 65536u},  // terrain 16384 sites: 3.60 -> 3.57 ms
-    {131072ull, 0, 256, 6, 16, 1, 65536u},  // terrain 65536 sites: 7.66 -> 7.66 ms
-    {524288ull, 0, 256, 6, 32, 1, 65536u},  // terrain 262144 sites: 26.99 -> 26.71 ms
-    {UINT64_MAX, 0, 256, 6, 16, 1, 65536u},  // terrain 1048576 sites: 88.99 -> 88.99 ms
-    {1317794ull, 1, 256, 6, 16, 1, 65536u},  // batch of 32 terrains, 1000..65536 sites (540264 in all): 48.22 -> 48.22 ms
-    {UINT64_MAX, 1, 256, 6, 16, 1, 65536u},  // batch of 64 terrains, 1000..262144 sites (3214323 in all): 330.26 -> 330.26 ms
+    {32768ull, 0, 256, 6, 16, 1, 65536u},  // terrain 16384 sites: 2.81 -> 2.76 ms
+    {131072ull, 0, 256, 4, 32, 1, 65536u},  // terrain 65536 sites: 6.16 -> 6.00 ms
+    {524288ull, 0, 256, 6, 32, 1, 65536u},  // terrain 262144 sites: 19.15 -> 19.15 ms
+    {UINT64_MAX, 0, 256, 6, 16, 1, 65536u},  // terrain 1048576 sites: 53.92 -> 53.92 ms
+    {1317794ull, 1, 256, 6, 16, 1, 65536u},  // batch of 32 terrains, 1000..65536 sites (540264 in all): 35.01 -> 35.01 ms
+    {UINT64_MAX, 1, 256, 6, 16, 1, 65536u},  // batch of 64 terrains, 1000..262144 sites (3214323 in all): 234.59 -> 234.59 ms
 };
