@@ -59,10 +59,9 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool, active: floa
         # next 4 + dnext 8; col 4 + len 8 per entry (the basin marks of changed sites aside)
         "k_receivers": 37 + 12 * deg,
         # every site: its previous basin 4 and that basin's mark 1, writes act 1; active:
-        # row_ptr 4 + next 4 + previous area 8, writes cmask 8 + hc 4; col 4 per entry
-        "k_children": 6 + a * (28 + 4 * deg),
-        # act 1; active: next 4 + hc 4, writes the two pairs 16; others copy their root 8
-        "k_chain_init": 1 + a * 24 + (1 - a) * 8,
+        # row_ptr 4 + next 4 + previous area 8, writes cmask 8 + hc 4 and the two chain
+        # pointers 16 (written by the parent); col 4 per entry; others copy their root 8
+        "k_children": 6 + a * (44 + 4 * deg) + (1 - a) * 8,
         # act 1; active: the two pairs read 16 + written 16
         "k_chain_jump": 1 + a * 32,
         # act 1; active: hc 4 (the tails' extra reads are not counted)
@@ -286,7 +285,7 @@ def profiled_roofline(L, ctx, t, n, nnz, clamp):
     t.solve()
     st = t.last_stats()
     active = min(1.0, st["recomputed_sites"] / (float(n) * max(st["iterations"], 1))) if n else 1.0
-    names = ["k_receivers", "k_children", "k_chain_init", "k_chain_jump", "k_bucket_count",
+    names = ["k_receivers", "k_children", "k_chain_jump", "k_bucket_count",
              "k_bucket_scan", "k_bucket_place", "k_chain_perm", "k_da_fold", "k_tau_fold<false>",
              "k_tau_fold<true>", "k_da_tail", "k_tau_tail<false>", "k_tau_tail<true>", "k_tau_scatter",
              "k_clear_iter", "k_sub_jump", "k_lake_keys", "k_lake_flip", "k_fill_u32", "k_fill_u64", "k_copy_u32"]

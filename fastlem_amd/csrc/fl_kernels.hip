@@ -12,7 +12,7 @@
 //     k_lake_flip                               reverse the lake's flow path
 //   tree decomposition (heavy paths, rebuilt every iteration):
 //     k_children        children of each site in row (slot) order
-//     k_chain_init/jump chain head, position in chain, light depth, root
+//     k_chain_jump      chain head, position in chain, light depth, root
 //     k_bucket_*        chains bucketed by light depth, laid out contiguously
 //     k_chain_perm      sites scattered to chain-major positions, with
 //                                             each site's light children
@@ -276,19 +276,33 @@ __global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const dou
 // Child slots and heavy child of every active site; act[u] (read by the
 // iteration's later kernels): u belongs to a dirty basin (every site when
 // `dirty` is null) of a terrain still iterating (tid, t_active: per-site
-// terrain and per-terrain mask of a union, or null).
+// terrain and per-terrain mask of a union, or null).  Also the chain and
+// root pointers for pointer jumping, written by each site for its children
+// (every child of an active site is active, and every active site is a root
+// or a child of one):
+//   cp: (chain parent, 1), (v, 0) at a chain head
+//   rt: (receiver, 1 at a head that is not a root: a light edge), (v, 0) at a root
+// An inactive site keeps its basin: its root is carried to this iteration's.
 __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const double* __restrict__ prev_area,
                            unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc,
                            uint8_t* __restrict__ act, const uint8_t* __restrict__ dirty,
                            const uint32_t* __restrict__ root_prev, const uint32_t* __restrict__ tid,
-                           const uint8_t* __restrict__ t_active) {
+                           const uint8_t* __restrict__ t_active, unsigned long long* __restrict__ cp,
+                           unsigned long long* __restrict__ rt, uint32_t* __restrict__ root_out) {
   uint32_t u = gid();
   if (u >= n) return;
   bool a = !dirty || dirty[root_prev[u]];
   if (tid) a = a && t_active[tid[u]];
   act[u] = a ? 1 : 0;
-  if (!a) return;
+  if (!a) {
+    root_out[u] = root_prev[u];
+    return;
+  }
+  if (next[u] == u) {
+    cp[u] = pair(u, 0u);
+    rt[u] = pair(u, 0u);
+  }
   uint32_t b = rp[u], e = rp[u + 1];
   uint32_t heavy = FL_NONE;
   unsigned long long m = 0;
@@ -302,31 +316,17 @@ __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const ui
         heavy = v;
       }
       if (k - b < 64) m |= 1ull << (k - b);
+      // a light child heads its chain (the heavy one is set below)
+      cp[v] = pair(v, 0u);
+      rt[v] = pair(u, 1u);
     }
   }
   cmask[u] = m;
   hc[u] = heavy;
-}
-
-// Chain and root pointers for pointer jumping (active sites).
-//   cp: (chain parent, 1), (v, 0) at a chain head
-//   rt: (receiver, 1 at a head that is not a root: a light edge), (v, 0) at a root
-// An inactive site keeps its basin: its root is carried to this iteration's.
-__global__ void k_chain_init(uint32_t n, const uint32_t* __restrict__ next, const uint32_t* __restrict__ hc,
-                             unsigned long long* __restrict__ cp, unsigned long long* __restrict__ rt,
-                             const uint8_t* __restrict__ act, const uint32_t* __restrict__ root_prev,
-                             uint32_t* __restrict__ root_out) {
-  uint32_t v = gid();
-  if (v >= n) return;
-  if (!act[v]) {
-    root_out[v] = root_prev[v];
-    return;
+  if (heavy != FL_NONE) {
+    cp[heavy] = pair(u, 1u);
+    rt[heavy] = pair(u, 0u);
   }
-  uint32_t p = next[v];
-  bool root = (p == v);
-  bool head = root || hc[p] != v;
-  cp[v] = head ? pair(v, 0u) : pair(p, 1u);
-  rt[v] = pair(p, (head && !root) ? 1u : 0u);
 }
 
 // One doubling round on both pointer systems, in place:
@@ -1678,9 +1678,8 @@ void fl_launch_dirty(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& 
 void fl_launch_children(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, bool full, const uint32_t* tid,
                         const uint8_t* t_active) {
   FL_LAUNCH(k_children, t.n, s, t.n, t.row_ptr, t.col, w.next, w.A, (unsigned long long*)w.cmask, w.hc, w.act,
-            full ? (const uint8_t*)nullptr : (const uint8_t*)w.dirty, w.root_prev, tid, t_active);
-  FL_LAUNCH(k_chain_init, t.n, s, t.n, w.next, w.hc, (unsigned long long*)w.cp, (unsigned long long*)w.rt, w.act,
-            w.root_prev, w.root);
+            full ? (const uint8_t*)nullptr : (const uint8_t*)w.dirty, w.root_prev, tid, t_active,
+            (unsigned long long*)w.cp, (unsigned long long*)w.rt, w.root);
 }
 
 void fl_launch_chain_jump(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t round) {
