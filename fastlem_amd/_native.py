@@ -53,7 +53,8 @@ class SolveStats(ctypes.Structure):
                 ("create_validate_ms", ctypes.c_double), ("create_relabel_ms", ctypes.c_double),
                 ("create_upload_ms", ctypes.c_double), ("create_on_device", ctypes.c_uint32),
                 ("create_clusters", ctypes.c_uint32), ("recomputed_sites", ctypes.c_uint64),
-                ("first_sweep_rounds", ctypes.c_uint32), ("max_sweep_rounds", ctypes.c_uint32)]
+                ("first_sweep_rounds", ctypes.c_uint32), ("max_sweep_rounds", ctypes.c_uint32),
+                ("rehinted", ctypes.c_uint32)]
 
 
 class ServerCounts(ctypes.Structure):
@@ -118,6 +119,7 @@ def _declare(lib):
         "fl_ctx_set_batch_compaction": (i32, [vp, u32]),
         "fl_ctx_set_pipelining": (i32, [vp, i32]),
         "fl_ctx_set_recompute_all": (i32, [vp, i32]),
+        "fl_ctx_set_rehint_rounds": (i32, [vp, u32]),
         "fl_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(f64),
                                      ctypes.POINTER(u64)]),
         "fl_ctx_reset_profile": (i32, [vp]),

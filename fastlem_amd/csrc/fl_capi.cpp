@@ -75,6 +75,7 @@ struct fl_ctx {
   bool graphs = false;                   // pipelined iterations replayed from captured graphs
   bool forbid_graphs = false;            // no graph replay whatever the tuning (allocation-free calls)
   bool recompute_all = false;            // every basin in every iteration (no dirty-basin selection)
+  uint32_t rehint_rounds = 0;            // sweep rounds beyond which an iteration re-hints (0: 2 log2 n + 8)
   fl_profiler* prof = nullptr;  // non-null while profiling is enabled
   hipStream_t stream = nullptr;
   uint32_t* host_ctrl = nullptr;  // pinned readback of the control block
@@ -137,6 +138,7 @@ struct fl_terrain {
   uint32_t stat_create_device = 0, stat_create_clusters = 0;
   uint64_t stat_recomputed = 0;  // sites of the recomputed basins, over the solve's iterations
   uint32_t stat_rounds_first = 0, stat_rounds_max = 0;  // sweep rounds: the solve's first iteration, any
+  uint32_t stat_rehinted = 0;  // iterations decomposed again with subtree sizes (too deep)
   uint64_t bytes = 0;
   // where its device arrays come from (the device pool unless a caller
   // supplies a region); region buffers are never freed one by one
@@ -717,6 +719,17 @@ uint64_t tail_positions(const std::vector<uint32_t>& off, uint32_t from, uint32_
   return units;
 }
 
+// A decomposition far deeper than subtree-size heavy children allow (light
+// depth <= log2 n): the previous drainage areas described another tree (a
+// river that moved takes its whole length of light edges), and the sweeps
+// would run one round per light edge.
+bool deep_rounds(const fl_ctx* c, uint32_t rounds, uint32_t n) {
+  if (c->rehint_rounds) return rounds > c->rehint_rounds;
+  uint32_t lg = 1;
+  while (lg < 32 && (1u << lg) < n) lg++;
+  return rounds > 2 * lg + 8;
+}
+
 // One outer iteration (generator.rs:141-209), synchronous form: the host
 // confirms pointer-jumping convergence and reads the bucket offsets before
 // the sweeps.  The decomposition is built speculatively before the lake test
@@ -754,6 +767,15 @@ int32_t one_iteration(fl_ctx* c, fl_terrain* T, bool full, bool keep, bool* chan
       FLCHK(join_lakes(c, T));
       fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
       FLCHK(decompose(c, T, full, true));
+    }
+    if (deep_rounds(c, T->last_rounds, n)) {
+      // decompose again with subtree sizes as the hint
+      std::string err;
+      fl_launch_clear_iter(s, w, std::min<uint32_t>(T->prev_nb + 1, 2 * n + 2));
+      if (fl_device_subtree_sizes(s, n, w.next, T->d_tid, T->d_active, w.A, &err, &T->arena) != FL_OK)
+        return set_err(FL_ERR_HIP, "%s", err.c_str());
+      T->stat_rehinted++;
+      FLCHK(decompose(c, T, full, false));
     }
   }
   uint32_t nb = T->prev_nb;
@@ -943,6 +965,12 @@ int32_t finish_fast(fl_ctx* c, fl_terrain* T, const Flight& f, bool* valid, bool
   *valid = true;
   T->prev_nb = nb;
   T->last_rounds = nb / 2;
+  // the next iterations would speculate as many rounds: they wait for a
+  // synchronous one with subtree sizes (the flights already enqueued land)
+  if (deep_rounds(c, T->last_rounds, n)) {
+    T->hint_stale = true;
+    T->stat_rehinted++;
+  }
   if (nb <= FL_CTRL_HEAD) {
     const uint32_t* head = hc + FL_NUM_FLAGS + FL_NUM_COUNTERS;
     T->last_boff.assign(head, head + nb + 1);
@@ -1737,6 +1765,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
   T->stat_recomputed = 0;
   T->stat_rounds_first = 0;
   T->stat_rounds_max = 0;
+  T->stat_rehinted = 0;
   T->stat_lake_iterations = 0;
   T->stat_compactions = 0;
   T->stat_redone = 0;
@@ -1795,11 +1824,14 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     }
   };
   while (active) {
-    if (cur != T) cur->stat_syncs = 0;
+    if (cur != T) {
+      cur->stat_syncs = 0;
+      cur->stat_rehinted = 0;
+    }
     // the first iteration, those whose heavy-child hint is stale (after lake
     // joins: a deep decomposition there would also make a poor speculation
     // for the iterations after it) and the unpipelined form
-    if (it == 0 || cur->hint_stale || !c->pipelined) {
+    if (it == 0 || (cur->hint_stale && q.empty()) || !c->pipelined) {
       bool changed = false, lake = false;
       std::vector<uint8_t> mask(cur->t_active);
       FLCHK(upload_mask(mask));
@@ -1819,7 +1851,7 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
     } else {
       // keep two iterations in flight: the host enqueues the next one while
       // the device runs the current one
-      while (!drain && q.size() < 2) {
+      while (!drain && !cur->hint_stale && q.size() < 2) {
         std::vector<uint8_t> mask(cur->nterr);
         bool any = false;
         for (uint32_t g = 0; g < cur->nterr; g++) {
@@ -1869,7 +1901,10 @@ int32_t solve_union(fl_ctx* c, fl_terrain* T, const uint32_t* max_it) {
       hin = (f.in + 1) % 3;
       it++;
     }
-    if (cur != T) T->stat_syncs += cur->stat_syncs;
+    if (cur != T) {
+      T->stat_syncs += cur->stat_syncs;
+      T->stat_rehinted += cur->stat_rehinted;
+    }
     if (!active) break;
     bool want = cur->nterr > 1 && active_sites * 2 < cur->t.n && cur->t.n - active_sites >= c->compact_min_idle;
     if (want) drain = true;
@@ -2154,7 +2189,7 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
     }
   };
   const double* result = nullptr;  // the buffer holding the confirmed iteration's elevations
-  if (L->sync_next || cur->hint_stale || !c->pipelined) {
+  if (L->sync_next || (cur->hint_stale && L->q.empty()) || !c->pipelined) {
     std::vector<uint8_t> mask(cur->t_active);
     FLCHK(lane_upload_mask(L, mask));
     set_slots(cur, L->hin);
@@ -2172,7 +2207,7 @@ int32_t fl_lane_step(fl_lane* L, std::vector<fl_lane_member>& stopped) {
   } else {
     // a drain for a join whose terrains went to another lane: go on
     if (L->drain && L->q.empty() && !L->compact_wanted) L->drain = false;
-    while (!L->drain && L->q.size() < 2) {
+    while (!L->drain && !cur->hint_stale && L->q.size() < 2) {
       std::vector<uint8_t> mask(cur->nterr);
       bool any = false;
       for (uint32_t g = 0; g < cur->nterr; g++) {
@@ -2482,6 +2517,12 @@ extern "C" int32_t fl_ctx_set_recompute_all(fl_ctx* c, int32_t all) {
   return FL_OK;
 }
 
+extern "C" int32_t fl_ctx_set_rehint_rounds(fl_ctx* c, uint32_t rounds) {
+  if (!c) return set_err(FL_ERR_INVALID, "null context");
+  c->rehint_rounds = rounds;
+  return FL_OK;
+}
+
 extern "C" int32_t fl_ctx_set_tuning(fl_ctx* c, const fl_tuning* t) {
   if (!c) return set_err(FL_ERR_INVALID, "null context");
   if (!t) {
@@ -2638,6 +2679,7 @@ extern "C" int32_t fl_terrain_last_stats(const fl_terrain* T, fl_solve_stats* ou
   out->recomputed_sites = T->stat_recomputed;
   out->first_sweep_rounds = T->stat_rounds_first;
   out->max_sweep_rounds = T->stat_rounds_max;
+  out->rehinted = T->stat_rehinted;
   return FL_OK;
 }
 

@@ -142,6 +142,13 @@ int32_t fl_ctx_set_pipelining(fl_ctx *ctx, int32_t enable);
  * elevations bit for bit.  1 recomputes every basin in every iteration, as
  * the reference does (for measurement).  Results do not depend on it. */
 int32_t fl_ctx_set_recompute_all(fl_ctx *ctx, int32_t all);
+/* The heavy child of a site continues its chain; it is chosen by the previous
+ * iteration's drainage areas, or by exact subtree sizes where those describe
+ * another tree.  An iteration whose decomposition needs more than `rounds`
+ * sweep rounds is decomposed again with subtree sizes (0, the default:
+ * 2 log2(n) + 8 for n sites; sizes keep the rounds within log2(n) + 1).
+ * Results do not depend on it. */
+int32_t fl_ctx_set_rehint_rounds(fl_ctx *ctx, uint32_t rounds);
 
 /* Launch tuning.  None of these change any result (every setting gives the
  * same bits); they shape the work:
@@ -236,6 +243,9 @@ typedef struct fl_solve_stats {
   uint32_t first_sweep_rounds; /* sweep rounds (light depth of the heavy-chain
                                   decomposition, + 1) of the first iteration */
   uint32_t max_sweep_rounds;   /*   and the most of any iteration */
+  uint32_t rehinted;           /* iterations decomposed again with subtree
+                                  sizes as the heavy-child hint (the previous
+                                  drainage areas gave a far deeper one) */
 } fl_solve_stats;
 int32_t fl_terrain_last_stats(const fl_terrain *t, fl_solve_stats *out);
 

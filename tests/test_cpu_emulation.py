@@ -62,7 +62,7 @@ def test_emulated_capped_oscillation(emu_lib):
     assert _bits_equal(got, ref)
 
 
-def _emu_solve(L, terrs, compact=None, recompute_all=None):
+def _emu_solve(L, terrs, compact=None, recompute_all=None, rehint=None, pipelining=None):
     """fl_batch_create/solve/download through the emulated library."""
     from cpu_emu import emu
     from fastlem_amd import _native
@@ -80,6 +80,10 @@ def _emu_solve(L, terrs, compact=None, recompute_all=None):
             emu.check(L, L.fl_ctx_set_batch_compaction(ctx, compact))
         if recompute_all is not None:
             emu.check(L, L.fl_ctx_set_recompute_all(ctx, recompute_all))
+        if rehint is not None:
+            emu.check(L, L.fl_ctx_set_rehint_rounds(ctx, rehint))
+        if pipelining is not None:
+            emu.check(L, L.fl_ctx_set_pipelining(ctx, pipelining))
         its = np.zeros(len(terrs), np.uint32)
         emu.check(L, L.fl_batch_solve(ctx, b, _native.ptr(its, ctypes.c_uint32)))
         for i, t in enumerate(terrs):
@@ -185,6 +189,31 @@ def test_emulated_first_iteration_light_depth(emu_lib):
         st = _emu_solve.last_stats
         assert 2 <= st.first_sweep_rounds <= int(np.log2(n)) + 1, (st.first_sweep_rounds, n)
         assert st.max_sweep_rounds >= st.first_sweep_rounds
+
+
+@pytest.mark.parametrize("pipelining", [1, 0])
+def test_emulated_rehinted_decompositions(emu_lib, pipelining):
+    """An iteration whose decomposition is deeper than the bound is decomposed
+    again with subtree sizes: with a bound of one round every iteration is,
+    in the synchronous and the pipelined form (where a confirmed deep flight
+    stops the enqueueing until a synchronous iteration re-hints), and the bits
+    and iteration counts stay the oracle's, alone and in a batch with caps."""
+    from fastlem_amd.device import DeviceBatch
+    from fastlem_amd.workloads import config_fbm, config_plain
+    from oracle import flo
+
+    fbm = DeviceBatch.terrain_of(config_fbm(3000, 55.0, seed=8))
+    plain = DeviceBatch.terrain_of(config_plain(1200), max_iteration=6)
+    for terrs in ([fbm], [plain, fbm]):
+        outs, its = _emu_solve(emu_lib, terrs, rehint=1, pipelining=pipelining)
+        st = _emu_solve.last_stats
+        assert st.rehinted > 0
+        for t, got, it in zip(terrs, outs, its):
+            g = flo.Graph(t["row_ptr"], t["col"], t["length"])
+            ref, rit, _ = flo.generate(g, t["area"], t["erodibility"], t["uplift"], t["init_elevation"],
+                                       t["outlets"], max_iteration=t.get("max_iteration", 0xFFFFFFFF))
+            assert it == rit
+            assert _bits_equal(got, ref)
 
 
 def test_emulated_batch_matches_solo(emu_lib):

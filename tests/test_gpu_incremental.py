@@ -1,5 +1,6 @@
 """On the GPU: the paths that skip or move work give the reference's bits.
 
+* Decompositions redone with subtree sizes as the heavy-child hint.
 * Recomputing only the basins whose drainage tree changed against
   recomputing every basin (fl_ctx_set_recompute_all) and against the oracle,
   with and without max slopes, in the synchronous, pipelined and redo-heavy
@@ -97,6 +98,34 @@ def test_dirty_basins_match_every_basin(kind, plain10k, fbm20k, strip):
         L.fl_ctx_set_pipelining(ctx, 1)
         L.fl_ctx_set_recompute_all(ctx, 0)
         t.close()
+
+
+def test_rehinted_decompositions_match_oracle(fbm20k, strip):
+    """With a one-round bound every iteration is decomposed again with
+    subtree sizes (synchronously, or after a confirmed flight stops the
+    enqueueing): the oracle's bits in every pipelining mode."""
+    from oracle import flo
+    from fastlem_amd import _native
+    from fastlem_amd.device import DeviceTerrain
+    from fastlem_amd.generator import context
+
+    L = _native.lib()
+    ctx = context()
+    for w in (fbm20k, strip):
+        ref, rit, _ = flo.generate(graph_of(w), w.model.areas(), w.erodibility, w.uplift, init_of(w), w.outlets())
+        t = DeviceTerrain.from_workload(w)
+        try:
+            for mode in (1, 0, 2):
+                assert L.fl_ctx_set_pipelining(ctx, mode) == 0
+                assert L.fl_ctx_set_rehint_rounds(ctx, 1) == 0
+                L.fl_terrain_invalidate_cache(t._h)
+                assert t.solve() == rit, mode
+                assert _bits_equal(ref, t.elevations()), mode
+                assert t.last_stats()["rehinted"] > 0
+        finally:
+            L.fl_ctx_set_pipelining(ctx, 1)
+            L.fl_ctx_set_rehint_rounds(ctx, 0)
+            t.close()
 
 
 def test_batch_dirty_basins_with_caps(plain10k, fbm20k):
