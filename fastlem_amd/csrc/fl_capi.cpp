@@ -125,6 +125,9 @@ struct fl_terrain {
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   uint32_t gJ[3] = {0, 0, 0}, gR[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
   int gF[3] = {0, 0, 0};
+  // launch tuning a graph was captured with (workgroup size, long-chain
+  // cutoff): another tuning captures afresh, so that it is the one timed
+  uint32_t gB[3] = {0, 0, 0}, gL[3] = {0, 0, 0};
   uint32_t* h_tslot[2] = {nullptr, nullptr};  // pinned per-terrain changed flags of iterations in flight
   uint8_t* h_mask = nullptr;                  // pinned staging of the active mask
   bool graphs_failed = false;
@@ -645,7 +648,7 @@ int32_t decompose(fl_ctx* c, fl_terrain* T, bool full, bool joined) {
   fl_launch_children(s, T->t, w, full, T->d_tid, T->d_active);
   uint32_t round = 0;
   uint32_t target = std::max<uint32_t>(T->jump_rounds, 1);
-  uint32_t quiet = FL_NONE_ROUND;  // first round in which no pointer moved
+  uint32_t quiet = FL_NONE_ROUND;  // first round that left every pair at its fixed point
   for (;;) {
     for (; round < target; round++) fl_launch_chain_jump(s, T->t, w, round);
     FLCHK(read_ctrl(c, T));
@@ -658,7 +661,7 @@ int32_t decompose(fl_ctx* c, fl_terrain* T, bool full, bool joined) {
     target = round + 2;
     if (target >= FL_MAX_JUMP_ROUNDS) return set_err(FL_ERR_HIP, "tree decomposition did not converge");
   }
-  // quiet + 1 rounds suffice (the quiet round confirms convergence)
+  // quiet + 1 rounds suffice (the quiet round completes convergence)
   T->jump_rounds = quiet + 1;
   w.guard = w.ctrl + FL_CTRL_JUMPS + (round - 1);  // a round at or after the quiet one: zero
   fl_launch_chain_count(s, T->t, w);
@@ -857,7 +860,8 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
   if (T->graphs_failed) return false;
   int p = T->w.h == T->hb[0] ? 0 : (T->w.h == T->hb[1] ? 1 : 2);
   const int mode = (full ? 1 : 0) | (keep ? 2 : 0);
-  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S || T->gF[p] != mode) {
+  if (!T->gexec[p] || T->gJ[p] != J || T->gR[p] != R || T->gS[p] != S || T->gF[p] != mode ||
+      T->gB[p] != fl_site_block || T->gL[p] != T->w.long_min) {
     if (T->gexec[p]) {
       hipGraphExecDestroy(T->gexec[p]);
       T->gexec[p] = nullptr;
@@ -881,6 +885,8 @@ bool launch_fast_graph(fl_ctx* c, fl_terrain* T, uint32_t J, uint32_t R, uint32_
     T->gR[p] = R;
     T->gS[p] = S;
     T->gF[p] = mode;
+    T->gB[p] = fl_site_block;
+    T->gL[p] = T->w.long_min;
   }
   return hipGraphLaunch(T->gexec[p], c->stream) == hipSuccess;
 }

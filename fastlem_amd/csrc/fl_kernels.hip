@@ -47,6 +47,7 @@
 #define FL_NONE 0xffffffffu
 #define FL_LIST 4                   // light children listed per chain position
 #define FL_LIST_OVERFLOW 0x10000u   // pcnt flag: more light children than listed
+#define FL_ROW 8                    // graph row entries loaded together
 
 namespace {
 
@@ -61,14 +62,22 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-__device__ __forceinline__ uint32_t gid() { return xcd_block() * blockDim.x + threadIdx.x; }
+// bs: the workgroup size the kernel was launched with, passed as an argument
+// (blockDim.x is read from the dispatch's implicit arguments by a vector load
+// that every wavefront would wait for before its first site load)
+__device__ __forceinline__ uint32_t gid(uint32_t bs) { return xcd_block() * bs + threadIdx.x; }
 
-// (pointer, count) pairs of the pointer jumping: pointer in the low word.
+// (pointer, count) pairs of the pointer jumping: pointer in the low word,
+// count in the high word below FL_FIN, the mark of a pair whose pointer is
+// known to be a fixed point (a chain head, a root).  Counts stay below 2^31
+// (they count sites).
+#define FL_FIN 0x80000000u
 __device__ __forceinline__ unsigned long long pair(uint32_t ptr, uint32_t cnt) {
   return ((unsigned long long)cnt << 32) | ptr;
 }
 __device__ __forceinline__ uint32_t pair_ptr(unsigned long long p) { return (uint32_t)p; }
-__device__ __forceinline__ uint32_t pair_cnt(unsigned long long p) { return (uint32_t)(p >> 32); }
+__device__ __forceinline__ uint32_t pair_cnt(unsigned long long p) { return (uint32_t)(p >> 32) & ~FL_FIN; }
+__device__ __forceinline__ bool pair_fin(unsigned long long p) { return (p >> 63) != 0; }
 
 // Is slot s of row k (starting at b) a child of k?
 __device__ __forceinline__ bool is_child(unsigned long long m, uint32_t deg, uint32_t s, uint32_t b,
@@ -112,45 +121,80 @@ __device__ __forceinline__ double first_len(const uint32_t* __restrict__ rp, con
 // marks the previous basins of itself, its new and its old receiver as
 // recomputed (the marks of k_dirty, made here without a launch of their
 // own).
-__global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+__global__ void k_receivers(uint32_t bs, uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                             const double* __restrict__ len, const double* __restrict__ h,
                             const uint8_t* __restrict__ is_outlet, uint32_t* __restrict__ next,
                             double* __restrict__ dnext, int* __restrict__ flags,
                             const uint8_t* __restrict__ chg_prev, const uint32_t* __restrict__ next_prev,
                             const double* __restrict__ dnext_prev, const uint32_t* __restrict__ root_prev,
                             uint8_t* __restrict__ dirty) {
-  uint32_t i = gid();
+  uint32_t i = gid(bs);
   if (i >= n) return;
-  if (is_outlet[i]) {
+  // the site's own words are loaded together, before the outlet test
+  const bool prev = chg_prev || dirty;
+  const uint8_t outlet = is_outlet[i];
+  const uint32_t b = rp[i], e = rp[i + 1];
+  const uint32_t pj = prev ? next_prev[i] : 0u;
+  const double pd = prev ? dnext_prev[i] : 0.0;
+  if (outlet) {
     next[i] = i;
     dnext[i] = 1.0;
     return;
   }
-  const uint32_t b = rp[i], e = rp[i + 1];
+  // a row is read FL_ROW entries at a time, every load of a group issued
+  // before the first is used (indices past the row's end repeat its last
+  // entry and are not evaluated)
+  uint32_t c[FL_ROW];
+  if (e > b) {
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(b + j, e - 1)];
+  }
   if (chg_prev) {
-    bool same = chg_prev[i] == 0;
-    for (uint32_t k = b; k < e && same; ++k) same = chg_prev[col[k]] == 0;
-    if (same) {
-      const uint32_t pj = next_prev[i];
+    uint32_t any = chg_prev[i];
+    for (uint32_t k = b; k < e; k += FL_ROW) {
+      if (k != b) {
+#pragma unroll
+        for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
+      }
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) any |= chg_prev[c[j]];
+      if (any) break;
+    }
+    if (!any) {
       if (pj == i) flags[FL_FLAG_LAKE] = 1;
       next[i] = pj;
-      dnext[i] = dnext_prev[i];
+      dnext[i] = pd;
       return;
+    }
+    if (e - b > FL_ROW && e > b) {
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) c[j] = col[b + j];
     }
   }
   const double hi = h[i];
   double best = 0.0, d = 1.0;
   uint32_t bj = i;
-  for (uint32_t k = b; k < e; ++k) {
-    uint32_t j = col[k];
-    double hj = h[j];
-    if (hi > hj) {
-      double lk = len[k];
-      double s = (hi - hj) / lk;
-      if (s > best) {
-        best = s;
-        bj = j;
-        d = lk;  // rows hold no duplicate edges: this slot is has_edge(i, bj)'s
+  for (uint32_t k = b; k < e; k += FL_ROW) {
+    if (k != b) {
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
+    }
+    double hj[FL_ROW], lk[FL_ROW];
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) {
+      hj[j] = h[c[j]];
+      lk[j] = len[min(k + j, e - 1)];
+    }
+    // slots in row order: the earliest of equal slopes is kept
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) {
+      if (k + j < e && hi > hj[j]) {
+        double s = (hi - hj[j]) / lk[j];
+        if (s > best) {
+          best = s;
+          bj = c[j];
+          d = lk[j];  // rows hold no duplicate edges: this slot is has_edge(i, bj)'s
+        }
       }
     }
   }
@@ -158,8 +202,7 @@ __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const u
   next[i] = bj;
   dnext[i] = d;
   if (dirty) {
-    const uint32_t pj = next_prev[i];
-    if (bj != pj || __double_as_longlong(d) != __double_as_longlong(dnext_prev[i])) {
+    if (bj != pj || __double_as_longlong(d) != __double_as_longlong(pd)) {
       dirty[root_prev[i]] = 1;
       dirty[root_prev[bj]] = 1;
       dirty[root_prev[pj]] = 1;
@@ -169,25 +212,25 @@ __global__ void k_receivers(uint32_t n, const uint32_t* __restrict__ rp, const u
 
 // ------------------------------------------------------------ lake join ---
 
-__global__ void k_copy_u32(uint32_t n, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
-  uint32_t i = gid();
+__global__ void k_copy_u32(uint32_t bs, uint32_t n, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  uint32_t i = gid(bs);
   if (i < n) dst[i] = src[i];
 }
 
-__global__ void k_fill_u64(uint32_t n, uint64_t* __restrict__ dst, uint64_t v) {
-  uint32_t i = gid();
+__global__ void k_fill_u64(uint32_t bs, uint32_t n, uint64_t* __restrict__ dst, uint64_t v) {
+  uint32_t i = gid(bs);
   if (i < n) dst[i] = v;
 }
 
-__global__ void k_fill_u32(uint32_t n, uint32_t* __restrict__ dst, uint32_t v) {
-  uint32_t i = gid();
+__global__ void k_fill_u32(uint32_t bs, uint32_t n, uint32_t* __restrict__ dst, uint32_t v) {
+  uint32_t i = gid(bs);
   if (i < n) dst[i] = v;
 }
 
 // One pointer-jumping round: out[i] = in[in[i]]; raises *flag if anything moved.
-__global__ void k_sub_jump(uint32_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+__global__ void k_sub_jump(uint32_t bs, uint32_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                            int* __restrict__ flag) {
-  uint32_t i = gid();
+  uint32_t i = gid(bs);
   if (i >= n) return;
   uint32_t s = in[i];
   uint32_t ss = in[s];
@@ -200,10 +243,10 @@ __global__ void k_sub_jump(uint32_t n, const uint32_t* __restrict__ in, uint32_t
 // (visit rank of i) << 32 | slot to that lake.  The minimum key names the
 // site i* and the slot of j* through which the reference's traversal first
 // touches the lake (stream_tree.rs:204-230).
-__global__ void k_lake_keys(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+__global__ void k_lake_keys(uint32_t bs, uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                             const uint32_t* __restrict__ rank, const uint32_t* __restrict__ sub,
                             const uint8_t* __restrict__ is_outlet, unsigned long long* __restrict__ key) {
-  uint32_t i = gid();
+  uint32_t i = gid(bs);
   if (i >= n) return;
   uint32_t r = rank[i];
   if (r == FL_NONE) return;
@@ -222,11 +265,11 @@ __global__ void k_lake_keys(uint32_t n, const uint32_t* __restrict__ rp, const u
 // to the pit so that j* drains into i* (stream_tree.rs:214-229) and refresh
 // the receiver distances of the reversed sites.  Lakes are disjoint, so all
 // flips run concurrently without conflicts.
-__global__ void k_lake_flip(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+__global__ void k_lake_flip(uint32_t bs, uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                             const double* __restrict__ len, const uint32_t* __restrict__ order,
                             const unsigned long long* __restrict__ key, const uint8_t* __restrict__ is_outlet,
                             uint32_t* __restrict__ next, double* __restrict__ dnext) {
-  uint32_t s = gid();
+  uint32_t s = gid(bs);
   if (s >= n) return;
   if (is_outlet[s] || next[s] != s) return;
   unsigned long long kv = key[s];
@@ -260,10 +303,10 @@ __global__ void k_lake_flip(uint32_t n, const uint32_t* __restrict__ rp, const u
 // after a lake join changed receivers).  Any other basin keeps every parent pointer and distance of
 // its tree, and no site of another basin attaches to it, so its drainage
 // areas, response times and elevations come out bit for bit as before.
-__global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const double* __restrict__ dnext,
+__global__ void k_dirty(uint32_t bs, uint32_t n, const uint32_t* __restrict__ next, const double* __restrict__ dnext,
                         const uint32_t* __restrict__ next_prev, const double* __restrict__ dnext_prev,
                         const uint32_t* __restrict__ root_prev, uint8_t* __restrict__ dirty) {
-  uint32_t v = gid();
+  uint32_t v = gid(bs);
   if (v >= n) return;
   const uint32_t a = next[v], b = next_prev[v];
   if (a != b || __double_as_longlong(dnext[v]) != __double_as_longlong(dnext_prev[v])) {
@@ -283,14 +326,14 @@ __global__ void k_dirty(uint32_t n, const uint32_t* __restrict__ next, const dou
 //   cp: (chain parent, 1), (v, 0) at a chain head
 //   rt: (receiver, 1 at a head that is not a root: a light edge), (v, 0) at a root
 // An inactive site keeps its basin: its root is carried to this iteration's.
-__global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
+__global__ void k_children(uint32_t bs, uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                            const uint32_t* __restrict__ next, const double* __restrict__ prev_area,
                            unsigned long long* __restrict__ cmask, uint32_t* __restrict__ hc,
                            uint8_t* __restrict__ act, const uint8_t* __restrict__ dirty,
                            const uint32_t* __restrict__ root_prev, const uint32_t* __restrict__ tid,
                            const uint8_t* __restrict__ t_active, unsigned long long* __restrict__ cp,
                            unsigned long long* __restrict__ rt, uint32_t* __restrict__ root_out) {
-  uint32_t u = gid();
+  uint32_t u = gid(bs);
   if (u >= n) return;
   bool a = !dirty || dirty[root_prev[u]];
   if (tid) a = a && t_active[tid[u]];
@@ -300,25 +343,38 @@ __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const ui
     return;
   }
   if (next[u] == u) {
-    cp[u] = pair(u, 0u);
-    rt[u] = pair(u, 0u);
+    cp[u] = pair(u, FL_FIN);
+    rt[u] = pair(u, FL_FIN);
   }
   uint32_t b = rp[u], e = rp[u + 1];
   uint32_t heavy = FL_NONE;
   unsigned long long m = 0;
   double best = 0.0;
-  for (uint32_t k = b; k < e; ++k) {
-    uint32_t v = col[k];
-    if (next[v] == u) {
-      double av = prev_area[v];
-      if (heavy == FL_NONE || av > best) {
-        best = av;
-        heavy = v;
+  for (uint32_t k = b; k < e; k += FL_ROW) {
+    // FL_ROW neighbours at a time, every load issued before the first is
+    // used (indices past the row's end repeat its last entry, not evaluated)
+    uint32_t c[FL_ROW], nx[FL_ROW];
+    double av[FL_ROW];
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) {
+      nx[j] = next[c[j]];
+      av[j] = prev_area[c[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) {
+      if (k + j < e && nx[j] == u) {
+        const uint32_t v = c[j];
+        if (heavy == FL_NONE || av[j] > best) {
+          best = av[j];
+          heavy = v;
+        }
+        if (k + j - b < 64) m |= 1ull << (k + j - b);
+        // a light child heads its chain (the heavy one is set below)
+        cp[v] = pair(v, FL_FIN);
+        rt[v] = pair(u, 1u);
       }
-      if (k - b < 64) m |= 1ull << (k - b);
-      // a light child heads its chain (the heavy one is set below)
-      cp[v] = pair(v, 0u);
-      rt[v] = pair(u, 1u);
     }
   }
   cmask[u] = m;
@@ -330,21 +386,27 @@ __global__ void k_children(uint32_t n, const uint32_t* __restrict__ rp, const ui
 }
 
 // One doubling round on both pointer systems, in place:
-// (a, d) <- (a', d + d') with (a', d') the pair of a.  A pair read while its
+// (a, d) <- (a'', d + d' + d'') with (a', d') the pair of a and (a'', d'')
+// that of a' (two hops: the reach triples per round).  A pair read while its
 // owner rewrites it is either its old or its new value (aligned 64-bit
 // accesses are single-copy atomic), and both are (ancestor, count) pairs of
-// the same path, so rounds may mix them.  A site whose pointers reach a
-// fixed point (a head, a root) stops writing.  After convergence
+// the same path, so rounds may mix them.  After convergence
 // cp[v] = (chain head, position in chain), rt[v] = (root of v's tree, light
-// edges between v and the root = light depth of v's chain).  Movement is
-// flagged in the round's own control word (zeroed at the start of the
-// iteration), so rounds need no separate clearing launch; a round without
-// movement proves convergence (a pointer at a fixed point never moves), and
-// the rounds launched after it return at once.
-__global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long long* rt,
+// edges between v and the root = light depth of v's chain).
+//
+// Fixed points (a head's cp, a root's rt: pairs (x, 0) of x itself) carry
+// FL_FIN, and so does a pair that reaches one: a site finding its pointer at
+// a fixed point marks its pair, and a hop inherits the mark of the pair it
+// copies (the pair of a' is marked when a'' is a fixed point).  A marked
+// pair never changes and its site loads nothing more.  A round flags, in its
+// own control word (zeroed at the start of the iteration), whether some
+// pair is still unmarked after it: a round without the flag proves
+// convergence in the round that completes it, and the rounds launched after
+// it return at once.
+__global__ void k_chain_jump(uint32_t bs, uint32_t n, unsigned long long* cp, unsigned long long* rt,
                              uint32_t* __restrict__ flag_set, const uint32_t* __restrict__ prev_flag,
                              const uint8_t* __restrict__ act) {
-  uint32_t v = gid();
+  uint32_t v = gid(bs);
   // the site's own pairs are read together with the previous round's flag
   // (an inactive site's pairs are stale and unused: no active site points
   // at one, as a dirty basin holds its sites' whole paths)
@@ -354,28 +416,37 @@ __global__ void k_chain_jump(uint32_t n, unsigned long long* cp, unsigned long l
     c = __atomic_load_n(cp + v, __ATOMIC_RELAXED);
     r = __atomic_load_n(rt + v, __ATOMIC_RELAXED);
   }
-  // a round after a round without movement has nothing to do
+  // a round after a round that left every pair marked has nothing to do
   if (prev_flag && *prev_flag == 0) return;
-  bool moved = false;
+  bool open = false;
   if (on) {
-    // two hops a round: (a, d) <- (a'', d + d' + d'') with (a', d') the
-    // pair of a and (a'', d'') that of a' (the reach triples per round)
+    const bool oc = !pair_fin(c), orr = !pair_fin(r);
     const uint32_t a = pair_ptr(c), b = pair_ptr(r);
-    const unsigned long long ca = __atomic_load_n(cp + a, __ATOMIC_RELAXED);
-    const unsigned long long rb = __atomic_load_n(rt + b, __ATOMIC_RELAXED);
-    const bool mc = pair_ptr(ca) != a, mr = pair_ptr(rb) != b;
+    const unsigned long long ca = oc ? __atomic_load_n(cp + a, __ATOMIC_RELAXED) : 0ull;
+    const unsigned long long rb = orr ? __atomic_load_n(rt + b, __ATOMIC_RELAXED) : 0ull;
+    // at a fixed point already (a is a head, b a root): mark; else hop
+    const bool fc = oc && pair_ptr(ca) == a, fr = orr && pair_ptr(rb) == b;
+    const bool mc = oc && !fc, mr = orr && !fr;
     const unsigned long long ca2 = mc ? __atomic_load_n(cp + pair_ptr(ca), __ATOMIC_RELAXED) : 0ull;
     const unsigned long long rb2 = mr ? __atomic_load_n(rt + pair_ptr(rb), __ATOMIC_RELAXED) : 0ull;
+    if (fc) __atomic_store_n(cp + v, c | (1ull << 63), __ATOMIC_RELAXED);
+    if (fr) __atomic_store_n(rt + v, r | (1ull << 63), __ATOMIC_RELAXED);
     if (mc) {
-      __atomic_store_n(cp + v, pair(pair_ptr(ca2), pair_cnt(c) + pair_cnt(ca) + pair_cnt(ca2)), __ATOMIC_RELAXED);
-      moved = true;
+      const bool f = pair_fin(ca2);
+      __atomic_store_n(cp + v,
+                       pair(pair_ptr(ca2), (pair_cnt(c) + pair_cnt(ca) + pair_cnt(ca2)) | (f ? FL_FIN : 0u)),
+                       __ATOMIC_RELAXED);
+      open = open || !f;
     }
     if (mr) {
-      __atomic_store_n(rt + v, pair(pair_ptr(rb2), pair_cnt(r) + pair_cnt(rb) + pair_cnt(rb2)), __ATOMIC_RELAXED);
-      moved = true;
+      const bool f = pair_fin(rb2);
+      __atomic_store_n(rt + v,
+                       pair(pair_ptr(rb2), (pair_cnt(r) + pair_cnt(rb) + pair_cnt(rb2)) | (f ? FL_FIN : 0u)),
+                       __ATOMIC_RELAXED);
+      open = open || !f;
     }
   }
-  if (__any(moved) && (threadIdx.x & 63u) == 0) *flag_set = 1;
+  if (__any(open) && (threadIdx.x & 63u) == 0) *flag_set = 1;
 }
 
 // Buckets: 2 * light depth + (chain length >= long_min), so every round
@@ -398,7 +469,7 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
-  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += FL_BUCKET_BLOCK) {
     s_c[t] = 0;
     s_n[t] = 0;
   }
@@ -423,7 +494,7 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += FL_BUCKET_BLOCK) {
     if (s_c[t]) {
       atomicAdd(bcnt + t, s_c[t]);
       atomicAdd(bnodes + t, s_n[t]);
@@ -436,10 +507,12 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
 // present (2 * (counters[FL_CNT_MAXLD] + 1) buckets), one workgroup.  The first
 // FL_CTRL_HEAD + 1 offsets are mirrored into the control block; offsets up to
 // `fill` are written.
-__global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ ctrl,
+#define FL_SCAN_BLOCK 1024  // k_bucket_scan: one workgroup of this size
+__global__ void __launch_bounds__(FL_SCAN_BLOCK)
+k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ ctrl,
                               const uint32_t* __restrict__ bcnt, const uint32_t* __restrict__ bnodes,
                               uint32_t* __restrict__ boff, uint32_t* __restrict__ bnoff) {
-  __shared__ uint32_t s_a[1024], s_b[1024];
+  __shared__ uint32_t s_a[FL_SCAN_BLOCK], s_b[FL_SCAN_BLOCK];
   __shared__ uint32_t carry_a, carry_b;
   uint32_t* counters = ctrl + FL_NUM_FLAGS;
   uint32_t nb = 2 * (counters[FL_CNT_MAXLD] + 1);
@@ -451,13 +524,13 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ 
     carry_b = 0;
   }
   __syncthreads();
-  for (uint32_t base = 0; base < nb; base += blockDim.x) {
+  for (uint32_t base = 0; base < nb; base += FL_SCAN_BLOCK) {
     uint32_t i = base + threadIdx.x;
     uint32_t a = i < nb ? bcnt[i] : 0, b = i < nb ? bnodes[i] : 0;
     s_a[threadIdx.x] = a;
     s_b[threadIdx.x] = b;
     __syncthreads();
-    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+    for (uint32_t off = 1; off < FL_SCAN_BLOCK; off <<= 1) {
       uint32_t ta = threadIdx.x >= off ? s_a[threadIdx.x - off] : 0;
       uint32_t tb = threadIdx.x >= off ? s_b[threadIdx.x - off] : 0;
       __syncthreads();
@@ -475,7 +548,7 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ 
       }
     }
     __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) {
+    if (threadIdx.x == FL_SCAN_BLOCK - 1) {
       carry_a += s_a[threadIdx.x];
       carry_b += s_b[threadIdx.x];
     }
@@ -493,7 +566,7 @@ __global__ void k_bucket_scan(uint32_t n, uint32_t fill, uint32_t* __restrict__ 
   }
   // offsets nb+1..fill repeat the totals: rounds launched beyond the deepest
   // light depth find empty ranges
-  for (uint32_t i = nb + 1 + threadIdx.x; i <= fill; i += blockDim.x) {
+  for (uint32_t i = nb + 1 + threadIdx.x; i <= fill; i += FL_SCAN_BLOCK) {
     boff[i] = carry_a;
     bnoff[i] = carry_b;
   }
@@ -523,7 +596,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
                uint32_t* __restrict__ head_base, uint32_t long_min, const uint8_t* __restrict__ act) {
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
-  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += FL_BUCKET_BLOCK) {
     s_c[t] = 0;
     s_n[t] = 0;
   }
@@ -551,7 +624,7 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += FL_BUCKET_BLOCK) {
     if (s_c[t]) {
       s_bc[t] = atomicAdd(bcur + t, s_c[t]);
       s_bn[t] = atomicAdd(bncur + t, s_n[t]);
@@ -593,7 +666,7 @@ struct fl_perm_out {
   uint4* cmeta;
 };
 
-__global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
+__global__ void k_chain_perm(uint32_t bs, const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
                              const unsigned long long* __restrict__ rt, const uint32_t* __restrict__ next,
                              const uint8_t* __restrict__ is_outlet, const uint32_t* __restrict__ head_base,
                              const uint32_t* __restrict__ head_chain, fl_perm_out o,
@@ -607,7 +680,7 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
                              const uint32_t* __restrict__ col, const unsigned long long* __restrict__ cmask,
                              const uint32_t* __restrict__ hc, const uint8_t* __restrict__ act,
                              uint32_t* __restrict__ root) {
-  uint32_t v = gid();
+  uint32_t v = gid(bs);
   // read together with the guard (one load latency)
   const uint32_t mld = *maxld;
   const bool on = v < n && act[v];
@@ -617,7 +690,7 @@ __global__ void k_chain_perm(const uint32_t* __restrict__ guard, uint32_t n, con
   // the bucket histograms are used up (count, scan, place): clear the
   // entries this decomposition touched for the next one
   const uint32_t nb = 2u * (mld + 1u);
-  for (uint32_t i = v; i <= nb && i < 2u * n + 4u; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = v; i <= nb && i < 2u * n + 4u; i += gridDim.x * bs) {
     bcnt[i] = 0;
     bnodes[i] = 0;
     bcur[i] = 0;
@@ -1106,7 +1179,7 @@ k_da_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, con
   if (*guard != 0 || top > 0xfffffffeu) return;  // pointer jumping has not converged: the iteration is redone
   top = top < cap ? top : cap;
   __shared__ fl_da_lds L[FL_TAIL_WAVES];
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = FL_TAIL_WAVES;
   for (uint32_t l = top + 1; l-- > lsplit;) {
     const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
     da_round<2>(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, g);
@@ -1384,7 +1457,7 @@ k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, co
   if (*guard != 0 || top > 0xfffffffeu) return;  // pointer jumping has not converged: the iteration is redone
   top = top < cap ? top : cap;
   __shared__ fl_tau_lds L[FL_TAIL_WAVES];
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = blockDim.x >> 6;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, nwaves = FL_TAIL_WAVES;
   for (uint32_t l = lsplit; l <= top; ++l) {
     const uint32_t c0 = boff[2 * l], c1 = boff[2 * l + 1], c2 = boff[2 * l + 2];
     tau_round<clamp>(c0, c1, c2, wave, nwaves, lane, L[wave], cmeta, xp, taup, up, tanp, dp, hop, hnp);
@@ -1401,7 +1474,7 @@ k_tau_tail(const uint32_t* __restrict__ guard, uint32_t lsplit, uint32_t cap, co
 // basin were settled by k_chain_perm.  In a batch (tid != null) a terrain
 // that stopped iterating keeps its elevations and each terrain has its own
 // flag.
-__global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
+__global__ void k_tau_scatter(uint32_t bs, const uint32_t* __restrict__ guard, uint32_t n, const unsigned long long* __restrict__ cp,
                               const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                               const uint32_t* __restrict__ head_base, const double* __restrict__ taup,
                               const double* __restrict__ hnp, const double* __restrict__ U,
@@ -1409,7 +1482,7 @@ __global__ void k_tau_scatter(const uint32_t* __restrict__ guard, uint32_t n, co
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged,
                               uint8_t* __restrict__ chg, uint8_t* __restrict__ dirty) {
-  const uint32_t v = gid();
+  const uint32_t v = gid(bs);
   // the basin marks were read by this iteration's k_children: clear them for
   // the next iteration's receivers (also when the iteration is redone: a
   // redo marks again from its own receivers)
@@ -1468,10 +1541,10 @@ __device__ __forceinline__ uint32_t shift_site(uint32_t x, const fl_compact_map&
 // New sites [v0, v1) of a union of n_new sites and nnz_new entries, taken
 // from the terrains of one source union (m, terrain q of the map becomes
 // terrain tid0 + q of the new union).
-__global__ void k_compact_sites(uint32_t v0, uint32_t v1, uint32_t n_new, uint32_t nnz_new, uint32_t tid0,
+__global__ void k_compact_sites(uint32_t bs, uint32_t v0, uint32_t v1, uint32_t n_new, uint32_t nnz_new, uint32_t tid0,
                                 fl_compact_map m, fl_dev_terrain ot, fl_dev_work ow, fl_dev_terrain nt,
                                 fl_dev_work nw, uint32_t* __restrict__ new_tid) {
-  uint32_t v = v0 + gid();
+  uint32_t v = v0 + gid(bs);
   if (v >= v1) return;
   uint32_t q = terrain_of(m, v);
   uint32_t src = v - m.new_off[q] + m.old_off[q];
@@ -1490,9 +1563,9 @@ __global__ void k_compact_sites(uint32_t v0, uint32_t v1, uint32_t n_new, uint32
   new_tid[v] = tid0 + q;
 }
 
-__global__ void k_compact_entries(uint32_t v0, uint32_t v1, fl_compact_map m, fl_dev_terrain ot,
+__global__ void k_compact_entries(uint32_t bs, uint32_t v0, uint32_t v1, fl_compact_map m, fl_dev_terrain ot,
                                   fl_dev_terrain nt) {
-  uint32_t v = v0 + gid();
+  uint32_t v = v0 + gid(bs);
   if (v >= v1) return;
   uint32_t q = terrain_of(m, v);
   uint32_t src = v - m.new_off[q] + m.old_off[q];
@@ -1558,10 +1631,10 @@ k_basin_order(uint32_t outlet, uint32_t cap, const uint32_t* __restrict__ rp, co
 }
 
 // Zero the control block and the first `nb` bucket entries.
-__global__ void k_clear_iter(uint32_t nb, uint32_t* __restrict__ ctrl, uint32_t* __restrict__ bcnt,
+__global__ void k_clear_iter(uint32_t bs, uint32_t nb, uint32_t* __restrict__ ctrl, uint32_t* __restrict__ bcnt,
                              uint32_t* __restrict__ bnodes, uint32_t* __restrict__ bcur,
                              uint32_t* __restrict__ bncur) {
-  uint32_t i = gid();
+  uint32_t i = gid(bs);
   if (i < FL_CTRL_WORDS) ctrl[i] = 0;
   if (i < nb) {
     bcnt[i] = 0;
@@ -1640,10 +1713,13 @@ fl_profiler::~fl_profiler() {
 // Per-site kernels: one thread per item in workgroups of fl_site_block
 // threads (the tuned workgroup size of the call in progress).
 thread_local uint32_t fl_site_block = 256;
-#define FL_LAUNCH_U(kernel, count, units, stream, ...)                                                      \
-  do {                                                                                                        \
-    if ((count) > 0)                                                                                          \
-      FL_LAUNCH_GRID(kernel, grid_for((count), fl_site_block), dim3(fl_site_block), (units), (stream), __VA_ARGS__); \
+// The workgroup size is also the kernel's first argument, taken into a local
+// first: a launch recorded for later replay holds the value of its time.
+#define FL_LAUNCH_U(kernel, count, units, stream, ...)                                                    \
+  do {                                                                                                      \
+    const uint32_t bs_ = fl_site_block;                                                                     \
+    if ((count) > 0)                                                                                        \
+      FL_LAUNCH_GRID(kernel, grid_for((count), bs_), dim3(bs_), (units), (stream), bs_, __VA_ARGS__);      \
   } while (0)
 #define FL_LAUNCH(kernel, count, stream, ...) FL_LAUNCH_U(kernel, count, count, stream, __VA_ARGS__)
 
@@ -1698,7 +1774,7 @@ void fl_launch_chain_count(hipStream_t s, const fl_dev_terrain& t, const fl_dev_
 
 void fl_launch_chain_layout(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w, uint32_t rounds) {
   uint64_t fill = std::min<uint64_t>(2ull * rounds, 2ull * t.n + 2);
-  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(1024), 0, s, t.n, (uint32_t)fill, w.ctrl, w.bcnt, w.bnodes, w.boff,
+  FL_LAUNCH_GRID(k_bucket_scan, dim3(1), dim3(FL_SCAN_BLOCK), 0, s, t.n, (uint32_t)fill, w.ctrl, w.bcnt, w.bnodes, w.boff,
                  w.bnoff);
   uint32_t per = FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD;
   FL_LAUNCH_GRID(k_bucket_place, dim3((t.n + per - 1) / per), dim3(FL_BUCKET_BLOCK), t.n, s, w.guard, t.n, w.hc,
