@@ -125,8 +125,10 @@ struct fl_dev_work {
   uint32_t* root = nullptr;             // [n] this iteration's, for the next one
   uint8_t* dirty = nullptr;             // [n] by root: basin to recompute
   uint8_t* act = nullptr;               // [n] site recomputed in this iteration
-  const uint8_t* chg_prev = nullptr;    // [n] elevation changed by the previous iteration
-  uint8_t* chg = nullptr;               // [n] ... by this one
+  // [n] the previous iteration changed the elevation of the site or of a
+  // neighbour (zeroed by k_receivers, marked by k_tau_scatter)
+  const uint8_t* chg_prev = nullptr;
+  uint8_t* chg = nullptr;               // [n] ... this one
 };
 
 // Workgroup size of the per-site kernels launched by the calling thread (a

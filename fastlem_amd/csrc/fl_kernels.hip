@@ -112,9 +112,11 @@ __device__ __forceinline__ double first_len(const uint32_t* __restrict__ rp, con
 // (h_i - h_j)/len, earliest slot on ties; i itself for outlets and pits.
 // flags[FL_FLAG_LAKE] is raised when a non-outlet site has no receiver.
 // A receiver depends on the site's and its neighbours' elevations only: with
-// chg_prev (per-site flags of the elevations the previous iteration changed,
-// whose tree had no lake joins), a site whose neighbourhood did not change
-// keeps its previous receiver and distance.
+// chg_prev (per-site flags, raised where the previous iteration, whose tree
+// had no lake joins, changed the elevation of the site or of a neighbour), a
+// site whose neighbourhood did not change keeps its previous receiver and
+// distance.  chg_out (this iteration's flags) is zeroed here for
+// k_tau_scatter to mark.
 //
 // With `dirty` (cleared by the previous iteration's k_tau_scatter), a site
 // whose receiver or receiver distance differs from the previous tree's
@@ -127,9 +129,10 @@ __global__ void k_receivers(uint32_t bs, uint32_t n, const uint32_t* __restrict_
                             double* __restrict__ dnext, int* __restrict__ flags,
                             const uint8_t* __restrict__ chg_prev, const uint32_t* __restrict__ next_prev,
                             const double* __restrict__ dnext_prev, const uint32_t* __restrict__ root_prev,
-                            uint8_t* __restrict__ dirty) {
+                            uint8_t* __restrict__ dirty, uint8_t* __restrict__ chg_out) {
   uint32_t i = gid(bs);
   if (i >= n) return;
+  chg_out[i] = 0;  // this iteration's k_tau_scatter marks the changed neighbourhoods
   // the site's own words are loaded together, before the outlet test
   const bool prev = chg_prev || dirty;
   const uint8_t outlet = is_outlet[i];
@@ -141,44 +144,22 @@ __global__ void k_receivers(uint32_t bs, uint32_t n, const uint32_t* __restrict_
     dnext[i] = 1.0;
     return;
   }
+  if (chg_prev && chg_prev[i] == 0) {
+    if (pj == i) flags[FL_FLAG_LAKE] = 1;
+    next[i] = pj;
+    dnext[i] = pd;
+    return;
+  }
   // a row is read FL_ROW entries at a time, every load of a group issued
   // before the first is used (indices past the row's end repeat its last
   // entry and are not evaluated)
   uint32_t c[FL_ROW];
-  if (e > b) {
-#pragma unroll
-    for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(b + j, e - 1)];
-  }
-  if (chg_prev) {
-    uint32_t any = chg_prev[i];
-    for (uint32_t k = b; k < e; k += FL_ROW) {
-      if (k != b) {
-#pragma unroll
-        for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
-      }
-#pragma unroll
-      for (int j = 0; j < FL_ROW; ++j) any |= chg_prev[c[j]];
-      if (any) break;
-    }
-    if (!any) {
-      if (pj == i) flags[FL_FLAG_LAKE] = 1;
-      next[i] = pj;
-      dnext[i] = pd;
-      return;
-    }
-    if (e - b > FL_ROW && e > b) {
-#pragma unroll
-      for (int j = 0; j < FL_ROW; ++j) c[j] = col[b + j];
-    }
-  }
   const double hi = h[i];
   double best = 0.0, d = 1.0;
   uint32_t bj = i;
   for (uint32_t k = b; k < e; k += FL_ROW) {
-    if (k != b) {
 #pragma unroll
-      for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
-    }
+    for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, e - 1)];
     double hj[FL_ROW], lk[FL_ROW];
 #pragma unroll
     for (int j = 0; j < FL_ROW; ++j) {
@@ -1481,7 +1462,8 @@ __global__ void k_tau_scatter(uint32_t bs, const uint32_t* __restrict__ guard, u
                               const double* __restrict__ h, double* __restrict__ h2, double* __restrict__ tau,
                               int* __restrict__ flags, int clamp, const uint32_t* __restrict__ tid,
                               const uint8_t* __restrict__ act, uint32_t* __restrict__ tchanged,
-                              uint8_t* __restrict__ chg, uint8_t* __restrict__ dirty) {
+                              uint8_t* __restrict__ chg, uint8_t* __restrict__ dirty,
+                              const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col) {
   const uint32_t v = gid(bs);
   // the basin marks were read by this iteration's k_children: clear them for
   // the next iteration's receivers (also when the iteration is redone: a
@@ -1490,6 +1472,7 @@ __global__ void k_tau_scatter(uint32_t bs, const uint32_t* __restrict__ guard, u
   // the site's pairs are read together with the guard (one load latency)
   const bool on = v < n && act[v];
   const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
+  const uint32_t rb = on ? rp[v] : 0u, re = on ? rp[v + 1] : 0u;  // the row, for the change marks
   if (*guard != 0) return;  // pointer jumping has not converged: the iteration is redone
   bool changed = false;
   const uint32_t o = pair_ptr(rv);
@@ -1514,9 +1497,20 @@ __global__ void k_tau_scatter(uint32_t bs, const uint32_t* __restrict__ guard, u
     changed = nh != h[v];
     if (changed && tchanged) tchanged[g] = 1;
   }
-  // per-site change flags for the next iteration's receivers (sites outside
-  // every basin kept their elevation in k_chain_perm)
-  if (v < n) chg[v] = changed ? 1 : 0;
+  // neighbourhood change flags for the next iteration's receivers (zeroed by
+  // this iteration's k_receivers; sites outside every basin kept their
+  // elevation in k_chain_perm): a changed site marks itself and its
+  // neighbours (rows are symmetric)
+  if (changed) {
+    chg[v] = 1;
+    for (uint32_t k = rb; k < re; k += FL_ROW) {
+      uint32_t c[FL_ROW];  // loaded together (past the row's end: its last entry again)
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) c[j] = col[min(k + j, re - 1)];
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) chg[c[j]] = 1;
+    }
+  }
   if (__any(changed) && (threadIdx.x & 63u) == 0) flags[FL_FLAG_CHANGED] = 1;
 }
 
@@ -1727,7 +1721,7 @@ void fl_launch_receivers(hipStream_t s, const fl_dev_terrain& t, const fl_dev_wo
                          bool mark) {
   FL_LAUNCH(k_receivers, t.n, s, t.n, t.row_ptr, t.col, t.len, w.h, t.is_outlet, w.next, w.dnext, w.flags,
             keep_unchanged ? w.chg_prev : (const uint8_t*)nullptr, w.next_prev, w.dnext_prev, w.root_prev,
-            mark ? w.dirty : (uint8_t*)nullptr);
+            mark ? w.dirty : (uint8_t*)nullptr, w.chg);
 }
 
 void fl_launch_sub_init(hipStream_t s, const fl_dev_terrain& t, const fl_dev_work& w) {
@@ -1836,7 +1830,7 @@ void fl_launch_tau_scatter(hipStream_t s, const fl_dev_terrain& t, const fl_dev_
                            const uint32_t* tid, const uint8_t* active, uint32_t* tchanged) {
   FL_LAUNCH(k_tau_scatter, t.n, s, w.guard, t.n, (const unsigned long long*)w.cp, (const unsigned long long*)w.rt,
             t.is_outlet, w.head_base, w.taup, w.hnp, t.uplift, w.h, w.h2, w.tau, w.flags, clamp ? 1 : 0, tid, w.act,
-            tchanged, w.chg, w.dirty);
+            tchanged, w.chg, w.dirty, t.row_ptr, t.col);
 }
 
 void fl_launch_clear_iter(hipStream_t s, const fl_dev_work& w, uint32_t nb) {
