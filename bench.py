@@ -55,9 +55,11 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool, active: floa
     deg = nnz / max(n, 1)
     a = active
     table = {
-        # row_ptr 4 + h 8 + is_outlet 1 + the previous receiver and distance 12, writes
-        # next 4 + dnext 8; col 4 + len 8 per entry (the basin marks of changed sites aside)
-        "k_receivers": 37 + 12 * deg,
+        # every site: is_outlet 1 + its neighbourhood flag 1 + the previous receiver and
+        # distance 12, writes next 4 + dnext 8 + this iteration's flag 1; sites near a
+        # change (about the recomputed share) also row_ptr 8 + h 8 and col 4 + len 8 per
+        # entry (the basin marks of changed sites aside)
+        "k_receivers": 27 + a * (16 + 12 * deg),
         # every site: its previous basin 4 and that basin's mark 1, writes act 1; active:
         # row_ptr 4 + next 4 + previous area 8, writes cmask 8 + hc 4 and the two chain
         # pointers 16 (written by the parent); col 4 per entry; others copy their root 8
@@ -81,10 +83,11 @@ def kernel_bytes_per_unit(name: str, n: int, nnz: int, clamp: bool, active: floa
         "k_da_tail": 56,
         "k_tau_tail<false>": 16,
         "k_tau_tail<true>": 48,
-        # every site: act 1, writes the changed flag 1 and clears a basin mark 1; active:
-        # rt 8 + cp 8 + head_base 4 + tau 8 + h 8 + U 8 (or the chain-major h 8), writes
-        # tau 8 + h 8 (the roots' values are shared); others copy h 16
-        "k_tau_scatter": 3 + a * 60 + (1 - a) * 16,
+        # every site: act 1, clears a basin mark 1; active: rt 8 + cp 8 + head_base 4 +
+        # tau 8 + h 8 + U 8 (or the chain-major h 8), writes tau 8 + h 8 (the roots' values
+        # are shared), and a changed site (taken as every active one) row_ptr 8, col 4 per
+        # entry and its neighbourhood's flags 1 + 1 per entry; others copy h 16
+        "k_tau_scatter": 2 + a * (69 + 5 * deg) + (1 - a) * 16,
     }
     return float(table.get(name, 0.0))
 
