@@ -366,9 +366,10 @@ __global__ void k_children(uint32_t bs, uint32_t n, const uint32_t* __restrict__
   }
 }
 
-// One doubling round on both pointer systems, in place:
-// (a, d) <- (a'', d + d' + d'') with (a', d') the pair of a and (a'', d'')
-// that of a' (two hops: the reach triples per round).  A pair read while its
+// One jumping round on both pointer systems, in place:
+// (a, d) <- (a''', d + d' + d'' + d''') with (a', d') the pair of a,
+// (a'', d'') that of a' and (a''', d''') that of a'' (three hops: the reach
+// quadruples per round).  A pair read while its
 // owner rewrites it is either its old or its new value (aligned 64-bit
 // accesses are single-copy atomic), and both are (ancestor, count) pairs of
 // the same path, so rounds may mix them.  After convergence
@@ -378,7 +379,7 @@ __global__ void k_children(uint32_t bs, uint32_t n, const uint32_t* __restrict__
 // Fixed points (a head's cp, a root's rt: pairs (x, 0) of x itself) carry
 // FL_FIN, and so does a pair that reaches one: a site finding its pointer at
 // a fixed point marks its pair, and a hop inherits the mark of the pair it
-// copies (the pair of a' is marked when a'' is a fixed point).  A marked
+// copies (the pair of a'' is marked when a''' is a fixed point).  A marked
 // pair never changes and its site loads nothing more.  A round flags, in its
 // own control word (zeroed at the start of the iteration), whether some
 // pair is still unmarked after it: a round without the flag proves
@@ -410,20 +411,26 @@ __global__ void k_chain_jump(uint32_t bs, uint32_t n, unsigned long long* cp, un
     const bool mc = oc && !fc, mr = orr && !fr;
     const unsigned long long ca2 = mc ? __atomic_load_n(cp + pair_ptr(ca), __ATOMIC_RELAXED) : 0ull;
     const unsigned long long rb2 = mr ? __atomic_load_n(rt + pair_ptr(rb), __ATOMIC_RELAXED) : 0ull;
+    // a third hop (a fixed point's pair is (x, 0) of x itself: hops past one
+    // stay there and add nothing)
+    const unsigned long long ca3 = mc ? __atomic_load_n(cp + pair_ptr(ca2), __ATOMIC_RELAXED) : 0ull;
+    const unsigned long long rb3 = mr ? __atomic_load_n(rt + pair_ptr(rb2), __ATOMIC_RELAXED) : 0ull;
     if (fc) __atomic_store_n(cp + v, c | (1ull << 63), __ATOMIC_RELAXED);
     if (fr) __atomic_store_n(rt + v, r | (1ull << 63), __ATOMIC_RELAXED);
     if (mc) {
-      const bool f = pair_fin(ca2);
-      __atomic_store_n(cp + v,
-                       pair(pair_ptr(ca2), (pair_cnt(c) + pair_cnt(ca) + pair_cnt(ca2)) | (f ? FL_FIN : 0u)),
-                       __ATOMIC_RELAXED);
+      const bool f = pair_fin(ca3);
+      __atomic_store_n(
+          cp + v,
+          pair(pair_ptr(ca3), (pair_cnt(c) + pair_cnt(ca) + pair_cnt(ca2) + pair_cnt(ca3)) | (f ? FL_FIN : 0u)),
+          __ATOMIC_RELAXED);
       open = open || !f;
     }
     if (mr) {
-      const bool f = pair_fin(rb2);
-      __atomic_store_n(rt + v,
-                       pair(pair_ptr(rb2), (pair_cnt(r) + pair_cnt(rb) + pair_cnt(rb2)) | (f ? FL_FIN : 0u)),
-                       __ATOMIC_RELAXED);
+      const bool f = pair_fin(rb3);
+      __atomic_store_n(
+          rt + v,
+          pair(pair_ptr(rb3), (pair_cnt(r) + pair_cnt(rb) + pair_cnt(rb2) + pair_cnt(rb3)) | (f ? FL_FIN : 0u)),
+          __ATOMIC_RELAXED);
       open = open || !f;
     }
   }
