@@ -669,11 +669,26 @@ __global__ void k_chain_perm(uint32_t bs, const uint32_t* __restrict__ guard, ui
                              const uint32_t* __restrict__ hc, const uint8_t* __restrict__ act,
                              uint32_t* __restrict__ root) {
   uint32_t v = gid(bs);
-  // read together with the guard (one load latency)
+  // the site's own words are read together with the guard (one load
+  // latency; a site of a lake basin reads some it does not use)
   const uint32_t mld = *maxld;
   const bool on = v < n && act[v];
-  const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
-  const uint32_t hq = on ? head_chain[v] : 0u;  // meaningful at a chain head only
+  unsigned long long rv = 0ull, cv = 0ull, m = 0ull;
+  uint32_t hq = 0u, b = 0u, e = 0u, hk = 0u, p = 0u;
+  double Kv = 0.0, dv = 0.0, av = 0.0;
+  if (on) {
+    rv = rt[v];
+    cv = cp[v];
+    hq = head_chain[v];  // meaningful at a chain head only
+    b = rp[v];
+    e = rp[v + 1];
+    hk = hc[v];
+    m = cmask[v];
+    p = next[v];
+    Kv = K[v];
+    dv = dnext[v];
+    av = area[v];
+  }
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   // the bucket histograms are used up (count, scan, place): clear the
   // entries this decomposition touched for the next one
@@ -685,21 +700,31 @@ __global__ void k_chain_perm(uint32_t bs, const uint32_t* __restrict__ guard, ui
     bncur[i] = 0;
   }
   if (!on) return;
-  uint32_t r = pair_ptr(rv);
+  const uint32_t r = pair_ptr(rv), hd = pair_ptr(cv), deg = e - b;
   root[v] = r;
-  if (!is_outlet[r]) {
-    A[v] = area[v];
+  // the second level of loads, issued together: the root's outlet flag, the
+  // chain's base, the row (up to FL_ROW entries) and, at a chain head, the
+  // receiver's pair and the outlet's base
+  const uint8_t outlet = is_outlet[r];
+  const uint32_t base = head_base[hd];
+  const bool head = hd == v;
+  const unsigned long long cpp = head ? cp[p] : 0ull;
+  const uint32_t hbr = head ? head_base[r] : 0u;
+  uint32_t cc[FL_ROW];
+  const bool short_row = deg <= FL_ROW;
+  if (short_row && deg > 0) {
+#pragma unroll
+    for (int j = 0; j < FL_ROW; ++j) cc[j] = col[min(b + j, e - 1)];
+  }
+  if (!outlet) {
+    A[v] = av;
     tau[v] = 0.0;
     h2[v] = h[v];
     return;
   }
-  unsigned long long c = cv;
-  uint32_t hd = pair_ptr(c);
-  uint32_t base = head_base[hd];
-  const uint32_t P = base + pair_cnt(c);
+  const uint32_t P = base + pair_cnt(cv);
   o.perm[P] = v;
-  const double dv = dnext[v];
-  o.pK[P] = K[v];
+  o.pK[P] = Kv;
   o.pdn[P] = dv;
   if (clamp) {
     o.up[P] = U[v];
@@ -712,31 +737,50 @@ __global__ void k_chain_perm(uint32_t bs, const uint32_t* __restrict__ guard, ui
   // their chains' heads to fill; more set the overflow bit and the fold
   // reads the graph instead
   {
-    const uint32_t b = rp[v], deg = rp[v + 1] - b, hk = hc[v];
-    const unsigned long long m = cmask[v];
     uint32_t npre = 0, npost = 0;
     bool after = false;
-    for (uint32_t s = deg; s-- > 0;) {
-      if (!is_child(m, deg, s, b, col, next, v)) continue;
-      uint32_t ch = col[b + s];
-      if (ch == hk) {
-        after = true;
-        continue;
+    if (short_row) {
+      // the children's bases loaded together (a row of at most FL_ROW
+      // entries: the child mask covers it)
+      uint32_t hb[FL_ROW];
+#pragma unroll
+      for (int j = 0; j < FL_ROW; ++j) {
+        hb[j] = 0u;
+        if (j < (int)deg && ((m >> j) & 1ull) && cc[j] != hk) hb[j] = head_base[cc[j]];
       }
-      uint32_t idx = npre + npost;
-      if (idx < FL_LIST) o.pdest[head_base[ch]] = FL_LIST * P + idx;
-      if (after) ++npost;
-      else ++npre;
+#pragma unroll
+      for (int j = FL_ROW - 1; j >= 0; --j) {
+        if (j >= (int)deg || !((m >> j) & 1ull)) continue;
+        if (cc[j] == hk) {
+          after = true;
+          continue;
+        }
+        uint32_t idx = npre + npost;
+        if (idx < FL_LIST) o.pdest[hb[j]] = FL_LIST * P + idx;
+        if (after) ++npost;
+        else ++npre;
+      }
+    } else {
+      for (uint32_t s = deg; s-- > 0;) {
+        if (!is_child(m, deg, s, b, col, next, v)) continue;
+        uint32_t ch = col[b + s];
+        if (ch == hk) {
+          after = true;
+          continue;
+        }
+        uint32_t idx = npre + npost;
+        if (idx < FL_LIST) o.pdest[head_base[ch]] = FL_LIST * P + idx;
+        if (after) ++npost;
+        else ++npre;
+      }
     }
     bool over = npre + npost > FL_LIST;
     o.pcnt[P] = over ? FL_LIST_OVERFLOW : (npre | (npost << 8));
-    o.parea[P] = area[v];
+    o.parea[P] = av;
   }
-  if (hd == v) {
-    uint32_t p = next[v];
-    unsigned long long cpp = cp[p];
+  if (head) {
     const uint32_t pp = (p == v) ? FL_NONE : head_base[pair_ptr(cpp)] + pair_cnt(cpp);
-    reinterpret_cast<uint2*>(o.cmeta)[2ull * hq + 1] = make_uint2(pp, head_base[r]);
+    reinterpret_cast<uint2*>(o.cmeta)[2ull * hq + 1] = make_uint2(pp, hbr);
   }
 }
 
