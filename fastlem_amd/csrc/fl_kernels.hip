@@ -316,18 +316,21 @@ __global__ void k_children(uint32_t bs, uint32_t n, const uint32_t* __restrict__
                            unsigned long long* __restrict__ rt, uint32_t* __restrict__ root_out) {
   uint32_t u = gid(bs);
   if (u >= n) return;
-  bool a = !dirty || dirty[root_prev[u]];
-  if (tid) a = a && t_active[tid[u]];
+  // the site's own words together (an inactive site reads its receiver and
+  // row for nothing: one load latency less for the active ones)
+  const uint32_t ru = root_prev[u], nu = next[u], b = rp[u], e = rp[u + 1];
+  const uint32_t tu = tid ? tid[u] : 0u;
+  bool a = !dirty || dirty[ru];
+  if (tid) a = a && t_active[tu];
   act[u] = a ? 1 : 0;
   if (!a) {
-    root_out[u] = root_prev[u];
+    root_out[u] = ru;
     return;
   }
-  if (next[u] == u) {
+  if (nu == u) {
     cp[u] = pair(u, FL_FIN);
     rt[u] = pair(u, FL_FIN);
   }
-  uint32_t b = rp[u], e = rp[u + 1];
   uint32_t heavy = FL_NONE;
   unsigned long long m = 0;
   double best = 0.0;
@@ -1520,33 +1523,52 @@ __global__ void k_tau_scatter(uint32_t bs, const uint32_t* __restrict__ guard, u
   // the next iteration's receivers (also when the iteration is redone: a
   // redo marks again from its own receivers)
   if (v < n) dirty[v] = 0;
-  // the site's pairs are read together with the guard (one load latency)
-  const bool on = v < n && act[v];
-  const unsigned long long rv = on ? rt[v] : 0ull, cv = on ? cp[v] : 0ull;
-  const uint32_t rb = on ? rp[v] : 0u, re = on ? rp[v + 1] : 0u;  // the row, for the change marks
+  // the site's own words are read together with the guard (one load
+  // latency), then the second level (outlet flag, positions, the outlet's
+  // elevation and uplift) together
+  const bool in = v < n;
+  const bool on = in && act[v];
+  const double hv = in ? h[v] : 0.0;
+  unsigned long long rv = 0ull, cv = 0ull;
+  uint32_t rb = 0u, re = 0u, g = 0u;  // the row, for the change marks
+  double uv = 0.0;
+  if (on) {
+    rv = rt[v];
+    cv = cp[v];
+    rb = rp[v];
+    re = rp[v + 1];
+    uv = U[v];
+    g = tid ? tid[v] : 0u;
+  }
   if (*guard != 0) return;  // pointer jumping has not converged: the iteration is redone
   bool changed = false;
-  const uint32_t o = pair_ptr(rv);
-  if (v < n && !on) {
+  if (in && !on) {
     // a site of a basin that keeps its tree (or of a terrain that stopped)
     // keeps its elevation
-    h2[v] = h[v];
-  } else if (on && is_outlet[o]) {
-    const uint32_t g = tid ? tid[v] : 0;
+    h2[v] = hv;
+  } else if (on) {
+    const uint32_t o = pair_ptr(rv);
+    const uint8_t outlet = is_outlet[o];
+    // (a site of a lake basin reads these for nothing; its values were
+    // written by k_chain_perm)
     const uint32_t P = head_base[pair_ptr(cv)] + pair_cnt(cv);
-    const double tk = taup[P];
-    double nh;
-    if (clamp) {
-      nh = hnp[P];
-    } else {
-      const double to = taup[head_base[o]];
-      const double ho = h[o] + U[o] * rust_max(to - to, 0.0);
-      nh = (v == o) ? ho : ho + U[v] * rust_max(tk - to, 0.0);
+    const uint32_t Po = clamp ? 0u : head_base[o];
+    const double h_o = clamp ? 0.0 : h[o], u_o = clamp ? 0.0 : U[o];
+    if (outlet) {
+      const double tk = taup[P];
+      double nh;
+      if (clamp) {
+        nh = hnp[P];
+      } else {
+        const double to = taup[Po];
+        const double ho = h_o + u_o * rust_max(to - to, 0.0);
+        nh = (v == o) ? ho : ho + uv * rust_max(tk - to, 0.0);
+      }
+      tau[v] = tk;
+      h2[v] = nh;
+      changed = nh != hv;
+      if (changed && tchanged) tchanged[g] = 1;
     }
-    tau[v] = tk;
-    h2[v] = nh;
-    changed = nh != h[v];
-    if (changed && tchanged) tchanged[g] = 1;
   }
   // neighbourhood change flags for the next iteration's receivers (zeroed by
   // this iteration's k_receivers; sites outside every basin kept their
