@@ -448,6 +448,39 @@ __global__ void k_chain_jump(uint32_t bs, uint32_t n, unsigned long long* cp, un
 #define FL_BUCKET_BLOCK 1024
 #define FL_BUCKET_PER_THREAD 2
 
+// The chain tails among a thread's FL_BUCKET_PER_THREAD sites of the
+// workgroup's range starting at `base`, with their pairs: activity and heavy
+// child of every site loaded first (clamped in range, all together), the
+// pairs of the tails second, the outlet flag of their roots third.
+struct fl_tail {
+  unsigned long long r, c;
+  bool t;
+};
+__device__ __forceinline__ void load_tails(uint32_t base, uint32_t n, const uint32_t* __restrict__ hc,
+                                           const unsigned long long* __restrict__ cp,
+                                           const unsigned long long* __restrict__ rt,
+                                           const uint8_t* __restrict__ is_outlet, const uint8_t* __restrict__ act,
+                                           fl_tail (&q)[FL_BUCKET_PER_THREAD]) {
+  uint8_t a[FL_BUCKET_PER_THREAD];
+  uint32_t h[FL_BUCKET_PER_THREAD];
+#pragma unroll
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
+    const uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x, vv = v < n ? v : n - 1;
+    a[j] = act[vv];
+    h[j] = hc[vv];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
+    const uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
+    q[j].t = v < n && a[j] && h[j] == FL_NONE;
+    q[j].r = q[j].t ? rt[v] : 0ull;
+    q[j].c = q[j].t ? cp[v] : 0ull;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j)
+    if (q[j].t) q[j].t = is_outlet[pair_ptr(q[j].r)] != 0;
+}
+
 // Light-depth histogram of the chains (count and total length), aggregated
 // per workgroup in LDS before one global atomic per (workgroup, bucket).
 // Each chain is counted by its tail (the member without a heavy child),
@@ -457,6 +490,9 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
                const unsigned long long* __restrict__ rt, const uint8_t* __restrict__ is_outlet,
                uint32_t* __restrict__ bcnt, uint32_t* __restrict__ bnodes, uint32_t* __restrict__ maxld,
                uint32_t long_min, const uint8_t* __restrict__ act) {
+  const uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  fl_tail q[FL_BUCKET_PER_THREAD];
+  load_tails(base, n, hc, cp, rt, is_outlet, act, q);  // in flight while the guard is read
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS];
   __shared__ uint32_t s_max;
@@ -466,22 +502,17 @@ k_bucket_count(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
   }
   if (threadIdx.x == 0) s_max = 0;
   __syncthreads();
-  uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
-    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    if (v < n && act[v] && hc[v] == FL_NONE) {
-      unsigned long long r = rt[v];
-      if (is_outlet[pair_ptr(r)]) {
-        uint32_t lw = pair_cnt(r), len = pair_cnt(cp[v]) + 1, l = 2u * lw + (len >= long_min ? 1u : 0u);
-        if (l < FL_LDS_BUCKETS) {
-          atomicAdd(s_c + l, 1u);
-          atomicAdd(s_n + l, len);
-        } else {
-          atomicAdd(bcnt + l, 1u);
-          atomicAdd(bnodes + l, len);
-        }
-        atomicMax(&s_max, lw);
+    if (q[j].t) {
+      uint32_t lw = pair_cnt(q[j].r), len = pair_cnt(q[j].c) + 1, l = 2u * lw + (len >= long_min ? 1u : 0u);
+      if (l < FL_LDS_BUCKETS) {
+        atomicAdd(s_c + l, 1u);
+        atomicAdd(s_n + l, len);
+      } else {
+        atomicAdd(bcnt + l, 1u);
+        atomicAdd(bnodes + l, len);
       }
+      atomicMax(&s_max, lw);
     }
   }
   __syncthreads();
@@ -585,6 +616,9 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
                const uint32_t* __restrict__ bnoff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ bncur,
                uint4* __restrict__ cmeta, uint32_t* __restrict__ head_chain, uint32_t* __restrict__ pdest,
                uint32_t* __restrict__ head_base, uint32_t long_min, const uint8_t* __restrict__ act) {
+  const uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
+  fl_tail tl[FL_BUCKET_PER_THREAD];
+  load_tails(base, n, hc, cp, rt, is_outlet, act, tl);  // in flight while the guard is read
   if (*guard) return;  // pointer jumping has not converged: the iteration is redone
   __shared__ uint32_t s_c[FL_LDS_BUCKETS], s_n[FL_LDS_BUCKETS], s_bc[FL_LDS_BUCKETS], s_bn[FL_LDS_BUCKETS];
   for (uint32_t t = threadIdx.x; t < FL_LDS_BUCKETS; t += FL_BUCKET_BLOCK) {
@@ -592,25 +626,20 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
     s_n[t] = 0;
   }
   __syncthreads();
-  uint32_t base = xcd_block() * (FL_BUCKET_BLOCK * FL_BUCKET_PER_THREAD);
   uint32_t lc[FL_BUCKET_PER_THREAD], ln[FL_BUCKET_PER_THREAD];
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
-    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
     lc[j] = FL_NONE;
     ln[j] = 0;
-    if (v < n && act[v] && hc[v] == FL_NONE) {
-      unsigned long long r = rt[v];
-      if (is_outlet[pair_ptr(r)]) {
-        unsigned long long c = cp[v];
-        uint32_t len = pair_cnt(c) + 1, l = 2u * pair_cnt(r) + (len >= long_min ? 1u : 0u);
-        if (l < FL_LDS_BUCKETS) {
-          lc[j] = atomicAdd(s_c + l, 1u);
-          ln[j] = atomicAdd(s_n + l, len);
-        } else {
-          uint32_t q = boff[l] + atomicAdd(bcur + l, 1u);
-          uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
-          place_chain(q, pos, len, pair_ptr(c), cmeta, head_chain, head_base, pdest);
-        }
+    if (tl[j].t) {
+      const unsigned long long c = tl[j].c;
+      uint32_t len = pair_cnt(c) + 1, l = 2u * pair_cnt(tl[j].r) + (len >= long_min ? 1u : 0u);
+      if (l < FL_LDS_BUCKETS) {
+        lc[j] = atomicAdd(s_c + l, 1u);
+        ln[j] = atomicAdd(s_n + l, len);
+      } else {
+        uint32_t q = boff[l] + atomicAdd(bcur + l, 1u);
+        uint32_t pos = bnoff[l] + atomicAdd(bncur + l, len);
+        place_chain(q, pos, len, pair_ptr(c), cmeta, head_chain, head_base, pdest);
       }
     }
   }
@@ -624,10 +653,9 @@ k_bucket_place(const uint32_t* __restrict__ guard, uint32_t n, const uint32_t* _
   __syncthreads();
   for (uint32_t j = 0; j < FL_BUCKET_PER_THREAD; ++j) {
     if (lc[j] == FL_NONE) continue;
-    uint32_t v = base + j * FL_BUCKET_BLOCK + threadIdx.x;
-    unsigned long long c = cp[v];
+    const unsigned long long c = tl[j].c;
     uint32_t len = pair_cnt(c) + 1;
-    uint32_t l = 2u * pair_cnt(rt[v]) + (len >= long_min ? 1u : 0u);
+    uint32_t l = 2u * pair_cnt(tl[j].r) + (len >= long_min ? 1u : 0u);
     uint32_t q = boff[l] + s_bc[l] + lc[j];
     uint32_t pos = bnoff[l] + s_bn[l] + ln[j];
     place_chain(q, pos, len, pair_ptr(c), cmeta, head_chain, head_base, pdest);
