@@ -16,10 +16,10 @@ struct fl_tuning_row {
 };
 
 static const fl_tuning_row fl_tuning_table[] = {
-    {32768ull, 0, 256, 6, 16, 1, 65536u},  // terrain 16384 sites: 2.81 -> 2.76 ms
-    {131072ull, 0, 256, 4, 32, 1, 65536u},  // terrain 65536 sites: 6.16 -> 6.00 ms
-    {524288ull, 0, 256, 6, 32, 1, 65536u},  // terrain 262144 sites: 19.15 -> 19.15 ms
-    {UINT64_MAX, 0, 256, 6, 16, 1, 65536u},  // terrain 1048576 sites: 53.92 -> 53.92 ms
-    {1317794ull, 1, 256, 6, 16, 1, 65536u},  // batch of 32 terrains, 1000..65536 sites (540264 in all): 35.01 -> 35.01 ms
-    {UINT64_MAX, 1, 256, 6, 16, 1, 65536u},  // batch of 64 terrains, 1000..262144 sites (3214323 in all): 234.59 -> 234.59 ms
+    {32768ull, 0, 128, 6, 32, 1, 65536u},  // terrain 16384 sites: 2.41 -> 2.35 ms
+    {131072ull, 0, 256, 4, 32, 1, 65536u},  // terrain 65536 sites: 5.46 -> 5.46 ms
+    {524288ull, 0, 256, 6, 32, 1, 65536u},  // terrain 262144 sites: 18.23 -> 18.23 ms
+    {UINT64_MAX, 0, 256, 6, 16, 1, 65536u},  // terrain 1048576 sites: 48.06 -> 48.06 ms
+    {1317794ull, 1, 128, 6, 16, 1, 65536u},  // batch of 32 terrains, 1000..65536 sites (540264 in all): 33.04 -> 32.47 ms
+    {UINT64_MAX, 1, 256, 6, 16, 1, 65536u},  // batch of 64 terrains, 1000..262144 sites (3214323 in all): 216.62 -> 216.62 ms
 };
