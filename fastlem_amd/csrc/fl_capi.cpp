@@ -270,7 +270,7 @@ RowError first_row_error(uint64_t n, C check_row) {
 int32_t check_header(const fl_terrain_desc* d, uint32_t* max_row) {
   *max_row = 0;
   if (!d) return set_err(FL_ERR_INVALID, "null terrain description");
-  if (d->n >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "n=%llu exceeds 2^32-2 sites", (unsigned long long)d->n);
+  if (d->n > FL_MAX_SITES) return set_err(FL_ERR_TOO_LARGE, "n=%llu exceeds 2^31-1 sites", (unsigned long long)d->n);
   if (d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "nnz exceeds 2^32-2 entries");
   if (d->n == 0) return FL_OK;
   if (!d->row_ptr || (d->nnz && (!d->col || !d->length)) || !d->area || !d->erodibility || !d->uplift_rate ||
@@ -1433,7 +1433,8 @@ int32_t create_union(fl_ctx* c, const fl_terrain_desc* descs, uint32_t count, fl
     NNZ += descs[g].nnz;
     K += descs[g].n_outlets;
   }
-  if (N >= 0xffffffffull || NNZ >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "batch exceeds 2^32-2 sites/entries");
+  if (N > FL_MAX_SITES || NNZ >= 0xffffffffull)
+    return set_err(FL_ERR_TOO_LARGE, "batch exceeds 2^31-1 sites or 2^32-2 entries");
   HIPCHK(hipSetDevice(c->device));
   std::unique_ptr<fl_terrain, int32_t (*)(fl_terrain*)> T(new fl_terrain(), fl_terrain_destroy);
   T->ctx = c;
@@ -2030,6 +2031,13 @@ int32_t lane_gather(fl_lane* L, std::vector<std::pair<fl_terrain*, fl_lane_membe
   N->t_active.assign(G, 1);
   N->t_asym.assign(G, 0);
   bool clamp = false;
+  uint64_t total_sites = 0, total_entries = 0;
+  for (const Part& p : parts) {
+    total_sites += p.src->site_off[p.g + 1] - p.src->site_off[p.g];
+    total_entries += p.src->entry_off[p.g + 1] - p.src->entry_off[p.g];
+  }
+  if (total_sites > FL_MAX_SITES || total_entries >= 0xffffffffull)
+    return set_err(FL_ERR_TOO_LARGE, "lane union exceeds 2^31-1 sites or 2^32-2 entries");
   for (uint32_t q = 0; q < G; q++) {
     const Part& p = parts[q];
     fl_terrain* S = p.src;
@@ -2412,8 +2420,8 @@ extern "C" int32_t fl_generate_f64(fl_ctx* c, const fl_terrain_desc* d, double* 
 extern "C" int32_t fl_generate_f64_noalloc(fl_ctx* c, const fl_terrain_desc* d, double* elevations_out,
                                            uint32_t* iterations_out) {
   if (!c || !d) return set_err(FL_ERR_INVALID, "null context or description");
-  if (d->n >= 0xffffffffull || d->nnz >= 0xffffffffull)
-    return set_err(FL_ERR_TOO_LARGE, "n=%llu, nnz=%llu exceed 2^32-2", (unsigned long long)d->n,
+  if (d->n > FL_MAX_SITES || d->nnz >= 0xffffffffull)
+    return set_err(FL_ERR_TOO_LARGE, "n=%llu, nnz=%llu exceed 2^31-1 sites or 2^32-2 entries", (unsigned long long)d->n,
                    (unsigned long long)d->nnz);
   uint8_t* base = nullptr;
   uint64_t size = 0;
@@ -2711,7 +2719,7 @@ extern "C" int32_t fl_stage_visit_order(fl_ctx* c, fl_terrain* T, uint32_t* rank
 
 extern "C" int32_t fl_request_device_bytes(const fl_terrain_desc* d, uint64_t* bytes) {
   if (!d || !bytes) return set_err(FL_ERR_INVALID, "null argument");
-  if (d->n >= 0xffffffffull || d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "terrain too large");
+  if (d->n > FL_MAX_SITES || d->nnz >= 0xffffffffull) return set_err(FL_ERR_TOO_LARGE, "terrain too large");
   bool clamp = false;
   if (d->has_max_slope)
     for (uint64_t i = 0; i < d->n && !clamp; i++) clamp = d->has_max_slope[i] != 0;

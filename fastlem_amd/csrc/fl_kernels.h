@@ -40,6 +40,11 @@ enum { FL_MAX_JUMP_ROUNDS = 96 };  // one "pointers moved" word per doubling rou
 enum { FL_CTRL_JUMPS = FL_NUM_FLAGS + FL_NUM_COUNTERS + 2 * (FL_CTRL_HEAD + 1) };
 enum { FL_CTRL_WORDS = FL_CTRL_JUMPS + FL_MAX_JUMP_ROUNDS };
 
+// Most sites of a terrain or a union of terrains: the pointer jumping keeps
+// its counts (chain positions, light depths) below 2^31, the top bit of the
+// count word marking a pair at its fixed point.
+#define FL_MAX_SITES 0x7fffffffull
+
 // Read-only inputs of one terrain, resident in HBM.
 struct fl_dev_terrain {
   uint32_t n = 0;

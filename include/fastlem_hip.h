@@ -13,8 +13,9 @@
  *   - Every function returns int32_t: FL_OK (0) or a negative FL_ERR_* code;
  *     fl_last_error() returns the calling thread's last message.
  *   - Host pointers are borrowed for the duration of the call only.
- *   - Indices are uint32_t; a terrain may hold up to 2^32-1 sites and 2^32-1
- *     directed adjacency entries.
+ *   - Indices are uint32_t; a terrain, or a batch or served union of
+ *     terrains, may hold up to 2^31-1 sites and 2^32-2 directed adjacency
+ *     entries (FL_ERR_TOO_LARGE beyond).
  *   - Arithmetic is IEEE binary64 evaluated exactly as the reference writes
  *     it (no contraction), so results are bit-identical to the reference
  *     given the same inputs.

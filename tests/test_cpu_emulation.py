@@ -474,6 +474,33 @@ def _solve_with_tuning(L, ctx, h, tuning, n):
     return out, it.value
 
 
+def test_emulated_site_limit(emu_lib):
+    """Terrains beyond 2^31 - 1 sites (chain positions and light depths of
+    the pointer jumping stay below 2^31) are refused with FL_ERR_TOO_LARGE
+    before any array is read or any memory taken, by every entry point that
+    takes a description; batches and served unions sum their sites first."""
+    from fastlem_amd import _native
+
+    L = emu_lib
+    d = _native.TerrainDesc()
+    d.n = 1 << 31  # the arrays stay null: the size is refused first
+    b = ctypes.c_uint64()
+    assert L.fl_request_device_bytes(ctypes.byref(d), ctypes.byref(b)) == -4
+    assert b"too large" in L.fl_last_error()
+    ctx = ctypes.c_void_p()
+    assert L.fl_ctx_create(0, ctypes.byref(ctx)) == 0
+    try:
+        h = ctypes.c_void_p()
+        assert L.fl_terrain_create(ctx, ctypes.byref(d), ctypes.byref(h)) == -4
+        assert b"2^31-1" in L.fl_last_error()
+        out = np.empty(1)
+        it = ctypes.c_uint32()
+        assert L.fl_generate_f64_noalloc(ctx, ctypes.byref(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         ctypes.byref(it)) == -4
+    finally:
+        L.fl_ctx_destroy(ctx)
+
+
 def test_emulated_tunings_give_identical_results(emu_lib):
     """Every launch tuning -- workgroup size, the short/long chain switch
     (all chains short, all long), tail launches off and wide, pipelining
