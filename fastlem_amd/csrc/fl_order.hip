@@ -145,18 +145,37 @@ __global__ void k_bv_best(uint32_t n, const uint32_t* __restrict__ rp, const uin
                           unsigned long long* __restrict__ cbest, uint32_t* __restrict__ vbest) {
   uint32_t v = gid();
   if (v >= n) return;
-  uint32_t cv = comp[v], be = FL_NONE;
+  const uint32_t cv = comp[v], b0 = rp[v], e1 = rp[v + 1];
+  uint32_t be = FL_NONE;
   unsigned long long best = ~0ull;
-  for (uint32_t e = rp[v]; e < rp[v + 1]; ++e) {
-    if (comp[col[e]] == cv) continue;
-    unsigned long long b = len_bits(len[e]);
-    if (b < best) {
-      best = b;
-      be = e;
+  // eight entries at a time, every load of a group issued before the first
+  // is used (past the row's end: its last entry again, not evaluated)
+  for (uint32_t e = b0; e < e1; e += 8) {
+    uint32_t c[8], k[8];
+    double w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t ej = e + j < e1 ? e + j : e1 - 1;
+      c[j] = col[ej];
+      w[j] = len[ej];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) k[j] = comp[c[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (e + j >= e1 || k[j] == cv) continue;
+      const unsigned long long b = len_bits(w[j]);
+      if (b < best) {
+        best = b;
+        be = e + j;
+      }
     }
   }
   vbest[v] = be;
-  if (be != FL_NONE) atomicMin(cbest + cv, best);
+  // the sites of a large component all aim at one word: an atomic only where
+  // the word (possibly an earlier value: it only decreases) is not already
+  // as small
+  if (be != FL_NONE && best < __atomic_load_n(cbest + cv, __ATOMIC_RELAXED)) atomicMin(cbest + cv, best);
 }
 
 // The site owning its component's lightest edge records it (unique:
