@@ -203,17 +203,33 @@ __global__ void k_bv_hook(uint32_t n, const uint32_t* __restrict__ col, const ui
                           uint32_t* __restrict__ msf_a, uint32_t* __restrict__ msf_b,
                           unsigned long long* __restrict__ msf_w, uint32_t* __restrict__ count) {
   uint32_t c = gid();
-  if (c >= n) return;
-  parent[c] = c;
-  if (comp[c] != c || cbest[c] == ~0ull) return;
-  uint32_t e = cedge[c];
-  uint32_t t = comp[col[e]];
-  if (cbest[t] == cbest[c] && c < t) return;
-  parent[c] = t;
-  uint32_t slot = atomicAdd(count, 1u);
-  msf_a[slot] = csrc[c];
-  msf_b[slot] = col[e];
-  msf_w[slot] = cbest[c];
+  bool hook = false;
+  uint32_t e = 0, t = 0;
+  unsigned long long w = 0;
+  if (c < n) {
+    parent[c] = c;
+    w = cbest[c];
+    if (comp[c] == c && w != ~0ull) {
+      e = cedge[c];
+      t = comp[col[e]];
+      hook = !(cbest[t] == w && c < t);
+    }
+  }
+  if (hook) parent[c] = t;
+  // forest slots: one counter atomic per wavefront (the order of the slots
+  // is immaterial: the forest's edges are sorted by length afterwards)
+  const unsigned long long m = __ballot(hook);
+  if (m == 0) return;
+  const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (hook) {
+    const uint32_t slot = base + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+    msf_a[slot] = csrc[c];
+    msf_b[slot] = col[e];
+    msf_w[slot] = w;
+  }
 }
 
 // One in-place doubling round on the hook pointers; a round after one that
