@@ -343,8 +343,15 @@ __global__ void k_list_jump(uint32_t M, unsigned long long* list, uint32_t* __re
     unsigned long long x = __atomic_load_n(list + p, __ATOMIC_RELAXED);
     uint32_t nx = pr_ptr(x);
     if (nx != FL_NONE) {
+      // two hops: (p'', c + c' + c''), the reach triples per round
       unsigned long long y = __atomic_load_n(list + nx, __ATOMIC_RELAXED);
-      __atomic_store_n(list + p, pr(pr_ptr(y), pr_cnt(x) + pr_cnt(y)), __ATOMIC_RELAXED);
+      const uint32_t ny = pr_ptr(y);
+      if (ny != FL_NONE) {
+        unsigned long long z = __atomic_load_n(list + ny, __ATOMIC_RELAXED);
+        __atomic_store_n(list + p, pr(pr_ptr(z), pr_cnt(x) + pr_cnt(y) + pr_cnt(z)), __ATOMIC_RELAXED);
+      } else {
+        __atomic_store_n(list + p, pr(FL_NONE, pr_cnt(x) + pr_cnt(y)), __ATOMIC_RELAXED);
+      }
       moved = true;
     }
   }
@@ -441,7 +448,11 @@ __global__ void k_q_jump(uint32_t m, const uint32_t* __restrict__ child, const u
     if (c != FL_NONE) {
       const uint32_t qc = __atomic_load_n(q + c, __ATOMIC_RELAXED);
       if (wt[qc] < j) {
-        __atomic_store_n(q + c, __atomic_load_n(q + qc, __ATOMIC_RELAXED), __ATOMIC_RELAXED);
+        // up to two steps a round: past qc, and past its q when that is
+        // lighter too
+        const uint32_t q2 = __atomic_load_n(q + qc, __ATOMIC_RELAXED);
+        const uint32_t q3 = wt[q2] < j ? __atomic_load_n(q + q2, __ATOMIC_RELAXED) : q2;
+        __atomic_store_n(q + c, q3, __ATOMIC_RELAXED);
         moved = true;
       }
     }
@@ -510,8 +521,15 @@ __global__ void k_qtour_jump(uint32_t M, const uint32_t* __restrict__ qs, const 
     unsigned long long x = __atomic_load_n(qt + e, __ATOMIC_RELAXED);
     uint32_t nx = pr_ptr(x);
     if (nx != FL_NONE) {
+      // two hops, as k_list_jump
       unsigned long long y = __atomic_load_n(qt + nx, __ATOMIC_RELAXED);
-      __atomic_store_n(qt + e, pr(pr_ptr(y), pr_cnt(x) + pr_cnt(y)), __ATOMIC_RELAXED);
+      const uint32_t ny = pr_ptr(y);
+      if (ny != FL_NONE) {
+        unsigned long long z = __atomic_load_n(qt + ny, __ATOMIC_RELAXED);
+        __atomic_store_n(qt + e, pr(pr_ptr(z), pr_cnt(x) + pr_cnt(y) + pr_cnt(z)), __ATOMIC_RELAXED);
+      } else {
+        __atomic_store_n(qt + e, pr(FL_NONE, pr_cnt(x) + pr_cnt(y)), __ATOMIC_RELAXED);
+      }
       moved = true;
     }
   }
@@ -779,10 +797,10 @@ int fl_device_ranks(hipStream_t s, uint32_t n, const uint32_t* rp, const uint32_
                                             s));
   hipLaunchKernelGGL(k_arc_pos, G2m, B, 0, s, 2 * m, r.skey, r.sarc, r.pos, r.start);
   hipLaunchKernelGGL(k_arc_tour, G2m, B, 0, s, 2 * m, r.skey, r.sarc, r.pos, r.start, is_outlet, r.tour);
-  // a list of L elements is ranked after ceil(log2 L) rounds (each round at
-  // least doubles every pointer's reach); one more proves it
+  // a list of L elements is ranked after ceil(log3 L) rounds (each round at
+  // least triples every pointer's reach); one more proves it
   uint32_t lg = 1;
-  while ((1ull << lg) < 2ull * m) lg++;
+  for (uint64_t reach = 3; reach < 2ull * m; reach *= 3) lg++;
   hipLaunchKernelGGL(k_fill32, dim3(1), B, 0, s, kMaxFlags, r.flags, 0u);
   int rc = jump_to_fixpoint(
       s, r.flags, kMaxFlags, lg + 1,
@@ -875,11 +893,11 @@ int fl_device_subtree_sizes(hipStream_t s, uint32_t n, const uint32_t* next, con
   hipLaunchKernelGGL(k_fill32, dim3(1), B, 0, s, kMaxFlags, flags, 0u);
   hipLaunchKernelGGL(k_st_link, Gn, B, 0, s, n, next, tid, t_active, head, sib);
   hipLaunchKernelGGL(k_st_tour, Gn, B, 0, s, n, next, tid, t_active, head, sib, tour);
-  // a list of L arcs is ranked after ceil(log2 L) rounds (each round at
-  // least doubles every pointer's reach), rounds after a quiet one return
+  // a list of L arcs is ranked after ceil(log3 L) rounds (each round at
+  // least triples every pointer's reach), rounds after a quiet one return
   // at once; nothing is read back: the sizes are a hint, the stream goes on
   uint32_t lg = 1;
-  while ((1ull << lg) < 2ull * n) lg++;
+  for (uint64_t reach = 3; reach < 2ull * n; reach *= 3) lg++;
   for (uint32_t r = 0; r < lg && r < kMaxFlags; r++)
     hipLaunchKernelGGL(k_list_jump, G2, B, 0, s, 2u * n, tour, flags + r, r ? flags + r - 1 : nullptr);
   hipLaunchKernelGGL(k_st_size, Gn, B, 0, s, n, next, tid, t_active, tour, size);
