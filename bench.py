@@ -350,7 +350,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solve runs on one GPU; --gpus > 1 is not supported "
-                                   "(terrains shard across GPUs without communication, see DESIGN.md)"}))
+                                   "(independent terrains would shard across GPUs without communication)"}))
         return 2
 
     if args.mode == "copy":
